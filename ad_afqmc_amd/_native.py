@@ -1,0 +1,113 @@
+"""ctypes binding of libafq.so (the C ABI declared in include/afq.h).
+
+The library is the only compute path: there is no Python or CPU fallback.  If
+libafq.so is missing or fails to load, `lib()` raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import threading
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libafq.so")
+
+AFQ_OK = 0
+AFQ_WALKERS_RESTRICTED, AFQ_WALKERS_UNRESTRICTED = 0, 1
+AFQ_TRIAL_RHF, AFQ_TRIAL_UHF = 0, 1
+AFQ_PREC_FP64, AFQ_PREC_FP32, AFQ_PREC_BF16X3 = 0, 1, 2
+PRECISIONS = {"fp64": AFQ_PREC_FP64, "fp32": AFQ_PREC_FP32, "bf16x3": AFQ_PREC_BF16X3}
+(AFQ_INT_H1, AFQ_INT_ROT_H1, AFQ_INT_ROT_CHOL, AFQ_INT_MF_SHIFTS, AFQ_INT_H0_PROP, AFQ_INT_EXP_H1) = range(6)
+
+
+class JobDesc(C.Structure):
+    _fields_ = [
+        ("norb", C.c_int32),
+        ("nelec_up", C.c_int32),
+        ("nelec_dn", C.c_int32),
+        ("nchol", C.c_int32),
+        ("n_walkers", C.c_int32),
+        ("walker_type", C.c_int32),
+        ("trial_type", C.c_int32),
+        ("n_exp_terms", C.c_int32),
+        ("precision", C.c_int32),
+        ("reserved", C.c_int32),
+        ("dt", C.c_double),
+    ]
+
+
+class AfqError(RuntimeError):
+    """Raised when a libafq.so entry point returns a non-zero status."""
+
+    def __init__(self, fn: str, code: int, msg: str):
+        super().__init__(f"{fn} failed ({code}): {msg}")
+        self.code = code
+
+
+P = C.c_void_p
+_SIGNATURES = {
+    "afq_version": ([], C.c_char_p),
+    "afq_last_error": ([], C.c_char_p),
+    "afq_job_create": ([C.POINTER(P), C.POINTER(JobDesc)], C.c_int),
+    "afq_job_destroy": ([P], C.c_int),
+    "afq_job_set_batch": ([P, C.c_int32], C.c_int),
+    "afq_job_set_hamiltonian": ([P, C.c_double, P, P, P], C.c_int),
+    "afq_job_set_trial": ([P, P, P, P, P], C.c_int),
+    "afq_build_measurement_intermediates": ([P, P], C.c_int),
+    "afq_build_propagation_intermediates": ([P, P], C.c_int),
+    "afq_get_intermediate": ([P, C.c_int, C.c_int, P, P], C.c_int),
+    "afq_set_intermediate": ([P, C.c_int, C.c_int, P, P], C.c_int),
+    "afq_calc_overlap": ([P, P, P, P, P], C.c_int),
+    "afq_calc_green": ([P, P, P, P, P, P], C.c_int),
+    "afq_calc_force_bias": ([P, P, P, P, P], C.c_int),
+    "afq_calc_energy": ([P, P, P, P, P], C.c_int),
+    "afq_apply_trotprop": ([P, P, P, P, P], C.c_int),
+    "afq_propagate_constrained": ([P, P, P, P, P, P, P, P, P], C.c_int),
+    "afq_orthonormalize": ([P, P, P, P], C.c_int),
+    "afq_stochastic_reconfiguration": ([P, P, P, P, P, P, P], C.c_int),
+    "afq_block": ([P, P, P, P, P, P, C.c_int, C.c_uint64, C.POINTER(C.c_uint64), P, P], C.c_int),
+    "afq_normal_fill": ([P, C.c_int64, C.c_uint64, C.c_uint64, P], C.c_int),
+    "afq_walkers_orthonormalize": ([P, C.c_int32, C.c_int32, C.c_int32, P], C.c_int),
+    "afq_walkers_sr_workspace": ([C.c_int32, C.c_int64], C.c_int64),
+    "afq_walkers_sr": ([P, C.c_int64, P, C.c_int64, C.c_int32, P, P, P, P, P], C.c_int),
+}
+
+_lib = None
+_lock = threading.Lock()
+
+
+def lib():
+    """Loads libafq.so once; raises if it is absent (no fallback exists)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    with _lock:
+        if _lib is None:
+            if not os.path.exists(LIB_PATH):
+                raise RuntimeError(
+                    f"{LIB_PATH} is missing: build it with `python -m ad_afqmc_amd.csrc.build` "
+                    "(there is no CPU fallback)"
+                )
+            L = C.CDLL(LIB_PATH)
+            for name, (args, res) in _SIGNATURES.items():
+                fn = getattr(L, name)
+                fn.argtypes = args
+                fn.restype = res
+            _lib = L
+    return _lib
+
+
+def exported_symbols():
+    return list(_SIGNATURES)
+
+
+def check(fn: str, rc: int):
+    if rc != AFQ_OK:
+        msg = lib().afq_last_error().decode(errors="replace")
+        raise AfqError(fn, rc, msg)
+
+
+def call(fn: str, *args):
+    rc = getattr(lib(), fn)(*args)
+    check(fn, rc)
+    return rc

@@ -1,0 +1,132 @@
+// Internal declarations of the MI355X AFQMC engine (not part of the C ABI).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdint>
+#include <string>
+
+#include "afq.h"
+
+typedef double v4d __attribute__((ext_vector_type(4)));
+typedef float v4f __attribute__((ext_vector_type(4)));
+
+namespace afq {
+
+// ---------------------------------------------------------------- errors
+void set_error(const std::string& msg);
+int fail(int code, const std::string& msg);
+
+#define AFQ_HIP(expr)                                                                 \
+  do {                                                                                \
+    hipError_t e_ = (expr);                                                           \
+    if (e_ != hipSuccess)                                                             \
+      return ::afq::fail(AFQ_E_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+#define AFQ_TRY(expr)            \
+  do {                           \
+    int rc_ = (expr);            \
+    if (rc_ != AFQ_OK) return rc_; \
+  } while (0)
+
+inline int launch_status(const char* what) {
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return fail(AFQ_E_HIP, std::string(what) + ": " + hipGetErrorString(e));
+  return AFQ_OK;
+}
+
+// ---------------------------------------------------------------- GEMM
+// Batched C[b] = alpha * opA(A[b]) * opB(B[b]) + beta * C[b]   (and D[b] += result
+// when D != nullptr).  Operands are real (double) or interleaved complex
+// (double2); C (and D) are complex when either operand is.  Row-major:
+//   A element (m,k) at A[m*lda+k]        (a_trans: A[k*lda+m])
+//   B element (k,n) at B[k*ldb+n]        (b_trans: B[n*ldb+k])
+//   C element (m,n) at C[m*ldc+n]
+// Leading dimensions and batch strides count elements (real or complex).
+struct Gemm {
+  int M = 0, N = 0, K = 0, batch = 1;
+  bool a_cplx = false, b_cplx = false, a_trans = false, b_trans = false;
+  const void* A = nullptr;
+  long lda = 0, sA = 0;
+  const void* B = nullptr;
+  long ldb = 0, sB = 0;
+  void* C = nullptr;
+  long ldc = 0, sC = 0;
+  void* D = nullptr;
+  long ldd = 0, sD = 0;
+  double2 alpha = {1.0, 0.0};
+  double2 beta = {0.0, 0.0};
+};
+int gemm_f64(const Gemm& g, hipStream_t stream);
+
+// ---------------------------------------------------------------- small dense linear algebra
+// In-place Gauss-Jordan with partial pivoting (|re|+|im| pivot metric, first
+// maximum wins, as LAPACK's izamax) on `count` n x n complex matrices.  Writes
+// det (count,) and, if inv != nullptr, the inverse.  n <= 64.
+int batched_det_inv(const double2* S, long strideS, int n, int count, double2* det, double2* inv, long strideInv,
+                    hipStream_t stream);
+
+// Classical Gram-Schmidt with re-orthogonalisation (CGS2) on `count` row-major
+// n_rows x n_cols complex panels (leading dimension ld, batch stride sW), in place.
+int batched_cgs2(double2* Wp, long ld, long sW, int n_rows, int n_cols, int count, hipStream_t stream);
+
+}  // namespace afq
+
+// ---------------------------------------------------------------- job
+struct afq_job {
+  afq_job_desc d{};
+  int N = 0, G = 0, W = 0;  // W: active walkers (<= Wcap)
+  int Wcap = 0;
+  int nocc[2] = {0, 0};
+  int wcols = 0;             // columns of the restricted walker buffer (nelec_up)
+  bool restricted = true;    // restricted walkers (one buffer)
+  bool rhf = true;           // RHF trial (mo_dn == mo_up)
+  bool one_spin = true;      // spin-dn quantities equal spin-up (RHF trial and restricted walkers)
+  int nspin_meas = 1;        // spin channels measured (1 when one_spin)
+  int nspin_prop = 1;        // walker buffers propagated
+
+  double h0 = 0.0;
+  bool have_ham = false, have_trial = false, have_meas = false, have_prop = false;
+  // per-item flags for intermediates supplied through afq_set_intermediate
+  bool have_rot_h1[2] = {false, false}, have_rot_chol[2] = {false, false};
+  bool have_mf = false, have_h0p = false, have_exp[2] = {false, false};
+
+  // resident data
+  double* h1 = nullptr;        // (2,N,N) symmetrised after measurement build
+  double* chol = nullptr;      // (G,N*N)
+  double* mo[2] = {nullptr, nullptr};      // (N, nocc_s)
+  double* rdm1 = nullptr;      // (2,N,N)
+  double* rot_h1[2] = {nullptr, nullptr};  // (nocc_s, N)
+  double* rot_chol[2] = {nullptr, nullptr}; // (G, nocc_s, N)
+  double2* mf_shifts = nullptr; // (G,)
+  double* mreal = nullptr;      // (2G,) m_g = sum L_g rho, and -m_g
+  double2* h0_prop = nullptr;   // device scalar
+  double2 h0_prop_host = {0, 0};
+  double* exp_h1[2] = {nullptr, nullptr}; // (N,N)
+
+  // scratch (per job)
+  double2* S[2] = {nullptr, nullptr};     // (W, nocc, nocc)
+  double2* Sinv[2] = {nullptr, nullptr};  // (W, nocc, nocc)
+  double2* det[2] = {nullptr, nullptr};   // (W,)
+  double2* green[2] = {nullptr, nullptr}; // (W, nocc, N)
+  double2* fb = nullptr;                  // (W, G)
+  double2* xs = nullptr;                  // (W, G) shifted fields
+  double2* terms = nullptr;               // (W, 2) shift_term, fb_term
+  double2* vhs = nullptr;                 // (W, N, N)
+  double2* tbuf[3] = {nullptr, nullptr, nullptr}; // (W, N, wmax) Taylor buffers
+  double2* ov_new = nullptr;              // (W,)
+  double2* coul = nullptr;                // (2, W, G) c_gamma per spin
+  double2* kpart = nullptr;               // (2, W, nchunk) exchange partials
+  double2* ene1 = nullptr;                // (W,)
+  double2* energies = nullptr;            // (W,)
+  double* fields = nullptr;               // (n_prop_steps_max, W, G) for afq_block
+  long fields_cap = 0;
+  double2* wgather = nullptr;             // (W, N, wmax) SR gather buffer
+  int32_t* sr_idx = nullptr;              // (W,)
+  double* cum = nullptr;                  // (W,) SR cumulative weights
+  double* dscratch = nullptr;             // small device scalars
+  int kchunk_max = 1;                     // exchange partials per walker
+  double* tmp_nn = nullptr;               // (4, N, N) setup temporaries
+};

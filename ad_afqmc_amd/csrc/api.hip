@@ -1,0 +1,833 @@
+// C ABI of the engine: job lifetime, setup intermediates, measurements,
+// propagation, re-orthonormalisation, population control and the device
+// block loop.  Every kernel runs on the caller's stream.
+#include <cmath>
+#include <cstring>
+#include <string>
+
+#include "kernels.h"
+
+namespace afq {
+static thread_local std::string g_err;
+void set_error(const std::string& m) { g_err = m; }
+int fail(int code, const std::string& m) {
+  g_err = m;
+  return code;
+}
+}  // namespace afq
+
+using namespace afq;
+
+namespace {
+
+inline hipStream_t S(void* s) { return reinterpret_cast<hipStream_t>(s); }
+
+int dalloc(void** p, size_t bytes) {
+  if (bytes == 0) bytes = 16;
+  hipError_t e = hipMalloc(p, bytes);
+  if (e != hipSuccess) return fail(AFQ_E_NOMEM, std::string("hipMalloc failed: ") + hipGetErrorString(e));
+  return AFQ_OK;
+}
+template <typename T>
+int dalloc_n(T** p, size_t n) {
+  return dalloc(reinterpret_cast<void**>(p), n * sizeof(T));
+}
+
+void release(afq_job* j) {
+  auto fr = [](void* p) {
+    if (p) (void)hipFree(p);
+  };
+  fr(j->h1);
+  fr(j->chol);
+  fr(j->mo[0]);
+  if (!j->rhf) fr(j->mo[1]);
+  fr(j->rdm1);
+  fr(j->rot_h1[0]);
+  if (!j->rhf) fr(j->rot_h1[1]);
+  fr(j->rot_chol[0]);
+  if (!j->rhf) fr(j->rot_chol[1]);
+  fr(j->mf_shifts);
+  fr(j->mreal);
+  fr(j->h0_prop);
+  fr(j->exp_h1[0]);
+  fr(j->exp_h1[1]);
+  for (int s = 0; s < 2; ++s) {
+    fr(j->S[s]);
+    fr(j->Sinv[s]);
+    fr(j->det[s]);
+    fr(j->green[s]);
+  }
+  fr(j->fb);
+  fr(j->xs);
+  fr(j->terms);
+  fr(j->vhs);
+  for (auto& t : j->tbuf) fr(t);
+  fr(j->ov_new);
+  fr(j->coul);
+  fr(j->kpart);
+  fr(j->ene1);
+  fr(j->energies);
+  fr(j->fields);
+  fr(j->wgather);
+  fr(j->sr_idx);
+  fr(j->cum);
+  fr(j->tmp_nn);
+  fr(j->dscratch);
+}
+
+// Walker panel of spin s: pointer and row stride (columns of the buffer).
+inline void panel(const afq_job* j, int s, const void* up, const void* dn, const double2** p, long* ld) {
+  if (j->restricted) {
+    *p = static_cast<const double2*>(up);
+    *ld = j->wcols;
+  } else {
+    *p = static_cast<const double2*>(s == 0 ? up : dn);
+    *ld = j->nocc[s];
+  }
+}
+
+int check_walkers(const afq_job* j, const void* up, const void* dn) {
+  if (!up) return fail(AFQ_E_INVALID, "walkers_up is NULL");
+  if (!j->restricted && !dn) return fail(AFQ_E_INVALID, "unrestricted walkers need walkers_dn");
+  return AFQ_OK;
+}
+
+// S_s = T_s^H W_s ; det_s ; optionally S_s^-1 and G_s = (W_s S_s^-1)^T.
+int green_pipeline(afq_job* j, const void* up, const void* dn, bool want_inv, bool want_green, hipStream_t st) {
+  const int N = j->N, W = j->W;
+  for (int s = 0; s < j->nspin_meas; ++s) {
+    const int O = j->nocc[s];
+    const double2* P;
+    long ld;
+    panel(j, s, up, dn, &P, &ld);
+    Gemm g;
+    g.M = O;
+    g.N = O;
+    g.K = N;
+    g.batch = W;
+    g.a_trans = true;
+    g.A = j->mo[s];
+    g.lda = O;
+    g.sA = 0;
+    g.b_cplx = true;
+    g.B = P;
+    g.ldb = ld;
+    g.sB = (long)N * ld;
+    g.C = j->S[s];
+    g.ldc = O;
+    g.sC = (long)O * O;
+    AFQ_TRY(gemm_f64(g, st));
+    AFQ_TRY(batched_det_inv(j->S[s], (long)O * O, O, W, j->det[s], want_inv ? j->Sinv[s] : nullptr, (long)O * O, st));
+    if (want_green) {
+      Gemm h;
+      h.M = O;
+      h.N = N;
+      h.K = O;
+      h.batch = W;
+      h.a_cplx = true;
+      h.a_trans = true;
+      h.A = j->Sinv[s];
+      h.lda = O;
+      h.sA = (long)O * O;
+      h.b_cplx = true;
+      h.b_trans = true;
+      h.B = P;
+      h.ldb = ld;
+      h.sB = (long)N * ld;
+      h.C = j->green[s];
+      h.ldc = N;
+      h.sC = (long)O * N;
+      AFQ_TRY(gemm_f64(h, st));
+    }
+  }
+  return AFQ_OK;
+}
+
+int overlap_from_dets(afq_job* j, double2* out, hipStream_t st) {
+  return k_overlap_combine(j->det[0], j->one_spin ? nullptr : j->det[1], out, j->W, st);
+}
+
+// fb = sum_s f_s * Gflat_s (W x O_s N) . rot_chol_s^T (O_s N x G), f = 2 for one spin channel.
+int force_bias_gemm(afq_job* j, double2* out, hipStream_t st) {
+  for (int s = 0; s < j->nspin_meas; ++s) {
+    const int O = j->nocc[s];
+    Gemm g;
+    g.M = j->W;
+    g.N = j->G;
+    g.K = O * j->N;
+    g.a_cplx = true;
+    g.A = j->green[s];
+    g.lda = (long)O * j->N;
+    g.b_trans = true;
+    g.B = j->rot_chol[s];
+    g.ldb = (long)O * j->N;
+    g.C = out;
+    g.ldc = j->G;
+    g.alpha = double2{j->one_spin ? 2.0 : 1.0, 0.0};
+    g.beta = double2{s > 0 ? 1.0 : 0.0, 0.0};
+    AFQ_TRY(gemm_f64(g, st));
+  }
+  return AFQ_OK;
+}
+
+int energy_pipeline(afq_job* j, const void* up, const void* dn, double2* out, hipStream_t st) {
+  AFQ_TRY(green_pipeline(j, up, dn, true, true, st));
+  const bool two = !j->one_spin;
+  AFQ_TRY(k_ene1(j->green[0], j->rot_h1[0], j->nocc[0], two ? j->green[1] : nullptr, two ? j->rot_h1[1] : nullptr,
+                 j->nocc[1], j->N, two ? 1.0 : 2.0, j->ene1, j->W, st));
+  int nch[2] = {1, 1};
+  for (int s = 0; s < j->nspin_meas; ++s)
+    AFQ_TRY(k_exchange(j->rot_chol[s], j->green[s], j->nocc[s], j->N, j->G, j->W, j->coul + (long)s * j->W * j->G,
+                       j->kpart + (long)s * j->W * j->kchunk_max, &nch[s], st));
+  return k_energy_finalize(j->ene1, j->coul, two ? j->coul + (long)j->W * j->G : nullptr, j->kpart,
+                           two ? j->kpart + (long)j->W * j->kchunk_max : nullptr, j->G, nch[0], nch[1], j->h0, out,
+                           j->W, st);
+}
+
+// W <- E . Taylor(VHS) . E . W for each walker buffer.
+int trotprop(afq_job* j, void* up, void* dn, const double2* xs, hipStream_t st) {
+  const int N = j->N, W = j->W;
+  const long NN = (long)N * N;
+  {
+    Gemm g;
+    g.M = W;
+    g.N = (int)NN;
+    g.K = j->G;
+    g.a_cplx = true;
+    g.A = xs;
+    g.lda = j->G;
+    g.B = j->chol;
+    g.ldb = NN;
+    g.C = j->vhs;
+    g.ldc = NN;
+    g.alpha = double2{0.0, std::sqrt(j->d.dt)};
+    AFQ_TRY(gemm_f64(g, st));
+  }
+  for (int s = 0; s < j->nspin_prop; ++s) {
+    double2* Wp = static_cast<double2*>(s == 0 ? up : dn);
+    const int c = j->restricted ? j->wcols : j->nocc[s];
+    const long per = (long)N * c;
+    const double* E = j->exp_h1[j->restricted ? 0 : s];
+    auto apply_E = [&](const double2* src, double2* dst) {
+      Gemm g;
+      g.M = N;
+      g.N = c;
+      g.K = N;
+      g.batch = W;
+      g.A = E;
+      g.lda = N;
+      g.sA = 0;
+      g.b_cplx = true;
+      g.B = src;
+      g.ldb = c;
+      g.sB = per;
+      g.C = dst;
+      g.ldc = c;
+      g.sC = per;
+      return gemm_f64(g, st);
+    };
+    double2* t0 = j->tbuf[0];
+    double2* t1 = j->tbuf[1];
+    double2* acc = j->tbuf[2];
+    AFQ_TRY(apply_E(Wp, t0));
+    AFQ_HIP(hipMemcpyAsync(acc, t0, sizeof(double2) * per * W, hipMemcpyDeviceToDevice, st));
+    double2* src = t0;
+    double2* dst = t1;
+    for (int n = 1; n < j->d.n_exp_terms; ++n) {
+      Gemm g;
+      g.M = N;
+      g.N = c;
+      g.K = N;
+      g.batch = W;
+      g.a_cplx = true;
+      g.A = j->vhs;
+      g.lda = N;
+      g.sA = NN;
+      g.b_cplx = true;
+      g.B = src;
+      g.ldb = c;
+      g.sB = per;
+      g.C = dst;
+      g.ldc = c;
+      g.sC = per;
+      g.D = acc;
+      g.ldd = c;
+      g.sD = per;
+      g.alpha = double2{1.0 / n, 0.0};
+      AFQ_TRY(gemm_f64(g, st));
+      double2* t = src;
+      src = dst;
+      dst = t;
+    }
+    AFQ_TRY(apply_E(acc, Wp));
+  }
+  return AFQ_OK;
+}
+
+int propagate(afq_job* j, void* up, void* dn, const double* fields, double* weights, double2* overlaps,
+              double* scalars, hipStream_t st) {
+  AFQ_TRY(green_pipeline(j, up, dn, true, true, st));
+  AFQ_TRY(force_bias_gemm(j, j->fb, st));
+  AFQ_TRY(k_field_shift(j->fb, fields, j->mf_shifts, j->G, std::sqrt(j->d.dt), j->xs, j->terms, j->W, st));
+  AFQ_TRY(trotprop(j, up, dn, j->xs, st));
+  AFQ_TRY(green_pipeline(j, up, dn, false, false, st));
+  AFQ_TRY(overlap_from_dets(j, j->ov_new, st));
+  return k_weight_update(j->ov_new, overlaps, j->terms, weights, scalars, j->h0_prop, j->d.dt, j->W, st);
+}
+
+int orthonormalize(afq_job* j, void* up, void* dn, hipStream_t st) {
+  const int N = j->N;
+  if (j->restricted) return batched_cgs2(static_cast<double2*>(up), j->wcols, (long)N * j->wcols, N, j->wcols, j->W, st);
+  AFQ_TRY(batched_cgs2(static_cast<double2*>(up), j->nocc[0], (long)N * j->nocc[0], N, j->nocc[0], j->W, st));
+  return batched_cgs2(static_cast<double2*>(dn), j->nocc[1], (long)N * j->nocc[1], N, j->nocc[1], j->W, st);
+}
+
+int require(const afq_job* j, bool meas, bool prop) {
+  if (!j) return fail(AFQ_E_INVALID, "job is NULL");
+  if (!j->have_trial) return fail(AFQ_E_STATE, "trial not set");
+  if (meas && !j->have_meas) return fail(AFQ_E_STATE, "measurement intermediates not built");
+  if (prop && !j->have_prop) return fail(AFQ_E_STATE, "propagation intermediates not built");
+  return AFQ_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* afq_version(void) { return "afq 0.1.0 gfx950"; }
+const char* afq_last_error(void) { return afq::g_err.c_str(); }
+
+int afq_job_create(afq_job_t* out, const afq_job_desc* d) {
+  if (!out || !d) return fail(AFQ_E_INVALID, "NULL argument");
+  *out = nullptr;
+  if (d->norb < 1 || d->nchol < 1 || d->n_walkers < 1 || d->nelec_up < 1 || d->nelec_dn < 1)
+    return fail(AFQ_E_INVALID, "sizes must be positive");
+  if (d->nelec_up > d->norb || d->nelec_dn > d->norb) return fail(AFQ_E_INVALID, "nelec exceeds norb");
+  if (d->walker_type != AFQ_WALKERS_RESTRICTED && d->walker_type != AFQ_WALKERS_UNRESTRICTED)
+    return fail(AFQ_E_INVALID, "unknown walker_type");
+  if (d->trial_type != AFQ_TRIAL_RHF && d->trial_type != AFQ_TRIAL_UHF) return fail(AFQ_E_INVALID, "unknown trial_type");
+  if (d->trial_type == AFQ_TRIAL_RHF && d->nelec_up != d->nelec_dn)
+    return fail(AFQ_E_INVALID, "RHF requires equal number of up and down electrons.");
+  if (d->walker_type == AFQ_WALKERS_RESTRICTED && d->nelec_dn > d->nelec_up)
+    return fail(AFQ_E_INVALID, "restricted walkers need nelec_dn <= nelec_up");
+  if (d->n_exp_terms < 1) return fail(AFQ_E_INVALID, "n_exp_terms must be >= 1");
+  if (!(d->dt > 0.0)) return fail(AFQ_E_INVALID, "dt must be positive");
+  if (d->precision != AFQ_PREC_FP64) return fail(AFQ_E_UNSUPPORTED, "precision mode not available in this build");
+  if (d->nelec_up > 64 || d->nelec_dn > 64) return fail(AFQ_E_UNSUPPORTED, "nocc > 64 not supported");
+  if (d->nchol > 65535 || d->n_walkers > 65535) return fail(AFQ_E_UNSUPPORTED, "nchol and n_walkers must be <= 65535");
+
+  afq_job* j = new afq_job();
+  j->d = *d;
+  j->N = d->norb;
+  j->G = d->nchol;
+  j->W = d->n_walkers;
+  j->Wcap = d->n_walkers;
+  j->nocc[0] = d->nelec_up;
+  j->nocc[1] = d->nelec_dn;
+  j->wcols = d->nelec_up;
+  j->restricted = d->walker_type == AFQ_WALKERS_RESTRICTED;
+  j->rhf = d->trial_type == AFQ_TRIAL_RHF;
+  j->one_spin = j->rhf && j->restricted;
+  j->nspin_meas = j->one_spin ? 1 : 2;
+  j->nspin_prop = j->restricted ? 1 : 2;
+
+  const long N = j->N, G = j->G, W = j->W;
+  const long wmax = std::max(j->wcols, std::max(j->nocc[0], j->nocc[1]));
+  int gc, R, C;
+  exchange_geometry(std::max(j->nocc[0], j->nocc[1]), &gc, &R, &C);
+  int kch = 1;
+  for (int s = 0; s < 2; ++s) {
+    int g2, r2, c2;
+    exchange_geometry(j->nocc[s], &g2, &r2, &c2);
+    kch = std::max(kch, (int)((G + g2 - 1) / g2));
+  }
+  j->kchunk_max = kch;
+  int rc = AFQ_OK;
+#define A_(p, n)                      \
+  if (rc == AFQ_OK) rc = dalloc_n(&(p), (size_t)(n));
+  A_(j->h1, 2 * N * N);
+  A_(j->chol, G * N * N);
+  A_(j->mo[0], N * j->nocc[0]);
+  if (!j->rhf) A_(j->mo[1], N * j->nocc[1]);
+  A_(j->rdm1, 2 * N * N);
+  A_(j->rot_h1[0], j->nocc[0] * N);
+  if (!j->rhf) A_(j->rot_h1[1], j->nocc[1] * N);
+  A_(j->rot_chol[0], G * j->nocc[0] * N);
+  if (!j->rhf) A_(j->rot_chol[1], G * j->nocc[1] * N);
+  A_(j->mf_shifts, G);
+  A_(j->mreal, 2 * G);
+  A_(j->h0_prop, 1);
+  A_(j->exp_h1[0], N * N);
+  if (!j->restricted) A_(j->exp_h1[1], N * N);
+  for (int s = 0; s < 2; ++s) {
+    const long O = j->nocc[s];
+    A_(j->S[s], W * O * O);
+    A_(j->Sinv[s], W * O * O);
+    A_(j->det[s], W);
+    A_(j->green[s], W * O * N);
+  }
+  A_(j->fb, W * G);
+  A_(j->xs, W * G);
+  A_(j->terms, 2 * W);
+  A_(j->vhs, W * N * N);
+  for (auto& t : j->tbuf) A_(t, W * N * wmax);
+  A_(j->ov_new, W);
+  A_(j->coul, 2 * W * G);
+  A_(j->kpart, 2 * W * kch);
+  A_(j->ene1, W);
+  A_(j->energies, W);
+  A_(j->wgather, W * N * wmax);
+  A_(j->sr_idx, W);
+  A_(j->cum, W);
+  A_(j->tmp_nn, 8 * N * N);
+  A_(j->dscratch, 8);
+#undef A_
+  if (rc != AFQ_OK) {
+    release(j);
+    delete j;
+    return rc;
+  }
+  if (j->rhf) {
+    j->mo[1] = j->mo[0];
+    j->rot_h1[1] = j->rot_h1[0];
+    j->rot_chol[1] = j->rot_chol[0];
+  }
+  *out = j;
+  return AFQ_OK;
+}
+
+int afq_job_destroy(afq_job_t j) {
+  if (!j) return AFQ_OK;
+  (void)hipDeviceSynchronize();
+  release(j);
+  delete j;
+  return AFQ_OK;
+}
+
+int afq_job_set_batch(afq_job_t j, int32_t n) {
+  if (!j) return fail(AFQ_E_INVALID, "job is NULL");
+  if (n < 1 || n > j->Wcap) return fail(AFQ_E_INVALID, "n_walkers outside [1, capacity]");
+  j->W = n;
+  return AFQ_OK;
+}
+
+int afq_job_set_hamiltonian(afq_job_t j, double h0, const double* h1, const double* chol, void* stream) {
+  if (!j) return fail(AFQ_E_INVALID, "job is NULL");
+  j->h0 = h0;
+  if (!h1 && !chol) return AFQ_OK;
+  if (!h1 || !chol) return fail(AFQ_E_INVALID, "h1 and chol must both be given or both be NULL");
+  const long N = j->N;
+  AFQ_HIP(hipMemcpyAsync(j->h1, h1, sizeof(double) * 2 * N * N, hipMemcpyDefault, S(stream)));
+  AFQ_HIP(hipMemcpyAsync(j->chol, chol, sizeof(double) * j->G * N * N, hipMemcpyDefault, S(stream)));
+  j->have_ham = true;
+  j->have_meas = j->have_prop = false;
+  return AFQ_OK;
+}
+
+int afq_job_set_trial(afq_job_t j, const double* mo_up, const double* mo_dn, const double* rdm1, void* stream) {
+  if (!j || !mo_up || !rdm1) return fail(AFQ_E_INVALID, "NULL argument");
+  if (!j->rhf && !mo_dn) return fail(AFQ_E_INVALID, "UHF trial needs mo_dn");
+  const long N = j->N;
+  AFQ_HIP(hipMemcpyAsync(j->mo[0], mo_up, sizeof(double) * N * j->nocc[0], hipMemcpyDefault, S(stream)));
+  if (!j->rhf)
+    AFQ_HIP(hipMemcpyAsync(j->mo[1], mo_dn, sizeof(double) * N * j->nocc[1], hipMemcpyDefault, S(stream)));
+  AFQ_HIP(hipMemcpyAsync(j->rdm1, rdm1, sizeof(double) * 2 * N * N, hipMemcpyDefault, S(stream)));
+  j->have_trial = true;
+  j->have_meas = j->have_prop = false;
+  return AFQ_OK;
+}
+
+int afq_build_measurement_intermediates(afq_job_t j, void* stream) {
+  if (!j) return fail(AFQ_E_INVALID, "job is NULL");
+  if (!j->have_ham || !j->have_trial) return fail(AFQ_E_STATE, "hamiltonian and trial must be set first");
+  hipStream_t st = S(stream);
+  const int N = j->N;
+  const long NN = (long)N * N;
+  // symmetrise h1 in place (via a temporary)
+  AFQ_TRY(k_sym_h1(j->h1, j->tmp_nn, N, st));
+  AFQ_HIP(hipMemcpyAsync(j->h1, j->tmp_nn, sizeof(double) * 2 * NN, hipMemcpyDeviceToDevice, st));
+  const int nsp = j->rhf ? 1 : 2;
+  for (int s = 0; s < nsp; ++s) {
+    const int O = j->nocc[s];
+    const double* hsrc = j->h1 + s * NN;
+    if (j->rhf) {
+      AFQ_TRY(k_axpby4(j->tmp_nn, j->h1, j->h1 + NN, 0.5, nullptr, nullptr, 0.0, NN, st));
+      hsrc = j->tmp_nn;
+    }
+    Gemm g;
+    g.M = O;
+    g.N = N;
+    g.K = N;
+    g.a_trans = true;
+    g.A = j->mo[s];
+    g.lda = O;
+    g.B = hsrc;
+    g.ldb = N;
+    g.C = j->rot_h1[s];
+    g.ldc = N;
+    AFQ_TRY(gemm_f64(g, st));
+    Gemm r;
+    r.M = O;
+    r.N = N;
+    r.K = N;
+    r.batch = j->G;
+    r.a_trans = true;
+    r.A = j->mo[s];
+    r.lda = O;
+    r.sA = 0;
+    r.B = j->chol;
+    r.ldb = N;
+    r.sB = NN;
+    r.C = j->rot_chol[s];
+    r.ldc = N;
+    r.sC = (long)O * N;
+    AFQ_TRY(gemm_f64(r, st));
+  }
+  j->have_meas = true;
+  return AFQ_OK;
+}
+
+int afq_build_propagation_intermediates(afq_job_t j, void* stream) {
+  if (!j) return fail(AFQ_E_INVALID, "job is NULL");
+  if (!j->have_ham || !j->have_trial) return fail(AFQ_E_STATE, "hamiltonian and trial must be set first");
+  hipStream_t st = S(stream);
+  const int N = j->N, G = j->G;
+  const long NN = (long)N * N;
+  double* rho = j->tmp_nn;           // [0]
+  double* v0 = j->tmp_nn + NN;       // [1]
+  double* v1 = j->tmp_nn + 2 * NN;   // [2]
+  double* hmod = j->tmp_nn + 3 * NN; // [3]
+  double* As = j->tmp_nn + 4 * NN;   // [4]
+  double* P = j->tmp_nn + 5 * NN;    // [5]
+  double* Q = j->tmp_nn + 6 * NN;    // [6]
+
+  // mf_shifts = i sum_pq L_g,pq rho_pq, rho = rdm1[0] + rdm1[1]
+  AFQ_TRY(k_axpby4(rho, j->rdm1, j->rdm1 + NN, 1.0, nullptr, nullptr, 0.0, NN, st));
+  {
+    Gemm g;
+    g.M = G;
+    g.N = 1;
+    g.K = (int)NN;
+    g.A = j->chol;
+    g.lda = NN;
+    g.B = rho;
+    g.ldb = 1;
+    g.C = j->mreal;
+    g.ldc = 1;
+    AFQ_TRY(gemm_f64(g, st));
+  }
+  double* mfr = j->mreal + G;
+  AFQ_TRY(k_mf(j->mreal, G, j->h0, j->mf_shifts, mfr, j->h0_prop, st));
+
+  // v0 = 1/2 sum_g L_g L_g, accumulated over gamma chunks of a permuted copy
+  {
+    long gch = std::max(1L, std::min((long)G, (64L << 20) / (long)(sizeof(double) * NN)));
+    double* perm = nullptr;
+    AFQ_TRY(dalloc_n(&perm, (size_t)(gch * NN)));
+    int rc = AFQ_OK;
+    for (int g0 = 0; g0 < G && rc == AFQ_OK; g0 += (int)gch) {
+      const int n = (int)std::min(gch, (long)(G - g0));
+      rc = k_permute_chunk(j->chol, perm, N, g0, n, st);
+      if (rc != AFQ_OK) break;
+      Gemm g;
+      g.M = N;
+      g.N = N;
+      g.K = n * N;
+      g.A = perm;
+      g.lda = (long)n * N;
+      g.B = j->chol + (long)g0 * NN;
+      g.ldb = N;
+      g.C = v0;
+      g.ldc = N;
+      g.alpha = double2{0.5, 0.0};
+      g.beta = double2{g0 == 0 ? 0.0 : 1.0, 0.0};
+      rc = gemm_f64(g, st);
+    }
+    (void)hipStreamSynchronize(st);
+    (void)hipFree(perm);
+    if (rc != AFQ_OK) return rc;
+  }
+  // v1 = sum_g Re(i mf_g) L_g
+  {
+    Gemm g;
+    g.M = 1;
+    g.N = (int)NN;
+    g.K = G;
+    g.A = mfr;
+    g.lda = G;
+    g.B = j->chol;
+    g.ldb = NN;
+    g.C = v1;
+    g.ldc = NN;
+    AFQ_TRY(gemm_f64(g, st));
+  }
+  const int nexp = j->restricted ? 1 : 2;
+  for (int s = 0; s < nexp; ++s) {
+    if (j->restricted)
+      AFQ_TRY(k_axpby4(hmod, j->h1, j->h1 + NN, 0.5, v0, v1, -1.0, NN, st));
+    else
+      AFQ_TRY(k_axpby4(hmod, j->h1 + s * NN, nullptr, 1.0, v0, v1, -1.0, NN, st));
+    // exp(A), A = -dt/2 h1_mod: scaling and squaring around a degree-18 Taylor/Horner core
+    AFQ_TRY(k_scale(As, hmod, -0.5 * j->d.dt, N, st));
+    AFQ_TRY(k_norm1(As, N, j->dscratch, st));
+    double nrm = 0.0;
+    AFQ_HIP(hipMemcpyAsync(&nrm, j->dscratch, sizeof(double), hipMemcpyDeviceToHost, st));
+    AFQ_HIP(hipStreamSynchronize(st));
+    int sq = 0;
+    while (nrm / std::ldexp(1.0, sq) > 0.25 && sq < 60) ++sq;
+    AFQ_TRY(k_scale(As, As, std::ldexp(1.0, -sq), N, st));
+    AFQ_TRY(k_identity(P, N, st));
+    for (int k = 18; k >= 1; --k) {
+      AFQ_TRY(k_identity(Q, N, st));
+      Gemm g;
+      g.M = N;
+      g.N = N;
+      g.K = N;
+      g.A = As;
+      g.lda = N;
+      g.B = P;
+      g.ldb = N;
+      g.C = Q;
+      g.ldc = N;
+      g.alpha = double2{1.0 / k, 0.0};
+      g.beta = double2{1.0, 0.0};
+      AFQ_TRY(gemm_f64(g, st));
+      std::swap(P, Q);
+    }
+    for (int i = 0; i < sq; ++i) {
+      Gemm g;
+      g.M = N;
+      g.N = N;
+      g.K = N;
+      g.A = P;
+      g.lda = N;
+      g.B = P;
+      g.ldb = N;
+      g.C = Q;
+      g.ldc = N;
+      AFQ_TRY(gemm_f64(g, st));
+      std::swap(P, Q);
+    }
+    AFQ_HIP(hipMemcpyAsync(j->exp_h1[s], P, sizeof(double) * NN, hipMemcpyDeviceToDevice, st));
+    // restore the named temporaries (P/Q may have swapped)
+    P = j->tmp_nn + 5 * NN;
+    Q = j->tmp_nn + 6 * NN;
+  }
+  AFQ_HIP(hipMemcpyAsync(&j->h0_prop_host, j->h0_prop, sizeof(double2), hipMemcpyDeviceToHost, st));
+  AFQ_HIP(hipStreamSynchronize(st));
+  j->have_prop = true;
+  return AFQ_OK;
+}
+
+int afq_get_intermediate(afq_job_t j, int which, int spin, void* dst, void* stream) {
+  if (!j || !dst) return fail(AFQ_E_INVALID, "NULL argument");
+  if (spin < 0 || spin > 1) return fail(AFQ_E_INVALID, "spin must be 0 or 1");
+  const long N = j->N, G = j->G, O = j->nocc[spin];
+  const void* src = nullptr;
+  size_t bytes = 0;
+  switch (which) {
+    case AFQ_INT_H1: src = j->h1; bytes = sizeof(double) * 2 * N * N; break;
+    case AFQ_INT_ROT_H1: src = j->rot_h1[spin]; bytes = sizeof(double) * O * N; break;
+    case AFQ_INT_ROT_CHOL: src = j->rot_chol[spin]; bytes = sizeof(double) * G * O * N; break;
+    case AFQ_INT_MF_SHIFTS: src = j->mf_shifts; bytes = sizeof(double2) * G; break;
+    case AFQ_INT_H0_PROP: src = j->h0_prop; bytes = sizeof(double2); break;
+    case AFQ_INT_EXP_H1:
+      src = j->exp_h1[j->restricted ? 0 : spin];
+      bytes = sizeof(double) * N * N;
+      break;
+    default: return fail(AFQ_E_INVALID, "unknown intermediate");
+  }
+  AFQ_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDefault, S(stream)));
+  return AFQ_OK;
+}
+
+int afq_set_intermediate(afq_job_t j, int which, int spin, const void* src, void* stream) {
+  if (!j || !src) return fail(AFQ_E_INVALID, "NULL argument");
+  if (spin < 0 || spin > 1) return fail(AFQ_E_INVALID, "spin must be 0 or 1");
+  if (j->rhf && spin == 1 && (which == AFQ_INT_ROT_H1 || which == AFQ_INT_ROT_CHOL))
+    return fail(AFQ_E_INVALID, "an RHF trial has spin-0 intermediates only");
+  const long N = j->N, G = j->G, O = j->nocc[spin];
+  void* dst = nullptr;
+  size_t bytes = 0;
+  switch (which) {
+    case AFQ_INT_H1: dst = j->h1; bytes = sizeof(double) * 2 * N * N; break;
+    case AFQ_INT_ROT_H1: dst = j->rot_h1[spin]; bytes = sizeof(double) * O * N; break;
+    case AFQ_INT_ROT_CHOL: dst = j->rot_chol[spin]; bytes = sizeof(double) * G * O * N; break;
+    case AFQ_INT_MF_SHIFTS: dst = j->mf_shifts; bytes = sizeof(double2) * G; break;
+    case AFQ_INT_H0_PROP: dst = j->h0_prop; bytes = sizeof(double2); break;
+    case AFQ_INT_EXP_H1:
+      if (j->restricted && spin == 1) return fail(AFQ_E_INVALID, "restricted walkers use one exp_h1");
+      dst = j->exp_h1[spin];
+      bytes = sizeof(double) * N * N;
+      break;
+    default: return fail(AFQ_E_INVALID, "unknown intermediate");
+  }
+  AFQ_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDefault, S(stream)));
+  switch (which) {
+    case AFQ_INT_ROT_H1: j->have_rot_h1[spin] = true; break;
+    case AFQ_INT_ROT_CHOL: j->have_rot_chol[spin] = true; break;
+    case AFQ_INT_MF_SHIFTS: j->have_mf = true; break;
+    case AFQ_INT_H0_PROP: j->have_h0p = true; break;
+    case AFQ_INT_EXP_H1: j->have_exp[spin] = true; break;
+    default: break;
+  }
+  const int nsp = j->rhf ? 1 : 2;
+  bool meas = true;
+  for (int s = 0; s < nsp; ++s) meas = meas && j->have_rot_h1[s] && j->have_rot_chol[s];
+  if (meas) j->have_meas = true;
+  const bool prop = j->have_mf && j->have_h0p && j->have_exp[0] && (j->restricted || j->have_exp[1]);
+  if (prop) j->have_prop = true;
+  return AFQ_OK;
+}
+
+int afq_calc_overlap(afq_job_t j, const void* up, const void* dn, void* ov, void* stream) {
+  AFQ_TRY(require(j, false, false));
+  AFQ_TRY(check_walkers(j, up, dn));
+  if (!ov) return fail(AFQ_E_INVALID, "overlaps is NULL");
+  AFQ_TRY(green_pipeline(j, up, dn, false, false, S(stream)));
+  return overlap_from_dets(j, static_cast<double2*>(ov), S(stream));
+}
+
+int afq_calc_green(afq_job_t j, const void* up, const void* dn, void* gup, void* gdn, void* stream) {
+  AFQ_TRY(require(j, false, false));
+  AFQ_TRY(check_walkers(j, up, dn));
+  if (!gup) return fail(AFQ_E_INVALID, "green_up is NULL");
+  hipStream_t st = S(stream);
+  AFQ_TRY(green_pipeline(j, up, dn, true, true, st));
+  const long W = j->W, N = j->N;
+  AFQ_HIP(hipMemcpyAsync(gup, j->green[0], sizeof(double2) * W * j->nocc[0] * N, hipMemcpyDeviceToDevice, st));
+  if (gdn)
+    AFQ_HIP(hipMemcpyAsync(gdn, j->green[j->one_spin ? 0 : 1], sizeof(double2) * W * j->nocc[1] * N,
+                           hipMemcpyDeviceToDevice, st));
+  return AFQ_OK;
+}
+
+int afq_calc_force_bias(afq_job_t j, const void* up, const void* dn, void* fb, void* stream) {
+  AFQ_TRY(require(j, true, false));
+  AFQ_TRY(check_walkers(j, up, dn));
+  if (!fb) return fail(AFQ_E_INVALID, "force_bias is NULL");
+  AFQ_TRY(green_pipeline(j, up, dn, true, true, S(stream)));
+  return force_bias_gemm(j, static_cast<double2*>(fb), S(stream));
+}
+
+int afq_calc_energy(afq_job_t j, const void* up, const void* dn, void* e, void* stream) {
+  AFQ_TRY(require(j, true, false));
+  AFQ_TRY(check_walkers(j, up, dn));
+  if (!e) return fail(AFQ_E_INVALID, "energies is NULL");
+  return energy_pipeline(j, up, dn, static_cast<double2*>(e), S(stream));
+}
+
+int afq_apply_trotprop(afq_job_t j, void* up, void* dn, const void* xs, void* stream) {
+  AFQ_TRY(require(j, false, true));
+  AFQ_TRY(check_walkers(j, up, dn));
+  if (!xs) return fail(AFQ_E_INVALID, "shifted_fields is NULL");
+  return trotprop(j, up, dn, static_cast<const double2*>(xs), S(stream));
+}
+
+int afq_propagate_constrained(afq_job_t j, void* up, void* dn, const double* fields, double* weights, void* overlaps,
+                              double* scalars, void* fb_out, void* stream) {
+  AFQ_TRY(require(j, true, true));
+  AFQ_TRY(check_walkers(j, up, dn));
+  if (!fields || !weights || !overlaps || !scalars) return fail(AFQ_E_INVALID, "NULL argument");
+  hipStream_t st = S(stream);
+  AFQ_TRY(propagate(j, up, dn, fields, weights, static_cast<double2*>(overlaps), scalars, st));
+  if (fb_out)
+    AFQ_HIP(hipMemcpyAsync(fb_out, j->fb, sizeof(double2) * (long)j->W * j->G, hipMemcpyDeviceToDevice, st));
+  return AFQ_OK;
+}
+
+int afq_orthonormalize(afq_job_t j, void* up, void* dn, void* stream) {
+  if (!j) return fail(AFQ_E_INVALID, "job is NULL");
+  AFQ_TRY(check_walkers(j, up, dn));
+  return orthonormalize(j, up, dn, S(stream));
+}
+
+int afq_stochastic_reconfiguration(afq_job_t j, void* up, void* dn, double* weights, const double* zeta,
+                                   int32_t* idx_out, void* stream) {
+  if (!j) return fail(AFQ_E_INVALID, "job is NULL");
+  AFQ_TRY(check_walkers(j, up, dn));
+  if (!weights || !zeta) return fail(AFQ_E_INVALID, "NULL argument");
+  hipStream_t st = S(stream);
+  AFQ_TRY(k_sr_index(weights, zeta, j->W, j->cum, j->sr_idx, st));
+  for (int s = 0; s < j->nspin_prop; ++s) {
+    double2* Wp = static_cast<double2*>(s == 0 ? up : dn);
+    const int c = j->restricted ? j->wcols : j->nocc[s];
+    const long per = (long)j->N * c;
+    AFQ_TRY(k_gather_walkers(Wp, j->wgather, j->sr_idx, per, j->W, st));
+    AFQ_HIP(hipMemcpyAsync(Wp, j->wgather, sizeof(double2) * per * j->W, hipMemcpyDeviceToDevice, st));
+  }
+  if (idx_out) AFQ_HIP(hipMemcpyAsync(idx_out, j->sr_idx, sizeof(int32_t) * j->W, hipMemcpyDeviceToDevice, st));
+  return AFQ_OK;
+}
+
+int afq_normal_fill(double* out, int64_t n, uint64_t seed, uint64_t counter, void* stream) {
+  if (!out || n < 0) return fail(AFQ_E_INVALID, "bad argument");
+  return k_normal_fill(out, n, seed, counter, S(stream));
+}
+
+int afq_block(afq_job_t j, void* up, void* dn, double* weights, void* overlaps, double* scalars, int n_prop_steps,
+              uint64_t seed, uint64_t* rng_counter, double* block_out, void* stream) {
+  AFQ_TRY(require(j, true, true));
+  AFQ_TRY(check_walkers(j, up, dn));
+  if (!weights || !overlaps || !scalars || !rng_counter || !block_out) return fail(AFQ_E_INVALID, "NULL argument");
+  if (n_prop_steps < 1) return fail(AFQ_E_INVALID, "n_prop_steps must be >= 1");
+  hipStream_t st = S(stream);
+  const long per_step = (long)j->W * j->G;
+  const long need = per_step * n_prop_steps;
+  if (need > j->fields_cap) {
+    AFQ_HIP(hipStreamSynchronize(st));
+    if (j->fields) (void)hipFree(j->fields);
+    j->fields = nullptr;
+    j->fields_cap = 0;
+    AFQ_TRY(dalloc_n(&j->fields, (size_t)need));
+    j->fields_cap = need;
+  }
+  AFQ_TRY(k_normal_fill(j->fields, need, seed, *rng_counter, st));
+  *rng_counter += 1;
+  double2* ov = static_cast<double2*>(overlaps);
+  for (int i = 0; i < n_prop_steps; ++i)
+    AFQ_TRY(propagate(j, up, dn, j->fields + i * per_step, weights, ov, scalars, st));
+  AFQ_TRY(orthonormalize(j, up, dn, st));
+  AFQ_TRY(green_pipeline(j, up, dn, false, false, st));
+  AFQ_TRY(overlap_from_dets(j, ov, st));
+  AFQ_TRY(energy_pipeline(j, up, dn, j->energies, st));
+  return k_block_finalize(j->energies, weights, scalars, std::sqrt(2.0 / j->d.dt), j->W, block_out, st);
+}
+
+int afq_walkers_orthonormalize(void* walkers, int32_t n_walkers, int32_t norb, int32_t ncols, void* stream) {
+  if (!walkers || n_walkers < 1 || norb < 1 || ncols < 1) return fail(AFQ_E_INVALID, "bad argument");
+  return batched_cgs2(static_cast<double2*>(walkers), ncols, (long)norb * ncols, norb, ncols, n_walkers, S(stream));
+}
+
+int64_t afq_walkers_sr_workspace(int32_t n_walkers, int64_t max_walker_elems) {
+  if (n_walkers < 1 || max_walker_elems < 1) return 0;
+  const int64_t cum = ((int64_t)sizeof(double) * n_walkers + 255) / 256 * 256;
+  const int64_t idx = ((int64_t)sizeof(int32_t) * n_walkers + 255) / 256 * 256;
+  return cum + idx + (int64_t)sizeof(double2) * n_walkers * max_walker_elems;
+}
+
+int afq_walkers_sr(void* up, int64_t elems_up, void* dn, int64_t elems_dn, int32_t W, double* weights,
+                   const double* zeta, int32_t* idx_out, void* workspace, void* stream) {
+  if (!up || !weights || !zeta || !workspace || W < 1 || elems_up < 1 || (dn && elems_dn < 1))
+    return fail(AFQ_E_INVALID, "bad argument");
+  hipStream_t st = S(stream);
+  char* ws = static_cast<char*>(workspace);
+  double* cum = reinterpret_cast<double*>(ws);
+  ws += ((int64_t)sizeof(double) * W + 255) / 256 * 256;
+  int32_t* idx = reinterpret_cast<int32_t*>(ws);
+  ws += ((int64_t)sizeof(int32_t) * W + 255) / 256 * 256;
+  double2* gather = reinterpret_cast<double2*>(ws);
+  AFQ_TRY(k_sr_index(weights, zeta, W, cum, idx, st));
+  void* bufs[2] = {up, dn};
+  const int64_t elems[2] = {elems_up, elems_dn};
+  for (int s = 0; s < 2; ++s) {
+    if (!bufs[s]) continue;
+    double2* Wp = static_cast<double2*>(bufs[s]);
+    AFQ_TRY(k_gather_walkers(Wp, gather, idx, elems[s], W, st));
+    AFQ_HIP(hipMemcpyAsync(Wp, gather, sizeof(double2) * elems[s] * W, hipMemcpyDeviceToDevice, st));
+  }
+  if (idx_out) AFQ_HIP(hipMemcpyAsync(idx_out, idx, sizeof(int32_t) * W, hipMemcpyDeviceToDevice, st));
+  return AFQ_OK;
+}
+
+}  // extern "C"

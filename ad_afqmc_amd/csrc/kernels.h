@@ -1,0 +1,35 @@
+// Launch helpers for the walker-step kernels in kernels.hip.
+#pragma once
+#include "afq_internal.h"
+
+namespace afq {
+
+int k_sym_h1(const double* in, double* out, int n, hipStream_t s);
+int k_axpby4(double* out, const double* x, const double* y, double alpha, const double* u, const double* v,
+             double beta, long n, hipStream_t s);
+int k_mf(const double* m, int G, double h0, double2* mf, double* mfr, double2* h0_prop, hipStream_t s);
+int k_permute_chunk(const double* L, double* out, int n, int g0, int gch, hipStream_t s);
+int k_norm1(const double* A, int n, double* out, hipStream_t s);
+int k_identity(double* out, int n, hipStream_t s);
+int k_scale(double* out, const double* A, double scale, int n, hipStream_t s);
+
+int k_overlap_combine(const double2* d0, const double2* d1, double2* out, int W, hipStream_t s);
+int k_ene1(const double2* G0, const double* rh0, int n0, const double2* G1, const double* rh1, int n1, int N, double f0,
+           double2* out, int W, hipStream_t s);
+void exchange_geometry(int O, int* gc, int* R, int* C);
+int k_exchange(const double* rc, const double2* Gm, int O, int N, int Gn, int W, double2* coul, double2* kpart,
+               int* nchunk_out, hipStream_t s);
+int k_energy_finalize(const double2* ene1, const double2* c0, const double2* c1, const double2* k0, const double2* k1,
+                      int Gn, int nch0, int nch1, double h0, double2* out, int W, hipStream_t s);
+
+int k_field_shift(const double2* fb, const double* x, const double2* mf, int Gn, double sqdt, double2* xs,
+                  double2* terms, int W, hipStream_t s);
+int k_weight_update(const double2* ov_new, double2* overlaps, const double2* terms, double* weights, double* scalars,
+                    const double2* h0_prop, double dt, int W, hipStream_t s);
+int k_block_finalize(const double2* energies, const double* weights, double* scalars, double clip, int W, double* out,
+                     hipStream_t s);
+int k_sr_index(double* weights, const double* zeta, int W, double* cum, int32_t* idx, hipStream_t s);
+int k_gather_walkers(const double2* src, double2* dst, const int32_t* idx, long per, int W, hipStream_t s);
+int k_normal_fill(double* out, long n, uint64_t seed, uint64_t counter, hipStream_t s);
+
+}  // namespace afq

@@ -1,0 +1,207 @@
+// Per-walker FP64 dense linear algebra kept in LDS / L2:
+//   * Gauss-Jordan determinant + inverse of the O x O overlap matrix T^H W
+//     (replaces jnp.linalg.det / jnp.linalg.inv, ad_afqmc/wavefunctions.py:1066,
+//     1100, 1266-1268, 1306-1311)
+//   * CGS2 re-orthonormalisation of an N x O walker panel (replaces the batched
+//     Householder QR of ad_afqmc/linalg_utils.py:51-72; R is discarded there).
+#include "afq_internal.h"
+
+namespace afq {
+namespace {
+
+__device__ inline double2 cmul(double2 a, double2 b) { return {a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x}; }
+__device__ inline double2 crecip(double2 a) {
+  // Smith's algorithm for 1/(a.x + i a.y)
+  if (fabs(a.x) >= fabs(a.y)) {
+    const double r = a.y / a.x, den = a.x + a.y * r;
+    return {1.0 / den, -r / den};
+  }
+  const double r = a.x / a.y, den = a.x * r + a.y;
+  return {r / den, -1.0 / den};
+}
+
+// One 256-thread work-group per matrix; the matrix lives in dynamic LDS (n*n
+// complex, n <= 64).  Partial pivoting selects the first row of maximal
+// |re|+|im| in the pivot column (LAPACK izamax convention).  The in-place
+// Gauss-Jordan sweep leaves the inverse with its columns permuted by the row
+// interchanges, undone at the end.
+__global__ __launch_bounds__(256) void gj_det_inv_kernel(const double2* __restrict__ S, long strideS, int n,
+                                                         double2* __restrict__ det, double2* __restrict__ inv,
+                                                         long strideInv) {
+  extern __shared__ double2 a[];
+  __shared__ double2 fcol[64];
+  __shared__ int piv[64];
+  __shared__ int pk;
+  const int tid = threadIdx.x;
+  const long b = blockIdx.x;
+  const double2* src = S + b * strideS;
+  const int nn = n * n;
+  for (int i = tid; i < nn; i += 256) a[i] = src[i];
+  __syncthreads();
+
+  double2 dprod = {1.0, 0.0};
+  bool neg = false;
+  for (int k = 0; k < n; ++k) {
+    if (tid < 64) {
+      double v = -1.0;
+      int idx = tid;
+      if (tid >= k && tid < n) {
+        const double2 x = a[tid * n + k];
+        v = fabs(x.x) + fabs(x.y);
+      }
+#pragma unroll
+      for (int off = 32; off > 0; off >>= 1) {
+        const double ov = __shfl_xor(v, off);
+        const int oi = __shfl_xor(idx, off);
+        if (ov > v || (ov == v && oi < idx)) {
+          v = ov;
+          idx = oi;
+        }
+      }
+      if (tid == 0) {
+        pk = idx;
+        piv[k] = idx;
+      }
+    }
+    __syncthreads();
+    const int p = pk;
+    if (p != k) {
+      for (int j = tid; j < n; j += 256) {
+        const double2 t = a[k * n + j];
+        a[k * n + j] = a[p * n + j];
+        a[p * n + j] = t;
+      }
+      __syncthreads();
+    }
+    const double2 d = a[k * n + k];
+    if (tid == 0) {
+      dprod = cmul(dprod, d);
+      if (p != k) neg = !neg;
+    }
+    const double2 dinv = crecip(d);
+    if (tid < n) fcol[tid] = a[tid * n + k];
+    __syncthreads();
+    if (tid < n) a[k * n + tid] = (tid == k) ? dinv : cmul(a[k * n + tid], dinv);
+    __syncthreads();
+    for (int idx = tid; idx < nn; idx += 256) {
+      const int i = idx / n, j = idx - i * n;
+      if (i == k) continue;
+      const double2 f = fcol[i];
+      const double2 rk = a[k * n + j];
+      if (j == k) {
+        a[idx] = double2{-(f.x * rk.x - f.y * rk.y), -(f.x * rk.y + f.y * rk.x)};
+      } else {
+        double2 x = a[idx];
+        x.x -= f.x * rk.x - f.y * rk.y;
+        x.y -= f.x * rk.y + f.y * rk.x;
+        a[idx] = x;
+      }
+    }
+    __syncthreads();
+  }
+  if (tid == 0) det[b] = neg ? double2{-dprod.x, -dprod.y} : dprod;
+  if (inv) {
+    // undo the column permutation: each thread owns whole rows, so no barrier
+    // is needed between the sequential swaps.
+    for (int i = tid; i < n; i += 256) {
+      for (int k = n - 1; k >= 0; --k) {
+        const int p = piv[k];
+        if (p != k) {
+          const double2 t = a[i * n + k];
+          a[i * n + k] = a[i * n + p];
+          a[i * n + p] = t;
+        }
+      }
+    }
+    __syncthreads();
+    double2* dst = inv + b * strideInv;
+    for (int i = tid; i < nn; i += 256) dst[i] = a[i];
+  }
+}
+
+__device__ inline double wave_sum(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+  return v;
+}
+
+// CGS2 on one panel per work-group, in global memory (the panel stays in L2).
+// Column j: two passes of r = Q[:, :j]^H a_j, a_j -= Q r, then a_j /= |a_j|.
+// The resulting R has a positive real diagonal.
+__global__ __launch_bounds__(256) void cgs2_kernel(double2* __restrict__ Wp, long ld, long sW, int nr, int nc) {
+  __shared__ double2 r[128];
+  __shared__ double red[4];
+  double2* Wb = Wp + (long)blockIdx.x * sW;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  for (int j = 0; j < nc; ++j) {
+    for (int pass = 0; pass < 2 && j > 0; ++pass) {
+      for (int k = wave; k < j; k += 4) {
+        double sr = 0.0, si = 0.0;
+        for (int p = lane; p < nr; p += 64) {
+          const double2 q = Wb[(long)p * ld + k], x = Wb[(long)p * ld + j];
+          sr += q.x * x.x + q.y * x.y;
+          si += q.x * x.y - q.y * x.x;
+        }
+        sr = wave_sum(sr);
+        si = wave_sum(si);
+        if (lane == 0) r[k] = double2{sr, si};
+      }
+      __syncthreads();
+      for (int p = tid; p < nr; p += 256) {
+        double2 x = Wb[(long)p * ld + j];
+        for (int k = 0; k < j; ++k) {
+          const double2 q = Wb[(long)p * ld + k], rk = r[k];
+          x.x -= q.x * rk.x - q.y * rk.y;
+          x.y -= q.x * rk.y + q.y * rk.x;
+        }
+        Wb[(long)p * ld + j] = x;
+      }
+      __syncthreads();
+    }
+    double s = 0.0;
+    for (int p = tid; p < nr; p += 256) {
+      const double2 x = Wb[(long)p * ld + j];
+      s += x.x * x.x + x.y * x.y;
+    }
+    s = wave_sum(s);
+    if (lane == 0) red[wave] = s;
+    __syncthreads();
+    const double scale = 1.0 / sqrt(red[0] + red[1] + red[2] + red[3]);
+    for (int p = tid; p < nr; p += 256) {
+      double2 x = Wb[(long)p * ld + j];
+      x.x *= scale;
+      x.y *= scale;
+      Wb[(long)p * ld + j] = x;
+    }
+    __syncthreads();
+  }
+}
+
+}  // namespace
+
+int batched_det_inv(const double2* S, long strideS, int n, int count, double2* det, double2* inv, long strideInv,
+                    hipStream_t stream) {
+  if (n < 1 || n > 64) return fail(AFQ_E_UNSUPPORTED, "det/inverse kernel supports 1 <= nocc <= 64");
+  if (count <= 0) return AFQ_OK;
+  const size_t lds = sizeof(double2) * n * n;
+  if (lds + 2048 > 65536) {
+    static bool raised = false;
+    if (!raised) {
+      hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(gj_det_inv_kernel),
+                                         hipFuncAttributeMaxDynamicSharedMemorySize, 96 * 1024);
+      if (e != hipSuccess) return fail(AFQ_E_HIP, std::string("det/inverse LDS attribute: ") + hipGetErrorString(e));
+      raised = true;
+    }
+  }
+  hipLaunchKernelGGL(gj_det_inv_kernel, dim3(count), dim3(256), lds, stream, S, strideS, n, det, inv, strideInv);
+  return launch_status("gj_det_inv_kernel");
+}
+
+int batched_cgs2(double2* Wp, long ld, long sW, int n_rows, int n_cols, int count, hipStream_t stream) {
+  if (n_cols < 1 || n_cols > 128 || n_cols > n_rows) return fail(AFQ_E_UNSUPPORTED, "CGS2 supports 1 <= nocc <= 128 <= N");
+  if (count <= 0) return AFQ_OK;
+  hipLaunchKernelGGL(cgs2_kernel, dim3(count), dim3(256), 0, stream, Wp, ld, sW, n_rows, n_cols);
+  return launch_status("cgs2_kernel");
+}
+
+}  // namespace afq

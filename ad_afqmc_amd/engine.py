@@ -1,0 +1,260 @@
+"""Job handle over libafq.so with PyTorch-ROCm tensors as device buffers.
+
+Every call is enqueued on torch's current HIP stream; nothing synchronises the
+host except `build_propagation` (the expm scaling exponent is chosen on the
+host) and explicit `.cpu()` conversions by the caller.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from typing import Optional, Tuple
+
+import numpy as np
+import torch
+
+from . import _native as nat
+
+
+def device() -> torch.device:
+    if not torch.cuda.is_available():
+        raise RuntimeError("ad_afqmc_amd runs on a HIP device; torch.cuda.is_available() is False")
+    return torch.device("cuda")
+
+
+def stream_ptr() -> C.c_void_p:
+    return C.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+def ptr(t: Optional[torch.Tensor]) -> C.c_void_p:
+    return C.c_void_p(0 if t is None else t.data_ptr())
+
+
+def as_f64(x) -> torch.Tensor:
+    if isinstance(x, torch.Tensor):
+        return x.to(device=device(), dtype=torch.float64).contiguous()
+    return torch.from_numpy(np.ascontiguousarray(np.asarray(x, dtype=np.float64))).to(device())
+
+
+def as_c128(x) -> torch.Tensor:
+    if isinstance(x, torch.Tensor):
+        return x.to(device=device(), dtype=torch.complex128).contiguous()
+    return torch.from_numpy(np.ascontiguousarray(np.asarray(x, dtype=np.complex128))).to(device())
+
+
+class EngineJob:
+    """One resident AFQMC job (Hamiltonian, trial, intermediates, scratch)."""
+
+    def __init__(
+        self,
+        norb: int,
+        nelec: Tuple[int, int],
+        nchol: int,
+        n_walkers: int,
+        walker_type: str = "restricted",
+        trial_type: str = "rhf",
+        n_exp_terms: int = 6,
+        dt: float = 0.01,
+        precision: str = "fp64",
+    ):
+        device()
+        if walker_type not in ("restricted", "unrestricted"):
+            raise NotImplementedError(f"walker_type {walker_type!r} is not supported by the engine")
+        if trial_type not in ("rhf", "uhf"):
+            raise NotImplementedError(f"trial {trial_type!r} is not supported by the engine")
+        self.norb, self.nelec, self.nchol = int(norb), (int(nelec[0]), int(nelec[1])), int(nchol)
+        self.capacity = self.batch = int(n_walkers)
+        self.walker_type, self.trial_type = walker_type, trial_type
+        self.n_exp_terms, self.dt, self.precision = int(n_exp_terms), float(dt), precision
+        desc = nat.JobDesc(
+            norb=self.norb,
+            nelec_up=self.nelec[0],
+            nelec_dn=self.nelec[1],
+            nchol=self.nchol,
+            n_walkers=self.capacity,
+            walker_type=nat.AFQ_WALKERS_RESTRICTED if walker_type == "restricted" else nat.AFQ_WALKERS_UNRESTRICTED,
+            trial_type=nat.AFQ_TRIAL_RHF if trial_type == "rhf" else nat.AFQ_TRIAL_UHF,
+            n_exp_terms=self.n_exp_terms,
+            precision=nat.PRECISIONS[precision],
+            reserved=0,
+            dt=self.dt,
+        )
+        h = C.c_void_p()
+        nat.call("afq_job_create", C.byref(h), C.byref(desc))
+        self._h = h
+        self._lib = nat.lib()
+
+    # ------------------------------------------------------------ lifetime
+    def close(self):
+        if getattr(self, "_h", None) is not None and self._h.value:
+            self._lib.afq_job_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def restricted(self) -> bool:
+        return self.walker_type == "restricted"
+
+    def set_batch(self, n: int):
+        if n != self.batch:
+            nat.call("afq_job_set_batch", self._h, int(n))
+            self.batch = int(n)
+
+    # ------------------------------------------------------------ setup
+    def set_hamiltonian(self, h0: float, h1=None, chol=None):
+        if h1 is None:
+            nat.call("afq_job_set_hamiltonian", self._h, float(h0), None, None, stream_ptr())
+            return
+        h1 = as_f64(h1).reshape(-1, self.norb, self.norb)
+        if h1.shape[0] == 1:
+            h1 = torch.cat([h1, h1])
+        chol = as_f64(chol).reshape(self.nchol, self.norb * self.norb)
+        nat.call("afq_job_set_hamiltonian", self._h, float(h0), ptr(h1), ptr(chol), stream_ptr())
+
+    def set_trial(self, mo_up, mo_dn, rdm1):
+        up = as_f64(mo_up)
+        dn = None if mo_dn is None else as_f64(mo_dn)
+        rdm = as_f64(rdm1)
+        nat.call("afq_job_set_trial", self._h, ptr(up), ptr(dn), ptr(rdm), stream_ptr())
+
+    def build_measurement(self):
+        nat.call("afq_build_measurement_intermediates", self._h, stream_ptr())
+
+    def build_propagation(self):
+        nat.call("afq_build_propagation_intermediates", self._h, stream_ptr())
+
+    def _int_shape(self, which: int, spin: int):
+        N, G, O = self.norb, self.nchol, self.nelec[spin]
+        return {
+            nat.AFQ_INT_H1: ((2, N, N), torch.float64),
+            nat.AFQ_INT_ROT_H1: ((O, N), torch.float64),
+            nat.AFQ_INT_ROT_CHOL: ((G, O, N), torch.float64),
+            nat.AFQ_INT_MF_SHIFTS: ((G,), torch.complex128),
+            nat.AFQ_INT_H0_PROP: ((), torch.complex128),
+            nat.AFQ_INT_EXP_H1: ((N, N), torch.float64),
+        }[which]
+
+    def get(self, which: int, spin: int = 0) -> torch.Tensor:
+        shape, dtype = self._int_shape(which, spin)
+        out = torch.empty(shape, dtype=dtype, device=device())
+        nat.call("afq_get_intermediate", self._h, which, spin, ptr(out), stream_ptr())
+        return out
+
+    def set(self, which: int, spin: int, value):
+        shape, dtype = self._int_shape(which, spin)
+        t = as_c128(value) if dtype == torch.complex128 else as_f64(value)
+        if tuple(t.shape) != tuple(shape):
+            t = t.reshape(shape)
+        nat.call("afq_set_intermediate", self._h, which, spin, ptr(t), stream_ptr())
+
+    # ------------------------------------------------------------ measurements
+    def _w(self, up: torch.Tensor, dn: Optional[torch.Tensor]):
+        n = up.shape[0]
+        if n > self.capacity:
+            raise ValueError(f"{n} walkers exceed the job capacity {self.capacity}")
+        self.set_batch(n)
+        return ptr(up), ptr(dn)
+
+    def overlap(self, up, dn=None) -> torch.Tensor:
+        pu, pd = self._w(up, dn)
+        out = torch.empty(up.shape[0], dtype=torch.complex128, device=up.device)
+        nat.call("afq_calc_overlap", self._h, pu, pd, ptr(out), stream_ptr())
+        return out
+
+    def green(self, up, dn=None):
+        pu, pd = self._w(up, dn)
+        W, N = up.shape[0], self.norb
+        gu = torch.empty((W, self.nelec[0], N), dtype=torch.complex128, device=up.device)
+        gd = torch.empty((W, self.nelec[1], N), dtype=torch.complex128, device=up.device)
+        nat.call("afq_calc_green", self._h, pu, pd, ptr(gu), ptr(gd), stream_ptr())
+        return gu, gd
+
+    def force_bias(self, up, dn=None) -> torch.Tensor:
+        pu, pd = self._w(up, dn)
+        out = torch.empty((up.shape[0], self.nchol), dtype=torch.complex128, device=up.device)
+        nat.call("afq_calc_force_bias", self._h, pu, pd, ptr(out), stream_ptr())
+        return out
+
+    def energy(self, up, dn=None) -> torch.Tensor:
+        pu, pd = self._w(up, dn)
+        out = torch.empty(up.shape[0], dtype=torch.complex128, device=up.device)
+        nat.call("afq_calc_energy", self._h, pu, pd, ptr(out), stream_ptr())
+        return out
+
+    # ------------------------------------------------------------ propagation (in place)
+    def trotprop(self, up, dn, shifted_fields: torch.Tensor):
+        pu, pd = self._w(up, dn)
+        nat.call("afq_apply_trotprop", self._h, pu, pd, ptr(shifted_fields), stream_ptr())
+
+    def propagate(self, up, dn, fields, weights, overlaps, scalars, fb_out=None):
+        pu, pd = self._w(up, dn)
+        nat.call(
+            "afq_propagate_constrained",
+            self._h,
+            pu,
+            pd,
+            ptr(fields),
+            ptr(weights),
+            ptr(overlaps),
+            ptr(scalars),
+            ptr(fb_out),
+            stream_ptr(),
+        )
+
+    def orthonormalize(self, up, dn=None):
+        pu, pd = self._w(up, dn)
+        nat.call("afq_orthonormalize", self._h, pu, pd, stream_ptr())
+
+    def stochastic_reconfiguration(self, up, dn, weights, zeta: torch.Tensor, idx_out=None):
+        pu, pd = self._w(up, dn)
+        nat.call("afq_stochastic_reconfiguration", self._h, pu, pd, ptr(weights), ptr(zeta), ptr(idx_out), stream_ptr())
+
+    def block(self, up, dn, weights, overlaps, scalars, n_prop_steps: int, seed: int, counter: int):
+        """One sampler block on the device; returns ({block_energy, block_weight} tensor, new counter)."""
+        pu, pd = self._w(up, dn)
+        out = torch.empty(2, dtype=torch.float64, device=up.device)
+        ctr = C.c_uint64(counter)
+        nat.call(
+            "afq_block",
+            self._h,
+            pu,
+            pd,
+            ptr(weights),
+            ptr(overlaps),
+            ptr(scalars),
+            int(n_prop_steps),
+            C.c_uint64(seed),
+            C.byref(ctr),
+            ptr(out),
+            stream_ptr(),
+        )
+        return out, int(ctr.value)
+
+
+def normal_fill(n: int, seed: int, counter: int) -> torch.Tensor:
+    """N(0,1) draws of the engine's Philox4x32-10 field generator."""
+    out = torch.empty(int(n), dtype=torch.float64, device=device())
+    nat.call("afq_normal_fill", ptr(out), int(n), C.c_uint64(seed), C.c_uint64(counter), stream_ptr())
+    return out
+
+
+def walkers_orthonormalize(w: torch.Tensor):
+    W, N, c = w.shape
+    nat.call("afq_walkers_orthonormalize", ptr(w), int(W), int(N), int(c), stream_ptr())
+
+
+def walkers_sr(up: torch.Tensor, dn: Optional[torch.Tensor], weights: torch.Tensor, zeta: torch.Tensor):
+    W = up.shape[0]
+    eu = up[0].numel()
+    ed = 0 if dn is None else dn[0].numel()
+    ws_bytes = nat.lib().afq_walkers_sr_workspace(int(W), int(max(eu, ed)))
+    ws = torch.empty(int(ws_bytes), dtype=torch.uint8, device=up.device)
+    idx = torch.empty(W, dtype=torch.int32, device=up.device)
+    nat.call(
+        "afq_walkers_sr", ptr(up), int(eu), ptr(dn), int(ed), int(W), ptr(weights), ptr(zeta), ptr(idx), ptr(ws), stream_ptr()
+    )
+    return idx

@@ -1,0 +1,182 @@
+/*
+ * afq.h -- C ABI of the MI355X phaseless-AFQMC walker engine (libafq.so).
+ *
+ * Every entry point is extern "C", takes plain pointers and sizes, returns an
+ * int status (AFQ_OK = 0, negative AFQ_E_* on failure; afq_last_error() gives
+ * the message of the calling thread's last failure) and is stream-ordered on
+ * the caller's HIP stream (`stream` is a hipStream_t passed as void*; NULL
+ * means the legacy default stream).  Calls do not synchronise the host unless
+ * stated.  Array arguments are DEVICE pointers unless stated.
+ *
+ * Complex arrays are interleaved complex128 (re, im) in the reference's
+ * row-major layouts:
+ *   restricted walkers    (W, N, nelec_up)                 one buffer
+ *   unrestricted walkers  (W, N, nelec_up), (W, N, nelec_dn) two buffers
+ *   Green's function      (W, nocc_s, N)
+ *   force bias            (W, G)
+ *   overlaps / energies   (W,)
+ *
+ * A job owns its resident Hamiltonian (h1, Cholesky vectors), trial, derived
+ * intermediates and scratch; it is not re-entrant (one stream at a time), and
+ * distinct jobs may run concurrently on distinct streams.
+ *
+ * Reference interfaces replaced (paths relative to the reference checkout):
+ *   afq_build_measurement_intermediates -> hamiltonian.build_measurement_intermediates
+ *        (ad_afqmc/hamiltonian.py:48-63; rhf/uhf._build_measurement_intermediates
+ *         ad_afqmc/wavefunctions.py:1215-1232, 1472-1496)
+ *   afq_build_propagation_intermediates -> hamiltonian.build_propagation_intermediates
+ *        (ad_afqmc/hamiltonian.py:65-81; propagator_afqmc._build_propagation_intermediates
+ *         ad_afqmc/propagation.py:378-438)
+ *   afq_calc_overlap      -> wave_function.calc_overlap      (ad_afqmc/wavefunctions.py:93-163)
+ *   afq_calc_green        -> rhf/uhf._calc_green             (ad_afqmc/wavefunctions.py:1089-1100, 1289-1312)
+ *   afq_calc_force_bias   -> wave_function.calc_force_bias   (ad_afqmc/wavefunctions.py:202-262)
+ *   afq_calc_energy       -> wave_function.calc_energy       (ad_afqmc/wavefunctions.py:264-394)
+ *   afq_apply_trotprop    -> propagator_afqmc._apply_trotprop (ad_afqmc/propagation.py:154-272)
+ *   afq_propagate_constrained -> propagator_afqmc.propagate_constrained (ad_afqmc/propagation.py:274-336)
+ *   afq_orthonormalize    -> RHFWalkers/UHFWalkers.orthonormalize (ad_afqmc/walkers.py:155-157, 260-262;
+ *                            ad_afqmc/linalg_utils.py:51-72)
+ *   afq_stochastic_reconfiguration -> stochastic_reconfiguration_local
+ *                            (ad_afqmc/walkers.py:139-145, 244-250; ad_afqmc/sr.py:24-48)
+ *   afq_block             -> sampler._block_scan (ad_afqmc/sampling.py:52-94) with on-device fields
+ */
+#ifndef AFQ_H
+#define AFQ_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct afq_job* afq_job_t;
+
+enum {
+  AFQ_OK = 0,
+  AFQ_E_INVALID = -1,     /* bad argument / size */
+  AFQ_E_NOMEM = -2,       /* device allocation failed */
+  AFQ_E_HIP = -3,         /* HIP runtime error */
+  AFQ_E_STATE = -4,       /* call order violated (e.g. intermediates not built) */
+  AFQ_E_UNSUPPORTED = -5  /* shape or option outside what the kernels support */
+};
+
+enum { AFQ_WALKERS_RESTRICTED = 0, AFQ_WALKERS_UNRESTRICTED = 1 };
+enum { AFQ_TRIAL_RHF = 0, AFQ_TRIAL_UHF = 1 };
+/* Arithmetic of the VHS / Taylor / exchange contractions.  LU, overlap ratios,
+ * QR and weight/energy accumulation stay FP64 in every mode. */
+enum { AFQ_PREC_FP64 = 0, AFQ_PREC_FP32 = 1, AFQ_PREC_BF16X3 = 2 };
+
+typedef struct {
+  int32_t norb;        /* N */
+  int32_t nelec_up;    /* occupied alpha orbitals */
+  int32_t nelec_dn;    /* occupied beta orbitals (== nelec_up for an RHF trial) */
+  int32_t nchol;       /* G */
+  int32_t n_walkers;   /* W */
+  int32_t walker_type; /* AFQ_WALKERS_* */
+  int32_t trial_type;  /* AFQ_TRIAL_* */
+  int32_t n_exp_terms; /* Taylor terms of exp(VHS), reference default 6 */
+  int32_t precision;   /* AFQ_PREC_* */
+  int32_t reserved;
+  double dt;
+} afq_job_desc;
+
+/* Intermediates readable with afq_get_intermediate (for inspection / tests). */
+enum {
+  AFQ_INT_H1 = 0,        /* (2,N,N) f64, symmetrised */
+  AFQ_INT_ROT_H1 = 1,    /* spin s: (nocc_s, N) f64 */
+  AFQ_INT_ROT_CHOL = 2,  /* spin s: (G, nocc_s, N) f64 */
+  AFQ_INT_MF_SHIFTS = 3, /* (G,) c128 */
+  AFQ_INT_H0_PROP = 4,   /* scalar c128 */
+  AFQ_INT_EXP_H1 = 5     /* spin s: (N,N) f64 (restricted walkers: spin 0 only) */
+};
+
+const char* afq_version(void);
+const char* afq_last_error(void);
+
+int afq_job_create(afq_job_t* job, const afq_job_desc* desc);
+int afq_job_destroy(afq_job_t job);
+
+/* Active walker count for the following calls, 1 <= n_walkers <= desc.n_walkers
+ * (the capacity the job was created with).  Default: the capacity. */
+int afq_job_set_batch(afq_job_t job, int32_t n_walkers);
+
+/* Sets h0 and copies h1 (2,N,N) and chol (G,N*N) (device or host pointers)
+ * into resident HBM.  h1/chol may be NULL to set h0 only (measurement-only use
+ * with intermediates supplied by afq_set_intermediate). */
+int afq_job_set_hamiltonian(afq_job_t job, double h0, const double* h1, const double* chol, void* stream);
+/* Trial orbitals mo_up (N,nelec_up), mo_dn (N,nelec_dn; NULL for RHF) and the
+ * trial 1-RDM rdm1 (2,N,N), device or host pointers. */
+int afq_job_set_trial(afq_job_t job, const double* mo_up, const double* mo_dn, const double* rdm1, void* stream);
+
+/* Builds rot_h1 / rot_chol (symmetrising h1 first, as the reference does). */
+int afq_build_measurement_intermediates(afq_job_t job, void* stream);
+/* Builds mf_shifts, h0_prop, exp_h1 = expm(-dt/2 h1_mod).  Synchronises the
+ * stream (the scaling exponent of expm is chosen on the host). */
+int afq_build_propagation_intermediates(afq_job_t job, void* stream);
+/* Copies intermediate `which` (AFQ_INT_*) of spin `spin` to dst. */
+int afq_get_intermediate(afq_job_t job, int which, int spin, void* dst, void* stream);
+/* Overwrites intermediate `which` of spin `spin` from src (the reference's
+ * measurement methods read whatever rot_h1 / rot_chol the caller's ham_data
+ * holds).  For an RHF trial only spin 0 exists.  Once rot_h1 and rot_chol of
+ * every spin are set the measurement intermediates count as built; once
+ * mf_shifts, h0_prop and every exp_h1 are set, the propagation ones do. */
+int afq_set_intermediate(afq_job_t job, int which, int spin, const void* src, void* stream);
+
+/* Measurements.  walkers_dn is NULL for restricted walkers. */
+int afq_calc_overlap(afq_job_t job, const void* walkers_up, const void* walkers_dn, void* overlaps, void* stream);
+int afq_calc_green(afq_job_t job, const void* walkers_up, const void* walkers_dn, void* green_up, void* green_dn,
+                   void* stream);
+int afq_calc_force_bias(afq_job_t job, const void* walkers_up, const void* walkers_dn, void* force_bias,
+                        void* stream);
+int afq_calc_energy(afq_job_t job, const void* walkers_up, const void* walkers_dn, void* energies, void* stream);
+
+/* W <- exp(-dt/2 h1) Taylor_n(exp(VHS)) exp(-dt/2 h1) W with
+ * VHS = i sqrt(dt) sum_g shifted_fields[w,g] L_g; shifted_fields (W,G) c128. */
+int afq_apply_trotprop(afq_job_t job, void* walkers_up, void* walkers_dn, const void* shifted_fields, void* stream);
+
+/* One phaseless step.  fields (W,G) f64; weights (W,) f64 in/out; overlaps
+ * (W,) c128 in/out; scalars (device f64[2]) = {e_estimate, pop_control_ene_shift},
+ * the second updated in place.  force_bias_out (W,G) c128 may be NULL. */
+int afq_propagate_constrained(afq_job_t job, void* walkers_up, void* walkers_dn, const double* fields,
+                              double* weights, void* overlaps, double* scalars, void* force_bias_out,
+                              void* stream);
+
+/* QR re-orthonormalisation in place (R discarded). */
+int afq_orthonormalize(afq_job_t job, void* walkers_up, void* walkers_dn, void* stream);
+
+/* Comb resampling by |weight| with offset zeta (device f64 scalar); permutes
+ * walkers in place and sets every weight to sum|w|/W.  indices_out (W,) int32
+ * may be NULL. */
+int afq_stochastic_reconfiguration(afq_job_t job, void* walkers_up, void* walkers_dn, double* weights,
+                                   const double* zeta, int32_t* indices_out, void* stream);
+
+/* One sampler block: n_prop_steps phaseless steps with on-device N(0,1)
+ * fields (Philox4x32-10 keyed by seed, counter advanced by *rng_counter),
+ * then QR, overlap and energy measurement, energy clipping, and the
+ * population-control shift update.  block_out (device f64[2]) receives
+ * {block_energy, block_weight}.  *rng_counter (host) is advanced. */
+int afq_block(afq_job_t job, void* walkers_up, void* walkers_dn, double* weights, void* overlaps,
+              double* scalars, int n_prop_steps, uint64_t seed, uint64_t* rng_counter, double* block_out,
+              void* stream);
+
+/* Fills out (n,) f64 with N(0,1) draws of the engine's field generator for
+ * (seed, counter) -- the stream afq_block uses. */
+int afq_normal_fill(double* out, int64_t n, uint64_t seed, uint64_t counter, void* stream);
+
+/* Job-free walker-batch operations (the reference's walker containers carry no
+ * Hamiltonian: RHFWalkers/UHFWalkers.orthonormalize, walkers.py:155-157,
+ * 260-262, and stochastic_reconfiguration_local, walkers.py:139-145, 244-250).
+ * `walkers` is one (n_walkers, norb, ncols) c128 buffer. */
+int afq_walkers_orthonormalize(void* walkers, int32_t n_walkers, int32_t norb, int32_t ncols, void* stream);
+/* Bytes of device workspace afq_walkers_sr needs. */
+int64_t afq_walkers_sr_workspace(int32_t n_walkers, int64_t max_walker_elems);
+/* Comb resampling of up to two walker buffers sharing one weight vector:
+ * elems_s = norb*ncols_s complex elements per walker (elems_dn = 0 and
+ * walkers_dn = NULL for one buffer).  weights set to sum|w|/W. */
+int afq_walkers_sr(void* walkers_up, int64_t elems_up, void* walkers_dn, int64_t elems_dn, int32_t n_walkers,
+                   double* weights, const double* zeta, int32_t* indices_out, void* workspace, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* AFQ_H */

@@ -1,0 +1,254 @@
+"""HIP engine (libafq.so through the reference-mirroring facade) vs the FP64
+oracle on identical inputs.  Tolerance: 1e-10 relative (max-abs error over
+max-abs value of each quantity), FP64 throughout."""
+import math
+import os
+
+import numpy as np
+import pytest
+import torch
+
+from ad_afqmc_amd import engine as E
+from ad_afqmc_amd import hamiltonian as hm
+from ad_afqmc_amd import propagation as pr
+from ad_afqmc_amd import synthetic as syn
+from ad_afqmc_amd import walkers as wk
+from ad_afqmc_amd import wavefunctions as wf
+from oracle import afqmc_oracle as O
+
+pytestmark = pytest.mark.gpu
+TOL = 1e-10
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+
+def rel(a, b):
+    a = np.asarray(a)
+    b = np.asarray(b)
+    return float(np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300))
+
+
+def npy(t):
+    if isinstance(t, (list, tuple)):
+        return [npy(x) for x in t]
+    return t.detach().cpu().numpy() if isinstance(t, torch.Tensor) else np.asarray(t)
+
+
+# ------------------------------------------------------------------ golden vectors
+def test_golden_rhf_engine():
+    d = np.load(os.path.join(HERE, "golden", "golden_wavefunctions.npz"))
+    trial = wf.rhf(10, (5, 5))
+    mo = d["mo"]
+    wave = {"mo_coeff": mo, "rdm1": np.array([mo @ mo.T] * 2)}
+    ham = {"h0": float(d["h0"]), "rot_h1": d["rot_h1"], "rot_chol": d["rot_chol"], "h1": np.array([d["h1"]] * 2),
+           "chol": d["chol"].reshape(5, -1), "ene0": 0.0}
+    w = wk.RHFWalkers(d["walker"][None])
+    ov = npy(trial.calc_overlap(w, wave))[0]
+    g = npy(trial.calc_green(w, wave))[0]
+    fb = npy(trial.calc_force_bias(w, ham, wave))[0]
+    e = npy(trial.calc_energy(w, ham, wave))[0]
+    assert math.isclose(ov.real, d["rhf_overlap"], rel_tol=1e-12)
+    assert math.isclose(g.sum().real, d["rhf_green_sum"], rel_tol=1e-12)
+    assert math.isclose(fb.sum().real, d["rhf_force_bias_sum"], rel_tol=1e-12)
+    assert math.isclose(e.real, d["rhf_energy"], rel_tol=1e-12)
+    t = O.RHFTrial(10, (5, 5), mo)
+    assert rel(fb, t.calc_force_bias(d["walker"][None], d["rot_chol"])[0]) < TOL
+    assert rel(e, t.calc_energy(d["walker"][None], d["h0"], d["rot_h1"], d["rot_chol"])[0]) < TOL
+
+
+def test_golden_uhf_engine():
+    d = np.load(os.path.join(HERE, "golden", "golden_wavefunctions.npz"))
+    trial = wf.uhf(10, (3, 2))
+    mu, md = d["mo_u_up"], d["mo_u_dn"]
+    wave = {"mo_coeff": [mu, md], "rdm1": np.array([mu @ mu.T, md @ md.T])}
+    rh = [d["rot_h1"][:3], d["rot_h1"][:2]]
+    rc = [np.ascontiguousarray(d["rot_chol"][:, :3]), np.ascontiguousarray(d["rot_chol"][:, :2])]
+    ham = {"h0": float(d["h0"]), "rot_h1": rh, "rot_chol": rc, "h1": np.array([d["h1"]] * 2),
+           "chol": d["chol"].reshape(5, -1), "ene0": 0.0}
+    w = wk.UHFWalkers([d["walker_up"][None], d["walker_dn"][None]])
+    ov = npy(trial.calc_overlap(w, wave))[0]
+    gu, gd = npy(trial.calc_green(w, wave))
+    fb = npy(trial.calc_force_bias(w, ham, wave))[0]
+    e = npy(trial.calc_energy(w, ham, wave))[0]
+    assert math.isclose(ov.real, d["uhf_overlap"], rel_tol=1e-11)
+    assert math.isclose(gu.sum().real + gd.sum().real, d["uhf_green_sum"], rel_tol=1e-11)
+    assert math.isclose(fb.sum().real, d["uhf_force_bias_sum"], rel_tol=1e-11)
+    assert math.isclose(e.real, d["uhf_energy"], rel_tol=1e-11)
+
+
+# ------------------------------------------------------------------ random systems
+CASES = [
+    # (trial, walker_type, N, nocc_up, nocc_dn, G, W)
+    ("rhf", "restricted", 12, 4, 4, 10, 6),
+    ("uhf", "unrestricted", 12, 4, 3, 10, 6),
+    ("rhf", "unrestricted", 10, 3, 3, 7, 5),
+    ("uhf", "restricted", 11, 4, 2, 9, 5),
+    ("rhf", "restricted", 70, 17, 17, 130, 9),
+    ("uhf", "unrestricted", 67, 21, 18, 97, 7),
+]
+
+
+def _setup(trial_kind, wtype, N, ou, od, G, W, seed=0, dt=0.01):
+    sysm = syn.make_system(N, ou, G, nocc_dn=od, uhf=(trial_kind == "uhf"), seed=seed)
+    wave = sysm.wave_data()
+    ham = sysm.ham_data()
+    trial = wf.rhf(N, (ou, ou)) if trial_kind == "rhf" else wf.uhf(N, (ou, od))
+    prop = pr.propagator_afqmc(dt=dt, n_walkers=W, walker_type=wtype)
+    H = hm.hamiltonian(N)
+    ham = H.build_measurement_intermediates(ham, trial, wave)
+    ham = H.build_propagation_intermediates(ham, prop, trial, wave)
+    # oracle side
+    if trial_kind == "rhf":
+        h1s, orh, orc = O.rhf_measurement_intermediates(sysm.h1, sysm.chol, sysm.mo_up)
+        otrial = O.RHFTrial(N, (ou, ou), sysm.mo_up)
+    else:
+        h1s, orh, orc = O.uhf_measurement_intermediates(sysm.h1, sysm.chol, sysm.mo_up, sysm.mo_dn)
+        otrial = O.UHFTrial(N, (ou, od), sysm.mo_up, sysm.mo_dn)
+    opi = O.propagation_intermediates(sysm.h0, h1s, sysm.chol, sysm.rdm1(), dt, wtype)
+    oham = dict(h0=sysm.h0, chol=sysm.chol, rot_h1=orh, rot_chol=orc, **{k: opi[k] for k in ("mf_shifts", "h0_prop", "exp_h1")})
+    if wtype == "restricted":
+        wn = syn.perturbed_walkers(sysm.mo_up, W, 0.1, seed + 1)
+        walkers, owalk = wk.RHFWalkers(wn), wn
+    else:
+        dn_mo = sysm.mo_up if sysm.mo_dn is None else sysm.mo_dn
+        wu = syn.perturbed_walkers(sysm.mo_up, W, 0.1, seed + 1)
+        wd = syn.perturbed_walkers(dn_mo, W, 0.1, seed + 2)
+        walkers, owalk = wk.UHFWalkers([wu, wd]), [wu, wd]
+    return sysm, trial, prop, ham, wave, walkers, otrial, oham, owalk, h1s
+
+
+@pytest.mark.parametrize("case", CASES, ids=[f"{c[0]}-{c[1]}-N{c[2]}" for c in CASES])
+def test_intermediates_and_measurements(case):
+    sysm, trial, prop, ham, wave, walkers, otrial, oham, owalk, h1s = _setup(*case)
+    assert rel(npy(ham["h1"]), h1s) < TOL
+    if case[0] == "rhf":
+        assert rel(npy(ham["rot_h1"]), oham["rot_h1"]) < TOL
+        assert rel(npy(ham["rot_chol"]), oham["rot_chol"]) < TOL
+    else:
+        for s in range(2):
+            assert rel(npy(ham["rot_h1"][s]), oham["rot_h1"][s]) < TOL
+            assert rel(npy(ham["rot_chol"][s]), oham["rot_chol"][s]) < TOL
+    assert rel(npy(ham["mf_shifts"]), oham["mf_shifts"]) < TOL
+    assert rel(npy(ham["h0_prop"]), oham["h0_prop"]) < TOL
+    assert rel(npy(ham["exp_h1"]), oham["exp_h1"]) < 1e-13 * 10
+    assert rel(npy(trial.calc_overlap(walkers, wave)), otrial.calc_overlap(owalk)) < TOL
+    assert rel(npy(trial.calc_force_bias(walkers, ham, wave)), otrial.calc_force_bias(owalk, oham["rot_chol"])) < TOL
+    e = npy(trial.calc_energy(walkers, ham, wave))
+    eo = otrial.calc_energy(owalk, oham["h0"], oham["rot_h1"], oham["rot_chol"])
+    assert rel(e, eo) < TOL
+
+
+@pytest.mark.parametrize("case", CASES, ids=[f"{c[0]}-{c[1]}-N{c[2]}" for c in CASES])
+def test_propagate_constrained_steps(case):
+    sysm, trial, prop, ham, wave, walkers, otrial, oham, owalk, _ = _setup(*case)
+    W, G = case[6], case[5]
+    rng = np.random.default_rng(42)
+    pd = prop.init_prop_data(trial, wave, ham, seed=3, init_walkers=walkers)
+    e_est = float(pd["e_estimate"].item())
+    st = dict(walkers=owalk, weights=np.ones(W), overlaps=otrial.calc_overlap(owalk), e_estimate=e_est,
+              pop_control_ene_shift=e_est)
+    assert rel(npy(pd["overlaps"]), st["overlaps"]) < TOL
+    for step in range(3):
+        fields = rng.standard_normal((W, G))
+        pd = prop.propagate_constrained(trial, ham, pd, fields, wave, return_force_bias=True)
+        st = O.propagate_constrained(otrial, st, fields, oham, prop.dt, prop.n_exp_terms)
+        assert rel(npy(pd["force_bias"]), st["force_bias"]) < TOL, step
+        if case[1] == "restricted":
+            assert rel(npy(pd["walkers"].data), st["walkers"]) < TOL, step
+        else:
+            assert rel(npy(pd["walkers"].data[0]), st["walkers"][0]) < TOL, step
+            assert rel(npy(pd["walkers"].data[1]), st["walkers"][1]) < TOL, step
+        assert rel(npy(pd["overlaps"]), st["overlaps"]) < TOL, step
+        assert rel(npy(pd["weights"]), st["weights"]) < TOL, step
+        assert abs(pd["pop_control_ene_shift"].item() - st["pop_control_ene_shift"]) < TOL * max(1, abs(e_est))
+
+
+def test_trotprop_matches_oracle():
+    sysm, trial, prop, ham, wave, walkers, otrial, oham, owalk, _ = _setup("uhf", "unrestricted", 20, 5, 4, 15, 4)
+    rng = np.random.default_rng(5)
+    xs = rng.standard_normal((4, 15)) + 0.3j * rng.standard_normal((4, 15))
+    out = prop.apply_trotprop(trial, ham, walkers, xs, wave)
+    ref = O.apply_trotprop(owalk, xs, oham["chol"], oham["exp_h1"], prop.dt, prop.n_exp_terms)
+    assert rel(npy(out.data[0]), ref[0]) < TOL
+    assert rel(npy(out.data[1]), ref[1]) < TOL
+
+
+def test_phaseless_clipping_known_answers():
+    # a walker whose overlap ratio flips sign gets cos(theta) < 0 -> weight 0;
+    # weights above 100 are zeroed.
+    sysm, trial, prop, ham, wave, walkers, otrial, oham, owalk, _ = _setup("rhf", "restricted", 12, 4, 4, 10, 6)
+    pd = prop.init_prop_data(trial, wave, ham, seed=3, init_walkers=walkers)
+    ov = pd["overlaps"].clone()
+    ov[1] = -ov[1]
+    w = pd["weights"].clone()
+    w[2] = 150.0
+    pd["overlaps"], pd["weights"] = ov, w
+    fields = np.zeros((6, 10))
+    out = prop.propagate_constrained(trial, ham, pd, fields, wave)
+    st = dict(walkers=owalk, weights=npy(w), overlaps=npy(ov), e_estimate=pd["e_estimate"].item(),
+              pop_control_ene_shift=pd["e_estimate"].item())
+    ref = O.propagate_constrained(otrial, st, fields, oham, prop.dt)
+    wout = npy(out["weights"])
+    assert wout[1] == 0.0 and ref["weights"][1] == 0.0
+    assert wout[2] == 0.0 and ref["weights"][2] == 0.0
+    assert rel(wout, ref["weights"]) < TOL
+
+
+def test_orthonormalize_matches_oracle_up_to_phases():
+    sysm, trial, prop, ham, wave, walkers, otrial, oham, owalk, _ = _setup("uhf", "unrestricted", 30, 7, 5, 12, 5)
+    q = walkers.orthonormalize()
+    qo = O.orthonormalize(owalk)
+    for s in range(2):
+        qg = npy(q.data[s])
+        # columns agree up to a sign (LAPACK's R diagonal is real with either sign)
+        m = np.einsum("wpi,wpj->wij", qg.conj(), qo[s])
+        assert np.allclose(np.abs(m), np.eye(qg.shape[2])[None], atol=1e-10)
+        assert np.allclose(np.einsum("wpi,wpj->wij", qg.conj(), qg), np.eye(qg.shape[2])[None], atol=1e-12)
+    # overlaps, Green's functions and energies are identical after QR
+    assert rel(npy(trial.calc_overlap(q, wave)), otrial.calc_overlap(qo)) < TOL
+    e = npy(trial.calc_energy(q, ham, wave))
+    assert rel(e, otrial.calc_energy(qo, oham["h0"], oham["rot_h1"], oham["rot_chol"])) < TOL
+
+
+def test_stochastic_reconfiguration_matches_oracle():
+    rng = np.random.default_rng(7)
+    W, N, Oo = 37, 8, 3
+    data = rng.standard_normal((W, N, Oo)) + 1j * rng.standard_normal((W, N, Oo))
+    weights = rng.random(W) * 2
+    weights[[3, 9, 20]] = 0.0
+    zeta = 0.3141
+    w2, wt2 = wk.RHFWalkers(data).stochastic_reconfiguration_local(weights, zeta)
+    idx, wref = O.sr_indices(weights, zeta)
+    assert np.array_equal(npy(w2.data), data[idx])
+    assert rel(npy(wt2), wref) < 1e-14
+
+
+def test_normal_fill_moments_and_determinism():
+    x = npy(E.normal_fill(1_000_001, seed=11, counter=5))
+    y = npy(E.normal_fill(1_000_001, seed=11, counter=5))
+    z = npy(E.normal_fill(1000, seed=11, counter=6))
+    assert np.array_equal(x, y)
+    assert not np.allclose(x[:1000], z)
+    assert abs(x.mean()) < 5e-3 and abs(x.var() - 1) < 5e-3
+    assert abs(np.mean(x**3)) < 1e-2 and abs(np.mean(x**4) - 3) < 3e-2
+
+
+def test_h6_driver_energy_statistical():
+    """Reference H6 fixture (UHF trial, unrestricted walkers, 50 walkers,
+    tests/test_driver.py:74-86: -3.238196747261496).  The field stream differs
+    from JAX's, so agreement is statistical."""
+    from ad_afqmc_amd import driver, sampling
+
+    d = np.load(os.path.join(HERE, "golden", "h6_fcidump_chol.npz"))
+    nelec, norb, ms, nchol = (int(x) for x in d["header"])
+    nup, ndn = (nelec + abs(ms)) // 2, (nelec - abs(ms)) // 2
+    mo = d["mo_coeff"]
+    trial = wf.uhf(norb, (nup, ndn))
+    wave = {"mo_coeff": [mo[0][:, :nup], mo[1][:, :ndn]]}
+    wave["rdm1"] = np.array([wave["mo_coeff"][0] @ wave["mo_coeff"][0].T, wave["mo_coeff"][1] @ wave["mo_coeff"][1].T])
+    h1 = d["hcore"].reshape(norb, norb)
+    ham = {"h0": float(d["energy_core"]), "h1": np.array([h1, h1]), "chol": d["chol"].reshape(nchol, -1), "ene0": 0.0}
+    prop = pr.propagator_afqmc(dt=0.01, n_walkers=50, walker_type="unrestricted")
+    smp = sampling.sampler(n_prop_steps=50, n_ene_blocks=1, n_sr_blocks=10, n_blocks=40)
+    e, err = driver.afqmc_energy(ham, hm.hamiltonian(norb), prop, trial, wave, smp,
+                                 dict(n_eql=1, seed=98), verbose=False)
+    assert abs(e - (-3.238196747261496)) < 5 * err + 2e-3, (e, err)
