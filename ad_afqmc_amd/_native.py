@@ -18,6 +18,22 @@ AFQ_TRIAL_RHF, AFQ_TRIAL_UHF = 0, 1
 AFQ_PREC_FP64, AFQ_PREC_FP32, AFQ_PREC_BF16X3 = 0, 1, 2
 PRECISIONS = {"fp64": AFQ_PREC_FP64, "fp32": AFQ_PREC_FP32, "bf16x3": AFQ_PREC_BF16X3}
 (AFQ_INT_H1, AFQ_INT_ROT_H1, AFQ_INT_ROT_CHOL, AFQ_INT_MF_SHIFTS, AFQ_INT_H0_PROP, AFQ_INT_EXP_H1) = range(6)
+KERNEL_CLASSES = [
+    "vhs",
+    "taylor",
+    "exp_h1",
+    "overlap_gemm",
+    "det_inv",
+    "green",
+    "force_bias",
+    "field_shift",
+    "weight",
+    "exchange",
+    "energy_misc",
+    "qr",
+    "sr",
+    "rng",
+]
 
 
 class JobDesc(C.Structure):
@@ -67,6 +83,8 @@ _SIGNATURES = {
     "afq_stochastic_reconfiguration": ([P, P, P, P, P, P, P], C.c_int),
     "afq_block": ([P, P, P, P, P, P, C.c_int, C.c_uint64, C.POINTER(C.c_uint64), P, P], C.c_int),
     "afq_normal_fill": ([P, C.c_int64, C.c_uint64, C.c_uint64, P], C.c_int),
+    "afq_job_profile": ([P, C.c_int, C.c_int], C.c_int),
+    "afq_job_profile_read": ([P, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_int64)], C.c_int),
     "afq_walkers_orthonormalize": ([P, C.c_int32, C.c_int32, C.c_int32, P], C.c_int),
     "afq_walkers_sr_workspace": ([C.c_int32, C.c_int64], C.c_int64),
     "afq_walkers_sr": ([P, C.c_int64, P, C.c_int64, C.c_int32, P, P, P, P, P], C.c_int),

@@ -6,6 +6,8 @@
 #include <algorithm>
 #include <cstdint>
 #include <string>
+#include <utility>
+#include <vector>
 
 #include "afq.h"
 
@@ -128,5 +130,43 @@ struct afq_job {
   double* cum = nullptr;                  // (W,) SR cumulative weights
   double* dscratch = nullptr;             // small device scalars
   int kchunk_max = 1;                     // exchange partials per walker
-  double* tmp_nn = nullptr;               // (4, N, N) setup temporaries
+  double* tmp_nn = nullptr;               // (8, N, N) setup temporaries
+
+  // live per-class kernel timing (afq_job_profile)
+  struct Prof {
+    bool on = false;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> spans;
+    std::vector<hipEvent_t> pool;
+  } prof[AFQ_K_NCLASSES];
 };
+
+namespace afq {
+// Brackets the launches issued in its scope with events when the class is enabled.
+struct ProfScope {
+  afq_job* j;
+  int cls;
+  hipStream_t st;
+  hipEvent_t b = nullptr, e = nullptr;
+  static hipEvent_t take(afq_job::Prof& p) {
+    if (!p.pool.empty()) {
+      hipEvent_t ev = p.pool.back();
+      p.pool.pop_back();
+      return ev;
+    }
+    hipEvent_t ev = nullptr;
+    if (hipEventCreate(&ev) != hipSuccess) return nullptr;
+    return ev;
+  }
+  ProfScope(afq_job* job, int c, hipStream_t s) : j(job), cls(c), st(s) {
+    if (!j->prof[cls].on) return;
+    b = take(j->prof[cls]);
+    e = take(j->prof[cls]);
+    if (b && e) (void)hipEventRecord(b, st);
+  }
+  ~ProfScope() {
+    if (!b || !e) return;
+    (void)hipEventRecord(e, st);
+    j->prof[cls].spans.emplace_back(b, e);
+  }
+};
+}  // namespace afq

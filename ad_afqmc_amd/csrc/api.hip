@@ -73,6 +73,15 @@ void release(afq_job* j) {
   fr(j->cum);
   fr(j->tmp_nn);
   fr(j->dscratch);
+  for (auto& p : j->prof) {
+    for (auto& sp : p.spans) {
+      (void)hipEventDestroy(sp.first);
+      (void)hipEventDestroy(sp.second);
+    }
+    for (auto ev : p.pool) (void)hipEventDestroy(ev);
+    p.spans.clear();
+    p.pool.clear();
+  }
 }
 
 // Walker panel of spin s: pointer and row stride (columns of the buffer).
@@ -116,9 +125,16 @@ int green_pipeline(afq_job* j, const void* up, const void* dn, bool want_inv, bo
     g.C = j->S[s];
     g.ldc = O;
     g.sC = (long)O * O;
-    AFQ_TRY(gemm_f64(g, st));
-    AFQ_TRY(batched_det_inv(j->S[s], (long)O * O, O, W, j->det[s], want_inv ? j->Sinv[s] : nullptr, (long)O * O, st));
+    {
+      ProfScope ps(j, AFQ_K_OVLP_GEMM, st);
+      AFQ_TRY(gemm_f64(g, st));
+    }
+    {
+      ProfScope ps(j, AFQ_K_DETINV, st);
+      AFQ_TRY(batched_det_inv(j->S[s], (long)O * O, O, W, j->det[s], want_inv ? j->Sinv[s] : nullptr, (long)O * O, st));
+    }
     if (want_green) {
+      ProfScope ps(j, AFQ_K_GREEN, st);
       Gemm h;
       h.M = O;
       h.N = N;
@@ -144,6 +160,7 @@ int green_pipeline(afq_job* j, const void* up, const void* dn, bool want_inv, bo
 }
 
 int overlap_from_dets(afq_job* j, double2* out, hipStream_t st) {
+  ProfScope ps(j, AFQ_K_ENERGY_MISC, st);
   return k_overlap_combine(j->det[0], j->one_spin ? nullptr : j->det[1], out, j->W, st);
 }
 
@@ -165,6 +182,7 @@ int force_bias_gemm(afq_job* j, double2* out, hipStream_t st) {
     g.ldc = j->G;
     g.alpha = double2{j->one_spin ? 2.0 : 1.0, 0.0};
     g.beta = double2{s > 0 ? 1.0 : 0.0, 0.0};
+    ProfScope ps(j, AFQ_K_FORCE_BIAS, st);
     AFQ_TRY(gemm_f64(g, st));
   }
   return AFQ_OK;
@@ -173,12 +191,18 @@ int force_bias_gemm(afq_job* j, double2* out, hipStream_t st) {
 int energy_pipeline(afq_job* j, const void* up, const void* dn, double2* out, hipStream_t st) {
   AFQ_TRY(green_pipeline(j, up, dn, true, true, st));
   const bool two = !j->one_spin;
-  AFQ_TRY(k_ene1(j->green[0], j->rot_h1[0], j->nocc[0], two ? j->green[1] : nullptr, two ? j->rot_h1[1] : nullptr,
-                 j->nocc[1], j->N, two ? 1.0 : 2.0, j->ene1, j->W, st));
+  {
+    ProfScope ps(j, AFQ_K_ENERGY_MISC, st);
+    AFQ_TRY(k_ene1(j->green[0], j->rot_h1[0], j->nocc[0], two ? j->green[1] : nullptr, two ? j->rot_h1[1] : nullptr,
+                   j->nocc[1], j->N, two ? 1.0 : 2.0, j->ene1, j->W, st));
+  }
   int nch[2] = {1, 1};
-  for (int s = 0; s < j->nspin_meas; ++s)
+  for (int s = 0; s < j->nspin_meas; ++s) {
+    ProfScope ps(j, AFQ_K_EXCHANGE, st);
     AFQ_TRY(k_exchange(j->rot_chol[s], j->green[s], j->nocc[s], j->N, j->G, j->W, j->coul + (long)s * j->W * j->G,
                        j->kpart + (long)s * j->W * j->kchunk_max, &nch[s], st));
+  }
+  ProfScope ps(j, AFQ_K_ENERGY_MISC, st);
   return k_energy_finalize(j->ene1, j->coul, two ? j->coul + (long)j->W * j->G : nullptr, j->kpart,
                            two ? j->kpart + (long)j->W * j->kchunk_max : nullptr, j->G, nch[0], nch[1], j->h0, out,
                            j->W, st);
@@ -201,6 +225,7 @@ int trotprop(afq_job* j, void* up, void* dn, const double2* xs, hipStream_t st) 
     g.C = j->vhs;
     g.ldc = NN;
     g.alpha = double2{0.0, std::sqrt(j->d.dt)};
+    ProfScope ps(j, AFQ_K_VHS, st);
     AFQ_TRY(gemm_f64(g, st));
   }
   for (int s = 0; s < j->nspin_prop; ++s) {
@@ -224,6 +249,7 @@ int trotprop(afq_job* j, void* up, void* dn, const double2* xs, hipStream_t st) 
       g.C = dst;
       g.ldc = c;
       g.sC = per;
+      ProfScope ps(j, AFQ_K_EXPH1, st);
       return gemm_f64(g, st);
     };
     double2* t0 = j->tbuf[0];
@@ -254,7 +280,10 @@ int trotprop(afq_job* j, void* up, void* dn, const double2* xs, hipStream_t st) 
       g.ldd = c;
       g.sD = per;
       g.alpha = double2{1.0 / n, 0.0};
-      AFQ_TRY(gemm_f64(g, st));
+      {
+        ProfScope ps(j, AFQ_K_TAYLOR, st);
+        AFQ_TRY(gemm_f64(g, st));
+      }
       double2* t = src;
       src = dst;
       dst = t;
@@ -268,15 +297,20 @@ int propagate(afq_job* j, void* up, void* dn, const double* fields, double* weig
               double* scalars, hipStream_t st) {
   AFQ_TRY(green_pipeline(j, up, dn, true, true, st));
   AFQ_TRY(force_bias_gemm(j, j->fb, st));
-  AFQ_TRY(k_field_shift(j->fb, fields, j->mf_shifts, j->G, std::sqrt(j->d.dt), j->xs, j->terms, j->W, st));
+  {
+    ProfScope ps(j, AFQ_K_FIELD, st);
+    AFQ_TRY(k_field_shift(j->fb, fields, j->mf_shifts, j->G, std::sqrt(j->d.dt), j->xs, j->terms, j->W, st));
+  }
   AFQ_TRY(trotprop(j, up, dn, j->xs, st));
   AFQ_TRY(green_pipeline(j, up, dn, false, false, st));
   AFQ_TRY(overlap_from_dets(j, j->ov_new, st));
+  ProfScope ps(j, AFQ_K_WEIGHT, st);
   return k_weight_update(j->ov_new, overlaps, j->terms, weights, scalars, j->h0_prop, j->d.dt, j->W, st);
 }
 
 int orthonormalize(afq_job* j, void* up, void* dn, hipStream_t st) {
   const int N = j->N;
+  ProfScope ps(j, AFQ_K_QR, st);
   if (j->restricted) return batched_cgs2(static_cast<double2*>(up), j->wcols, (long)N * j->wcols, N, j->wcols, j->W, st);
   AFQ_TRY(batched_cgs2(static_cast<double2*>(up), j->nocc[0], (long)N * j->nocc[0], N, j->nocc[0], j->W, st));
   return batched_cgs2(static_cast<double2*>(dn), j->nocc[1], (long)N * j->nocc[1], N, j->nocc[1], j->W, st);
@@ -333,14 +367,7 @@ int afq_job_create(afq_job_t* out, const afq_job_desc* d) {
 
   const long N = j->N, G = j->G, W = j->W;
   const long wmax = std::max(j->wcols, std::max(j->nocc[0], j->nocc[1]));
-  int gc, R, C;
-  exchange_geometry(std::max(j->nocc[0], j->nocc[1]), &gc, &R, &C);
-  int kch = 1;
-  for (int s = 0; s < 2; ++s) {
-    int g2, r2, c2;
-    exchange_geometry(j->nocc[s], &g2, &r2, &c2);
-    kch = std::max(kch, (int)((G + g2 - 1) / g2));
-  }
+  const int kch = exchange_nchunk((int)G);
   j->kchunk_max = kch;
   int rc = AFQ_OK;
 #define A_(p, n)                      \
@@ -748,6 +775,7 @@ int afq_stochastic_reconfiguration(afq_job_t j, void* up, void* dn, double* weig
   AFQ_TRY(check_walkers(j, up, dn));
   if (!weights || !zeta) return fail(AFQ_E_INVALID, "NULL argument");
   hipStream_t st = S(stream);
+  ProfScope ps(j, AFQ_K_SR, st);
   AFQ_TRY(k_sr_index(weights, zeta, j->W, j->cum, j->sr_idx, st));
   for (int s = 0; s < j->nspin_prop; ++s) {
     double2* Wp = static_cast<double2*>(s == 0 ? up : dn);
@@ -782,7 +810,10 @@ int afq_block(afq_job_t j, void* up, void* dn, double* weights, void* overlaps, 
     AFQ_TRY(dalloc_n(&j->fields, (size_t)need));
     j->fields_cap = need;
   }
-  AFQ_TRY(k_normal_fill(j->fields, need, seed, *rng_counter, st));
+  {
+    ProfScope ps(j, AFQ_K_RNG, st);
+    AFQ_TRY(k_normal_fill(j->fields, need, seed, *rng_counter, st));
+  }
   *rng_counter += 1;
   double2* ov = static_cast<double2*>(overlaps);
   for (int i = 0; i < n_prop_steps; ++i)
@@ -791,7 +822,34 @@ int afq_block(afq_job_t j, void* up, void* dn, double* weights, void* overlaps, 
   AFQ_TRY(green_pipeline(j, up, dn, false, false, st));
   AFQ_TRY(overlap_from_dets(j, ov, st));
   AFQ_TRY(energy_pipeline(j, up, dn, j->energies, st));
+  ProfScope ps(j, AFQ_K_ENERGY_MISC, st);
   return k_block_finalize(j->energies, weights, scalars, std::sqrt(2.0 / j->d.dt), j->W, block_out, st);
+}
+
+int afq_job_profile(afq_job_t j, int cls, int enable) {
+  if (!j) return fail(AFQ_E_INVALID, "job is NULL");
+  if (cls < 0 || cls >= AFQ_K_NCLASSES) return fail(AFQ_E_INVALID, "unknown kernel class");
+  j->prof[cls].on = enable != 0;
+  return AFQ_OK;
+}
+
+int afq_job_profile_read(afq_job_t j, int cls, double* total_ms, int64_t* launches) {
+  if (!j || !total_ms || !launches) return fail(AFQ_E_INVALID, "NULL argument");
+  if (cls < 0 || cls >= AFQ_K_NCLASSES) return fail(AFQ_E_INVALID, "unknown kernel class");
+  auto& p = j->prof[cls];
+  double sum = 0.0;
+  for (auto& sp : p.spans) {
+    AFQ_HIP(hipEventSynchronize(sp.second));
+    float ms = 0.f;
+    AFQ_HIP(hipEventElapsedTime(&ms, sp.first, sp.second));
+    sum += ms;
+    p.pool.push_back(sp.first);
+    p.pool.push_back(sp.second);
+  }
+  *total_ms = sum;
+  *launches = (int64_t)p.spans.size();
+  p.spans.clear();
+  return AFQ_OK;
 }
 
 int afq_walkers_orthonormalize(void* walkers, int32_t n_walkers, int32_t norb, int32_t ncols, void* stream) {

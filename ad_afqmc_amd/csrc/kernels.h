@@ -16,7 +16,7 @@ int k_scale(double* out, const double* A, double scale, int n, hipStream_t s);
 int k_overlap_combine(const double2* d0, const double2* d1, double2* out, int W, hipStream_t s);
 int k_ene1(const double2* G0, const double* rh0, int n0, const double2* G1, const double* rh1, int n1, int N, double f0,
            double2* out, int W, hipStream_t s);
-void exchange_geometry(int O, int* gc, int* R, int* C);
+int exchange_nchunk(int Gn);
 int k_exchange(const double* rc, const double2* Gm, int O, int N, int Gn, int W, double2* coul, double2* kpart,
                int* nchunk_out, hipStream_t s);
 int k_energy_finalize(const double2* ene1, const double2* c0, const double2* c1, const double2* k0, const double2* k1,

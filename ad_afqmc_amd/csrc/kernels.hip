@@ -143,108 +143,6 @@ __global__ void ene1_kernel(const double2* G0, const double* rh0, int n0, const 
   if (threadIdx.x == 0) out[w] = acc;
 }
 
-// Fused exchange: for walker w = blockIdx.y and gamma chunk blockIdx.x,
-// f_g = rot_chol_g (O x N) . G_w^T (N x O) on MFMA for gc gammas at once (rows
-// (g,i) stacked), then c_g = tr f_g and K = sum_g sum_ij f_gij f_gji from the
-// tile parked in LDS.  Writes coul[w, g] and kpart[w, chunk].
-constexpr int EXK = 16;
-__global__ __launch_bounds__(256) void exchange_f64_kernel(const double* __restrict__ rc, const double2* __restrict__ Gm,
-                                                           int O, int N, int Gn, int gc, int R, int C, int nchunk,
-                                                           double2* __restrict__ coul, double2* __restrict__ kpart) {
-  extern __shared__ double lds[];
-  // staging: As[EXK][R+1] (real), Bs[2][EXK][C+1]; epilogue: f[R][C] complex (aliases)
-  double* As = lds;
-  double* Bs = lds + EXK * (R + 1);
-  double2* f = reinterpret_cast<double2*>(lds);
-  __shared__ double2 kred[4];
-
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int chunk = blockIdx.x;
-  const long w = blockIdx.y;
-  const int g0 = chunk * gc;
-  const int ng = min(gc, Gn - g0);
-  const int rows = ng * O;  // valid rows
-  const int nbr = R / 16, nbc = C / 16, nblk = nbr * nbc;
-  const double* Ab = rc + (long)g0 * O * N;
-  const double2* Bb = Gm + w * (long)O * N;
-
-  v4d accR[4], accI[4];
-#pragma unroll
-  for (int t = 0; t < 4; ++t) {
-    accR[t] = v4d{0.0, 0.0, 0.0, 0.0};
-    accI[t] = v4d{0.0, 0.0, 0.0, 0.0};
-  }
-
-  for (int p0 = 0; p0 < N; p0 += EXK) {
-    for (int e = tid; e < R * EXK; e += 256) {
-      const int r = e / EXK, k = e % EXK;
-      const int p = p0 + k;
-      As[k * (R + 1) + r] = (r < rows && p < N) ? Ab[(long)r * N + p] : 0.0;
-    }
-    for (int e = tid; e < C * EXK; e += 256) {
-      const int j = e / EXK, k = e % EXK;
-      const int p = p0 + k;
-      double2 v = {0.0, 0.0};
-      if (j < O && p < N) v = Bb[(long)j * N + p];
-      Bs[k * (C + 1) + j] = v.x;
-      Bs[EXK * (C + 1) + k * (C + 1) + j] = v.y;
-    }
-    __syncthreads();
-#pragma unroll
-    for (int kk = 0; kk < EXK; kk += 4) {
-      const int kr = kk + (lane >> 4), cidx = lane & 15;
-#pragma unroll
-      for (int t = 0; t < 4; ++t) {
-        const int blk = wave + 4 * t;
-        if (blk < nblk) {
-          const int br = blk / nbc, bc = blk % nbc;
-          const double a = As[kr * (R + 1) + br * 16 + cidx];
-          const double bre = Bs[kr * (C + 1) + bc * 16 + cidx];
-          const double bim = Bs[EXK * (C + 1) + kr * (C + 1) + bc * 16 + cidx];
-          accR[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, bre, accR[t], 0, 0, 0);
-          accI[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, bim, accI[t], 0, 0, 0);
-        }
-      }
-    }
-    __syncthreads();
-  }
-  // park the valid O x O blocks of f in LDS: f[(gl*O + i)*O + j]
-#pragma unroll
-  for (int t = 0; t < 4; ++t) {
-    const int blk = wave + 4 * t;
-    if (blk < nblk) {
-      const int br = blk / nbc, bc = blk % nbc;
-#pragma unroll
-      for (int v = 0; v < 4; ++v) {
-        const int r = br * 16 + (lane >> 4) + 4 * v;
-        const int c = bc * 16 + (lane & 15);
-        if (r < rows && c < O) f[r * O + c] = double2{accR[t][v], accI[t][v]};
-      }
-    }
-  }
-  __syncthreads();
-  double2 kacc = {0.0, 0.0};
-  for (int gl = wave; gl < ng; gl += 4) {
-    const int base = gl * O;
-    double2 kk = {0.0, 0.0}, cc = {0.0, 0.0};
-    for (int e = lane; e < O * O; e += 64) {
-      const int i = e / O, j = e % O;
-      const double2 x = f[(base + i) * O + j], y = f[(base + j) * O + i];
-      kk = cadd(kk, cmul(x, y));
-    }
-    for (int i = lane; i < O; i += 64) cc = cadd(cc, f[(base + i) * O + i]);
-    kk.x = wave_sum(kk.x);
-    kk.y = wave_sum(kk.y);
-    cc.x = wave_sum(cc.x);
-    cc.y = wave_sum(cc.y);
-    if (lane == 0) coul[w * Gn + g0 + gl] = cc;
-    kacc = cadd(kacc, kk);
-  }
-  if (lane == 0) kred[wave] = kacc;
-  __syncthreads();
-  if (tid == 0) kpart[w * nchunk + chunk] = cadd(cadd(kred[0], kred[1]), cadd(kred[2], kred[3]));
-}
-
 // E = h0 + ene1 + 1/2 sum_g (c_up + c_dn)^2 - 1/2 (K_up + K_dn)
 __global__ void energy_finalize_kernel(const double2* ene1, const double2* c0, const double2* c1, const double2* k0,
                                        const double2* k1, int Gn, int nch0, int nch1, double h0, double2* out) {
@@ -508,43 +406,6 @@ int k_ene1(const double2* G0, const double* rh0, int n0, const double2* G1, cons
            double2* out, int W, hipStream_t s) {
   hipLaunchKernelGGL(ene1_kernel, dim3(W), dim3(256), 0, s, G0, rh0, n0, G1, rh1, n1, N, f0, out);
   return launch_status("ene1_kernel");
-}
-// gammas per work-group: the stacked (g,i) x j tile must fit 16 MFMA blocks
-// (4 per wave) and the parked f (g*O*O complex) 64 KB of LDS.
-void exchange_geometry(int O, int* gc, int* R, int* C) {
-  const int c = ((O + 15) / 16) * 16;
-  int g = 1;
-  while (true) {
-    const int r = (((g + 1) * O + 15) / 16) * 16;
-    if (r * c > 4096 || (long)(g + 1) * O * O * 16 > 65536) break;
-    ++g;
-  }
-  *gc = g;
-  *R = ((g * O + 15) / 16) * 16;
-  *C = c;
-}
-int k_exchange(const double* rc, const double2* Gm, int O, int N, int Gn, int W, double2* coul, double2* kpart,
-               int* nchunk_out, hipStream_t s) {
-  if (O < 1 || O > 64) return fail(AFQ_E_UNSUPPORTED, "exchange kernel supports 1 <= nocc <= 64");
-  int gc, R, C;
-  exchange_geometry(O, &gc, &R, &C);
-  const int nchunk = (Gn + gc - 1) / gc;
-  *nchunk_out = nchunk;
-  const size_t stage = sizeof(double) * (EXK * (R + 1) + 2 * EXK * (C + 1));
-  const size_t park = sizeof(double2) * gc * O * O;
-  const size_t lds = stage > park ? stage : park;
-  if (lds + 256 > 65536) {
-    static bool raised = false;
-    if (!raised) {
-      hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(exchange_f64_kernel),
-                                         hipFuncAttributeMaxDynamicSharedMemorySize, 96 * 1024);
-      if (e != hipSuccess) return fail(AFQ_E_HIP, std::string("exchange LDS attribute: ") + hipGetErrorString(e));
-      raised = true;
-    }
-  }
-  hipLaunchKernelGGL(exchange_f64_kernel, dim3(nchunk, W), dim3(256), lds, s, rc, Gm, O, N, Gn, gc, R, C, nchunk,
-                     coul, kpart);
-  return launch_status("exchange_f64_kernel");
 }
 int k_energy_finalize(const double2* ene1, const double2* c0, const double2* c1, const double2* k0, const double2* k1,
                       int Gn, int nch0, int nch1, double h0, double2* out, int W, hipStream_t s) {

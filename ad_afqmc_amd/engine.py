@@ -213,6 +213,16 @@ class EngineJob:
         pu, pd = self._w(up, dn)
         nat.call("afq_stochastic_reconfiguration", self._h, pu, pd, ptr(weights), ptr(zeta), ptr(idx_out), stream_ptr())
 
+    # ------------------------------------------------------------ live kernel timing
+    def profile(self, kernel_class: str, enable: bool = True):
+        nat.call("afq_job_profile", self._h, nat.KERNEL_CLASSES.index(kernel_class), int(enable))
+
+    def profile_read(self, kernel_class: str):
+        """(summed ms, launches) of the class since the last read (synchronises)."""
+        ms, n = C.c_double(), C.c_int64()
+        nat.call("afq_job_profile_read", self._h, nat.KERNEL_CLASSES.index(kernel_class), C.byref(ms), C.byref(n))
+        return float(ms.value), int(n.value)
+
     def block(self, up, dn, weights, overlaps, scalars, n_prop_steps: int, seed: int, counter: int):
         """One sampler block on the device; returns ({block_energy, block_weight} tensor, new counter)."""
         pu, pd = self._w(up, dn)

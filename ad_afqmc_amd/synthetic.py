@@ -47,11 +47,15 @@ class System:
 
 
 def make_system(norb: int, nocc_up: int, nchol: int, nocc_dn: Optional[int] = None, uhf: bool = False,
-                seed: int = 0) -> System:
+                seed: int = 0, l_scale: float = 1.0) -> System:
+    """l_scale multiplies every s_g.  l_scale = 1 is the generator as stated;
+    its two-body scale grows like sqrt(N) (|sqrt(dt) sum_g x_g L_g| ~ 7 at
+    N=200, G=1200, dt=0.005), so walkers are killed within a few steps.
+    l_scale = 1/sqrt(N) gives O(1) Cholesky-vector norms, as in molecules."""
     rng = np.random.default_rng(seed)
     N, G = norb, nchol
     h1 = np.diag(np.linspace(-2.0, 1.0, N)) + 0.05 * _sym(rng.standard_normal((N, N)))
-    sig = 0.5 * np.exp(-np.arange(G) / (0.2 * G))
+    sig = l_scale * 0.5 * np.exp(-np.arange(G) / (0.2 * G))
     chol = np.empty((G, N * N))
     for g in range(G):  # one gamma at a time keeps peak memory at the size of chol
         a = rng.standard_normal((N, N)) * sig[g]

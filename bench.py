@@ -1,0 +1,270 @@
+#!/usr/bin/env python
+"""Walker-step throughput of the phaseless-AFQMC engine on one MI355X.
+
+One bench step = one device sampler block on the synthetic RHF workload
+(N=200, O=40, G=1200, 512 walkers, dt=0.005): 10 phaseless steps with
+on-device fields, QR re-orthonormalisation, overlap and local-energy
+measurement, population-control update, then comb resampling.
+value = walkers * 10 * steps / wall time of the timed region (walker-steps/s).
+
+Prints one JSON line.  --gpus N > 1 is refused: this engine runs on one GPU.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+PEAKS = {  # MI355X published dense peaks
+    "fp64": 78.6e12,
+    "fp32": 157.3e12,
+    "bf16": 2.5e15,
+    "hbm": 8.0e12,
+}
+
+
+def flops_per_walker_step(N, Oa, Ob, G, m, one_spin, K=5):
+    """Algorithmic real flops per walker-step (complex x real MAC = 4, complex x
+    complex MAC = 8), with the measurement (QR + energy) amortised over m steps."""
+    occ = [Oa] if one_spin else [Oa, Ob]
+    sumO = sum(occ)
+    f = {}
+    f["vhs"] = 4.0 * G * N * N
+    f["taylor"] = 8.0 * K * N * N * sumO
+    f["exp_h1"] = 8.0 * N * N * sumO
+    green = sum(4.0 * o * o * N + 8.0 * o**3 + 8.0 * N * o * o for o in occ)
+    f["green_fb"] = green
+    f["force_bias"] = 4.0 * G * N * sumO
+    f["overlap"] = sum(4.0 * o * o * N + 8.0 / 3.0 * o**3 for o in occ)
+    f["energy"] = (sum(4.0 * G * o * o * N + 8.0 * G * o * o for o in occ) + green) / m
+    f["qr"] = sum(8.0 * (2.0 * N * o * o - 2.0 / 3.0 * o**3) for o in occ) / m
+    return sum(f.values()), f
+
+
+def launch_flops(cls, N, c_cols, occ, G, W):
+    """Algorithmic flops of ONE launch of a kernel class (per spin buffer)."""
+    if cls == "vhs":
+        return 4.0 * G * N * N * W
+    if cls == "taylor":
+        return 8.0 * N * N * c_cols * W
+    if cls == "exp_h1":
+        return 4.0 * N * N * c_cols * W
+    if cls == "exchange":
+        return 4.0 * G * occ * occ * N * W
+    if cls == "force_bias":
+        return 4.0 * G * occ * N * W
+    if cls == "green":
+        return 8.0 * occ * occ * N * W
+    if cls == "overlap_gemm":
+        return 4.0 * occ * occ * N * W
+    return None
+
+
+def cpu_baseline(cfg, sample_walkers=8, seed=0):
+    """The FP64 oracle timed on one host core on a bounded sample: a few walkers
+    of the same workload through one block (10 steps + QR + energy)."""
+    import numpy as np
+
+    try:
+        from threadpoolctl import threadpool_limits
+    except Exception:  # pragma: no cover
+        threadpool_limits = None
+    from ad_afqmc_amd import synthetic as syn
+    from oracle import afqmc_oracle as O
+
+    s = syn.make_system(cfg["norb"], cfg["nocc_up"], cfg["nchol"], seed=seed)
+    dt, nsteps, W = cfg["dt"], cfg["n_prop_steps"], sample_walkers
+    h1s, rh, rc = O.rhf_measurement_intermediates(s.h1, s.chol, s.mo_up)
+    pi = O.propagation_intermediates(s.h0, h1s, s.chol, s.rdm1(), dt)
+    trial = O.RHFTrial(s.norb, s.nelec, s.mo_up)
+    ham = dict(h0=s.h0, chol=s.chol, rot_h1=rh, rot_chol=rc, mf_shifts=pi["mf_shifts"], h0_prop=pi["h0_prop"],
+               exp_h1=pi["exp_h1"])
+    w0 = O.init_walkers(s.rdm1(), s.nelec, W, "restricted")
+    e0 = float(np.mean(trial.calc_energy(w0, s.h0, rh, rc).real))
+    state = dict(walkers=w0, weights=np.ones(W), overlaps=trial.calc_overlap(w0), e_estimate=e0,
+                 pop_control_ene_shift=e0)
+    rng = np.random.default_rng(seed)
+    fields = rng.standard_normal((nsteps, W, s.nchol))
+    ctx = threadpool_limits(limits=1) if threadpool_limits else None
+    try:
+        t0 = time.perf_counter()
+        state, be, bw = O.block_scan(trial, state, fields, ham, dt)
+        t = time.perf_counter() - t0
+    finally:
+        if ctx is not None:
+            ctx.unregister() if hasattr(ctx, "unregister") else None
+    return {
+        "value": W * nsteps / t,
+        "unit": "walker-steps/s",
+        "cores": 1,
+        "kind": "port",
+        "sample": f"{W} walkers x 1 block ({nsteps} steps + QR + energy) of the same workload, FP64 NumPy oracle, "
+        f"1 BLAS thread, {t:.1f} s",
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10, help="timed blocks")
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--config", default="rhf_n200_o40_g1200_w512")
+    ap.add_argument("--precision", default="fp64")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-sample-walkers", type=int, default=8)
+    ap.add_argument("--breakdown", action="store_true", help="print per-class kernel times to stderr")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this engine runs on one GPU; --gpus above 1 is not supported", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+
+    from ad_afqmc_amd import engine as E
+    from ad_afqmc_amd import hamiltonian as hm
+    from ad_afqmc_amd import propagation as pr
+    from ad_afqmc_amd import synthetic as syn
+    from ad_afqmc_amd import wavefunctions as wf
+    from ad_afqmc_amd.binding import engine_for
+
+    cfg = syn.CONFIGS[args.config]
+    N, G, W = cfg["norb"], cfg["nchol"], cfg["n_walkers"]
+    uhf = cfg["uhf"]
+    Oa = cfg["nocc_up"]
+    Ob = cfg.get("nocc_dn", Oa)
+    nsteps, dt = cfg["n_prop_steps"], cfg["dt"]
+    print(f"[bench] building {args.config}", file=sys.stderr, flush=True)
+    s = syn.make_system(N, Oa, G, nocc_dn=Ob, uhf=uhf, seed=0)
+    wave, ham = s.wave_data(), s.ham_data()
+    trial = wf.uhf(N, (Oa, Ob)) if uhf else wf.rhf(N, (Oa, Oa))
+    wtype = "unrestricted" if uhf else "restricted"
+    prop = pr.propagator_afqmc(dt=dt, n_walkers=W, walker_type=wtype, precision=args.precision)
+    H = hm.hamiltonian(N)
+    ham = H.build_measurement_intermediates(ham, trial, wave)
+    ham = H.build_propagation_intermediates(ham, prop, trial, wave)
+    pd = prop.init_prop_data(trial, wave, ham, seed=0)
+    job = engine_for(trial, wave, ham, wtype, W, prop)
+    up, dn = (t.clone() if t is not None else None for t in pd["walkers"].tensors())
+    weights = pd["weights"].clone()
+    overlaps = job.overlap(up, dn)
+    scalars = torch.stack([pd["e_estimate"], pd["e_estimate"].clone()]).contiguous()
+    zetas = torch.rand(args.warmup + args.steps + 8, dtype=torch.float64, device=up.device)
+    counter = 0
+
+    def one_block(i):
+        nonlocal counter
+        out, counter = job.block(up, dn, weights, overlaps, scalars, nsteps, 1234, counter)
+        job.stochastic_reconfiguration(up, dn, weights, zetas[i : i + 1])
+        return out
+
+    # warmup with every kernel class timed, to find the dominant one
+    for c in E.nat.KERNEL_CLASSES:
+        job.profile(c, True)
+    for i in range(args.warmup):
+        one_block(i)
+    torch.cuda.synchronize()
+    breakdown = {c: job.profile_read(c) for c in E.nat.KERNEL_CLASSES}
+    for c in E.nat.KERNEL_CLASSES:
+        job.profile(c, False)
+    dominant = max(breakdown, key=lambda c: breakdown[c][0])
+    if args.breakdown:
+        tot = sum(v[0] for v in breakdown.values())
+        for c, (ms, n) in sorted(breakdown.items(), key=lambda kv: -kv[1][0]):
+            print(f"[bench] {c:14s} {ms / max(args.warmup, 1):9.3f} ms/block  {n // max(args.warmup, 1):5d} launches/block"
+                  f"  {100 * ms / max(tot, 1e-9):5.1f}%", file=sys.stderr)
+
+    # timed region: only the dominant class is bracketed by events
+    job.profile(dominant, True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    outs = []
+    for i in range(args.steps):
+        outs.append(one_block(args.warmup + i))
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    dom_ms, dom_n = job.profile_read(dominant)
+    job.profile(dominant, False)
+    blocks = torch.stack(outs).cpu().numpy()
+    finite = bool(np.all(np.isfinite(blocks)))
+
+    walker_steps = W * nsteps * args.steps
+    value = walker_steps / elapsed
+    one_spin = not uhf
+    f_total, _ = flops_per_walker_step(N, Oa, Ob, G, nsteps, one_spin)
+    occ = Oa
+    c_cols = Oa
+    per_launch = launch_flops(dominant, N, c_cols, occ, G, W)
+    avg_s = dom_ms / max(dom_n, 1) / 1e3
+    peak = PEAKS["fp64"] if args.precision == "fp64" else PEAKS["fp32"]
+    roof = None
+    if per_launch is not None and avg_s > 0:
+        ach = per_launch / avg_s
+        roof = {
+            "kernel": dominant,
+            "bound": "mfma",
+            "achieved": ach / 1e12,
+            "peak": peak / 1e12,
+            "unit": "TFLOP/s",
+            "frac": ach / peak,
+            "traffic": None,
+            "avg_launch_ms": avg_s * 1e3,
+            "flops_per_launch": per_launch,
+        }
+        tfile = os.path.join(ROOT, "profiles", "traffic.json")
+        if os.path.exists(tfile):
+            try:
+                t = json.load(open(tfile)).get(f"{args.config}:{args.precision}:{dominant}")
+                if t is not None:
+                    roof["traffic"] = t
+            except Exception:
+                pass
+    line = {
+        "metric": "walker-steps/s at <=0.1 mHa vs FP64 oracle; p50/p99 job latency under load",
+        "value": value,
+        "unit": "walker-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": elapsed / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64" if args.precision == "fp64" else args.precision,
+        "data": "synthetic",
+        "config": {
+            "workload": args.config,
+            "norb": N,
+            "nocc": [Oa, Ob] if uhf else [Oa],
+            "nchol": G,
+            "walkers": W,
+            "dt": dt,
+            "steps_per_block": nsteps,
+            "trial": "uhf" if uhf else "rhf",
+            "precision": args.precision,
+            "parallelism": "single-gpu",
+        },
+        "model_flops_per_walker_step": f_total,
+        "achieved_tflops_whole_step": f_total * value / 1e12,
+        "roofline": roof,
+        "kernel_breakdown_ms_per_block": {c: v[0] / max(args.warmup, 1) for c, v in breakdown.items()},
+        # The stated generator kills the walker population (see synthetic.make_system);
+        # the work per block is the same dense arithmetic either way.
+        "population_finite": finite,
+        "mean_block_energy": float(np.mean(blocks[:, 0])) if finite else None,
+    }
+    if not args.no_cpu_baseline:
+        print("[bench] cpu baseline (oracle sample)", file=sys.stderr, flush=True)
+        line["cpu_baseline"] = cpu_baseline(cfg, args.cpu_sample_walkers) if not uhf else None
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -162,6 +162,31 @@ int afq_block(afq_job_t job, void* walkers_up, void* walkers_dn, double* weights
  * (seed, counter) -- the stream afq_block uses. */
 int afq_normal_fill(double* out, int64_t n, uint64_t seed, uint64_t counter, void* stream);
 
+/* Live kernel timing.  Kernel classes of the walker step: */
+enum {
+  AFQ_K_VHS = 0,        /* VHS = i sqrt(dt) x~ . L GEMM */
+  AFQ_K_TAYLOR = 1,     /* one Taylor term VHS . X (per spin buffer) */
+  AFQ_K_EXPH1 = 2,      /* exp(-dt/2 h1) . W */
+  AFQ_K_OVLP_GEMM = 3,  /* S = T^H W */
+  AFQ_K_DETINV = 4,     /* Gauss-Jordan det / inverse of S */
+  AFQ_K_GREEN = 5,      /* G = (W S^-1)^T */
+  AFQ_K_FORCE_BIAS = 6, /* fb = G . rot_chol^T */
+  AFQ_K_FIELD = 7,      /* field shifts */
+  AFQ_K_WEIGHT = 8,     /* phaseless weight update */
+  AFQ_K_EXCHANGE = 9,   /* fused Coulomb/exchange */
+  AFQ_K_ENERGY_MISC = 10,
+  AFQ_K_QR = 11,
+  AFQ_K_SR = 12,
+  AFQ_K_RNG = 13,
+  AFQ_K_NCLASSES = 14
+};
+/* When enabled for a class, every launch of that class on the job's calls is
+ * bracketed by hipEventRecord on the launch stream.  afq_job_profile_read
+ * synchronises on the recorded events, returns the summed elapsed ms and
+ * launch count of the class, and clears its record. */
+int afq_job_profile(afq_job_t job, int kernel_class, int enable);
+int afq_job_profile_read(afq_job_t job, int kernel_class, double* total_ms, int64_t* launches);
+
 /* Job-free walker-batch operations (the reference's walker containers carry no
  * Hamiltonian: RHFWalkers/UHFWalkers.orthonormalize, walkers.py:155-157,
  * 260-262, and stochastic_reconfiguration_local, walkers.py:139-145, 244-250).

@@ -231,7 +231,10 @@ def vhs_from_fields(shifted_fields: np.ndarray, chol: np.ndarray, dt: float) -> 
     """VHS = i sqrt(dt) (x~ @ L).reshape(N,N), propagation.py:196-203, 215-222."""
     n2 = chol.reshape(chol.shape[0], -1).shape[1]
     n = int(round(math.sqrt(n2)))
-    v = shifted_fields.astype(np.complex128) @ chol.reshape(-1, n2).astype(np.float64)
+    L = chol.reshape(-1, n2).astype(np.float64)
+    x = shifted_fields.astype(np.complex128)
+    # complex @ real as two real products (same values as the complex product)
+    v = (x.real @ L) + 1.0j * (x.imag @ L)
     return 1.0j * math.sqrt(dt) * v.reshape(-1, n, n)
 
 

@@ -88,7 +88,9 @@ CASES = [
 
 
 def _setup(trial_kind, wtype, N, ou, od, G, W, seed=0, dt=0.01):
-    sysm = syn.make_system(N, ou, G, nocc_dn=od, uhf=(trial_kind == "uhf"), seed=seed)
+    # Cholesky scale ~ 1/sqrt(N) keeps the walker population alive at this dt
+    sysm = syn.make_system(N, ou, G, nocc_dn=od, uhf=(trial_kind == "uhf"), seed=seed,
+                           l_scale=min(1.0, 2.0 / math.sqrt(N)))
     wave = sysm.wave_data()
     ham = sysm.ham_data()
     trial = wf.rhf(N, (ou, ou)) if trial_kind == "rhf" else wf.uhf(N, (ou, od))
@@ -105,13 +107,15 @@ def _setup(trial_kind, wtype, N, ou, od, G, W, seed=0, dt=0.01):
         otrial = O.UHFTrial(N, (ou, od), sysm.mo_up, sysm.mo_dn)
     opi = O.propagation_intermediates(sysm.h0, h1s, sysm.chol, sysm.rdm1(), dt, wtype)
     oham = dict(h0=sysm.h0, chol=sysm.chol, rot_h1=orh, rot_chol=orc, **{k: opi[k] for k in ("mf_shifts", "h0_prop", "exp_h1")})
+    # perturbation ~ 0.3/sqrt(N) per element keeps the walkers' energies near the trial's
+    amp = 0.3 / math.sqrt(N)
     if wtype == "restricted":
-        wn = syn.perturbed_walkers(sysm.mo_up, W, 0.1, seed + 1)
+        wn = syn.perturbed_walkers(sysm.mo_up, W, amp, seed + 1)
         walkers, owalk = wk.RHFWalkers(wn), wn
     else:
         dn_mo = sysm.mo_up if sysm.mo_dn is None else sysm.mo_dn
-        wu = syn.perturbed_walkers(sysm.mo_up, W, 0.1, seed + 1)
-        wd = syn.perturbed_walkers(dn_mo, W, 0.1, seed + 2)
+        wu = syn.perturbed_walkers(sysm.mo_up, W, amp, seed + 1)
+        wd = syn.perturbed_walkers(dn_mo, W, amp, seed + 2)
         walkers, owalk = wk.UHFWalkers([wu, wd]), [wu, wd]
     return sysm, trial, prop, ham, wave, walkers, otrial, oham, owalk, h1s
 
@@ -159,6 +163,7 @@ def test_propagate_constrained_steps(case):
             assert rel(npy(pd["walkers"].data[1]), st["walkers"][1]) < TOL, step
         assert rel(npy(pd["overlaps"]), st["overlaps"]) < TOL, step
         assert rel(npy(pd["weights"]), st["weights"]) < TOL, step
+        assert np.count_nonzero(st["weights"]) >= W // 2, step  # the step is not trivially clipped
         assert abs(pd["pop_control_ene_shift"].item() - st["pop_control_ene_shift"]) < TOL * max(1, abs(e_est))
 
 
