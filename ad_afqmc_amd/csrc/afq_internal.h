@@ -60,8 +60,22 @@ struct Gemm {
   long ldd = 0, sD = 0;
   double2 alpha = {1.0, 0.0};
   double2 beta = {0.0, 0.0};
+  // optional split-K workspace (doubles); used only for unbatched products
+  // with too few output tiles to fill the GPU
+  void* ws = nullptr;
+  long ws_elems = 0;
 };
 int gemm_f64(const Gemm& g, hipStream_t stream);
+// C = alpha * sum of `ksplit` split-K partial planes in ws + beta * C (+ D), fixed order.
+int gemm_splitk_reduce(const Gemm& g, const double* ws, int ksplit, bool cplx, hipStream_t stream);
+
+// Reduced-precision products of the walker step (FP32, or split-BF16 x3), FP32 accumulation:
+//   VHS        A complex double (W x G) . B float (G x N^2)          -> C complex float
+//   TAYLOR_D   A complex float (N x N) . B complex double (N x c)    -> C complex float, D complex double +=
+//   TAYLOR_F   A complex float (N x N) . B complex float (N x c)     -> C complex float, D complex double +=
+//   FORCE_BIAS A complex double (W x ON) . B float^T (G x ON)        -> C complex double
+enum class LpKind { VHS, TAYLOR_D, TAYLOR_F, FORCE_BIAS };
+int gemm_lp(LpKind kind, bool bf16x3, const Gemm& g, hipStream_t stream);
 
 // ---------------------------------------------------------------- small dense linear algebra
 // In-place Gauss-Jordan with partial pivoting (|re|+|im| pivot metric, first
@@ -131,6 +145,14 @@ struct afq_job {
   double* dscratch = nullptr;             // small device scalars
   int kchunk_max = 1;                     // exchange partials per walker
   double* tmp_nn = nullptr;               // (8, N, N) setup temporaries
+  // reduced-precision mode (desc.precision != FP64)
+  bool lp = false, bf16x3 = false;
+  float* chol32 = nullptr;                // (G, N*N)
+  float* rot_chol32[2] = {nullptr, nullptr}; // (G, nocc_s, N)
+  float2* vhs32 = nullptr;                // (W, N, N)
+  float2* t32[2] = {nullptr, nullptr};    // (W, N, wmax) Taylor terms
+  double* gemm_ws = nullptr;              // split-K workspace
+  long gemm_ws_elems = 0;
 
   // live per-class kernel timing (afq_job_profile)
   struct Prof {

@@ -72,6 +72,13 @@ void release(afq_job* j) {
   fr(j->sr_idx);
   fr(j->cum);
   fr(j->tmp_nn);
+  fr(j->gemm_ws);
+  fr(j->chol32);
+  fr(j->rot_chol32[0]);
+  if (!j->rhf) fr(j->rot_chol32[1]);
+  fr(j->vhs32);
+  fr(j->t32[0]);
+  fr(j->t32[1]);
   fr(j->dscratch);
   for (auto& p : j->prof) {
     for (auto& sp : p.spans) {
@@ -182,8 +189,15 @@ int force_bias_gemm(afq_job* j, double2* out, hipStream_t st) {
     g.ldc = j->G;
     g.alpha = double2{j->one_spin ? 2.0 : 1.0, 0.0};
     g.beta = double2{s > 0 ? 1.0 : 0.0, 0.0};
+    g.ws = j->gemm_ws;
+    g.ws_elems = j->gemm_ws_elems;
     ProfScope ps(j, AFQ_K_FORCE_BIAS, st);
-    AFQ_TRY(gemm_f64(g, st));
+    if (j->lp) {
+      g.B = j->rot_chol32[s];
+      AFQ_TRY(gemm_lp(LpKind::FORCE_BIAS, j->bf16x3, g, st));
+    } else {
+      AFQ_TRY(gemm_f64(g, st));
+    }
   }
   return AFQ_OK;
 }
@@ -199,8 +213,13 @@ int energy_pipeline(afq_job* j, const void* up, const void* dn, double2* out, hi
   int nch[2] = {1, 1};
   for (int s = 0; s < j->nspin_meas; ++s) {
     ProfScope ps(j, AFQ_K_EXCHANGE, st);
-    AFQ_TRY(k_exchange(j->rot_chol[s], j->green[s], j->nocc[s], j->N, j->G, j->W, j->coul + (long)s * j->W * j->G,
-                       j->kpart + (long)s * j->W * j->kchunk_max, &nch[s], st));
+    double2* coul = j->coul + (long)s * j->W * j->G;
+    double2* kpart = j->kpart + (long)s * j->W * j->kchunk_max;
+    if (j->lp)
+      AFQ_TRY(k_exchange_lp(j->rot_chol32[s], j->green[s], j->nocc[s], j->N, j->G, j->W, j->bf16x3, coul, kpart,
+                            &nch[s], st));
+    else
+      AFQ_TRY(k_exchange(j->rot_chol[s], j->green[s], j->nocc[s], j->N, j->G, j->W, coul, kpart, &nch[s], st));
   }
   ProfScope ps(j, AFQ_K_ENERGY_MISC, st);
   return k_energy_finalize(j->ene1, j->coul, two ? j->coul + (long)j->W * j->G : nullptr, j->kpart,
@@ -208,8 +227,93 @@ int energy_pipeline(afq_job* j, const void* up, const void* dn, double2* out, hi
                            j->W, st);
 }
 
+// Reduced precision: VHS and the Taylor terms in complex64 on FP32 / split-BF16
+// MFMA, exp(-dt/2 h1) applications and the running Taylor sum in complex128
+// (the reference's vhs_mixed_precision split, propagation.py:154-176, 196-203).
+int trotprop_lp(afq_job* j, void* up, void* dn, const double2* xs, hipStream_t st) {
+  const int N = j->N, W = j->W;
+  const long NN = (long)N * N;
+  {
+    Gemm g;
+    g.M = W;
+    g.N = (int)NN;
+    g.K = j->G;
+    g.a_cplx = true;
+    g.A = xs;
+    g.lda = j->G;
+    g.B = j->chol32;
+    g.ldb = NN;
+    g.C = j->vhs32;
+    g.ldc = NN;
+    g.alpha = double2{0.0, std::sqrt(j->d.dt)};
+    ProfScope ps(j, AFQ_K_VHS, st);
+    AFQ_TRY(gemm_lp(LpKind::VHS, j->bf16x3, g, st));
+  }
+  for (int s = 0; s < j->nspin_prop; ++s) {
+    double2* Wp = static_cast<double2*>(s == 0 ? up : dn);
+    const int c = j->restricted ? j->wcols : j->nocc[s];
+    const long per = (long)N * c;
+    const double* E = j->exp_h1[j->restricted ? 0 : s];
+    auto apply_E = [&](const double2* src, double2* dst) {
+      Gemm g;
+      g.M = N;
+      g.N = c;
+      g.K = N;
+      g.batch = W;
+      g.A = E;
+      g.lda = N;
+      g.sA = 0;
+      g.b_cplx = true;
+      g.B = src;
+      g.ldb = c;
+      g.sB = per;
+      g.C = dst;
+      g.ldc = c;
+      g.sC = per;
+      ProfScope ps(j, AFQ_K_EXPH1, st);
+      return gemm_f64(g, st);
+    };
+    double2* t0 = j->tbuf[0];
+    double2* acc = j->tbuf[2];
+    AFQ_TRY(apply_E(Wp, t0));
+    AFQ_HIP(hipMemcpyAsync(acc, t0, sizeof(double2) * per * W, hipMemcpyDeviceToDevice, st));
+    int cur = -1;  // -1: previous term is t0 (complex128)
+    for (int n = 1; n < j->d.n_exp_terms; ++n) {
+      const int nxt = (cur == 0) ? 1 : 0;
+      Gemm g;
+      g.M = N;
+      g.N = c;
+      g.K = N;
+      g.batch = W;
+      g.a_cplx = true;
+      g.A = j->vhs32;
+      g.lda = N;
+      g.sA = NN;
+      g.b_cplx = true;
+      g.B = cur < 0 ? static_cast<const void*>(t0) : static_cast<const void*>(j->t32[cur]);
+      g.ldb = c;
+      g.sB = per;
+      g.C = j->t32[nxt];
+      g.ldc = c;
+      g.sC = per;
+      g.D = acc;
+      g.ldd = c;
+      g.sD = per;
+      g.alpha = double2{1.0 / n, 0.0};
+      {
+        ProfScope ps(j, AFQ_K_TAYLOR, st);
+        AFQ_TRY(gemm_lp(cur < 0 ? LpKind::TAYLOR_D : LpKind::TAYLOR_F, j->bf16x3, g, st));
+      }
+      cur = nxt;
+    }
+    AFQ_TRY(apply_E(acc, Wp));
+  }
+  return AFQ_OK;
+}
+
 // W <- E . Taylor(VHS) . E . W for each walker buffer.
 int trotprop(afq_job* j, void* up, void* dn, const double2* xs, hipStream_t st) {
+  if (j->lp) return trotprop_lp(j, up, dn, xs, st);
   const int N = j->N, W = j->W;
   const long NN = (long)N * N;
   {
@@ -346,7 +450,8 @@ int afq_job_create(afq_job_t* out, const afq_job_desc* d) {
     return fail(AFQ_E_INVALID, "restricted walkers need nelec_dn <= nelec_up");
   if (d->n_exp_terms < 1) return fail(AFQ_E_INVALID, "n_exp_terms must be >= 1");
   if (!(d->dt > 0.0)) return fail(AFQ_E_INVALID, "dt must be positive");
-  if (d->precision != AFQ_PREC_FP64) return fail(AFQ_E_UNSUPPORTED, "precision mode not available in this build");
+  if (d->precision != AFQ_PREC_FP64 && d->precision != AFQ_PREC_FP32 && d->precision != AFQ_PREC_BF16X3)
+    return fail(AFQ_E_INVALID, "unknown precision");
   if (d->nelec_up > 64 || d->nelec_dn > 64) return fail(AFQ_E_UNSUPPORTED, "nocc > 64 not supported");
   if (d->nchol > 65535 || d->n_walkers > 65535) return fail(AFQ_E_UNSUPPORTED, "nchol and n_walkers must be <= 65535");
 
@@ -364,6 +469,8 @@ int afq_job_create(afq_job_t* out, const afq_job_desc* d) {
   j->one_spin = j->rhf && j->restricted;
   j->nspin_meas = j->one_spin ? 1 : 2;
   j->nspin_prop = j->restricted ? 1 : 2;
+  j->lp = d->precision != AFQ_PREC_FP64;
+  j->bf16x3 = d->precision == AFQ_PREC_BF16X3;
 
   const long N = j->N, G = j->G, W = j->W;
   const long wmax = std::max(j->wcols, std::max(j->nocc[0], j->nocc[1]));
@@ -396,8 +503,20 @@ int afq_job_create(afq_job_t* out, const afq_job_desc* d) {
   A_(j->fb, W * G);
   A_(j->xs, W * G);
   A_(j->terms, 2 * W);
-  A_(j->vhs, W * N * N);
-  for (auto& t : j->tbuf) A_(t, W * N * wmax);
+  if (j->lp) {
+    // FP32 copies of L and rot_chol, complex64 VHS and Taylor terms; the
+    // complex128 Taylor buffers hold exp(-dt/2 h1) W and the running sum
+    A_(j->chol32, G * N * N);
+    A_(j->rot_chol32[0], G * j->nocc[0] * N);
+    if (!j->rhf) A_(j->rot_chol32[1], G * j->nocc[1] * N);
+    A_(j->vhs32, W * N * N);
+    for (auto& t : j->t32) A_(t, W * N * wmax);
+    A_(j->tbuf[0], W * N * wmax);
+    A_(j->tbuf[2], W * N * wmax);
+  } else {
+    A_(j->vhs, W * N * N);
+    for (auto& t : j->tbuf) A_(t, W * N * wmax);
+  }
   A_(j->ov_new, W);
   A_(j->coul, 2 * W * G);
   A_(j->kpart, 2 * W * kch);
@@ -407,6 +526,9 @@ int afq_job_create(afq_job_t* out, const afq_job_desc* d) {
   A_(j->sr_idx, W);
   A_(j->cum, W);
   A_(j->tmp_nn, 8 * N * N);
+  // split-K partials: up to 16 slices of the (W, G) complex force bias, or of an N x N setup product
+  j->gemm_ws_elems = 16 * std::max(2 * W * G, N * N);
+  A_(j->gemm_ws, j->gemm_ws_elems);
   A_(j->dscratch, 8);
 #undef A_
   if (rc != AFQ_OK) {
@@ -418,6 +540,7 @@ int afq_job_create(afq_job_t* out, const afq_job_desc* d) {
     j->mo[1] = j->mo[0];
     j->rot_h1[1] = j->rot_h1[0];
     j->rot_chol[1] = j->rot_chol[0];
+    j->rot_chol32[1] = j->rot_chol32[0];
   }
   *out = j;
   return AFQ_OK;
@@ -446,6 +569,7 @@ int afq_job_set_hamiltonian(afq_job_t j, double h0, const double* h1, const doub
   const long N = j->N;
   AFQ_HIP(hipMemcpyAsync(j->h1, h1, sizeof(double) * 2 * N * N, hipMemcpyDefault, S(stream)));
   AFQ_HIP(hipMemcpyAsync(j->chol, chol, sizeof(double) * j->G * N * N, hipMemcpyDefault, S(stream)));
+  if (j->lp) AFQ_TRY(k_to_f32(j->chol, j->chol32, (long)j->G * N * N, S(stream)));
   j->have_ham = true;
   j->have_meas = j->have_prop = false;
   return AFQ_OK;
@@ -510,6 +634,9 @@ int afq_build_measurement_intermediates(afq_job_t j, void* stream) {
     r.sC = (long)O * N;
     AFQ_TRY(gemm_f64(r, st));
   }
+  if (j->lp)
+    for (int s = 0; s < nsp; ++s)
+      AFQ_TRY(k_to_f32(j->rot_chol[s], j->rot_chol32[s], (long)j->G * j->nocc[s] * N, st));
   j->have_meas = true;
   return AFQ_OK;
 }
@@ -567,6 +694,8 @@ int afq_build_propagation_intermediates(afq_job_t j, void* stream) {
       g.C = v0;
       g.ldc = N;
       g.alpha = double2{0.5, 0.0};
+      g.ws = j->gemm_ws;
+      g.ws_elems = j->gemm_ws_elems;
       g.beta = double2{g0 == 0 ? 0.0 : 1.0, 0.0};
       rc = gemm_f64(g, st);
     }
@@ -692,7 +821,10 @@ int afq_set_intermediate(afq_job_t j, int which, int spin, const void* src, void
   AFQ_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDefault, S(stream)));
   switch (which) {
     case AFQ_INT_ROT_H1: j->have_rot_h1[spin] = true; break;
-    case AFQ_INT_ROT_CHOL: j->have_rot_chol[spin] = true; break;
+    case AFQ_INT_ROT_CHOL:
+      j->have_rot_chol[spin] = true;
+      if (j->lp) AFQ_TRY(k_to_f32(j->rot_chol[spin], j->rot_chol32[spin], G * O * N, S(stream)));
+      break;
     case AFQ_INT_MF_SHIFTS: j->have_mf = true; break;
     case AFQ_INT_H0_PROP: j->have_h0p = true; break;
     case AFQ_INT_EXP_H1: j->have_exp[spin] = true; break;

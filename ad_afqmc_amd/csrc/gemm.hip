@@ -1,10 +1,22 @@
 // Batched FP64 GEMM on CDNA4 matrix cores (v_mfma_f64_16x16x4f64).
 //
-// One 256-thread work-group computes a 64x64 tile of C; the four waves each own
-// a 32x32 quadrant made of 2x2 MFMA 16x16 blocks.  K advances in slabs of 16
-// staged through LDS.  Complex operands are split into real/imaginary planes
-// in LDS, so a complex x real product costs two real MFMAs per block and a
-// complex x complex product four, with FP64 accumulation throughout.
+// A work-group of WM x WN waves computes a BM x BN tile of C, each wave a
+// TM x TN grid of 16x16 MFMA blocks (BM = 16 TM WM, BN = 16 TN WN).  K advances
+// in slabs of 16 through two LDS buffers: the next slab is fetched from global
+// memory into registers while the current one is multiplied, then stored to
+// the other buffer (one barrier per slab).  Complex operands are split into
+// real/imaginary LDS planes; complex x real costs two real MFMAs per block,
+// complex x complex four, all accumulated in FP64.
+//
+// LDS rows are padded so their stride is 16 (mod 32) doubles: the four k rows
+// read by one MFMA fragment then fall in disjoint bank halves.
+//
+// Two tile shapes: 64x64 (2x2 waves of 2x2 blocks) for large products, and
+// "panel" tiles 64 x 16*TN (4x1 waves of 1 x TN blocks) for products whose N is
+// a walker's occupied-orbital count, so at most 15 padded columns are wasted.
+// Large single products are ordered XCD-aware (work-groups sharing a B panel
+// land on the same XCD) and may be split along K into a workspace that a
+// second kernel reduces in a fixed order.
 //
 // MFMA f64 16x16x4 lane layout (gfx950): lane l feeds A[l%16][l/16] and
 // B[l/16][l%16]; accumulator element v of lane l is D[l/16 + 4v][l%16].
@@ -13,134 +25,213 @@
 namespace afq {
 namespace {
 
-constexpr int BM = 64, BN = 64, BK = 16, PAD = 1;
+constexpr int BK = 16;
 
 __device__ inline double2 cmul(double2 a, double2 b) { return {a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x}; }
 
-template <bool AC, bool BC, bool AT, bool BT>
-__global__ __launch_bounds__(256) void gemm_f64_kernel(Gemm g) {
+constexpr int padded(int w) { return ((w % 32) == 16) ? w : w + ((48 - (w % 32)) % 32); }
+
+template <bool AC, bool BC, bool AT, bool BT, int WM, int WN, int TM, int TN>
+struct Cfg {
+  static constexpr int NT = 64 * WM * WN;
+  static constexpr int BM = 16 * TM * WM, BN = 16 * TN * WN;
+  static constexpr int LDA = padded(BM), LDB = padded(BN);
+  static constexpr int PA = AC ? 2 : 1, PB = BC ? 2 : 1;
+  // quads (4 consecutive elements along the contiguous dimension) per tile
+  static constexpr int QA = BM * BK / 4, QB = BN * BK / 4;
+  static constexpr int RA = (QA + NT - 1) / NT, RB = (QB + NT - 1) / NT;
+};
+
+// Loads quad q of the A tile (rows m0.., k0..) into registers.
+template <class C, bool AC, bool AT>
+__device__ inline void fetch_a(double (&r)[C::RA][4][2], const double* A, long lda, int M, int K, int m0, int k0,
+                               int tid) {
+#pragma unroll
+  for (int i = 0; i < C::RA; ++i) {
+    const int q = tid + i * C::NT;
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+      r[i][t][0] = 0.0;
+      r[i][t][1] = 0.0;
+    }
+    if (q >= C::QA) continue;
+    int m, k;
+    if (!AT) {  // k contiguous: quad = 4 consecutive k of row m
+      m = q % C::BM;
+      k = (q / C::BM) * 4;
+    } else {  // m contiguous: quad = 4 consecutive m of column k
+      m = (q % (C::BM / 4)) * 4;
+      k = q / (C::BM / 4);
+    }
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+      const int gm = m0 + m + (AT ? t : 0), gk = k0 + k + (AT ? 0 : t);
+      if (gm < M && gk < K) {
+        const long off = AT ? (long)gk * lda + gm : (long)gm * lda + gk;
+        if (AC) {
+          const double2 v = reinterpret_cast<const double2*>(A)[off];
+          r[i][t][0] = v.x;
+          r[i][t][1] = v.y;
+        } else {
+          r[i][t][0] = A[off];
+        }
+      }
+    }
+  }
+}
+
+template <class C, bool AC, bool AT>
+__device__ inline void store_a(double* As, const double (&r)[C::RA][4][2], int tid) {
+#pragma unroll
+  for (int i = 0; i < C::RA; ++i) {
+    const int q = tid + i * C::NT;
+    if (q >= C::QA) continue;
+    int m, k;
+    if (!AT) {
+      m = q % C::BM;
+      k = (q / C::BM) * 4;
+    } else {
+      m = (q % (C::BM / 4)) * 4;
+      k = q / (C::BM / 4);
+    }
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+      const int mm = m + (AT ? t : 0), kk = k + (AT ? 0 : t);
+      As[kk * C::LDA + mm] = r[i][t][0];
+      if (AC) As[BK * C::LDA + kk * C::LDA + mm] = r[i][t][1];
+    }
+  }
+}
+
+template <class C, bool BC, bool BT>
+__device__ inline void fetch_b(double (&r)[C::RB][4][2], const double* B, long ldb, int N, int K, int n0, int k0,
+                               int tid) {
+#pragma unroll
+  for (int i = 0; i < C::RB; ++i) {
+    const int q = tid + i * C::NT;
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+      r[i][t][0] = 0.0;
+      r[i][t][1] = 0.0;
+    }
+    if (q >= C::QB) continue;
+    int n, k;
+    if (!BT) {  // n contiguous: quad = 4 consecutive n of row k
+      n = (q % (C::BN / 4)) * 4;
+      k = q / (C::BN / 4);
+    } else {  // k contiguous: quad = 4 consecutive k of column n
+      n = q % C::BN;
+      k = (q / C::BN) * 4;
+    }
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+      const int gn = n0 + n + (BT ? 0 : t), gk = k0 + k + (BT ? t : 0);
+      if (gn < N && gk < K) {
+        const long off = BT ? (long)gn * ldb + gk : (long)gk * ldb + gn;
+        if (BC) {
+          const double2 v = reinterpret_cast<const double2*>(B)[off];
+          r[i][t][0] = v.x;
+          r[i][t][1] = v.y;
+        } else {
+          r[i][t][0] = B[off];
+        }
+      }
+    }
+  }
+}
+
+template <class C, bool BC, bool BT>
+__device__ inline void store_b(double* Bs, const double (&r)[C::RB][4][2], int tid) {
+#pragma unroll
+  for (int i = 0; i < C::RB; ++i) {
+    const int q = tid + i * C::NT;
+    if (q >= C::QB) continue;
+    int n, k;
+    if (!BT) {
+      n = (q % (C::BN / 4)) * 4;
+      k = q / (C::BN / 4);
+    } else {
+      n = q % C::BN;
+      k = (q / C::BN) * 4;
+    }
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+      const int nn = n + (BT ? 0 : t), kk = k + (BT ? t : 0);
+      Bs[kk * C::LDB + nn] = r[i][t][0];
+      if (BC) Bs[BK * C::LDB + kk * C::LDB + nn] = r[i][t][1];
+    }
+  }
+}
+
+// grid: x = tiles (XCD-remapped when xcd_remap), y = batch, z = K split.
+template <bool AC, bool BC, bool AT, bool BT, int WM, int WN, int TM, int TN>
+__global__ __launch_bounds__(64 * WM * WN) void gemm_f64_kernel(Gemm g, int tiles_m, int tiles_n, int xcd_remap,
+                                                                 long T8, int kchunk, double* ws) {
+  using C = Cfg<AC, BC, AT, BT, WM, WN, TM, TN>;
   constexpr bool CC = AC || BC;
-  __shared__ double As[AC ? 2 : 1][BK][BM + PAD];
-  __shared__ double Bs[BC ? 2 : 1][BK][BN + PAD];
+  __shared__ double As[2][C::PA * BK * C::LDA];
+  __shared__ double Bs[2][C::PB * BK * C::LDB];
 
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int wm = wave >> 1, wn = wave & 1;
-  const int m0 = blockIdx.y * BM, n0 = blockIdx.x * BN;
-  const long b = blockIdx.z;
+  const int wm = wave / WN, wn = wave % WN;
+  long tile = blockIdx.x;
+  if (xcd_remap) tile = (tile & 7) * T8 + (tile >> 3);
+  const int mt = (int)(tile % tiles_m);
+  const long ntl = tile / tiles_m;
+  if (ntl >= tiles_n) return;  // uniform per work-group
+  const int m0 = mt * C::BM, n0 = (int)ntl * C::BN;
+  const long b = blockIdx.y;
+  const int kz = blockIdx.z;
+  const int kbeg = kz * kchunk;
+  const int kend = min(g.K, kbeg + kchunk);
   const double* A = static_cast<const double*>(g.A) + b * g.sA * (AC ? 2 : 1);
   const double* B = static_cast<const double*>(g.B) + b * g.sB * (BC ? 2 : 1);
 
-  v4d accR[2][2], accI[2][2];
+  v4d accR[TM][TN], accI[TM][TN];
 #pragma unroll
-  for (int i = 0; i < 2; ++i)
+  for (int i = 0; i < TM; ++i)
 #pragma unroll
-    for (int j = 0; j < 2; ++j) {
+    for (int j = 0; j < TN; ++j) {
       accR[i][j] = v4d{0.0, 0.0, 0.0, 0.0};
       accI[i][j] = v4d{0.0, 0.0, 0.0, 0.0};
     }
 
-  for (int k0 = 0; k0 < g.K; k0 += BK) {
-    // ---- stage A (BM x BK) into As[plane][k][m]
-    if (!AT) {
-      const int m = tid >> 2, kq = (tid & 3) * 4;
-      const int gm = m0 + m;
-#pragma unroll
-      for (int t = 0; t < 4; ++t) {
-        const int k = kq + t, gk = k0 + k;
-        double re = 0.0, im = 0.0;
-        if (gm < g.M && gk < g.K) {
-          const long off = (long)gm * g.lda + gk;
-          if (AC) {
-            const double2 v = reinterpret_cast<const double2*>(A)[off];
-            re = v.x;
-            im = v.y;
-          } else {
-            re = A[off];
-          }
-        }
-        As[0][k][m] = re;
-        if (AC) As[AC ? 1 : 0][k][m] = im;
-      }
-    } else {
-      const int k = tid >> 4, mq = (tid & 15) * 4;
-      const int gk = k0 + k;
-#pragma unroll
-      for (int t = 0; t < 4; ++t) {
-        const int m = mq + t, gm = m0 + m;
-        double re = 0.0, im = 0.0;
-        if (gm < g.M && gk < g.K) {
-          const long off = (long)gk * g.lda + gm;
-          if (AC) {
-            const double2 v = reinterpret_cast<const double2*>(A)[off];
-            re = v.x;
-            im = v.y;
-          } else {
-            re = A[off];
-          }
-        }
-        As[0][k][m] = re;
-        if (AC) As[AC ? 1 : 0][k][m] = im;
-      }
+  double ra[C::RA][4][2], rb[C::RB][4][2];
+  // K is processed as [kbeg, kend); the fetch helpers clip against kend.
+  fetch_a<C, AC, AT>(ra, A, g.lda, g.M, kend, m0, kbeg, tid);
+  fetch_b<C, BC, BT>(rb, B, g.ldb, g.N, kend, n0, kbeg, tid);
+  store_a<C, AC, AT>(As[0], ra, tid);
+  store_b<C, BC, BT>(Bs[0], rb, tid);
+  __syncthreads();
+  int buf = 0;
+  for (int k0 = kbeg; k0 < kend; k0 += BK) {
+    const bool more = k0 + BK < kend;
+    if (more) {
+      fetch_a<C, AC, AT>(ra, A, g.lda, g.M, kend, m0, k0 + BK, tid);
+      fetch_b<C, BC, BT>(rb, B, g.ldb, g.N, kend, n0, k0 + BK, tid);
     }
-    // ---- stage B (BK x BN) into Bs[plane][k][n]
-    if (!BT) {
-      const int k = tid >> 4, nq = (tid & 15) * 4;
-      const int gk = k0 + k;
-#pragma unroll
-      for (int t = 0; t < 4; ++t) {
-        const int n = nq + t, gn = n0 + n;
-        double re = 0.0, im = 0.0;
-        if (gn < g.N && gk < g.K) {
-          const long off = (long)gk * g.ldb + gn;
-          if (BC) {
-            const double2 v = reinterpret_cast<const double2*>(B)[off];
-            re = v.x;
-            im = v.y;
-          } else {
-            re = B[off];
-          }
-        }
-        Bs[0][k][n] = re;
-        if (BC) Bs[BC ? 1 : 0][k][n] = im;
-      }
-    } else {
-      const int n = tid >> 2, kq = (tid & 3) * 4;
-      const int gn = n0 + n;
-#pragma unroll
-      for (int t = 0; t < 4; ++t) {
-        const int k = kq + t, gk = k0 + k;
-        double re = 0.0, im = 0.0;
-        if (gn < g.N && gk < g.K) {
-          const long off = (long)gn * g.ldb + gk;
-          if (BC) {
-            const double2 v = reinterpret_cast<const double2*>(B)[off];
-            re = v.x;
-            im = v.y;
-          } else {
-            re = B[off];
-          }
-        }
-        Bs[0][k][n] = re;
-        if (BC) Bs[BC ? 1 : 0][k][n] = im;
-      }
-    }
-    __syncthreads();
-
+    const double* as = As[buf];
+    const double* bs = Bs[buf];
 #pragma unroll
     for (int kk = 0; kk < BK; kk += 4) {
       const int kr = kk + (lane >> 4), c = lane & 15;
-      double ar[2], ai[2], br[2], bi[2];
+      double ar[TM], ai[TM], br[TN], bi[TN];
 #pragma unroll
-      for (int i = 0; i < 2; ++i) {
-        ar[i] = As[0][kr][wm * 32 + i * 16 + c];
-        ai[i] = AC ? As[AC ? 1 : 0][kr][wm * 32 + i * 16 + c] : 0.0;
-        br[i] = Bs[0][kr][wn * 32 + i * 16 + c];
-        bi[i] = BC ? Bs[BC ? 1 : 0][kr][wn * 32 + i * 16 + c] : 0.0;
+      for (int i = 0; i < TM; ++i) {
+        const int col = (wm * TM + i) * 16 + c;
+        ar[i] = as[kr * C::LDA + col];
+        ai[i] = AC ? as[BK * C::LDA + kr * C::LDA + col] : 0.0;
       }
 #pragma unroll
-      for (int i = 0; i < 2; ++i)
+      for (int j = 0; j < TN; ++j) {
+        const int col = (wn * TN + j) * 16 + c;
+        br[j] = bs[kr * C::LDB + col];
+        bi[j] = BC ? bs[BK * C::LDB + kr * C::LDB + col] : 0.0;
+      }
 #pragma unroll
-        for (int j = 0; j < 2; ++j) {
+      for (int i = 0; i < TM; ++i)
+#pragma unroll
+        for (int j = 0; j < TN; ++j) {
           accR[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar[i], br[j], accR[i][j], 0, 0, 0);
           if (AC && BC) {
             accR[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(-ai[i], bi[j], accR[i][j], 0, 0, 0);
@@ -153,26 +244,51 @@ __global__ __launch_bounds__(256) void gemm_f64_kernel(Gemm g) {
           }
         }
     }
+    if (more) {
+      store_a<C, AC, AT>(As[buf ^ 1], ra, tid);
+      store_b<C, BC, BT>(Bs[buf ^ 1], rb, tid);
+    }
     __syncthreads();
+    buf ^= 1;
   }
 
   // ---- epilogue
+  if (ws) {  // split-K partial: raw accumulators to ws[kz][b][m][n]
+    const long plane = (long)g.M * g.N;
+    double* W = ws + ((long)kz * gridDim.y + b) * plane * (CC ? 2 : 1);
+#pragma unroll
+    for (int i = 0; i < TM; ++i)
+#pragma unroll
+      for (int j = 0; j < TN; ++j)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          const int m = m0 + (wm * TM + i) * 16 + (lane >> 4) + 4 * v;
+          const int n = n0 + (wn * TN + j) * 16 + (lane & 15);
+          if (m >= g.M || n >= g.N) continue;
+          const long off = (long)m * g.N + n;
+          if (CC)
+            reinterpret_cast<double2*>(W)[off] = double2{accR[i][j][v], accI[i][j][v]};
+          else
+            W[off] = accR[i][j][v];
+        }
+    return;
+  }
   const bool has_beta = (g.beta.x != 0.0 || g.beta.y != 0.0);
-  double* C = static_cast<double*>(g.C) + b * g.sC * (CC ? 2 : 1);
+  double* Cp = static_cast<double*>(g.C) + b * g.sC * (CC ? 2 : 1);
   double* Dp = g.D ? static_cast<double*>(g.D) + b * g.sD * (CC ? 2 : 1) : nullptr;
 #pragma unroll
-  for (int i = 0; i < 2; ++i)
+  for (int i = 0; i < TM; ++i)
 #pragma unroll
-    for (int j = 0; j < 2; ++j)
+    for (int j = 0; j < TN; ++j)
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
-        const int m = m0 + wm * 32 + i * 16 + (lane >> 4) + 4 * v;
-        const int n = n0 + wn * 32 + j * 16 + (lane & 15);
+        const int m = m0 + (wm * TM + i) * 16 + (lane >> 4) + 4 * v;
+        const int n = n0 + (wn * TN + j) * 16 + (lane & 15);
         if (m >= g.M || n >= g.N) continue;
         const long off = (long)m * g.ldc + n;
         if (CC) {
           double2 r = cmul(g.alpha, double2{accR[i][j][v], accI[i][j][v]});
-          double2* Cc = reinterpret_cast<double2*>(C);
+          double2* Cc = reinterpret_cast<double2*>(Cp);
           if (has_beta) {
             const double2 o = cmul(g.beta, Cc[off]);
             r.x += o.x;
@@ -188,35 +304,119 @@ __global__ __launch_bounds__(256) void gemm_f64_kernel(Gemm g) {
           }
         } else {
           double r = g.alpha.x * accR[i][j][v];
-          if (has_beta) r += g.beta.x * C[off];
-          C[off] = r;
+          if (has_beta) r += g.beta.x * Cp[off];
+          Cp[off] = r;
           if (Dp) Dp[(long)m * g.ldd + n] += r;
         }
       }
 }
 
-template <bool AC, bool BC, bool AT, bool BT>
+// C = alpha * sum_kz ws[kz] + beta * C  (fixed summation order)
+template <bool CC>
+__global__ void splitk_reduce_kernel(Gemm g, const double* ws, int ksplit) {
+  const long plane = (long)g.M * g.N;
+  const long idx = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  const long b = blockIdx.y;
+  if (idx >= plane) return;
+  const int m = (int)(idx / g.N), n = (int)(idx % g.N);
+  const bool has_beta = (g.beta.x != 0.0 || g.beta.y != 0.0);
+  if (CC) {
+    double2 s = {0.0, 0.0};
+    for (int z = 0; z < ksplit; ++z) {
+      const double2 v = reinterpret_cast<const double2*>(ws)[((long)z * gridDim.y + b) * plane + idx];
+      s.x += v.x;
+      s.y += v.y;
+    }
+    double2 r = cmul(g.alpha, s);
+    double2* Cc = reinterpret_cast<double2*>(static_cast<double*>(g.C) + b * g.sC * 2) + (long)m * g.ldc + n;
+    if (has_beta) {
+      const double2 o = cmul(g.beta, *Cc);
+      r.x += o.x;
+      r.y += o.y;
+    }
+    *Cc = r;
+    if (g.D) {
+      double2* Dc = reinterpret_cast<double2*>(static_cast<double*>(g.D) + b * g.sD * 2) + (long)m * g.ldd + n;
+      Dc->x += r.x;
+      Dc->y += r.y;
+    }
+  } else {
+    double s = 0.0;
+    for (int z = 0; z < ksplit; ++z) s += ws[((long)z * gridDim.y + b) * plane + idx];
+    double r = g.alpha.x * s;
+    double* Cr = static_cast<double*>(g.C) + b * g.sC + (long)m * g.ldc + n;
+    if (has_beta) r += g.beta.x * *Cr;
+    *Cr = r;
+    if (g.D) static_cast<double*>(g.D)[b * g.sD + (long)m * g.ldd + n] += r;
+  }
+}
+
+template <bool AC, bool BC, bool AT, bool BT, int WM, int WN, int TM, int TN>
 int launch(const Gemm& g, hipStream_t s) {
-  dim3 grid((g.N + BN - 1) / BN, (g.M + BM - 1) / BM, g.batch);
-  hipLaunchKernelGGL((gemm_f64_kernel<AC, BC, AT, BT>), grid, dim3(256), 0, s, g);
-  return launch_status("gemm_f64_kernel");
+  using C = Cfg<AC, BC, AT, BT, WM, WN, TM, TN>;
+  const int tm = (g.M + C::BM - 1) / C::BM, tn = (g.N + C::BN - 1) / C::BN;
+  const long tiles = (long)tm * tn;
+  int ksplit = 1;
+  const long wgs = tiles * g.batch;
+  if (g.ws && g.batch == 1 && wgs < 1024 && g.K >= 8 * BK) {
+    ksplit = (int)std::min<long>((2048 + wgs - 1) / wgs, (long)g.K / (4 * BK));
+    const long need = (long)ksplit * g.M * g.N * ((AC || BC) ? 2 : 1);
+    while (ksplit > 1 && need > g.ws_elems) {
+      --ksplit;
+    }
+    if ((long)ksplit * g.M * g.N * ((AC || BC) ? 2 : 1) > g.ws_elems) ksplit = 1;
+  }
+  int kchunk = g.K;
+  if (ksplit > 1) {
+    kchunk = ((g.K + ksplit - 1) / ksplit + BK - 1) / BK * BK;
+    ksplit = (g.K + kchunk - 1) / kchunk;
+  }
+  const int remap = (g.batch == 1 && tiles >= 64) ? 1 : 0;
+  const long T8 = remap ? (tiles + 7) / 8 : 0;
+  const long gx = remap ? T8 * 8 : tiles;
+  if (gx > 0x7fffffffL || g.batch > 65535 || ksplit > 65535) return fail(AFQ_E_UNSUPPORTED, "gemm_f64: grid too large");
+  dim3 grid((unsigned)gx, g.batch, ksplit);
+  double* ws = ksplit > 1 ? static_cast<double*>(g.ws) : nullptr;
+  hipLaunchKernelGGL((gemm_f64_kernel<AC, BC, AT, BT, WM, WN, TM, TN>), grid, dim3(C::NT), 0, s, g, tm, tn, remap, T8,
+                     kchunk, ws);
+  AFQ_TRY(launch_status("gemm_f64_kernel"));
+  if (ksplit > 1) return gemm_splitk_reduce(g, ws, ksplit, AC || BC, s);
+  return AFQ_OK;
+}
+
+template <bool AC, bool BC, bool AT, bool BT>
+int dispatch_shape(const Gemm& g, hipStream_t s) {
+  // panel tiles for N <= 64 (walker-column products)
+  if (g.N <= 16) return launch<AC, BC, AT, BT, 4, 1, 1, 1>(g, s);
+  if (g.N <= 32) return launch<AC, BC, AT, BT, 4, 1, 1, 2>(g, s);
+  if (g.N <= 48) return launch<AC, BC, AT, BT, 4, 1, 1, 3>(g, s);
+  if (g.N <= 64 && g.M > 64) return launch<AC, BC, AT, BT, 4, 1, 1, 4>(g, s);
+  return launch<AC, BC, AT, BT, 2, 2, 2, 2>(g, s);
 }
 
 template <bool AC, bool BC>
-int launch_t(const Gemm& g, hipStream_t s) {
-  if (g.a_trans) return g.b_trans ? launch<AC, BC, true, true>(g, s) : launch<AC, BC, true, false>(g, s);
-  return g.b_trans ? launch<AC, BC, false, true>(g, s) : launch<AC, BC, false, false>(g, s);
+int dispatch_t(const Gemm& g, hipStream_t s) {
+  if (g.a_trans) return g.b_trans ? dispatch_shape<AC, BC, true, true>(g, s) : dispatch_shape<AC, BC, true, false>(g, s);
+  return g.b_trans ? dispatch_shape<AC, BC, false, true>(g, s) : dispatch_shape<AC, BC, false, false>(g, s);
 }
 
 }  // namespace
 
+int gemm_splitk_reduce(const Gemm& g, const double* ws, int ksplit, bool cplx, hipStream_t s) {
+  const long plane = (long)g.M * g.N;
+  dim3 rg((unsigned)((plane + 255) / 256), g.batch);
+  if (cplx)
+    hipLaunchKernelGGL((splitk_reduce_kernel<true>), rg, dim3(256), 0, s, g, ws, ksplit);
+  else
+    hipLaunchKernelGGL((splitk_reduce_kernel<false>), rg, dim3(256), 0, s, g, ws, ksplit);
+  return launch_status("splitk_reduce_kernel");
+}
+
 int gemm_f64(const Gemm& g, hipStream_t s) {
   if (g.M <= 0 || g.N <= 0 || g.batch <= 0) return AFQ_OK;
   if (g.K <= 0) return fail(AFQ_E_INVALID, "gemm_f64: K must be positive");
-  if (g.batch > 65535 || (g.M + BM - 1) / BM > 65535)
-    return fail(AFQ_E_UNSUPPORTED, "gemm_f64: grid too large");
-  if (g.a_cplx) return g.b_cplx ? launch_t<true, true>(g, s) : launch_t<true, false>(g, s);
-  return g.b_cplx ? launch_t<false, true>(g, s) : launch_t<false, false>(g, s);
+  if (g.a_cplx) return g.b_cplx ? dispatch_t<true, true>(g, s) : dispatch_t<true, false>(g, s);
+  return g.b_cplx ? dispatch_t<false, true>(g, s) : dispatch_t<false, false>(g, s);
 }
 
 }  // namespace afq

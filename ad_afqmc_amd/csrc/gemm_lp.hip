@@ -1,0 +1,406 @@
+// Reduced-precision GEMM for the walker step (FP32 and split-BF16 x3 on
+// matrix cores, FP32 accumulation).
+//
+// Same tiling as the FP64 GEMM (WM x WN waves of TM x TN 16x16 blocks, two LDS
+// buffers with register prefetch), with operands staged in LDS as FP32 and a
+// 32-deep K slab.  Global operands may be double or float; they are rounded
+// to FP32 when staged.
+//   F32    : v_mfma_f32_16x16x4f32, eight k-steps of 4 per slab.
+//   BF16X3 : each FP32 operand x is split into hi = bf16(x), lo = bf16(x - hi);
+//            a product is hi*hi + hi*lo + lo*hi on v_mfma_f32_16x16x32_bf16
+//            (lane l holds 8 consecutive k), i.e. ~16 significant bits per
+//            operand and FP32 accumulation.
+// Output C may be float2 (complex64) or double2; D (double2) += alpha*result.
+//
+// MFMA f32 16x16 lane layout (gfx950): accumulator element v of lane l is
+// D[4*(l/16) + v][l%16]; f32 16x16x4 inputs A[l%16][l/16], B[l/16][l%16];
+// bf16 16x16x32 inputs A[l%16][8*(l/16)+t], B[8*(l/16)+t][l%16], t = 0..7.
+#include "afq_internal.h"
+
+typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
+
+namespace afq {
+namespace {
+
+constexpr int BKL = 32;
+
+__device__ inline double2 cmul(double2 a, double2 b) { return {a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x}; }
+
+// LDS row strides (floats).  F32 fragments read 4 rows k..k+3: stride = 16 (mod 64)
+// puts them in disjoint bank quarters.  BF16 fragments read rows 8 apart: 8*stride = 16 (mod 64).
+constexpr int pad_f32(int w) { return w + ((16 - (w % 64)) % 64 + 64) % 64; }
+constexpr int pad_bf16(int w) { return w + (((2 - (w % 8)) % 8) + 8) % 8; }
+
+template <typename T>
+__device__ inline float2 ld2(const T* p, long off, bool cplx) {
+  if (cplx) {
+    if constexpr (sizeof(T) == 8) {
+      const double2 v = reinterpret_cast<const double2*>(p)[off];
+      return float2{(float)v.x, (float)v.y};
+    } else {
+      return reinterpret_cast<const float2*>(p)[off];
+    }
+  }
+  return float2{(float)p[off], 0.f};
+}
+
+template <bool BF, bool AC, bool BC, bool AT, bool BT, typename TA, typename TB, int WM, int WN, int TM, int TN>
+struct CfgL {
+  static constexpr int NT = 64 * WM * WN;
+  static constexpr int BM = 16 * TM * WM, BN = 16 * TN * WN;
+  static constexpr int LDA = BF ? pad_bf16(BM) : pad_f32(BM);
+  static constexpr int LDB = BF ? pad_bf16(BN) : pad_f32(BN);
+  static constexpr int PA = AC ? 2 : 1, PB = BC ? 2 : 1;
+  static constexpr int QA = BM * BKL / 4, QB = BN * BKL / 4;
+  static constexpr int RA = (QA + NT - 1) / NT, RB = (QB + NT - 1) / NT;
+};
+
+template <class C, bool AC, bool AT, typename TA>
+__device__ inline void fetch_a(float (&r)[C::RA][4][2], const TA* A, long lda, int M, int K, int m0, int k0, int tid) {
+#pragma unroll
+  for (int i = 0; i < C::RA; ++i) {
+    const int q = tid + i * C::NT;
+#pragma unroll
+    for (int t = 0; t < 4; ++t) r[i][t][0] = r[i][t][1] = 0.f;
+    if (q >= C::QA) continue;
+    const int m = AT ? (q % (C::BM / 4)) * 4 : q % C::BM;
+    const int k = AT ? q / (C::BM / 4) : (q / C::BM) * 4;
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+      const int gm = m0 + m + (AT ? t : 0), gk = k0 + k + (AT ? 0 : t);
+      if (gm < M && gk < K) {
+        const float2 v = ld2(A, AT ? (long)gk * lda + gm : (long)gm * lda + gk, AC);
+        r[i][t][0] = v.x;
+        r[i][t][1] = v.y;
+      }
+    }
+  }
+}
+
+template <class C, bool AC, bool AT>
+__device__ inline void store_a(float* As, const float (&r)[C::RA][4][2], int tid) {
+#pragma unroll
+  for (int i = 0; i < C::RA; ++i) {
+    const int q = tid + i * C::NT;
+    if (q >= C::QA) continue;
+    const int m = AT ? (q % (C::BM / 4)) * 4 : q % C::BM;
+    const int k = AT ? q / (C::BM / 4) : (q / C::BM) * 4;
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+      const int mm = m + (AT ? t : 0), kk = k + (AT ? 0 : t);
+      As[kk * C::LDA + mm] = r[i][t][0];
+      if (AC) As[BKL * C::LDA + kk * C::LDA + mm] = r[i][t][1];
+    }
+  }
+}
+
+template <class C, bool BC, bool BT, typename TB>
+__device__ inline void fetch_b(float (&r)[C::RB][4][2], const TB* B, long ldb, int N, int K, int n0, int k0, int tid) {
+#pragma unroll
+  for (int i = 0; i < C::RB; ++i) {
+    const int q = tid + i * C::NT;
+#pragma unroll
+    for (int t = 0; t < 4; ++t) r[i][t][0] = r[i][t][1] = 0.f;
+    if (q >= C::QB) continue;
+    const int n = BT ? q % C::BN : (q % (C::BN / 4)) * 4;
+    const int k = BT ? (q / C::BN) * 4 : q / (C::BN / 4);
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+      const int gn = n0 + n + (BT ? 0 : t), gk = k0 + k + (BT ? t : 0);
+      if (gn < N && gk < K) {
+        const float2 v = ld2(B, BT ? (long)gn * ldb + gk : (long)gk * ldb + gn, BC);
+        r[i][t][0] = v.x;
+        r[i][t][1] = v.y;
+      }
+    }
+  }
+}
+
+template <class C, bool BC, bool BT>
+__device__ inline void store_b(float* Bs, const float (&r)[C::RB][4][2], int tid) {
+#pragma unroll
+  for (int i = 0; i < C::RB; ++i) {
+    const int q = tid + i * C::NT;
+    if (q >= C::QB) continue;
+    const int n = BT ? q % C::BN : (q % (C::BN / 4)) * 4;
+    const int k = BT ? (q / C::BN) * 4 : q / (C::BN / 4);
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+      const int nn = n + (BT ? 0 : t), kk = k + (BT ? t : 0);
+      Bs[kk * C::LDB + nn] = r[i][t][0];
+      if (BC) Bs[BKL * C::LDB + kk * C::LDB + nn] = r[i][t][1];
+    }
+  }
+}
+
+struct BfPair {
+  v8bf hi, lo;
+};
+// 8 consecutive k (rows of the [k][m] LDS layout) of column `col`, split hi/lo.
+__device__ inline BfPair frag_bf(const float* s, int ld, int k0, int col, bool neg) {
+  BfPair p;
+#pragma unroll
+  for (int t = 0; t < 8; ++t) {
+    float x = s[(k0 + t) * ld + col];
+    if (neg) x = -x;
+    const __bf16 h = (__bf16)x;
+    p.hi[t] = h;
+    p.lo[t] = (__bf16)(x - (float)h);
+  }
+  return p;
+}
+
+__device__ inline v4f mma3(const BfPair& a, const BfPair& b, v4f c) {
+  c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a.hi, b.hi, c, 0, 0, 0);
+  c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a.hi, b.lo, c, 0, 0, 0);
+  c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a.lo, b.hi, c, 0, 0, 0);
+  return c;
+}
+
+// grid: x = tiles (XCD-remapped when xcd_remap), y = batch, z = K split.
+template <bool BF, bool AC, bool BC, bool AT, bool BT, typename TA, typename TB, typename TCo, int WM, int WN, int TM,
+          int TN>
+__global__ __launch_bounds__(64 * WM * WN) void gemm_lp_kernel(Gemm g, int tiles_m, int tiles_n, int xcd_remap,
+                                                                long T8, int kchunk, double* ws) {
+  using C = CfgL<BF, AC, BC, AT, BT, TA, TB, WM, WN, TM, TN>;
+  constexpr bool CC = AC || BC;
+  __shared__ float As[2][C::PA * BKL * C::LDA];
+  __shared__ float Bs[2][C::PB * BKL * C::LDB];
+
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wm = wave / WN, wn = wave % WN;
+  long tile = blockIdx.x;
+  if (xcd_remap) tile = (tile & 7) * T8 + (tile >> 3);
+  const int mt = (int)(tile % tiles_m);
+  const long ntl = tile / tiles_m;
+  if (ntl >= tiles_n) return;
+  const int m0 = mt * C::BM, n0 = (int)ntl * C::BN;
+  const long b = blockIdx.y;
+  const int kz = blockIdx.z;
+  const int kbeg = kz * kchunk;
+  const int kend = min(g.K, kbeg + kchunk);
+  const TA* A = static_cast<const TA*>(g.A) + b * g.sA * (AC ? 2 : 1);
+  const TB* B = static_cast<const TB*>(g.B) + b * g.sB * (BC ? 2 : 1);
+
+  v4f accR[TM][TN], accI[TM][TN];
+#pragma unroll
+  for (int i = 0; i < TM; ++i)
+#pragma unroll
+    for (int j = 0; j < TN; ++j) {
+      accR[i][j] = v4f{0.f, 0.f, 0.f, 0.f};
+      accI[i][j] = v4f{0.f, 0.f, 0.f, 0.f};
+    }
+
+  float ra[C::RA][4][2], rb[C::RB][4][2];
+  fetch_a<C, AC, AT, TA>(ra, A, g.lda, g.M, kend, m0, kbeg, tid);
+  fetch_b<C, BC, BT, TB>(rb, B, g.ldb, g.N, kend, n0, kbeg, tid);
+  store_a<C, AC, AT>(As[0], ra, tid);
+  store_b<C, BC, BT>(Bs[0], rb, tid);
+  __syncthreads();
+  int buf = 0;
+  const int c16 = lane & 15, q4 = lane >> 4;
+  for (int k0 = kbeg; k0 < kend; k0 += BKL) {
+    const bool more = k0 + BKL < kend;
+    if (more) {
+      fetch_a<C, AC, AT, TA>(ra, A, g.lda, g.M, kend, m0, k0 + BKL, tid);
+      fetch_b<C, BC, BT, TB>(rb, B, g.ldb, g.N, kend, n0, k0 + BKL, tid);
+    }
+    const float* as = As[buf];
+    const float* bs = Bs[buf];
+    if constexpr (!BF) {
+#pragma unroll
+      for (int kk = 0; kk < BKL; kk += 4) {
+        const int kr = kk + q4;
+        float ar[TM], ai[TM], br[TN], bi[TN];
+#pragma unroll
+        for (int i = 0; i < TM; ++i) {
+          const int col = (wm * TM + i) * 16 + c16;
+          ar[i] = as[kr * C::LDA + col];
+          ai[i] = AC ? as[BKL * C::LDA + kr * C::LDA + col] : 0.f;
+        }
+#pragma unroll
+        for (int j = 0; j < TN; ++j) {
+          const int col = (wn * TN + j) * 16 + c16;
+          br[j] = bs[kr * C::LDB + col];
+          bi[j] = BC ? bs[BKL * C::LDB + kr * C::LDB + col] : 0.f;
+        }
+#pragma unroll
+        for (int i = 0; i < TM; ++i)
+#pragma unroll
+          for (int j = 0; j < TN; ++j) {
+            accR[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(ar[i], br[j], accR[i][j], 0, 0, 0);
+            if (AC && BC) {
+              accR[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(-ai[i], bi[j], accR[i][j], 0, 0, 0);
+              accI[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(ar[i], bi[j], accI[i][j], 0, 0, 0);
+              accI[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(ai[i], br[j], accI[i][j], 0, 0, 0);
+            } else if (AC) {
+              accI[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(ai[i], br[j], accI[i][j], 0, 0, 0);
+            } else if (BC) {
+              accI[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(ar[i], bi[j], accI[i][j], 0, 0, 0);
+            }
+          }
+      }
+    } else {
+      const int kq = 8 * q4;
+      BfPair fbr[TN], fbi[TN];
+#pragma unroll
+      for (int j = 0; j < TN; ++j) {
+        const int col = (wn * TN + j) * 16 + c16;
+        fbr[j] = frag_bf(bs, C::LDB, kq, col, false);
+        if (BC) fbi[j] = frag_bf(bs + BKL * C::LDB, C::LDB, kq, col, false);
+      }
+#pragma unroll
+      for (int i = 0; i < TM; ++i) {
+        const int col = (wm * TM + i) * 16 + c16;
+        const BfPair far_ = frag_bf(as, C::LDA, kq, col, false);
+        if (AC && BC) {
+          const BfPair fai = frag_bf(as + BKL * C::LDA, C::LDA, kq, col, false);
+          const BfPair fain = frag_bf(as + BKL * C::LDA, C::LDA, kq, col, true);
+#pragma unroll
+          for (int j = 0; j < TN; ++j) {
+            accR[i][j] = mma3(far_, fbr[j], accR[i][j]);
+            accR[i][j] = mma3(fain, fbi[j], accR[i][j]);
+            accI[i][j] = mma3(far_, fbi[j], accI[i][j]);
+            accI[i][j] = mma3(fai, fbr[j], accI[i][j]);
+          }
+        } else if (AC) {
+          const BfPair fai = frag_bf(as + BKL * C::LDA, C::LDA, kq, col, false);
+#pragma unroll
+          for (int j = 0; j < TN; ++j) {
+            accR[i][j] = mma3(far_, fbr[j], accR[i][j]);
+            accI[i][j] = mma3(fai, fbr[j], accI[i][j]);
+          }
+        } else {
+#pragma unroll
+          for (int j = 0; j < TN; ++j) {
+            accR[i][j] = mma3(far_, fbr[j], accR[i][j]);
+            if (BC) accI[i][j] = mma3(far_, fbi[j], accI[i][j]);
+          }
+        }
+      }
+    }
+    if (more) {
+      store_a<C, AC, AT>(As[buf ^ 1], ra, tid);
+      store_b<C, BC, BT>(Bs[buf ^ 1], rb, tid);
+    }
+    __syncthreads();
+    buf ^= 1;
+  }
+
+  if (ws) {  // split-K partial (double) to ws[kz][b][m][n]
+    const long plane = (long)g.M * g.N;
+    double* Wp = ws + ((long)kz * gridDim.y + b) * plane * (CC ? 2 : 1);
+#pragma unroll
+    for (int i = 0; i < TM; ++i)
+#pragma unroll
+      for (int j = 0; j < TN; ++j)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          const int m = m0 + (wm * TM + i) * 16 + 4 * q4 + v;
+          const int n = n0 + (wn * TN + j) * 16 + c16;
+          if (m >= g.M || n >= g.N) continue;
+          const long off = (long)m * g.N + n;
+          if (CC)
+            reinterpret_cast<double2*>(Wp)[off] = double2{(double)accR[i][j][v], (double)accI[i][j][v]};
+          else
+            Wp[off] = accR[i][j][v];
+        }
+    return;
+  }
+  const bool has_beta = (g.beta.x != 0.0 || g.beta.y != 0.0);
+  TCo* Cp = static_cast<TCo*>(g.C) + b * g.sC;
+  double2* Dp = g.D ? static_cast<double2*>(g.D) + b * g.sD : nullptr;
+#pragma unroll
+  for (int i = 0; i < TM; ++i)
+#pragma unroll
+    for (int j = 0; j < TN; ++j)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int m = m0 + (wm * TM + i) * 16 + 4 * q4 + v;
+        const int n = n0 + (wn * TN + j) * 16 + c16;
+        if (m >= g.M || n >= g.N) continue;
+        const long off = (long)m * g.ldc + n;
+        double2 r = cmul(g.alpha, double2{(double)accR[i][j][v], (double)accI[i][j][v]});
+        if (has_beta) {
+          const double2 o = cmul(g.beta, double2{(double)Cp[off].x, (double)Cp[off].y});
+          r.x += o.x;
+          r.y += o.y;
+        }
+        if constexpr (sizeof(TCo) == 8)
+          Cp[off] = TCo{(float)r.x, (float)r.y};
+        else
+          Cp[off] = TCo{r.x, r.y};
+        if (Dp) {
+          double2* Dc = Dp + (long)m * g.ldd + n;
+          double2 d = *Dc;
+          d.x += r.x;
+          d.y += r.y;
+          *Dc = d;
+        }
+      }
+}
+
+template <bool BF, bool AC, bool BC, bool AT, bool BT, typename TA, typename TB, typename TCo, int WM, int WN, int TM,
+          int TN>
+int launch(const Gemm& g, hipStream_t s) {
+  using C = CfgL<BF, AC, BC, AT, BT, TA, TB, WM, WN, TM, TN>;
+  const int tm = (g.M + C::BM - 1) / C::BM, tn = (g.N + C::BN - 1) / C::BN;
+  const long tiles = (long)tm * tn;
+  const bool CC = AC || BC;
+  int ksplit = 1;
+  const long wgs = tiles * g.batch;
+  // split-K partials are reduced into a double2 / double C only
+  if (sizeof(TCo) == 16 && g.ws && g.batch == 1 && wgs < 1024 && g.K >= 8 * BKL) {
+    ksplit = (int)std::min<long>((2048 + wgs - 1) / wgs, (long)g.K / (4 * BKL));
+    while (ksplit > 1 && (long)ksplit * g.M * g.N * (CC ? 2 : 1) > g.ws_elems) --ksplit;
+  }
+  int kchunk = g.K;
+  if (ksplit > 1) {
+    kchunk = ((g.K + ksplit - 1) / ksplit + BKL - 1) / BKL * BKL;
+    ksplit = (g.K + kchunk - 1) / kchunk;
+  }
+  const int remap = (g.batch == 1 && tiles >= 64) ? 1 : 0;
+  const long T8 = remap ? (tiles + 7) / 8 : 0;
+  const long gx = remap ? T8 * 8 : tiles;
+  if (gx > 0x7fffffffL || g.batch > 65535) return fail(AFQ_E_UNSUPPORTED, "gemm_lp: grid too large");
+  double* ws = ksplit > 1 ? static_cast<double*>(g.ws) : nullptr;
+  hipLaunchKernelGGL((gemm_lp_kernel<BF, AC, BC, AT, BT, TA, TB, TCo, WM, WN, TM, TN>),
+                     dim3((unsigned)gx, g.batch, ksplit), dim3(C::NT), 0, s, g, tm, tn, remap, T8, kchunk, ws);
+  AFQ_TRY(launch_status("gemm_lp_kernel"));
+  if (ksplit > 1) return gemm_splitk_reduce(g, ws, ksplit, CC, s);
+  return AFQ_OK;
+}
+
+template <bool BF, bool AC, bool BC, bool AT, bool BT, typename TA, typename TB, typename TCo>
+int dispatch_shape(const Gemm& g, hipStream_t s) {
+  if (g.N <= 16) return launch<BF, AC, BC, AT, BT, TA, TB, TCo, 4, 1, 1, 1>(g, s);
+  if (g.N <= 32) return launch<BF, AC, BC, AT, BT, TA, TB, TCo, 4, 1, 1, 2>(g, s);
+  if (g.N <= 48) return launch<BF, AC, BC, AT, BT, TA, TB, TCo, 4, 1, 1, 3>(g, s);
+  if (g.N <= 64 && g.M > 64) return launch<BF, AC, BC, AT, BT, TA, TB, TCo, 4, 1, 1, 4>(g, s);
+  return launch<BF, AC, BC, AT, BT, TA, TB, TCo, 2, 2, 2, 2>(g, s);
+}
+
+template <bool BF>
+int dispatch(LpKind kind, const Gemm& g, hipStream_t s) {
+  switch (kind) {
+    case LpKind::VHS:  // complex double x real float -> complex float
+      return dispatch_shape<BF, true, false, false, false, double, float, float2>(g, s);
+    case LpKind::TAYLOR_D:  // complex float x complex double -> complex float (+ D)
+      return dispatch_shape<BF, true, true, false, false, float, double, float2>(g, s);
+    case LpKind::TAYLOR_F:  // complex float x complex float -> complex float (+ D)
+      return dispatch_shape<BF, true, true, false, false, float, float, float2>(g, s);
+    case LpKind::FORCE_BIAS:  // complex double x real float (transposed) -> complex double
+      return dispatch_shape<BF, true, false, false, true, double, float, double2>(g, s);
+  }
+  return fail(AFQ_E_INVALID, "gemm_lp: unknown kind");
+}
+
+}  // namespace
+
+int gemm_lp(LpKind kind, bool bf16x3, const Gemm& g, hipStream_t s) {
+  if (g.M <= 0 || g.N <= 0 || g.batch <= 0) return AFQ_OK;
+  if (g.K <= 0) return fail(AFQ_E_INVALID, "gemm_lp: K must be positive");
+  return bf16x3 ? dispatch<true>(kind, g, s) : dispatch<false>(kind, g, s);
+}
+
+}  // namespace afq

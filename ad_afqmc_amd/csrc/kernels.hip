@@ -360,6 +360,11 @@ __global__ void normal_fill_kernel(double* out, long n, uint64_t seed, uint64_t 
   if (2 * pair + 1 < n) out[2 * pair + 1] = rad * s;
 }
 
+__global__ void to_f32_kernel(const double* src, float* dst, long n) {
+  const long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (i < n) dst[i] = (float)src[i];
+}
+
 inline unsigned blocks_for(long n, int t) { return (unsigned)((n + t - 1) / t); }
 
 }  // namespace
@@ -435,6 +440,11 @@ int k_sr_index(double* weights, const double* zeta, int W, double* cum, int32_t*
 int k_gather_walkers(const double2* src, double2* dst, const int32_t* idx, long per, int W, hipStream_t s) {
   hipLaunchKernelGGL(gather_walkers_kernel, dim3(blocks_for(per, 256), W), dim3(256), 0, s, src, dst, idx, per);
   return launch_status("gather_walkers_kernel");
+}
+int k_to_f32(const double* src, float* dst, long n, hipStream_t s) {
+  if (n <= 0) return AFQ_OK;
+  hipLaunchKernelGGL(to_f32_kernel, dim3(blocks_for(n, 256)), dim3(256), 0, s, src, dst, n);
+  return launch_status("to_f32_kernel");
 }
 int k_normal_fill(double* out, long n, uint64_t seed, uint64_t counter, hipStream_t s) {
   const long pairs = (n + 1) / 2;
