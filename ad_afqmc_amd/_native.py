@@ -66,6 +66,7 @@ _SIGNATURES = {
     "afq_last_error": ([], C.c_char_p),
     "afq_job_create": ([C.POINTER(P), C.POINTER(JobDesc)], C.c_int),
     "afq_job_destroy": ([P], C.c_int),
+    "afq_job_footprint": ([C.POINTER(JobDesc), C.c_int32], C.c_int64),
     "afq_job_set_batch": ([P, C.c_int32], C.c_int),
     "afq_job_set_hamiltonian": ([P, C.c_double, P, P, P], C.c_int),
     "afq_job_set_trial": ([P, P, P, P, P], C.c_int),

@@ -435,9 +435,11 @@ extern "C" {
 const char* afq_version(void) { return "afq 0.1.0 gfx950"; }
 const char* afq_last_error(void) { return afq::g_err.c_str(); }
 
-int afq_job_create(afq_job_t* out, const afq_job_desc* d) {
-  if (!out || !d) return fail(AFQ_E_INVALID, "NULL argument");
-  *out = nullptr;
+}  // extern "C"
+
+namespace {
+
+int validate_desc(const afq_job_desc* d) {
   if (d->norb < 1 || d->nchol < 1 || d->n_walkers < 1 || d->nelec_up < 1 || d->nelec_dn < 1)
     return fail(AFQ_E_INVALID, "sizes must be positive");
   if (d->nelec_up > d->norb || d->nelec_dn > d->norb) return fail(AFQ_E_INVALID, "nelec exceeds norb");
@@ -454,8 +456,10 @@ int afq_job_create(afq_job_t* out, const afq_job_desc* d) {
     return fail(AFQ_E_INVALID, "unknown precision");
   if (d->nelec_up > 64 || d->nelec_dn > 64) return fail(AFQ_E_UNSUPPORTED, "nocc > 64 not supported");
   if (d->nchol > 65535 || d->n_walkers > 65535) return fail(AFQ_E_UNSUPPORTED, "nchol and n_walkers must be <= 65535");
+  return AFQ_OK;
+}
 
-  afq_job* j = new afq_job();
+void init_job(afq_job* j, const afq_job_desc* d) {
   j->d = *d;
   j->N = d->norb;
   j->G = d->nchol;
@@ -471,14 +475,21 @@ int afq_job_create(afq_job_t* out, const afq_job_desc* d) {
   j->nspin_prop = j->restricted ? 1 : 2;
   j->lp = d->precision != AFQ_PREC_FP64;
   j->bf16x3 = d->precision == AFQ_PREC_BF16X3;
+  j->kchunk_max = exchange_nchunk(j->G);
+}
 
+// Allocates every device buffer of the job (dry: only sums the bytes).
+int alloc_job(afq_job* j, bool dry, int64_t* total) {
   const long N = j->N, G = j->G, W = j->W;
   const long wmax = std::max(j->wcols, std::max(j->nocc[0], j->nocc[1]));
-  const int kch = exchange_nchunk((int)G);
-  j->kchunk_max = kch;
+  const int kch = j->kchunk_max;
   int rc = AFQ_OK;
-#define A_(p, n)                      \
-  if (rc == AFQ_OK) rc = dalloc_n(&(p), (size_t)(n));
+  *total = 0;
+#define A_(p, n)                                                   \
+  do {                                                             \
+    *total += (int64_t)(n) * (int64_t)sizeof(*(p));                \
+    if (!dry && rc == AFQ_OK) rc = dalloc_n(&(p), (size_t)(n));   \
+  } while (0)
   A_(j->h1, 2 * N * N);
   A_(j->chol, G * N * N);
   A_(j->mo[0], N * j->nocc[0]);
@@ -531,6 +542,31 @@ int afq_job_create(afq_job_t* out, const afq_job_desc* d) {
   A_(j->gemm_ws, j->gemm_ws_elems);
   A_(j->dscratch, 8);
 #undef A_
+  return rc;
+}
+
+}  // namespace
+
+extern "C" {
+
+int64_t afq_job_footprint(const afq_job_desc* d, int32_t n_prop_steps) {
+  if (!d || validate_desc(d) != AFQ_OK) return -1;
+  afq_job j;
+  init_job(&j, d);
+  int64_t total = 0;
+  alloc_job(&j, true, &total);
+  if (n_prop_steps > 0) total += (int64_t)sizeof(double) * n_prop_steps * j.W * j.G;
+  return total;
+}
+
+int afq_job_create(afq_job_t* out, const afq_job_desc* d) {
+  if (!out || !d) return fail(AFQ_E_INVALID, "NULL argument");
+  *out = nullptr;
+  AFQ_TRY(validate_desc(d));
+  afq_job* j = new afq_job();
+  init_job(j, d);
+  int64_t total = 0;
+  const int rc = alloc_job(j, false, &total);
   if (rc != AFQ_OK) {
     release(j);
     delete j;

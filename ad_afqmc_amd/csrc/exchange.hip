@@ -78,25 +78,29 @@ __device__ inline void load_frag(Frag<MODE, NB, KS>& F, const TR* __restrict__ A
   const int kb = c + KE * (lane >> 4);
   const int r16 = lane & 15;
   const bool full = (kb + KE - 1 < N);
+  // Branch-free: rows >= O and columns >= N are clamped to valid addresses
+  // and their values masked to zero.
 #pragma unroll
   for (int bi = 0; bi < NB; ++bi) {
     const int row = bi * 16 + r16;
     const bool rok = row < O;
-    const TR* ap = A + (long)(rok ? row : 0) * N + kb;
-    const double2* bp = B + (long)(rok ? row : 0) * N + kb;
+    const long roff = (long)(rok ? row : O - 1) * N;
 #pragma unroll
     for (int s = 0; s < KE; ++s) {
       const bool ok = rok && (full || kb + s < N);
-      const double2 v = ok ? bp[s] : double2{0.0, 0.0};
-      F.a[bi][s] = ok ? (R)ap[s] : (R)0;
-      F.br[bi][s] = (R)v.x;
-      F.bi[bi][s] = (R)v.y;
+      const int col = full ? kb + s : min(kb + s, N - 1);
+      const double2 v = B[roff + col];
+      const TR a = A[roff + col];
+      F.a[bi][s] = ok ? (R)a : (R)0;
+      F.br[bi][s] = ok ? (R)v.x : (R)0;
+      F.bi[bi][s] = ok ? (R)v.y : (R)0;
     }
   }
 }
 
+// NB <= 3 fits two work-groups per CU (<= 256 VGPRs per lane); NB = 4 needs the whole file.
 template <int MODE, int NB, int KS, typename TR>
-__global__ __launch_bounds__(256) void exchange_kernel(const TR* __restrict__ rc, const double2* __restrict__ Gm, int O,
+__global__ __launch_bounds__(256, (NB <= 3 ? 2 : 1)) void exchange_kernel(const TR* __restrict__ rc, const double2* __restrict__ Gm, int O,
                                                        int N, int Gn, int W, int nchunk, int cpg, long T8,
                                                        double2* __restrict__ coul, double2* __restrict__ kpart) {
   using Acc = std::conditional_t<MODE == F64, v4d, v4f>;
@@ -240,11 +244,12 @@ int launch_nb(const TR* rc, const double2* Gm, int O, int N, int Gn, int W, int 
 template <int MODE, typename TR>
 int launch_mode(const TR* rc, const double2* Gm, int O, int N, int Gn, int W, int nchunk, double2* coul,
                 double2* kpart, hipStream_t s) {
-  // NB = 4 (49 <= O <= 64) uses 4-wide chunks in FP64 so the accumulators fit the register file
+  // chunk widths chosen so the FP64 fragments fit next to the accumulators:
+  // NB = 3 uses 8-wide chunks (two work-groups per CU), NB = 4 (49 <= O <= 64) 4-wide ones
   switch ((O + 15) / 16) {
     case 1: return launch_nb<MODE, 1, 4, TR>(rc, Gm, O, N, Gn, W, nchunk, coul, kpart, s);
     case 2: return launch_nb<MODE, 2, 4, TR>(rc, Gm, O, N, Gn, W, nchunk, coul, kpart, s);
-    case 3: return launch_nb<MODE, 3, 4, TR>(rc, Gm, O, N, Gn, W, nchunk, coul, kpart, s);
+    case 3: return launch_nb<MODE, 3, 2, TR>(rc, Gm, O, N, Gn, W, nchunk, coul, kpart, s);
     default: return launch_nb<MODE, 4, (MODE == F64 ? 1 : 4), TR>(rc, Gm, O, N, Gn, W, nchunk, coul, kpart, s);
   }
 }

@@ -1,0 +1,270 @@
+"""Continuous-batching service for many AFQMC jobs on one GPU.
+
+Requests (jobs of different N, O, G and walker counts) arrive over time.  A
+job is admitted as soon as a slot and its HBM footprint (as libafq.so would
+allocate it, `afq_job_footprint`) fit under the cap; admitted jobs keep their
+Hamiltonian, rotated Cholesky tensor and walkers resident and each runs its
+device blocks (afq_block: phaseless steps, QR, energy, then comb resampling)
+on its own HIP stream, so work of different jobs overlaps on the device.  A
+job holds at most one block in flight; the host issues the next block of every
+job whose previous block has finished, admits newly arrived jobs in between,
+and retires finished jobs (freeing their memory and slot) as soon as their
+last block completes.  The reference runs many-job workloads (LNO fragments)
+one after another through separate processes (ad_afqmc/lnoutils.py:175-177).
+
+Each job's arithmetic depends only on its own inputs and seed, so its energy is
+the same whether it runs alone or in the mix.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+import time
+from collections import deque
+from dataclasses import dataclass, field
+from typing import Callable, Dict, List, Optional
+
+import numpy as np
+
+
+@dataclass
+class JobSpec:
+    job_id: int
+    norb: int
+    nocc: int
+    nchol: int
+    n_walkers: int
+    n_steps: int = 300
+    dt: float = 0.005
+    seed: int = 0
+    precision: str = "fp64"
+    arrival: float = 0.0  # seconds after the start of the stream
+    l_scale: Optional[float] = None  # Cholesky scale (default 1/sqrt(norb))
+
+
+@dataclass
+class JobResult:
+    job_id: int
+    arrival: float
+    admitted: float = math.nan
+    finished: float = math.nan
+    energy: float = math.nan
+    block_energies: Optional[np.ndarray] = None
+    walker_steps: int = 0
+    error: Optional[str] = None
+
+    @property
+    def time_to_energy(self) -> float:
+        return self.finished - self.arrival
+
+
+def footprint_bytes(spec: JobSpec, n_prop_steps: int) -> int:
+    """Device bytes of a job: libafq.so's own buffers plus the caller-owned
+    walkers, weights and overlaps."""
+    from . import _native as nat
+
+    d = nat.JobDesc(
+        norb=spec.norb, nelec_up=spec.nocc, nelec_dn=spec.nocc, nchol=spec.nchol, n_walkers=spec.n_walkers,
+        walker_type=nat.AFQ_WALKERS_RESTRICTED, trial_type=nat.AFQ_TRIAL_RHF, n_exp_terms=6,
+        precision=nat.PRECISIONS[spec.precision], reserved=0, dt=spec.dt,
+    )
+    b = nat.lib().afq_job_footprint(C.byref(d), int(n_prop_steps))
+    if b < 0:
+        raise ValueError(f"job {spec.job_id}: invalid sizes ({nat.lib().afq_last_error().decode()})")
+    walkers = spec.n_walkers * spec.norb * spec.nocc * 16
+    return int(b + walkers + spec.n_walkers * 24 + 4096)
+
+
+class Scheduler:
+    """Admission and issue logic, independent of the device.
+
+    executor must provide start(spec) -> handle, ready(handle) -> bool (the
+    previous block finished), issue(handle) -> None (enqueue the next block),
+    done(handle) -> bool (all blocks issued and finished), finish(handle) ->
+    (energy, block_energies, walker_steps), release(handle)."""
+
+    def __init__(self, executor, hbm_cap_bytes: float = 200e9, max_active: int = 64, n_prop_steps: int = 10,
+                 footprint: Callable[[JobSpec, int], int] = footprint_bytes, clock=time.perf_counter):
+        self.ex = executor
+        self.cap = float(hbm_cap_bytes)
+        self.max_active = int(max_active)
+        self.n_prop_steps = int(n_prop_steps)
+        self.footprint = footprint
+        self.clock = clock
+        self.used = 0
+        self.peak_used = 0
+        self.max_concurrent = 0
+
+    def run(self, specs: List[JobSpec], realtime: bool = True) -> Dict[int, JobResult]:
+        pending = deque(sorted(specs, key=lambda s: s.arrival))
+        waiting: deque = deque()
+        active = {}  # job_id -> (spec, handle, bytes)
+        results: Dict[int, JobResult] = {s.job_id: JobResult(s.job_id, s.arrival) for s in specs}
+        t0 = self.clock()
+        now = lambda: (self.clock() - t0)
+        while pending or waiting or active:
+            t = now()
+            while pending and (pending[0].arrival <= t or not realtime):
+                waiting.append(pending.popleft())
+            # admission, first come first served; a job that can never fit is refused
+            while waiting:
+                spec = waiting[0]
+                try:
+                    need = self.footprint(spec, self.n_prop_steps)
+                except ValueError as e:
+                    results[spec.job_id].error = str(e)
+                    waiting.popleft()
+                    continue
+                if need > self.cap:
+                    results[spec.job_id].error = f"footprint {need / 1e9:.2f} GB exceeds the HBM cap {self.cap / 1e9:.2f} GB"
+                    waiting.popleft()
+                    continue
+                if self.used + need > self.cap or len(active) >= self.max_active:
+                    break
+                waiting.popleft()
+                handle = self.ex.start(spec)
+                self.used += need
+                self.peak_used = max(self.peak_used, self.used)
+                active[spec.job_id] = (spec, handle, need)
+                results[spec.job_id].admitted = now()
+            self.max_concurrent = max(self.max_concurrent, len(active))
+            # issue the next block of every job whose previous one finished
+            progressed = False
+            for jid, (spec, h, need) in list(active.items()):
+                if self.ex.done(h):
+                    e, blocks, ws = self.ex.finish(h)
+                    r = results[jid]
+                    r.finished, r.energy, r.block_energies, r.walker_steps = now(), e, blocks, ws
+                    self.ex.release(h)
+                    self.used -= need
+                    del active[jid]
+                    progressed = True
+                elif self.ex.ready(h):
+                    self.ex.issue(h)
+                    progressed = True
+            if not progressed:
+                if not active and pending and realtime:
+                    time.sleep(max(0.0, min(0.01, pending[0].arrival - now())))
+                else:
+                    time.sleep(0.0002)
+        return results
+
+
+class GpuExecutor:
+    """Runs each admitted job on its own HIP stream through libafq.so."""
+
+    def __init__(self, n_prop_steps: int = 10):
+        import torch
+
+        self.torch = torch
+        self.n_prop_steps = int(n_prop_steps)
+
+    def start(self, spec: JobSpec):
+        torch = self.torch
+        from . import hamiltonian as hm
+        from . import propagation as pr
+        from . import wavefunctions as wf
+        from .binding import engine_for
+
+        stream = torch.cuda.Stream()
+        with torch.cuda.stream(stream):
+            h = _device_system(spec)
+            trial = wf.rhf(spec.norb, (spec.nocc, spec.nocc))
+            prop = pr.propagator_afqmc(dt=spec.dt, n_walkers=spec.n_walkers, walker_type="restricted",
+                                       precision=spec.precision)
+            wave = {"mo_coeff": h["mo"], "rdm1": h["rdm1"]}
+            ham = {"h0": 0.0, "h1": h["h1"], "chol": h["chol"], "ene0": 0.0}
+            H = hm.hamiltonian(spec.norb)
+            ham = H.build_measurement_intermediates(ham, trial, wave)
+            ham = H.build_propagation_intermediates(ham, prop, trial, wave)
+            ham.pop("chol")  # the job keeps its own resident copy
+            pd = prop.init_prop_data(trial, wave, ham, seed=spec.seed)
+            job = engine_for(trial, wave, ham, "restricted", spec.n_walkers, prop)
+            up = pd["walkers"].data.clone()
+            weights = pd["weights"].clone()
+            overlaps = job.overlap(up, None)
+            e0 = pd["e_estimate"].reshape(())
+            scalars = torch.stack([e0, e0.clone()]).contiguous()
+            nblocks = max(1, -(-spec.n_steps // self.n_prop_steps))
+            zetas = torch.rand(nblocks, dtype=torch.float64, device="cuda",
+                               generator=torch.Generator(device="cuda").manual_seed(spec.seed + 17))
+            ev = torch.cuda.Event()
+            ev.record(stream)
+        return dict(spec=spec, stream=stream, job=job, wave=wave, ham=ham, up=up, weights=weights,
+                    overlaps=overlaps, scalars=scalars, zetas=zetas, counter=0, issued=0, nblocks=nblocks,
+                    outs=[], event=ev)
+
+    def ready(self, h) -> bool:
+        return h["issued"] < h["nblocks"] and h["event"].query()
+
+    def issue(self, h):
+        torch = self.torch
+        with torch.cuda.stream(h["stream"]):
+            out, h["counter"] = h["job"].block(h["up"], None, h["weights"], h["overlaps"], h["scalars"],
+                                               self.n_prop_steps, h["spec"].seed, h["counter"])
+            i = h["issued"]
+            h["job"].stochastic_reconfiguration(h["up"], None, h["weights"], h["zetas"][i : i + 1])
+            h["outs"].append(out)
+            h["event"] = torch.cuda.Event()
+            h["event"].record(h["stream"])
+        h["issued"] += 1
+
+    def done(self, h) -> bool:
+        return h["issued"] >= h["nblocks"] and h["event"].query()
+
+    def finish(self, h):
+        blocks = self.torch.stack(h["outs"]).cpu().numpy()
+        e = float(np.sum(blocks[:, 0] * blocks[:, 1]) / np.sum(blocks[:, 1]))
+        return e, blocks, h["spec"].n_walkers * self.n_prop_steps * h["nblocks"]
+
+    def release(self, h):
+        h["stream"].synchronize()
+        h["job"].close()
+        h.clear()
+
+
+def _device_system(spec: JobSpec):
+    """The synthetic generator of ad_afqmc_amd.synthetic, drawn on the device
+    from torch's generator seeded with the job seed (fast for large jobs)."""
+    import torch
+
+    N, O, G = spec.norb, spec.nocc, spec.nchol
+    gen = torch.Generator(device="cuda").manual_seed(int(spec.seed))
+    dev = torch.device("cuda")
+    x = torch.randn((N, N), generator=gen, dtype=torch.float64, device=dev)
+    h1 = torch.diag(torch.linspace(-2.0, 1.0, N, dtype=torch.float64, device=dev)) + 0.05 * (x + x.T) / 2
+    ls = spec.l_scale if spec.l_scale is not None else 1.0 / math.sqrt(N)
+    sig = ls * 0.5 * torch.exp(-torch.arange(G, dtype=torch.float64, device=dev) / (0.2 * G))
+    a = torch.randn((G, N, N), generator=gen, dtype=torch.float64, device=dev) * sig.view(G, 1, 1)
+    chol = ((a + a.transpose(1, 2)) / 2).reshape(G, N * N).contiguous()
+    mo = torch.linalg.eigh(h1)[1][:, :O].contiguous()
+    d = mo @ mo.T
+    return dict(h1=torch.stack([h1, h1]), chol=chol, mo=mo, rdm1=torch.stack([d, d]))
+
+
+def poisson_stream(n_jobs: int, rate: float, seed: int = 0, nmin: int = 30, nmax: int = 250, wmin: int = 64,
+                   wmax: int = 512, n_steps: int = 300, precision: str = "fp64") -> List[JobSpec]:
+    """Jobs with N ~ U{nmin..nmax}, O = N/5, G ~ 5N, W ~ U{wmin..wmax}, Poisson arrivals at `rate` per second."""
+    rng = np.random.default_rng(seed)
+    t = np.cumsum(rng.exponential(1.0 / rate, size=n_jobs))
+    specs = []
+    for i in range(n_jobs):
+        N = int(rng.integers(nmin, nmax + 1))
+        specs.append(JobSpec(job_id=i, norb=N, nocc=max(1, N // 5), nchol=5 * N, n_walkers=int(rng.integers(wmin, wmax + 1)),
+                             n_steps=n_steps, seed=1000 + i, precision=precision, arrival=float(t[i])))
+    return specs
+
+
+def summarize(results: Dict[int, JobResult], wall: float) -> dict:
+    ok = [r for r in results.values() if r.error is None and not math.isnan(r.finished)]
+    tte = np.array([r.time_to_energy for r in ok]) if ok else np.array([math.nan])
+    ws = sum(r.walker_steps for r in ok)
+    return {
+        "jobs": len(results),
+        "completed": len(ok),
+        "refused": sum(1 for r in results.values() if r.error is not None),
+        "wall_s": wall,
+        "walker_steps_per_s": ws / wall if wall > 0 else math.nan,
+        "p50_time_to_energy_s": float(np.percentile(tte, 50)),
+        "p99_time_to_energy_s": float(np.percentile(tte, 99)),
+    }

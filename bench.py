@@ -120,6 +120,11 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-sample-walkers", type=int, default=8)
     ap.add_argument("--breakdown", action="store_true", help="print per-class kernel times to stderr")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run 5 plain device copies of MIB MiB (to scale HBM counters) and exit")
+    ap.add_argument("--serving-jobs", type=int, default=16,
+                    help="jobs in the serving stream (Poisson arrivals at 4/s, N~U{30..250}, 300 steps); 0 skips it")
+    ap.add_argument("--serving-precision", default="bf16x3")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this engine runs on one GPU; --gpus above 1 is not supported", file=sys.stderr)
@@ -127,6 +132,16 @@ def main():
 
     import numpy as np
     import torch
+
+    if args.copy_calibration:
+        n = args.copy_calibration * (1 << 20) // 8
+        src = torch.ones(n, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(5):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_calibration_bytes_read": n * 8, "copy_calibration_bytes_written": n * 8}))
+        return
 
     from ad_afqmc_amd import engine as E
     from ad_afqmc_amd import hamiltonian as hm
@@ -260,6 +275,25 @@ def main():
         "population_finite": finite,
         "mean_block_energy": float(np.mean(blocks[:, 0])) if finite else None,
     }
+    if args.serving_jobs > 0:
+        from ad_afqmc_amd import serving
+
+        # release the benchmark job's memory before the stream starts
+        del job, up, dn, weights, overlaps, scalars, pd, ham, wave
+        wave = ham = None
+        torch.cuda.empty_cache()
+        print(f"[bench] serving stream: {args.serving_jobs} jobs", file=sys.stderr, flush=True)
+        specs = serving.poisson_stream(args.serving_jobs, rate=4.0, seed=0, precision=args.serving_precision)
+        sch = serving.Scheduler(serving.GpuExecutor(n_prop_steps=10), hbm_cap_bytes=200e9, max_active=64)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        res = sch.run(specs, realtime=True)
+        torch.cuda.synchronize()
+        summ = serving.summarize(res, time.perf_counter() - t0)
+        summ.update(arrival_rate_per_s=4.0, precision=args.serving_precision, hbm_cap_gb=200,
+                    max_concurrent_jobs=sch.max_concurrent, peak_admitted_gb=sch.peak_used / 1e9,
+                    workload="N~U{30..250}, O=N/5, G=5N, W~U{64..512}, 300 steps, Poisson 4 jobs/s")
+        line["serving"] = summ
     if not args.no_cpu_baseline:
         print("[bench] cpu baseline (oracle sample)", file=sys.stderr, flush=True)
         line["cpu_baseline"] = cpu_baseline(cfg, args.cpu_sample_walkers) if not uhf else None

@@ -94,6 +94,9 @@ const char* afq_last_error(void);
 
 int afq_job_create(afq_job_t* job, const afq_job_desc* desc);
 int afq_job_destroy(afq_job_t job);
+/* Device bytes afq_job_create would allocate for desc (plus the afq_block
+ * field buffer for n_prop_steps steps); -1 if desc is invalid.  No device call. */
+int64_t afq_job_footprint(const afq_job_desc* desc, int32_t n_prop_steps);
 
 /* Active walker count for the following calls, 1 <= n_walkers <= desc.n_walkers
  * (the capacity the job was created with).  Default: the capacity. */

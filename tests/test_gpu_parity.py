@@ -257,3 +257,49 @@ def test_h6_driver_energy_statistical():
     e, err = driver.afqmc_energy(ham, hm.hamiltonian(norb), prop, trial, wave, smp,
                                  dict(n_eql=1, seed=98), verbose=False)
     assert abs(e - (-3.238196747261496)) < 5 * err + 2e-3, (e, err)
+
+
+# ------------------------------------------------------------------ reduced precision
+LP_CASES = [
+    ("rhf", "restricted", 12, 4, 4, 10, 6),
+    ("uhf", "unrestricted", 67, 21, 18, 97, 7),
+    ("rhf", "restricted", 70, 17, 17, 130, 9),
+]
+# relative tolerances vs the FP64 oracle for one measurement / one step
+LP_TOL = {"fp32": 2e-5, "bf16x3": 5e-4}
+
+
+@pytest.mark.parametrize("precision", ["fp32", "bf16x3"])
+@pytest.mark.parametrize("case", LP_CASES, ids=[f"{c[0]}-{c[1]}-N{c[2]}" for c in LP_CASES])
+def test_reduced_precision_against_oracle(case, precision):
+    kind, wtype, N, ou, od, G, W = case
+    sysm, trial, prop, ham, wave, walkers, otrial, oham, owalk, _ = _setup(*case)
+    # same system, reduced-precision job (new wave_data so a new job is bound)
+    wave_lp = sysm.wave_data()
+    ham_lp = sysm.ham_data()
+    prop_lp = pr.propagator_afqmc(dt=prop.dt, n_walkers=W, walker_type=wtype, precision=precision)
+    H = hm.hamiltonian(N)
+    ham_lp = H.build_measurement_intermediates(ham_lp, trial, wave_lp)
+    ham_lp = H.build_propagation_intermediates(ham_lp, prop_lp, trial, wave_lp)
+    tol = LP_TOL[precision]
+    e = npy(trial.calc_energy(walkers, ham_lp, wave_lp))
+    eo = otrial.calc_energy(owalk, oham["h0"], oham["rot_h1"], oham["rot_chol"])
+    assert rel(e, eo) < tol
+    fb = npy(trial.calc_force_bias(walkers, ham_lp, wave_lp))
+    assert rel(fb, otrial.calc_force_bias(owalk, oham["rot_chol"])) < tol
+    pd = prop_lp.init_prop_data(trial, wave_lp, ham_lp, seed=3, init_walkers=walkers)
+    e_est = float(pd["e_estimate"].item())
+    st = dict(walkers=owalk, weights=np.ones(W), overlaps=otrial.calc_overlap(owalk), e_estimate=e_est,
+              pop_control_ene_shift=e_est)
+    fields = np.random.default_rng(42).standard_normal((W, G))
+    pd = prop_lp.propagate_constrained(trial, ham_lp, pd, fields, wave_lp)
+    st = O.propagate_constrained(otrial, st, fields, oham, prop.dt, prop.n_exp_terms)
+    if wtype == "restricted":
+        assert rel(npy(pd["walkers"].data), st["walkers"]) < tol
+    else:
+        assert rel(npy(pd["walkers"].data[0]), st["walkers"][0]) < tol
+        assert rel(npy(pd["walkers"].data[1]), st["walkers"][1]) < tol
+    assert rel(npy(pd["overlaps"]), st["overlaps"]) < 10 * tol
+    assert rel(npy(pd["weights"]), st["weights"]) < 10 * tol
+    # the FP64-kept parts are untouched: overlap of the same walkers is exact
+    assert rel(npy(trial.calc_overlap(walkers, wave_lp)), otrial.calc_overlap(owalk)) < TOL
