@@ -110,6 +110,81 @@ def cpu_baseline(cfg, sample_walkers=8, seed=0):
     }
 
 
+def precision_study(cfg_name: str, n_blocks: int, n_eql: int, l_coef: float = 0.3):
+    """Energy of each precision mode against FP64 on identical Philox fields and
+    SR offsets: the trajectories stay correlated, so the paired block-energy
+    differences measure the systematic error with a small error bar."""
+    import numpy as np
+    import torch
+
+    from ad_afqmc_amd import hamiltonian as hm
+    from ad_afqmc_amd import propagation as pr
+    from ad_afqmc_amd import stat_utils
+    from ad_afqmc_amd import synthetic as syn
+    from ad_afqmc_amd import wavefunctions as wf
+    from ad_afqmc_amd.binding import engine_for
+
+    cfg = syn.CONFIGS[cfg_name]
+    N, G, W = cfg["norb"], cfg["nchol"], cfg["n_walkers"]
+    Oa = cfg["nocc_up"]
+    Ob = cfg.get("nocc_dn", Oa)
+    uhf = cfg["uhf"]
+    # Cholesky scale l_coef/sqrt(N): a correlation energy small enough for a stable
+    # walker population at dt = 0.005 (the stated generator, l_coef = sqrt(N), kills it)
+    lsc = l_coef / math.sqrt(N)
+    s = syn.make_system(N, Oa, G, nocc_dn=Ob, uhf=uhf, seed=0, l_scale=lsc)
+    nsteps, dt = cfg["n_prop_steps"], cfg["dt"]
+    trial = wf.uhf(N, (Oa, Ob)) if uhf else wf.rhf(N, (Oa, Oa))
+    wtype = "unrestricted" if uhf else "restricted"
+    zetas = torch.rand(n_eql + n_blocks, dtype=torch.float64, device="cuda",
+                       generator=torch.Generator(device="cuda").manual_seed(7))
+    out = {}
+    for prec in ("fp64", "fp32", "bf16x3"):
+        wave, ham = s.wave_data(), s.ham_data()
+        prop = pr.propagator_afqmc(dt=dt, n_walkers=W, walker_type=wtype, precision=prec)
+        H = hm.hamiltonian(N)
+        ham = H.build_measurement_intermediates(ham, trial, wave)
+        ham = H.build_propagation_intermediates(ham, prop, trial, wave)
+        pd = prop.init_prop_data(trial, wave, ham, seed=0)
+        job = engine_for(trial, wave, ham, wtype, W, prop)
+        up, dn = (t.clone() if t is not None else None for t in pd["walkers"].tensors())
+        weights = pd["weights"].clone()
+        overlaps = job.overlap(up, dn)
+        scalars = torch.stack([pd["e_estimate"], pd["e_estimate"].clone()]).contiguous()
+        counter, blocks = 0, []
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for i in range(n_eql + n_blocks):
+            o, counter = job.block(up, dn, weights, overlaps, scalars, nsteps, 99, counter)
+            job.stochastic_reconfiguration(up, dn, weights, zetas[i : i + 1])
+            blocks.append(o)
+            # e_estimate follows the block energy as in the reference driver (driver.py:168)
+            scalars[0] = 0.9 * scalars[0] + 0.1 * o[0]
+        torch.cuda.synchronize()
+        t = time.perf_counter() - t0
+        b = torch.stack(blocks).cpu().numpy()[n_eql:]
+        mean, err = stat_utils.blocking_analysis(b[:, 1], b[:, 0])
+        out[prec] = dict(blocks=b, energy=float(mean), error=float(err) if err is not None else None,
+                         walker_steps_per_s=W * nsteps * (n_eql + n_blocks) / t, finite=bool(np.all(np.isfinite(b))))
+        del job, wave, ham, pd, up, dn, weights, overlaps, scalars
+        torch.cuda.empty_cache()
+    ref = out["fp64"]["blocks"]
+    summary = {"workload": cfg_name, "l_scale": f"{l_coef}/sqrt(N)",
+               "blocks": n_blocks, "equilibration_blocks": n_eql, "steps_per_block": nsteps}
+    for prec, r in out.items():
+        d = r["blocks"][:, 0] - ref[:, 0]
+        summary[prec] = {
+            "energy_ha": r["energy"],
+            "stat_error_ha": r["error"],
+            "delta_vs_fp64_mha": 1e3 * float(np.mean(d)),
+            "delta_stderr_mha": 1e3 * float(np.std(d, ddof=1) / math.sqrt(len(d))) if len(d) > 1 else None,
+            "max_abs_block_delta_mha": 1e3 * float(np.max(np.abs(d))),
+            "walker_steps_per_s": r["walker_steps_per_s"],
+            "finite": r["finite"],
+        }
+    return summary
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__)
     ap.add_argument("--gpus", type=int, default=1)
@@ -125,6 +200,11 @@ def main():
     ap.add_argument("--serving-jobs", type=int, default=16,
                     help="jobs in the serving stream (Poisson arrivals at 4/s, N~U{30..250}, 300 steps); 0 skips it")
     ap.add_argument("--serving-precision", default="bf16x3")
+    ap.add_argument("--precision-study", default="", metavar="WORKLOAD",
+                    help="only run the FP64 / FP32 / bf16x3 energy comparison on WORKLOAD and exit")
+    ap.add_argument("--study-blocks", type=int, default=50)
+    ap.add_argument("--study-eql", type=int, default=10)
+    ap.add_argument("--study-lcoef", type=float, default=0.3, help="Cholesky scale coefficient c in c/sqrt(N)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this engine runs on one GPU; --gpus above 1 is not supported", file=sys.stderr)
@@ -132,6 +212,11 @@ def main():
 
     import numpy as np
     import torch
+
+    if args.precision_study:
+        print(json.dumps({"precision_study": precision_study(args.precision_study, args.study_blocks,
+                                                             args.study_eql, args.study_lcoef)}), flush=True)
+        return
 
     if args.copy_calibration:
         n = args.copy_calibration * (1 << 20) // 8
