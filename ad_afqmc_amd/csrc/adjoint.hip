@@ -1,0 +1,794 @@
+// Reverse pass of a phaseless segment on the device (RHF trial, restricted
+// walkers, FP64): the value and the gradient with respect to h1 and the
+// Cholesky vectors of
+//
+//     F = sum_b E_b W_b / sum_b W_b
+//
+// over n_blocks blocks of n_steps phaseless steps with given fields and no
+// population control -- the function the reference differentiates with
+// jax.vjp in its "reverse" AD mode (sampling.py:300-341 _nosr_norot variant,
+// driver.py:865-876).  The segment is checkpointed once per block (walkers,
+// weights, population-control shift); in the reverse sweep each block's steps
+// are recomputed from its checkpoint and then pulled back in reverse order.
+//
+// Adjoint convention: a complex intermediate z carries g_z = dF/dRe z - i dF/dIm z,
+// so a holomorphic product y = A x pulls back as g_x = A^T g_y (transposes only,
+// which the GEMM supports directly); a real parameter p in z gets Re(g_z dz/dp).
+// Complex accumulators for real parameters are reduced with Re at the end.
+#include <cmath>
+#include <cstring>
+
+#include "kernels.h"
+#include "pipeline.h"
+
+namespace afq {
+namespace {
+
+__device__ inline double2 zmul(double2 a, double2 b) { return {a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x}; }
+__device__ inline double2 zdiv(double2 a, double2 b) {
+  const double d = b.x * b.x + b.y * b.y;
+  return {(a.x * b.x + a.y * b.y) / d, (a.y * b.x - a.x * b.y) / d};
+}
+__device__ inline double2 zexp(double2 a) {
+  const double e = exp(a.x);
+  return {e * cos(a.y), e * sin(a.y)};
+}
+__device__ inline double wsum(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+  return v;
+}
+// Sum over a 1024-thread work-group; every thread gets the total.
+__device__ inline double wg_sum(double v, double* red) {
+  v = wsum(v);
+  __syncthreads();
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+  __syncthreads();
+  double t = 0.0;
+  for (int i = 0; i < (int)(blockDim.x >> 6); ++i) t += red[i];
+  return t;
+}
+inline unsigned nblk(long n, int t = 256) { return (unsigned)((n + t - 1) / t); }
+
+// Step k of the weight update, pulled back: from the adjoints of w_{k+1}
+// (gw, in) and of the shift E_{k+1} (gesh, in) to the adjoints of w_k (gw,
+// out) and E_k (gesh, out), plus gd = g of log(ov'/ov) and of -sqrt(dt)
+// shift_term (they enter the importance function the same way), alpha = the
+// adjoint of its real exponent, and dt sum(alpha) added to the h0_prop adjoint.
+// Follows propagation.py:306-334 (phaseless factor, clipping, weight cap, shift).
+__global__ __launch_bounds__(1024) void adj_weight_kernel(const double2* ov_k, const double2* ov_n,
+                                                          const double2* terms, const double* w_k, const double* w_n,
+                                                          const double* sc_k, const double2* h0p_p, double dt, int W,
+                                                          double* gw, double* gesh, double2* gd, double* alpha,
+                                                          double* gh0p) {
+  __shared__ double red[16];
+  const double sqdt = sqrt(dt);
+  double loc = 0.0;
+  for (int w = threadIdx.x; w < W; w += blockDim.x) loc += w_n[w];
+  const double sw = wg_sum(loc, red);
+  const double geshn = *gesh;
+  const double pcs = sc_k[1];
+  const double2 h0p = *h0p_p;
+  double asum = 0.0;
+  for (int w = threadIdx.x; w < W; w += blockDim.x) {
+    const double gwn = gw[w] + geshn * (-0.1 / dt) / sw;
+    const double2 ratio = zdiv(ov_n[w], ov_k[w]);
+    const double2 sh = terms[2 * w], ft = terms[2 * w + 1];
+    const double2 arg = {-sqdt * sh.x + ft.x + dt * (pcs + h0p.x), -sqdt * sh.y + ft.y + dt * h0p.y};
+    const double2 imp = zmul(zexp(arg), ratio);
+    const double2 ph_arg = zmul(zexp(double2{-sqdt * sh.x, -sqdt * sh.y}), ratio);
+    const double theta = atan2(ph_arg.y, ph_arg.x);
+    const double aimp = hypot(imp.x, imp.y);
+    const double ph_raw = aimp * cos(theta);
+    const bool mph = !(isnan(ph_raw) || ph_raw < 1.0e-3 || ph_raw > 100.0);
+    const double ph = mph ? ph_raw : 0.0;
+    const bool mw = !(ph * w_k[w] > 100.0);
+    const double gph = mw ? gwn * w_k[w] : 0.0;
+    const double a = mph ? gph * ph_raw : 0.0;
+    const double b = mph ? gph * (-aimp * sin(theta)) : 0.0;
+    gw[w] = mw ? gwn * ph : 0.0;
+    gd[w] = double2{a, -b};
+    alpha[w] = a;
+    asum += a;
+  }
+  const double at = wg_sum(asum, red);
+  if (threadIdx.x == 0) {
+    *gesh = dt * at;
+    *gh0p += dt * at;
+  }
+}
+
+// g of the shifted fields from shift_term = sum_g xt_g (i m_g):  gxt = gsh * mf, gsh = -sqrt(dt) gd.
+__global__ void adj_gxt_init_kernel(const double2* gd, const double2* mf, double sqdt, int G, long n, double2* gxt) {
+  const long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double2 gsh = {-sqdt * gd[i / G].x, -sqdt * gd[i / G].y};
+  gxt[i] = zmul(gsh, mf[i % G]);
+}
+
+// gs = alpha xt - gxt (fb_term and xt = x - s), gfb = -i sqrt(dt) gs (s = -i sqrt(dt)(fb - m)),
+// and the m adjoint from shift_term and s, summed over walkers (one thread per gamma).
+__global__ void adj_fb_kernel(const double2* xt, const double2* gxt, const double2* gd, const double* alpha, double sqdt,
+                              int G, int W, double2* gfb, double* mbar) {
+  const int g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= G) return;
+  double macc = 0.0;
+  for (int w = 0; w < W; ++w) {
+    const long i = (long)w * G + g;
+    const double2 x = xt[i], gx = gxt[i];
+    const double a = alpha[w];
+    const double2 gs = {a * x.x - gx.x, a * x.y - gx.y};
+    gfb[i] = double2{sqdt * gs.y, -sqdt * gs.x};
+    const double2 gsh = {-sqdt * gd[w].x, -sqdt * gd[w].y};
+    const double2 p = zmul(gsh, x);
+    macc += -p.y - sqdt * gs.y;  // Re(i gsh xt) + Re(i sqrt(dt) gs)
+  }
+  mbar[g] += macc;
+}
+
+// X_w = c gd_w S_w^-T (O x O per walker)
+__global__ void adj_scaled_transpose_kernel(const double2* Sinv, const double2* gd, double c, int O, long n,
+                                            double2* X) {
+  const long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const long oo = (long)O * O;
+  const long w = i / oo, r = (i % oo) / O, q = i % O;
+  const double2 v = Sinv[w * oo + q * O + r];
+  X[i] = zmul(double2{c * gd[w].x, c * gd[w].y}, v);
+}
+
+// P_w += c gd_w I
+__global__ void adj_diag_add_kernel(double2* P, const double2* gd, double c, int O, long n) {
+  const long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const long w = i / O, r = i % O;
+  double2& p = P[w * (long)O * O + r * O + r];
+  p.x += c * gd[w].x;
+  p.y += c * gd[w].y;
+}
+
+// Block end (sampling.py:84-92 and the segment ratio F): from the adjoint of the
+// shift after the block to the energy weights kappa_w = dF/dE_w (zero where the
+// energy was clipped), the weight adjoints and the shift adjoint before it.
+__global__ __launch_bounds__(1024) void adj_block_end_kernel(const double2* energies, const double* w_end,
+                                                             const double* sc, const double* bo, const double* seg,
+                                                             double clip, int W, double* gesh, double* kappa,
+                                                             double* gw) {
+  const double gsa = *gesh;
+  const double e_est = sc[0], be = bo[0], bw = bo[1], F = seg[0], D = seg[1];
+  const double gbe = 0.1 * gsa + bw / D;
+  const double gbw = (be - F) / D;
+  for (int w = threadIdx.x; w < W; w += blockDim.x) {
+    const double e = energies[w].x;
+    const bool in = !(fabs(e - e_est) > clip);
+    const double ec = in ? e : e_est;
+    kappa[w] = in ? gbe * w_end[w] / bw : 0.0;
+    gw[w] += gbe * (ec - be) / bw + gbw;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) *gesh = 0.9 * gsa;
+}
+
+// F = sum E_b W_b / sum W_b (and D = sum W_b)
+__global__ void adj_seed_kernel(const double* bo, int nb, double* seg, double* energy_out) {
+  if (threadIdx.x != 0) return;
+  double num = 0.0, den = 0.0;
+  for (int b = 0; b < nb; ++b) {
+    num += bo[2 * b] * bo[2 * b + 1];
+    den += bo[2 * b + 1];
+  }
+  seg[0] = num / den;
+  seg[1] = den;
+  *energy_out = num / den;
+}
+
+// F~ = -2 kappa_w f (in place, layout [g][i][w][j]) and its copy in layout [w][i][g][j].
+__global__ void adj_fscale_kernel(double2* F, const double* kappa, int G, int O, int Wc, double2* Fp) {
+  const long n = (long)G * O * Wc * O;
+  const long idx = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (idx >= n) return;
+  const int j = (int)(idx % O);
+  long t = idx / O;
+  const int wl = (int)(t % Wc);
+  t /= Wc;
+  const int i = (int)(t % O);
+  const int g = (int)(t / O);
+  const double s = -2.0 * kappa[wl];
+  const double2 v = {s * F[idx].x, s * F[idx].y};
+  F[idx] = v;
+  Fp[(((long)wl * O + i) * G + g) * O + j] = v;
+}
+
+// Cc[w][g] = -2 tr F~_wg = 4 kappa_w c_wg
+__global__ void adj_cc_kernel(const double2* F, int G, int O, int Wc, double2* Cc) {
+  const long idx = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (idx >= (long)Wc * G) return;
+  const int wl = (int)(idx / G), g = (int)(idx % G);
+  double2 s = {0.0, 0.0};
+  for (int i = 0; i < O; ++i) {
+    const double2 v = F[(((long)g * O + i) * Wc + wl) * O + i];
+    s.x += v.x;
+    s.y += v.y;
+  }
+  Cc[idx] = double2{-2.0 * s.x, -2.0 * s.y};
+}
+
+// Gp[i][w][n] = G_w[i][n]
+__global__ void adj_gperm_kernel(const double2* Gm, int O, int N, int Wc, double2* Gp) {
+  const long idx = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (idx >= (long)Wc * O * N) return;
+  const int n = (int)(idx % N);
+  const long t = idx / N;
+  const int i = (int)(t % O), wl = (int)(t / O);
+  Gp[((long)i * Wc + wl) * N + n] = Gm[idx];
+}
+
+// gG_w += c kappa_w rot_h1
+__global__ void adj_outer_kernel(double2* gG, const double* kappa, const double* rh1, double c, long per, long n) {
+  const long idx = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (idx >= n) return;
+  gG[idx].x += c * kappa[idx / per] * rh1[idx % per];
+}
+
+__global__ void adj_conj_kernel(const double2* a, double2* b, long n) {
+  const long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (i < n) b[i] = double2{a[i].x, -a[i].y};
+}
+
+// out[e] = Re sum_w X[w][e]
+__global__ void adj_reduce_re_kernel(const double2* X, int W, long n, double* out) {
+  const long e = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (e >= n) return;
+  double s = 0.0;
+  for (int w = 0; w < W; ++w) s += X[(long)w * n + e].x;
+  out[e] = s;
+}
+
+__global__ void adj_re_kernel(const double2* X, long n, double* out) {
+  const long e = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (e < n) out[e] = X[e].x;
+}
+
+// M = [[A^T, E], [0, A^T]] (2n x 2n): expm(M) holds the Frechet derivative of
+// expm at A^T in direction E in its upper-right block.
+__global__ void adj_block_matrix_kernel(const double* A, const double* E, int n, double* M) {
+  const long idx = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  const long m = 2L * n;
+  if (idx >= m * m) return;
+  const int r = (int)(idx / m), c = (int)(idx % m);
+  double v = 0.0;
+  if (r < n && c < n)
+    v = A[(long)c * n + r];
+  else if (r < n)
+    v = E[(long)r * n + (c - n)];
+  else if (c >= n)
+    v = A[(long)(c - n) * n + (r - n)];
+  M[idx] = v;
+}
+
+__global__ void adj_extract_ur_kernel(const double* M, int n, double scale, double* out) {
+  const long idx = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (idx >= (long)n * n) return;
+  const int r = (int)(idx / n), c = (int)(idx % n);
+  out[idx] = scale * M[(long)r * 2 * n + n + c];
+}
+
+// mbar_g += h0pbar m_g + t_g
+__global__ void adj_mbar_final_kernel(double* mbar, const double* m, const double* gh0p, const double* t, int G) {
+  const int g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g < G) mbar[g] += (*gh0p) * m[g] + t[g];
+}
+
+// Lbar_g += m_g hmodbar + mbar_g rho
+__global__ void adj_lfinal_kernel(double* Lbar, const double* m, const double* mbar, const double* hmodbar,
+                                  const double* rho, int G, long nn) {
+  const long idx = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (idx >= (long)G * nn) return;
+  const long g = idx / nn, e = idx % nn;
+  Lbar[idx] += m[g] * hmodbar[e] + mbar[g] * rho[e];
+}
+
+// h1bar[s] = (X + X^T)/4 for s = 0, 1 (h1s = ((h1_0 + h1_0^T)/2 + (h1_1 + h1_1^T)/2)/2)
+__global__ void adj_h1_out_kernel(const double* X, int n, double* out) {
+  const long idx = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  const long nn = (long)n * n;
+  if (idx >= nn) return;
+  const int r = (int)(idx / n), c = (int)(idx % n);
+  const double v = 0.25 * (X[idx] + X[(long)c * n + r]);
+  out[idx] = v;
+  out[nn + idx] = v;
+}
+
+// ---------------------------------------------------------------- GEMM builder
+struct GB {
+  Gemm g;
+  GB(int M, int N, int K, int batch = 1) {
+    g.M = M;
+    g.N = N;
+    g.K = K;
+    g.batch = batch;
+  }
+  GB& a(const void* p, long lda, long sA, bool cplx, bool trans = false) {
+    g.A = p;
+    g.lda = lda;
+    g.sA = sA;
+    g.a_cplx = cplx;
+    g.a_trans = trans;
+    return *this;
+  }
+  GB& b(const void* p, long ldb, long sB, bool cplx, bool trans = false) {
+    g.B = p;
+    g.ldb = ldb;
+    g.sB = sB;
+    g.b_cplx = cplx;
+    g.b_trans = trans;
+    return *this;
+  }
+  GB& c(void* p, long ldc, long sC) {
+    g.C = p;
+    g.ldc = ldc;
+    g.sC = sC;
+    return *this;
+  }
+  GB& d(void* p, long ldd, long sD) {
+    g.D = p;
+    g.ldd = ldd;
+    g.sD = sD;
+    return *this;
+  }
+  GB& alpha(double re, double im = 0.0) {
+    g.alpha = double2{re, im};
+    return *this;
+  }
+  GB& beta(double re) {
+    g.beta = double2{re, 0.0};
+    return *this;
+  }
+  GB& ws(afq_job* j) {
+    g.ws = j->gemm_ws;
+    g.ws_elems = j->gemm_ws_elems;
+    return *this;
+  }
+  int run(hipStream_t st) { return gemm_f64(g, st); }
+};
+
+// Device arena carved in two passes (size, then pointers).
+struct Arena {
+  char* base = nullptr;
+  size_t off = 0;
+  template <typename T>
+  T* take(size_t n) {
+    off = (off + 255) & ~size_t(255);
+    T* p = base ? reinterpret_cast<T*>(base + off) : nullptr;
+    off += n * sizeof(T);
+    return p;
+  }
+};
+
+struct Buffers {
+  // checkpoints and block results
+  double2* ckW;
+  double* ckw;
+  double* cksc;
+  double* bo;
+  double* seg;
+  // tapes of one block
+  double2* Wt;
+  double* wt;
+  double* sct;
+  double2* ovt;
+  double2* xtt;
+  double2* termst;
+  // reverse sweep
+  double2 *Q, *Qc, *R, *Rinv, *detR;
+  double2* Tb;
+  double2 *W2, *gW, *gWk, *gW2, *gTa, *gTb;
+  double2 *gvhs, *EbarW;
+  double2 *gxt, *gfb, *gd, *gG, *P, *gS, *X;
+  double *alpha, *gw, *kappa, *mbar, *scal;  // scal: gesh, gh0p
+  double2 *LbarC, *rcbarC, *rh1barC;
+  // energy chunks
+  int Wc;
+  double2 *F, *Fp, *Gp, *Cc;
+  // setup
+  double *Ebar, *v0, *v1, *hmod, *Amat, *Mblk, *Mexp, *Mwork, *hmodbar, *h1sbar, *rh1bar, *rcbar, *rho, *tg;
+};
+
+void carve(Arena& ar, Buffers& b, const afq_job* j, int nb, int ns, int K) {
+  const long N = j->N, G = j->G, W = j->W, O = j->nocc[0];
+  const long per = N * O, NN = N * N;
+  b.ckW = ar.take<double2>(nb * W * per);
+  b.ckw = ar.take<double>(nb * W);
+  b.cksc = ar.take<double>(2 * nb);
+  b.bo = ar.take<double>(2 * nb);
+  b.seg = ar.take<double>(2);
+  b.Wt = ar.take<double2>((ns + 1) * W * per);
+  b.wt = ar.take<double>((ns + 1) * W);
+  b.sct = ar.take<double>(2 * (ns + 1));
+  b.ovt = ar.take<double2>((ns + 1) * W);
+  b.xtt = ar.take<double2>(ns * W * G);
+  b.termst = ar.take<double2>(ns * 2 * W);
+  b.Q = ar.take<double2>(W * per);
+  b.Qc = ar.take<double2>(W * per);
+  b.R = ar.take<double2>(W * O * O);
+  b.Rinv = ar.take<double2>(W * O * O);
+  b.detR = ar.take<double2>(W);
+  b.Tb = ar.take<double2>((K + 1) * W * per);
+  b.W2 = ar.take<double2>(W * per);
+  b.gW = ar.take<double2>(W * per);
+  b.gWk = ar.take<double2>(W * per);
+  b.gW2 = ar.take<double2>(W * per);
+  b.gTa = ar.take<double2>(W * per);
+  b.gTb = ar.take<double2>(W * per);
+  b.gvhs = ar.take<double2>(W * NN);
+  b.EbarW = ar.take<double2>(W * NN);
+  b.gxt = ar.take<double2>(W * G);
+  b.gfb = ar.take<double2>(W * G);
+  b.gd = ar.take<double2>(W);
+  b.gG = ar.take<double2>(W * per);
+  b.P = ar.take<double2>(W * O * O);
+  b.gS = ar.take<double2>(W * O * O);
+  b.X = ar.take<double2>(W * O * O);
+  b.alpha = ar.take<double>(W);
+  b.gw = ar.take<double>(W);
+  b.kappa = ar.take<double>(W);
+  b.mbar = ar.take<double>(G);
+  b.scal = ar.take<double>(4);
+  b.LbarC = ar.take<double2>(G * NN);
+  b.rcbarC = ar.take<double2>(G * per);
+  b.rh1barC = ar.take<double2>(per);
+  // exchange-adjoint chunks: F and its permuted copy at most ~512 MB each
+  const long per_w = G * O * O * (long)sizeof(double2);
+  b.Wc = (int)std::max(1L, std::min(W, (512L << 20) / std::max(1L, per_w)));
+  b.F = ar.take<double2>(G * O * b.Wc * O);
+  b.Fp = ar.take<double2>(G * O * b.Wc * O);
+  b.Gp = ar.take<double2>(b.Wc * per);
+  b.Cc = ar.take<double2>(b.Wc * G);
+  b.Ebar = ar.take<double>(NN);
+  b.v0 = ar.take<double>(NN);
+  b.v1 = ar.take<double>(NN);
+  b.hmod = ar.take<double>(NN);
+  b.Amat = ar.take<double>(NN);
+  b.Mblk = ar.take<double>(4 * NN);
+  b.Mexp = ar.take<double>(4 * NN);
+  b.Mwork = ar.take<double>(12 * NN);
+  b.hmodbar = ar.take<double>(NN);
+  b.h1sbar = ar.take<double>(NN);
+  b.rh1bar = ar.take<double>(per);
+  b.rcbar = ar.take<double>(G * per);
+  b.rho = ar.take<double>(NN);
+  b.tg = ar.take<double>(G);
+}
+
+#define LAUNCH(kernel, n, ...)                                                       \
+  do {                                                                               \
+    const long n_ = (n);                                                             \
+    if (n_ > 0) hipLaunchKernelGGL(kernel, dim3(nblk(n_)), dim3(256), 0, st, __VA_ARGS__); \
+    AFQ_TRY(launch_status(#kernel));                                                 \
+  } while (0)
+
+struct Seg {
+  afq_job* j;
+  Buffers b;
+  hipStream_t st;
+  int nb, ns, K;
+  long N, G, W, O, per, NN;
+  double dt, sqdt, e_est;
+  const double* fields;
+
+  const double* T() const { return j->mo[0]; }
+  const double* E() const { return j->exp_h1[0]; }
+  const double* fields_at(int blk, int k) const { return fields + ((long)blk * ns + k) * W * G; }
+
+  int cpy(void* dst, const void* src, size_t bytes) {
+    AFQ_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, st));
+    return AFQ_OK;
+  }
+  int zero(void* p, size_t bytes) {
+    AFQ_HIP(hipMemsetAsync(p, 0, bytes, st));
+    return AFQ_OK;
+  }
+  int overlap(double2* Wp, double2* ov) {
+    AFQ_TRY(green_pipeline(j, Wp, nullptr, false, false, st));
+    return overlap_from_dets(j, ov, st);
+  }
+
+  // Forward over all blocks with checkpoints (afq_block with the given fields).
+  int forward(const void* walkers, const double* weights, double* energy_out) {
+    double2* Wc = b.Q;  // current walkers
+    double* wc = b.wt;
+    double2* ov = b.ovt;
+    double* sc = b.sct;
+    AFQ_TRY(cpy(Wc, walkers, sizeof(double2) * W * per));
+    AFQ_TRY(cpy(wc, weights, sizeof(double) * W));
+    const double sc0[2] = {e_est, e_est};
+    AFQ_HIP(hipMemcpyAsync(sc, sc0, sizeof(sc0), hipMemcpyHostToDevice, st));
+    AFQ_TRY(overlap(Wc, ov));
+    for (int blk = 0; blk < nb; ++blk) {
+      AFQ_TRY(cpy(b.ckW + blk * W * per, Wc, sizeof(double2) * W * per));
+      AFQ_TRY(cpy(b.ckw + blk * W, wc, sizeof(double) * W));
+      AFQ_TRY(cpy(b.cksc + 2 * blk, sc, 2 * sizeof(double)));
+      for (int k = 0; k < ns; ++k) AFQ_TRY(propagate(j, Wc, nullptr, fields_at(blk, k), wc, ov, sc, st));
+      AFQ_TRY(orthonormalize(j, Wc, nullptr, st));
+      AFQ_TRY(overlap(Wc, ov));
+      AFQ_TRY(energy_pipeline(j, Wc, nullptr, j->energies, st));
+      AFQ_TRY(k_block_finalize(j->energies, wc, sc, std::sqrt(2.0 / dt), (int)W, b.bo + 2 * blk, st));
+    }
+    hipLaunchKernelGGL(adj_seed_kernel, dim3(1), dim3(64), 0, st, b.bo, nb, b.seg, energy_out);
+    return launch_status("adj_seed_kernel");
+  }
+
+  // Recomputes the tapes of block blk from its checkpoint.
+  int tape(int blk) {
+    AFQ_TRY(cpy(b.Wt, b.ckW + blk * W * per, sizeof(double2) * W * per));
+    AFQ_TRY(cpy(b.wt, b.ckw + blk * W, sizeof(double) * W));
+    AFQ_TRY(cpy(b.sct, b.cksc + 2 * blk, 2 * sizeof(double)));
+    AFQ_TRY(overlap(b.Wt, b.ovt));
+    for (int k = 0; k < ns; ++k) {
+      double2* Wn = b.Wt + (k + 1) * W * per;
+      double* wn = b.wt + (k + 1) * W;
+      double2* ovn = b.ovt + (k + 1) * W;
+      double* scn = b.sct + 2 * (k + 1);
+      AFQ_TRY(cpy(Wn, b.Wt + k * W * per, sizeof(double2) * W * per));
+      AFQ_TRY(cpy(wn, b.wt + k * W, sizeof(double) * W));
+      AFQ_TRY(cpy(ovn, b.ovt + k * W, sizeof(double2) * W));
+      AFQ_TRY(cpy(scn, b.sct + 2 * k, 2 * sizeof(double)));
+      AFQ_TRY(propagate(j, Wn, nullptr, fields_at(blk, k), wn, ovn, scn, st));
+      AFQ_TRY(cpy(b.xtt + k * W * G, j->xs, sizeof(double2) * W * G));
+      AFQ_TRY(cpy(b.termst + k * 2 * W, j->terms, sizeof(double2) * 2 * W));
+    }
+    return AFQ_OK;
+  }
+
+  // gWout += adjoint through G^T = W S^-1 and log det S (coefficient gd, or none),
+  // with j->green / j->Sinv holding G and S^-1 of the walkers concerned.
+  int green_backward(const double2* gG, const double2* gd, double2* gWout) {
+    const long OO = O * O;
+    AFQ_TRY(GB(O, O, N, W).a(j->green[0], N, per, true).b(gG, N, per, true, true).c(b.P, O, OO).run(st));
+    if (gd) LAUNCH(adj_diag_add_kernel, W * O, b.P, gd, 2.0, (int)O, W * O);
+    AFQ_TRY(GB(O, O, O, W).a(b.P, O, OO, true).b(j->Sinv[0], O, OO, true, true).c(b.gS, O, OO).alpha(-1.0).run(st));
+    AFQ_TRY(GB(N, O, O, W)
+                .a(gG, N, per, true, true)
+                .b(j->Sinv[0], O, OO, true, true)
+                .c(gWout, O, per)
+                .beta(1.0)
+                .run(st));
+    return GB(N, O, O, W).a(T(), O, 0, false).b(b.gS, O, OO, true).c(gWout, O, per).beta(1.0).run(st);
+  }
+
+  // Pulls kappa_w Re E_L(Q_w) back to gG (into b.gG) and the parameter accumulators;
+  // j->green / j->Sinv hold G and S^-1 at Q.  wavefunctions.py:1128-1150.
+  int energy_backward() {
+    const int Wc = b.Wc;
+    for (long w0 = 0; w0 < W; w0 += Wc) {
+      const int wc = (int)std::min<long>(Wc, W - w0);
+      const double2* Gc = j->green[0] + w0 * per;
+      const long fsz = G * O * wc * O;
+      // f[g][i][(w,j)] = rc_g G_w^T
+      AFQ_TRY(GB(O, wc * O, N, G)
+                  .a(j->rot_chol[0], N, per, false)
+                  .b(Gc, N, 0, true, true)
+                  .c(b.F, wc * O, O * wc * O)
+                  .run(st));
+      LAUNCH(adj_fscale_kernel, fsz, b.F, b.kappa + w0, (int)G, (int)O, wc, b.Fp);
+      LAUNCH(adj_cc_kernel, (long)wc * G, b.F, (int)G, (int)O, wc, b.Cc);
+      // gG_w = sum_g 4 kappa c_g rc_g - 2 kappa sum_g f_g rc_g + 2 kappa rot_h1
+      AFQ_TRY(GB(wc, per, G).a(b.Cc, G, 0, true).b(j->rot_chol[0], per, 0, false).c(b.gG + w0 * per, per, 0).run(st));
+      AFQ_TRY(GB(O, N, G * O, wc)
+                  .a(b.Fp, G * O, O * G * O, true)
+                  .b(j->rot_chol[0], N, 0, false)
+                  .c(b.gG + w0 * per, N, per)
+                  .beta(1.0)
+                  .run(st));
+      LAUNCH(adj_outer_kernel, wc * per, b.gG + w0 * per, b.kappa + w0, j->rot_h1[0], 2.0, per, wc * per);
+      // rc_g adjoint: sum_w 4 kappa c_wg G_w - 2 kappa f_wg^T G_w ; rot_h1 adjoint: 2 sum_w kappa G_w
+      AFQ_TRY(GB(G, per, wc).a(b.Cc, G, 0, true, true).b(Gc, per, 0, true).c(b.rcbarC, per, 0).beta(1.0).run(st));
+      LAUNCH(adj_gperm_kernel, wc * per, Gc, (int)O, (int)N, wc, b.Gp);
+      AFQ_TRY(GB(O, N, O * wc, G)
+                  .a(b.F, O, O * wc * O, true, true)
+                  .b(b.Gp, N, 0, true)
+                  .c(b.rcbarC, N, per)
+                  .beta(1.0)
+                  .run(st));
+      AFQ_TRY(GB(1, per, wc).a(b.kappa + w0, wc, 0, false).b(Gc, per, 0, true).c(b.rh1barC, per, 0).alpha(2.0).beta(1.0).run(
+          st));
+    }
+    return AFQ_OK;
+  }
+
+  // One propagate_constrained step pulled back (propagation.py:274-336).
+  // On entry b.gW / b.gw / scal[0] hold the adjoints of W_{k+1}, w_{k+1}, E_{k+1};
+  // on exit those of W_k, w_k, E_k.
+  int step_backward(int k) {
+    const long OO = O * O;
+    double2* Wk = b.Wt + k * W * per;
+    double2* Wn = b.Wt + (k + 1) * W * per;
+    const double2* xt = b.xtt + k * W * G;
+    hipLaunchKernelGGL(adj_weight_kernel, dim3(1), dim3(1024), 0, st, b.ovt + k * W, b.ovt + (k + 1) * W,
+                       b.termst + k * 2 * W, b.wt + k * W, b.wt + (k + 1) * W, b.sct + 2 * k, j->h0_prop, dt, (int)W,
+                       b.gw, b.scal, b.gd, b.alpha, b.scal + 1);
+    AFQ_TRY(launch_status("adj_weight_kernel"));
+    // log ov' = 2 log det(T^T W'): gW' += T (2 gd S'^-T)
+    AFQ_TRY(green_pipeline(j, Wn, nullptr, true, false, st));
+    LAUNCH(adj_scaled_transpose_kernel, W * OO, j->Sinv[0], b.gd, 2.0, (int)O, W * OO, b.X);
+    AFQ_TRY(GB(N, O, O, W).a(T(), O, 0, false).b(b.X, O, OO, true).c(b.gW, O, per).beta(1.0).run(st));
+    LAUNCH(adj_gxt_init_kernel, W * G, b.gd, j->mf_shifts, sqdt, (int)G, W * G, b.gxt);
+    // recompute VHS, the Taylor terms and their sum
+    const size_t wbytes = sizeof(double2) * W * per;
+    AFQ_TRY(GB(N, O, N, W).a(E(), N, 0, false).b(Wk, O, per, true).c(b.Tb, O, per).run(st));
+    AFQ_TRY(cpy(b.W2, b.Tb, wbytes));
+    if (K > 0) {
+      AFQ_TRY(GB(W, NN, G).a(xt, G, 0, true).b(j->chol, NN, 0, false).c(j->vhs, NN, 0).alpha(0.0, sqdt).run(st));
+      for (int n = 1; n <= K; ++n)
+        AFQ_TRY(GB(N, O, N, W)
+                    .a(j->vhs, N, NN, true)
+                    .b(b.Tb + (n - 1) * W * per, O, per, true)
+                    .c(b.Tb + n * W * per, O, per)
+                    .d(b.W2, O, per)
+                    .alpha(1.0 / n)
+                    .run(st));
+    }
+    // W' = E W2
+    AFQ_TRY(GB(N, O, N, W).a(E(), N, 0, false, true).b(b.gW, O, per, true).c(b.gW2, O, per).run(st));
+    AFQ_TRY(GB(N, N, O, W).a(b.gW, O, per, true).b(b.W2, O, per, true, true).c(b.EbarW, N, NN).beta(1.0).run(st));
+    // Taylor: gT_K = gW2 ; gT_{n-1} = gW2 + vhs^T gT_n / n ; gvhs = sum_n gT_n T_{n-1}^T / n
+    const double2* cur = b.gW2;
+    for (int n = K; n >= 1; --n) {
+      AFQ_TRY(GB(N, N, O, W)
+                  .a(cur, O, per, true)
+                  .b(b.Tb + (n - 1) * W * per, O, per, true, true)
+                  .c(b.gvhs, N, NN)
+                  .alpha(1.0 / n)
+                  .beta(n == K ? 0.0 : 1.0)
+                  .run(st));
+      double2* nxt = (cur == b.gTa) ? b.gTb : b.gTa;
+      AFQ_TRY(cpy(nxt, b.gW2, wbytes));
+      AFQ_TRY(GB(N, O, N, W).a(j->vhs, N, NN, true, true).b(cur, O, per, true).c(nxt, O, per).alpha(1.0 / n).beta(1.0).run(
+          st));
+      cur = nxt;
+    }
+    // W1 = E W_k
+    AFQ_TRY(GB(N, N, O, W).a(cur, O, per, true).b(Wk, O, per, true, true).c(b.EbarW, N, NN).beta(1.0).run(st));
+    AFQ_TRY(GB(N, O, N, W).a(E(), N, 0, false, true).b(cur, O, per, true).c(b.gWk, O, per).run(st));
+    // vhs = i sqrt(dt) sum_g xt_g L_g
+    if (K > 0) {
+      AFQ_TRY(GB(W, G, NN).a(b.gvhs, NN, 0, true).b(j->chol, NN, 0, false, true).c(b.gxt, G, 0).alpha(0.0, sqdt).beta(1.0).ws(
+          j).run(st));
+      AFQ_TRY(GB(G, NN, W).a(xt, G, 0, true, true).b(b.gvhs, NN, 0, true).c(b.LbarC, NN, 0).alpha(0.0, sqdt).beta(1.0).run(
+          st));
+    }
+    // shifts, force bias and the m adjoint
+    hipLaunchKernelGGL(adj_fb_kernel, dim3(nblk(G)), dim3(256), 0, st, xt, b.gxt, b.gd, b.alpha, sqdt, (int)G, (int)W,
+                       b.gfb, b.mbar);
+    AFQ_TRY(launch_status("adj_fb_kernel"));
+    // fb = 2 sum rc . G at W_k
+    AFQ_TRY(green_pipeline(j, Wk, nullptr, true, true, st));
+    AFQ_TRY(GB(W, per, G).a(b.gfb, G, 0, true).b(j->rot_chol[0], per, 0, false).c(b.gG, per, 0).alpha(2.0).run(st));
+    AFQ_TRY(GB(G, per, W).a(b.gfb, G, 0, true, true).b(j->green[0], per, 0, true).c(b.rcbarC, per, 0).alpha(2.0).beta(
+        1.0).run(st));
+    // Green's function and -log ov at W_k
+    AFQ_TRY(green_backward(b.gG, b.gd, b.gWk));
+    std::swap(b.gW, b.gWk);
+    return AFQ_OK;
+  }
+
+  int reverse() {
+    const size_t wbytes = sizeof(double2) * W * per;
+    AFQ_TRY(zero(b.LbarC, sizeof(double2) * G * NN));
+    AFQ_TRY(zero(b.rcbarC, sizeof(double2) * G * per));
+    AFQ_TRY(zero(b.rh1barC, sizeof(double2) * per));
+    AFQ_TRY(zero(b.EbarW, sizeof(double2) * W * NN));
+    AFQ_TRY(zero(b.mbar, sizeof(double) * G));
+    AFQ_TRY(zero(b.scal, sizeof(double) * 4));
+    AFQ_TRY(zero(b.gW, wbytes));
+    AFQ_TRY(zero(b.gw, sizeof(double) * W));
+    for (int blk = nb - 1; blk >= 0; --blk) {
+      AFQ_TRY(tape(blk));
+      // block end: Q = qr(W_end), energies at Q
+      double2* Wend = b.Wt + ns * W * per;
+      AFQ_TRY(cpy(b.Q, Wend, wbytes));
+      AFQ_TRY(orthonormalize(j, b.Q, nullptr, st));
+      AFQ_TRY(energy_pipeline(j, b.Q, nullptr, j->energies, st));
+      hipLaunchKernelGGL(adj_block_end_kernel, dim3(1), dim3(1024), 0, st, j->energies, b.wt + ns * W, b.sct,
+                         b.bo + 2 * blk, b.seg, std::sqrt(2.0 / dt), (int)W, b.scal, b.kappa, b.gw);
+      AFQ_TRY(launch_status("adj_block_end_kernel"));
+      AFQ_TRY(energy_backward());
+      AFQ_TRY(green_backward(b.gG, nullptr, b.gW));  // gQ = incoming gW + energy part
+      // gauge: Q = W_end R^-1 with R = Q^H W_end ; g_Wend = g_Q R^-T
+      const long OO = O * O;
+      LAUNCH(adj_conj_kernel, W * per, b.Q, b.Qc, W * per);
+      AFQ_TRY(GB(O, O, N, W).a(b.Qc, O, per, true, true).b(Wend, O, per, true).c(b.R, O, OO).run(st));
+      AFQ_TRY(batched_det_inv(b.R, OO, (int)O, (int)W, b.detR, b.Rinv, OO, st));
+      AFQ_TRY(GB(N, O, O, W).a(b.gW, O, per, true).b(b.Rinv, O, OO, true, true).c(b.gWk, O, per).run(st));
+      std::swap(b.gW, b.gWk);
+      for (int k = ns - 1; k >= 0; --k) AFQ_TRY(step_backward(k));
+    }
+    return AFQ_OK;
+  }
+
+  // Setup intermediates pulled back to h1 and L (wavefunctions.py:1215-1232,
+  // propagation.py:378-438).
+  int setup_backward(double* h1_bar, double* chol_bar) {
+    const long per_ = per;
+    LAUNCH(adj_reduce_re_kernel, NN, b.EbarW, (int)W, NN, b.Ebar);
+    // h1_mod = (h1_0 + h1_1)/2 - v0 - v1, A = -dt/2 h1_mod
+    AFQ_TRY(compute_v0(j, b.v0, st));
+    AFQ_TRY(GB(1, NN, G).a(j->mreal + G, G, 0, false).b(j->chol, NN, 0, false).c(b.v1, NN, 0).run(st));
+    AFQ_TRY(k_axpby4(b.hmod, j->h1, j->h1 + NN, 0.5, b.v0, b.v1, -1.0, NN, st));
+    AFQ_TRY(k_scale(b.Amat, b.hmod, -0.5 * dt, (int)N, st));
+    LAUNCH(adj_block_matrix_kernel, 4 * NN, b.Amat, b.Ebar, (int)N, b.Mblk);
+    AFQ_TRY(expm_device(b.Mblk, 2 * (int)N, b.Mexp, b.Mwork, j->dscratch, st));
+    LAUNCH(adj_extract_ur_kernel, NN, b.Mexp, (int)N, -0.5 * dt, b.hmodbar);
+    // h1s adjoint = hmodbar + T rot_h1bar
+    LAUNCH(adj_re_kernel, per_, b.rh1barC, per_, b.rh1bar);
+    AFQ_TRY(cpy(b.h1sbar, b.hmodbar, sizeof(double) * NN));
+    AFQ_TRY(GB(N, N, O).a(T(), O, 0, false).b(b.rh1bar, N, 0, false).c(b.h1sbar, N, 0).beta(1.0).run(st));
+    LAUNCH(adj_h1_out_kernel, NN, b.h1sbar, (int)N, h1_bar);
+    // m adjoint: + h0_prop' m (h0_prop = -h0 + sum m^2/2) + sum L_g . hmodbar (v1 = -sum m_g L_g)
+    AFQ_TRY(GB(G, 1, NN).a(j->chol, NN, 0, false).b(b.hmodbar, 1, 0, false).c(b.tg, 1, 0).run(st));
+    LAUNCH(adj_mbar_final_kernel, G, b.mbar, j->mreal, b.scal + 1, b.tg, (int)G);
+    // L adjoint
+    LAUNCH(adj_re_kernel, G * NN, b.LbarC, G * NN, chol_bar);
+    AFQ_TRY(GB(N, N, N, G).a(b.hmodbar, N, 0, false).b(j->chol, N, NN, false, true).c(chol_bar, N, NN).alpha(-0.5).beta(
+        1.0).run(st));
+    AFQ_TRY(GB(N, N, N, G).a(j->chol, N, NN, false, true).b(b.hmodbar, N, 0, false).c(chol_bar, N, NN).alpha(-0.5).beta(
+        1.0).run(st));
+    LAUNCH(adj_re_kernel, G * per_, b.rcbarC, G * per_, b.rcbar);
+    AFQ_TRY(GB(N, N, O, G).a(T(), O, 0, false).b(b.rcbar, N, per_, false).c(chol_bar, N, NN).beta(1.0).run(st));
+    AFQ_TRY(k_axpby4(b.rho, j->rdm1, j->rdm1 + NN, 1.0, nullptr, nullptr, 0.0, NN, st));
+    LAUNCH(adj_lfinal_kernel, G * NN, chol_bar, j->mreal, b.mbar, b.hmodbar, b.rho, (int)G, NN);
+    return AFQ_OK;
+  }
+};
+
+#undef LAUNCH
+
+}  // namespace
+}  // namespace afq
+
+using namespace afq;
+
+extern "C" int afq_segment_energy_vjp(afq_job_t j, const void* walkers, const double* weights, double e_estimate,
+                                      const double* fields, int32_t n_blocks, int32_t n_steps, double* energy_out,
+                                      double* h1_bar, double* chol_bar, void* stream) {
+  AFQ_TRY(require(j, true, true));
+  if (!walkers || !weights || !fields || !energy_out) return fail(AFQ_E_INVALID, "NULL argument");
+  if ((h1_bar == nullptr) != (chol_bar == nullptr))
+    return fail(AFQ_E_INVALID, "h1_bar and chol_bar must both be given or both be NULL");
+  if (n_blocks < 1 || n_steps < 1) return fail(AFQ_E_INVALID, "n_blocks and n_steps must be >= 1");
+  if (!j->rhf || !j->restricted) return fail(AFQ_E_UNSUPPORTED, "segment derivative: RHF trial and restricted walkers only");
+  if (j->lp) return fail(AFQ_E_UNSUPPORTED, "segment derivative: FP64 jobs only");
+  if (j->have_rot_h1[0] || j->have_rot_chol[0] || j->have_mf || j->have_h0p || j->have_exp[0])
+    return fail(AFQ_E_STATE, "segment derivative needs intermediates built from the job's h1 and chol");
+  Seg s;
+  s.j = j;
+  s.st = reinterpret_cast<hipStream_t>(stream);
+  s.nb = n_blocks;
+  s.ns = n_steps;
+  s.K = j->d.n_exp_terms - 1;
+  s.N = j->N;
+  s.G = j->G;
+  s.W = j->W;
+  s.O = j->nocc[0];
+  s.per = s.N * s.O;
+  s.NN = s.N * s.N;
+  s.dt = j->d.dt;
+  s.sqdt = std::sqrt(j->d.dt);
+  s.e_est = e_estimate;
+  s.fields = fields;
+  Arena sizer;
+  carve(sizer, s.b, j, n_blocks, n_steps, s.K);
+  void* base = nullptr;
+  AFQ_TRY(dalloc(&base, sizer.off));
+  Arena ar;
+  ar.base = static_cast<char*>(base);
+  carve(ar, s.b, j, n_blocks, n_steps, s.K);
+  int rc = s.forward(walkers, weights, energy_out);
+  if (rc == AFQ_OK && h1_bar) rc = s.reverse();
+  if (rc == AFQ_OK && h1_bar) rc = s.setup_backward(h1_bar, chol_bar);
+  const hipError_t e = hipStreamSynchronize(s.st);
+  (void)hipFree(base);
+  if (rc != AFQ_OK) return rc;
+  if (e != hipSuccess) return fail(AFQ_E_HIP, std::string("afq_segment_energy_vjp: ") + hipGetErrorString(e));
+  return AFQ_OK;
+}

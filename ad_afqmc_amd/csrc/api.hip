@@ -6,6 +6,7 @@
 #include <string>
 
 #include "kernels.h"
+#include "pipeline.h"
 
 namespace afq {
 static thread_local std::string g_err;
@@ -18,7 +19,7 @@ int fail(int code, const std::string& m) {
 
 using namespace afq;
 
-namespace {
+namespace afq {
 
 inline hipStream_t S(void* s) { return reinterpret_cast<hipStream_t>(s); }
 
@@ -27,10 +28,6 @@ int dalloc(void** p, size_t bytes) {
   hipError_t e = hipMalloc(p, bytes);
   if (e != hipSuccess) return fail(AFQ_E_NOMEM, std::string("hipMalloc failed: ") + hipGetErrorString(e));
   return AFQ_OK;
-}
-template <typename T>
-int dalloc_n(T** p, size_t n) {
-  return dalloc(reinterpret_cast<void**>(p), n * sizeof(T));
 }
 
 void release(afq_job* j) {
@@ -428,7 +425,88 @@ int require(const afq_job* j, bool meas, bool prop) {
   return AFQ_OK;
 }
 
-}  // namespace
+int compute_v0(afq_job* j, double* v0, hipStream_t st) {
+  const int N = j->N, G = j->G;
+  const long NN = (long)N * N;
+  // accumulated over gamma chunks of a permuted copy: v0 = perm (N x nN) . L_chunk (nN x N)
+  long gch = std::max(1L, std::min((long)G, (64L << 20) / (long)(sizeof(double) * NN)));
+  double* perm = nullptr;
+  AFQ_TRY(dalloc_n(&perm, (size_t)(gch * NN)));
+  int rc = AFQ_OK;
+  for (int g0 = 0; g0 < G && rc == AFQ_OK; g0 += (int)gch) {
+    const int n = (int)std::min(gch, (long)(G - g0));
+    rc = k_permute_chunk(j->chol, perm, N, g0, n, st);
+    if (rc != AFQ_OK) break;
+    Gemm g;
+    g.M = N;
+    g.N = N;
+    g.K = n * N;
+    g.A = perm;
+    g.lda = (long)n * N;
+    g.B = j->chol + (long)g0 * NN;
+    g.ldb = N;
+    g.C = v0;
+    g.ldc = N;
+    g.alpha = double2{0.5, 0.0};
+    g.ws = j->gemm_ws;
+    g.ws_elems = j->gemm_ws_elems;
+    g.beta = double2{g0 == 0 ? 0.0 : 1.0, 0.0};
+    rc = gemm_f64(g, st);
+  }
+  (void)hipStreamSynchronize(st);
+  (void)hipFree(perm);
+  return rc;
+}
+
+int expm_device(const double* A, int n, double* out, double* work, double* dscr, hipStream_t st) {
+  const long nn = (long)n * n;
+  double* As = work;
+  double* P = work + nn;
+  double* Q = work + 2 * nn;
+  AFQ_TRY(k_norm1(A, n, dscr, st));
+  double nrm = 0.0;
+  AFQ_HIP(hipMemcpyAsync(&nrm, dscr, sizeof(double), hipMemcpyDeviceToHost, st));
+  AFQ_HIP(hipStreamSynchronize(st));
+  int sq = 0;
+  while (nrm / std::ldexp(1.0, sq) > 0.25 && sq < 60) ++sq;
+  AFQ_TRY(k_scale(As, A, std::ldexp(1.0, -sq), n, st));
+  AFQ_TRY(k_identity(P, n, st));
+  for (int k = 18; k >= 1; --k) {
+    AFQ_TRY(k_identity(Q, n, st));
+    Gemm g;
+    g.M = n;
+    g.N = n;
+    g.K = n;
+    g.A = As;
+    g.lda = n;
+    g.B = P;
+    g.ldb = n;
+    g.C = Q;
+    g.ldc = n;
+    g.alpha = double2{1.0 / k, 0.0};
+    g.beta = double2{1.0, 0.0};
+    AFQ_TRY(gemm_f64(g, st));
+    std::swap(P, Q);
+  }
+  for (int i = 0; i < sq; ++i) {
+    Gemm g;
+    g.M = n;
+    g.N = n;
+    g.K = n;
+    g.A = P;
+    g.lda = n;
+    g.B = P;
+    g.ldb = n;
+    g.C = Q;
+    g.ldc = n;
+    AFQ_TRY(gemm_f64(g, st));
+    std::swap(P, Q);
+  }
+  AFQ_HIP(hipMemcpyAsync(out, P, sizeof(double) * nn, hipMemcpyDeviceToDevice, st));
+  return AFQ_OK;
+}
+
+}  // namespace afq
 
 extern "C" {
 
@@ -687,9 +765,7 @@ int afq_build_propagation_intermediates(afq_job_t j, void* stream) {
   double* v0 = j->tmp_nn + NN;       // [1]
   double* v1 = j->tmp_nn + 2 * NN;   // [2]
   double* hmod = j->tmp_nn + 3 * NN; // [3]
-  double* As = j->tmp_nn + 4 * NN;   // [4]
-  double* P = j->tmp_nn + 5 * NN;    // [5]
-  double* Q = j->tmp_nn + 6 * NN;    // [6]
+  double* As = j->tmp_nn + 4 * NN;   // [4]; [5..7] expm work
 
   // mf_shifts = i sum_pq L_g,pq rho_pq, rho = rdm1[0] + rdm1[1]
   AFQ_TRY(k_axpby4(rho, j->rdm1, j->rdm1 + NN, 1.0, nullptr, nullptr, 0.0, NN, st));
@@ -709,36 +785,8 @@ int afq_build_propagation_intermediates(afq_job_t j, void* stream) {
   double* mfr = j->mreal + G;
   AFQ_TRY(k_mf(j->mreal, G, j->h0, j->mf_shifts, mfr, j->h0_prop, st));
 
-  // v0 = 1/2 sum_g L_g L_g, accumulated over gamma chunks of a permuted copy
-  {
-    long gch = std::max(1L, std::min((long)G, (64L << 20) / (long)(sizeof(double) * NN)));
-    double* perm = nullptr;
-    AFQ_TRY(dalloc_n(&perm, (size_t)(gch * NN)));
-    int rc = AFQ_OK;
-    for (int g0 = 0; g0 < G && rc == AFQ_OK; g0 += (int)gch) {
-      const int n = (int)std::min(gch, (long)(G - g0));
-      rc = k_permute_chunk(j->chol, perm, N, g0, n, st);
-      if (rc != AFQ_OK) break;
-      Gemm g;
-      g.M = N;
-      g.N = N;
-      g.K = n * N;
-      g.A = perm;
-      g.lda = (long)n * N;
-      g.B = j->chol + (long)g0 * NN;
-      g.ldb = N;
-      g.C = v0;
-      g.ldc = N;
-      g.alpha = double2{0.5, 0.0};
-      g.ws = j->gemm_ws;
-      g.ws_elems = j->gemm_ws_elems;
-      g.beta = double2{g0 == 0 ? 0.0 : 1.0, 0.0};
-      rc = gemm_f64(g, st);
-    }
-    (void)hipStreamSynchronize(st);
-    (void)hipFree(perm);
-    if (rc != AFQ_OK) return rc;
-  }
+  // v0 = 1/2 sum_g L_g L_g
+  AFQ_TRY(compute_v0(j, v0, st));
   // v1 = sum_g Re(i mf_g) L_g
   {
     Gemm g;
@@ -759,51 +807,9 @@ int afq_build_propagation_intermediates(afq_job_t j, void* stream) {
       AFQ_TRY(k_axpby4(hmod, j->h1, j->h1 + NN, 0.5, v0, v1, -1.0, NN, st));
     else
       AFQ_TRY(k_axpby4(hmod, j->h1 + s * NN, nullptr, 1.0, v0, v1, -1.0, NN, st));
-    // exp(A), A = -dt/2 h1_mod: scaling and squaring around a degree-18 Taylor/Horner core
+    // exp(A), A = -dt/2 h1_mod
     AFQ_TRY(k_scale(As, hmod, -0.5 * j->d.dt, N, st));
-    AFQ_TRY(k_norm1(As, N, j->dscratch, st));
-    double nrm = 0.0;
-    AFQ_HIP(hipMemcpyAsync(&nrm, j->dscratch, sizeof(double), hipMemcpyDeviceToHost, st));
-    AFQ_HIP(hipStreamSynchronize(st));
-    int sq = 0;
-    while (nrm / std::ldexp(1.0, sq) > 0.25 && sq < 60) ++sq;
-    AFQ_TRY(k_scale(As, As, std::ldexp(1.0, -sq), N, st));
-    AFQ_TRY(k_identity(P, N, st));
-    for (int k = 18; k >= 1; --k) {
-      AFQ_TRY(k_identity(Q, N, st));
-      Gemm g;
-      g.M = N;
-      g.N = N;
-      g.K = N;
-      g.A = As;
-      g.lda = N;
-      g.B = P;
-      g.ldb = N;
-      g.C = Q;
-      g.ldc = N;
-      g.alpha = double2{1.0 / k, 0.0};
-      g.beta = double2{1.0, 0.0};
-      AFQ_TRY(gemm_f64(g, st));
-      std::swap(P, Q);
-    }
-    for (int i = 0; i < sq; ++i) {
-      Gemm g;
-      g.M = N;
-      g.N = N;
-      g.K = N;
-      g.A = P;
-      g.lda = N;
-      g.B = P;
-      g.ldb = N;
-      g.C = Q;
-      g.ldc = N;
-      AFQ_TRY(gemm_f64(g, st));
-      std::swap(P, Q);
-    }
-    AFQ_HIP(hipMemcpyAsync(j->exp_h1[s], P, sizeof(double) * NN, hipMemcpyDeviceToDevice, st));
-    // restore the named temporaries (P/Q may have swapped)
-    P = j->tmp_nn + 5 * NN;
-    Q = j->tmp_nn + 6 * NN;
+    AFQ_TRY(expm_device(As, N, j->exp_h1[s], j->tmp_nn + 5 * NN, j->dscratch, st));
   }
   AFQ_HIP(hipMemcpyAsync(&j->h0_prop_host, j->h0_prop, sizeof(double2), hipMemcpyDeviceToHost, st));
   AFQ_HIP(hipStreamSynchronize(st));

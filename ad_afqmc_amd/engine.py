@@ -244,6 +244,38 @@ class EngineJob:
         )
         return out, int(ctr.value)
 
+    def segment_energy_vjp(self, walkers, weights, e_estimate: float, fields, want_grad: bool = True):
+        """Segment energy F over fields (n_blocks, n_steps, W, G) and, with
+        want_grad, dF/dh1 (2,N,N) and dF/dchol (G,N*N) (afq_segment_energy_vjp).
+        Synchronises the stream."""
+        if not self.restricted:
+            raise NotImplementedError("the segment derivative supports restricted walkers only")
+        f = as_f64(fields)
+        if f.dim() != 4 or f.shape[2] != self.batch or f.shape[3] != self.nchol:
+            raise ValueError(f"fields must be (n_blocks, n_steps, {self.batch}, {self.nchol}), got {tuple(f.shape)}")
+        w = as_c128(walkers)
+        wt = as_f64(weights)
+        out = torch.empty(1, dtype=torch.float64, device=w.device)
+        gh1 = gL = None
+        if want_grad:
+            gh1 = torch.empty((2, self.norb, self.norb), dtype=torch.float64, device=w.device)
+            gL = torch.empty((self.nchol, self.norb * self.norb), dtype=torch.float64, device=w.device)
+        nat.call(
+            "afq_segment_energy_vjp",
+            self._h,
+            ptr(w),
+            ptr(wt),
+            float(e_estimate),
+            ptr(f),
+            int(f.shape[0]),
+            int(f.shape[1]),
+            ptr(out),
+            ptr(gh1),
+            ptr(gL),
+            stream_ptr(),
+        )
+        return out[0], gh1, gL
+
 
 def normal_fill(n: int, seed: int, counter: int) -> torch.Tensor:
     """N(0,1) draws of the engine's Philox4x32-10 field generator."""

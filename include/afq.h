@@ -38,6 +38,8 @@
  *   afq_stochastic_reconfiguration -> stochastic_reconfiguration_local
  *                            (ad_afqmc/walkers.py:139-145, 244-250; ad_afqmc/sr.py:24-48)
  *   afq_block             -> sampler._block_scan (ad_afqmc/sampling.py:52-94) with on-device fields
+ *   afq_segment_energy_vjp -> jax.vjp of sampler.propagate_phaseless_ad_nosr_norot
+ *                            (ad_afqmc/sampling.py:300-341; ad_afqmc/driver.py:865-876, 842-864)
  */
 #ifndef AFQ_H
 #define AFQ_H
@@ -202,6 +204,26 @@ int64_t afq_walkers_sr_workspace(int32_t n_walkers, int64_t max_walker_elems);
  * walkers_dn = NULL for one buffer).  weights set to sum|w|/W. */
 int afq_walkers_sr(void* walkers_up, int64_t elems_up, void* walkers_dn, int64_t elems_dn, int32_t n_walkers,
                    double* weights, const double* zeta, int32_t* indices_out, void* workspace, void* stream);
+
+/* Value and reverse-mode gradient of the segment energy
+ *     F = sum_b E_b W_b / sum_b W_b
+ * over n_blocks blocks of n_steps phaseless steps (afq_block's step, QR,
+ * overlap, energy and block bookkeeping) with the given fields
+ * (n_blocks, n_steps, W, G) and no population control, starting from
+ * `walkers` (restricted, not modified) and `weights` (W,), with the
+ * population-control shift starting at, and the energy clipping centred on,
+ * e_estimate.  The intermediates are
+ * functions of the job's h1 and Cholesky vectors (they must have been built by
+ * afq_build_*_intermediates, not supplied); the gradient is taken with respect
+ * to those inputs:
+ *   energy_out (1,) device, h1_bar (2, N, N), chol_bar (G, N*N).
+ * h1_bar = chol_bar = NULL evaluates F only.  The fields are constants (the
+ * reference draws them from a key that carries no tangent); clipping masks
+ * are held fixed.  RHF trial, restricted walkers, FP64 jobs.  Allocates a
+ * device workspace for the call and synchronises the stream. */
+int afq_segment_energy_vjp(afq_job_t job, const void* walkers, const double* weights, double e_estimate,
+                           const double* fields, int32_t n_blocks, int32_t n_steps, double* energy_out,
+                           double* h1_bar, double* chol_bar, void* stream);
 
 #ifdef __cplusplus
 }
