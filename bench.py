@@ -182,7 +182,72 @@ def precision_study(cfg_name: str, n_blocks: int, n_eql: int, l_coef: float = 0.
             "walker_steps_per_s": r["walker_steps_per_s"],
             "finite": r["finite"],
         }
+    summary.update(_restart_paired(s, cfg, trial, wtype, n_blocks, n_eql, zetas))
     return summary
+
+
+def _restart_paired(s, cfg, trial, wtype, n_blocks, n_eql, zetas):
+    """Per-block error without trajectory divergence: along the FP64 trajectory,
+    each block is re-run from the same FP64 state (walkers, weights, overlaps,
+    shift, field counter) in each reduced-precision mode, and the local energies
+    of the same FP64 walkers are measured in each mode.  The means of these
+    differences are the systematic errors per block; their spread is small
+    because both sides start from identical states."""
+    import numpy as np
+    import torch
+
+    from ad_afqmc_amd import hamiltonian as hm
+    from ad_afqmc_amd import propagation as pr
+    from ad_afqmc_amd.binding import engine_for
+
+    N, W = cfg["norb"], cfg["n_walkers"]
+    nsteps, dt = cfg["n_prop_steps"], cfg["dt"]
+    jobs, waves = {}, {}
+    for prec in ("fp64", "fp32", "bf16x3"):
+        wave, ham = s.wave_data(), s.ham_data()
+        prop = pr.propagator_afqmc(dt=dt, n_walkers=W, walker_type=wtype, precision=prec)
+        H = hm.hamiltonian(N)
+        ham = H.build_measurement_intermediates(ham, trial, wave)
+        ham = H.build_propagation_intermediates(ham, prop, trial, wave)
+        jobs[prec] = engine_for(trial, wave, ham, wtype, W, prop)
+        waves[prec] = (wave, ham, prop)
+    wave, ham, prop = waves["fp64"]
+    pd = prop.init_prop_data(trial, wave, ham, seed=0)
+    up, dn = (t.clone() if t is not None else None for t in pd["walkers"].tensors())
+    weights = pd["weights"].clone()
+    J = jobs["fp64"]
+    overlaps = J.overlap(up, dn)
+    scalars = torch.stack([pd["e_estimate"], pd["e_estimate"].clone()]).contiguous()
+    counter = 0
+    blk = {p: [] for p in jobs}
+    ene = {p: [] for p in jobs}
+    cl = lambda t: None if t is None else t.clone()
+    for i in range(n_eql + n_blocks):
+        state = (cl(up), cl(dn), weights.clone(), overlaps.clone(), scalars.clone(), counter)
+        if i >= n_eql:
+            for p, job in jobs.items():
+                e = job.energy(up, dn).real
+                ene[p].append(float((weights * e).sum() / weights.sum()))
+        o, counter = J.block(up, dn, weights, overlaps, scalars, nsteps, 99, counter)
+        if i >= n_eql:
+            blk["fp64"].append(float(o[0]))
+            for p in ("fp32", "bf16x3"):
+                u2, d2, w2, ov2, sc2, c0 = (cl(x) if isinstance(x, torch.Tensor) else x for x in state)
+                o2, _ = jobs[p].block(u2, d2, w2, ov2, sc2, nsteps, 99, c0)
+                blk[p].append(float(o2[0]))
+        J.stochastic_reconfiguration(up, dn, weights, zetas[i : i + 1])
+        scalars[0] = 0.9 * scalars[0] + 0.1 * o[0]
+    out = {}
+    for p in ("fp32", "bf16x3"):
+        res = {}
+        for name, series in (("restart_paired_block", blk), ("energy_same_walkers", ene)):
+            d = 1e3 * (np.array(series[p]) - np.array(series["fp64"]))
+            res[name] = {"delta_mha": float(np.mean(d)), "stderr_mha": float(np.std(d, ddof=1) / math.sqrt(len(d))),
+                         "max_abs_mha": float(np.max(np.abs(d)))}
+        out[p + "_per_block"] = res
+    for j in jobs.values():
+        j.close()
+    return out
 
 
 def main():
