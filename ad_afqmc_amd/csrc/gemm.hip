@@ -386,6 +386,12 @@ int launch(const Gemm& g, hipStream_t s) {
 
 template <bool AC, bool BC, bool AT, bool BT>
 int dispatch_shape(const Gemm& g, hipStream_t s) {
+  // Large unbatched products with a wide N (the VHS product, N = norb^2): 16 waves
+  // of 2x2 blocks (64 x 256 tiles) halve the operand traffic per flop of 64 x 64
+  // tiles and keep 16 waves in flight per work-group (measured 1.73 ms vs 2.31 ms
+  // at W=512, G=1200, N=200).
+  if constexpr (!BC && !BT)  // (a complex B tile of this width would not fit in LDS)
+    if (g.batch == 1 && g.N >= 4096 && g.M >= 64 && g.K >= 64) return launch<AC, BC, AT, BT, 2, 8, 2, 2>(g, s);
   // panel tiles for N <= 64 (walker-column products)
   if (g.N <= 16) return launch<AC, BC, AT, BT, 4, 1, 1, 1>(g, s);
   if (g.N <= 32) return launch<AC, BC, AT, BT, 4, 1, 1, 2>(g, s);

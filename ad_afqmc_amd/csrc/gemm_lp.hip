@@ -373,6 +373,9 @@ int launch(const Gemm& g, hipStream_t s) {
 
 template <bool BF, bool AC, bool BC, bool AT, bool BT, typename TA, typename TB, typename TCo>
 int dispatch_shape(const Gemm& g, hipStream_t s) {
+  if constexpr (!BC && !BT)  // wide real-B tiles (the VHS product), as in the FP64 GEMM
+    if (g.batch == 1 && g.N >= 4096 && g.M >= 64 && g.K >= 64)
+      return launch<BF, AC, BC, AT, BT, TA, TB, TCo, 2, 8, 2, 2>(g, s);
   if (g.N <= 16) return launch<BF, AC, BC, AT, BT, TA, TB, TCo, 4, 1, 1, 1>(g, s);
   if (g.N <= 32) return launch<BF, AC, BC, AT, BT, TA, TB, TCo, 4, 1, 1, 2>(g, s);
   if (g.N <= 48) return launch<BF, AC, BC, AT, BT, TA, TB, TCo, 4, 1, 1, 3>(g, s);
