@@ -151,6 +151,10 @@ struct afq_job {
   float* rot_chol32[2] = {nullptr, nullptr}; // (G, nocc_s, N)
   float2* vhs32 = nullptr;                // (W, N, N)
   float2* t32[2] = {nullptr, nullptr};    // (W, N, wmax) Taylor terms
+  // split-BF16 x3 exchange operands (bf16 bit patterns): rot_chol (G, 2, Opad, Npad)
+  // per spin and one walker batch of G (W, 4, Opad, Npad)
+  uint16_t* rcs[2] = {nullptr, nullptr};
+  uint16_t* gsplit = nullptr;
   double* gemm_ws = nullptr;              // split-K workspace
   long gemm_ws_elems = 0;
 

@@ -76,6 +76,9 @@ void release(afq_job* j) {
   fr(j->vhs32);
   fr(j->t32[0]);
   fr(j->t32[1]);
+  fr(j->rcs[0]);
+  if (!j->rhf) fr(j->rcs[1]);
+  fr(j->gsplit);
   fr(j->dscratch);
   for (auto& p : j->prof) {
     for (auto& sp : p.spans) {
@@ -212,9 +215,11 @@ int energy_pipeline(afq_job* j, const void* up, const void* dn, double2* out, hi
     ProfScope ps(j, AFQ_K_EXCHANGE, st);
     double2* coul = j->coul + (long)s * j->W * j->G;
     double2* kpart = j->kpart + (long)s * j->W * j->kchunk_max;
-    if (j->lp)
-      AFQ_TRY(k_exchange_lp(j->rot_chol32[s], j->green[s], j->nocc[s], j->N, j->G, j->W, j->bf16x3, coul, kpart,
-                            &nch[s], st));
+    if (j->bf16x3) {
+      AFQ_TRY(k_split_bf16(reinterpret_cast<const double*>(j->green[s]), true, j->nocc[s], j->N, j->W, j->gsplit, st));
+      AFQ_TRY(k_exchange_bf16(j->rcs[s], j->gsplit, j->nocc[s], j->N, j->G, j->W, coul, kpart, &nch[s], st));
+    } else if (j->lp)
+      AFQ_TRY(k_exchange_lp(j->rot_chol32[s], j->green[s], j->nocc[s], j->N, j->G, j->W, coul, kpart, &nch[s], st));
     else
       AFQ_TRY(k_exchange(j->rot_chol[s], j->green[s], j->nocc[s], j->N, j->G, j->W, coul, kpart, &nch[s], st));
   }
@@ -602,6 +607,12 @@ int alloc_job(afq_job* j, bool dry, int64_t* total) {
     for (auto& t : j->t32) A_(t, W * N * wmax);
     A_(j->tbuf[0], W * N * wmax);
     A_(j->tbuf[2], W * N * wmax);
+    if (j->bf16x3) {
+      const long np = bf16_npad(j->N);
+      A_(j->rcs[0], G * 2 * bf16_opad(j->nocc[0]) * np);
+      if (!j->rhf) A_(j->rcs[1], G * 2 * bf16_opad(j->nocc[1]) * np);
+      A_(j->gsplit, W * 4 * bf16_opad((int)std::max(j->nocc[0], j->nocc[1])) * np);
+    }
   } else {
     A_(j->vhs, W * N * N);
     for (auto& t : j->tbuf) A_(t, W * N * wmax);
@@ -655,6 +666,7 @@ int afq_job_create(afq_job_t* out, const afq_job_desc* d) {
     j->rot_h1[1] = j->rot_h1[0];
     j->rot_chol[1] = j->rot_chol[0];
     j->rot_chol32[1] = j->rot_chol32[0];
+    j->rcs[1] = j->rcs[0];
   }
   *out = j;
   return AFQ_OK;
@@ -749,8 +761,10 @@ int afq_build_measurement_intermediates(afq_job_t j, void* stream) {
     AFQ_TRY(gemm_f64(r, st));
   }
   if (j->lp)
-    for (int s = 0; s < nsp; ++s)
+    for (int s = 0; s < nsp; ++s) {
       AFQ_TRY(k_to_f32(j->rot_chol[s], j->rot_chol32[s], (long)j->G * j->nocc[s] * N, st));
+      if (j->bf16x3) AFQ_TRY(k_split_bf16(j->rot_chol[s], false, j->nocc[s], N, j->G, j->rcs[s], st));
+    }
   j->have_meas = true;
   return AFQ_OK;
 }
@@ -866,6 +880,7 @@ int afq_set_intermediate(afq_job_t j, int which, int spin, const void* src, void
     case AFQ_INT_ROT_CHOL:
       j->have_rot_chol[spin] = true;
       if (j->lp) AFQ_TRY(k_to_f32(j->rot_chol[spin], j->rot_chol32[spin], G * O * N, S(stream)));
+      if (j->bf16x3) AFQ_TRY(k_split_bf16(j->rot_chol[spin], false, (int)O, (int)N, G, j->rcs[spin], S(stream)));
       break;
     case AFQ_INT_MF_SHIFTS: j->have_mf = true; break;
     case AFQ_INT_H0_PROP: j->have_h0p = true; break;

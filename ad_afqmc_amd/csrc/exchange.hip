@@ -11,14 +11,15 @@
 // Arithmetic of f:
 //   F64    : v_mfma_f64_16x16x4f64
 //   F32    : v_mfma_f32_16x16x4f32 (rot_chol read as FP32, G rounded to FP32)
-//   BF16X3 : v_mfma_f32_16x16x32_bf16 on hi/lo bf16 splits, hi*hi + hi*lo + lo*hi
+//   BF16X3 : v_mfma_f32_16x16x32_bf16 on pre-split hi/lo bf16 planes,
+//            hi*hi + hi*lo + lo*hi (exchange_bf16_kernel)
 //
 // Fragments go straight from global memory to registers in MFMA layout.  The
 // K dimension (orbital index p) is permuted within each chunk so that every
 // lane reads consecutive elements of one row: for the 4-deep MFMAs lane l of
 // slab s reads p = c + KS*(l/16) + s (KS slabs per chunk); the 32-deep bf16
-// MFMA already takes 8 consecutive k per lane.  The next chunk is loaded while
-// the current one is multiplied.
+// MFMA takes 8 consecutive k per lane.  In the F64/F32 kernel the next chunk
+// is loaded while the current one is multiplied.
 //
 // Work-group order is XCD-aware: work-groups are handed to the eight XCDs
 // round-robin, so the logical index L = xcd * (T/8) + (bid/8) gives each XCD a
@@ -33,7 +34,7 @@ typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
 namespace afq {
 namespace {
 
-enum Mode { F64 = 0, F32 = 1, BF16X3 = 2 };
+enum Mode { F64 = 0, F32 = 1 };
 
 __device__ inline double wave_sum(double v) {
 #pragma unroll
@@ -41,29 +42,11 @@ __device__ inline double wave_sum(double v) {
   return v;
 }
 
-struct Bf8 {
-  v8bf hi, lo;
-};
-__device__ inline void split8(const float (&x)[8], Bf8& o) {
-#pragma unroll
-  for (int t = 0; t < 8; ++t) {
-    const __bf16 h = (__bf16)x[t];
-    o.hi[t] = h;
-    o.lo[t] = (__bf16)(x[t] - (float)h);
-  }
-}
-__device__ inline v4f mma3(const Bf8& a, const Bf8& b, v4f c) {
-  c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a.hi, b.hi, c, 0, 0, 0);
-  c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a.hi, b.lo, c, 0, 0, 0);
-  c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a.lo, b.hi, c, 0, 0, 0);
-  return c;
-}
-
 // Per-mode fragment of one chunk: NB row blocks of A (rot_chol rows) and of
 // B (Green's function rows), KE elements per lane.
 template <int MODE, int NB, int KS>
 struct Frag {
-  static constexpr int KE = MODE == BF16X3 ? 8 : KS;
+  static constexpr int KE = KS;
   using R = std::conditional_t<MODE == F64, double, float>;
   R a[NB][KE];
   R br[NB][KE];
@@ -139,37 +122,20 @@ __global__ __launch_bounds__(256, (NB <= 3 ? 2 : 1)) void exchange_kernel(const 
   load_frag<MODE, NB, KS, TR>(cur, A, B, O, N, 0, lane);
   for (int c = 0; c < N; c += CW) {
     if (c + CW < N) load_frag<MODE, NB, KS, TR>(nxt, A, B, O, N, c + CW, lane);
-    if constexpr (MODE == BF16X3) {
-      Bf8 fa[NB], fr[NB], fi[NB];
 #pragma unroll
-      for (int bi = 0; bi < NB; ++bi) {
-        split8(cur.a[bi], fa[bi]);
-        split8(cur.br[bi], fr[bi]);
-        split8(cur.bi[bi], fi[bi]);
-      }
+    for (int s = 0; s < KS; ++s)
 #pragma unroll
       for (int bi = 0; bi < NB; ++bi)
 #pragma unroll
         for (int bj = 0; bj < NB; ++bj) {
-          accR[bi][bj] = mma3(fa[bi], fr[bj], accR[bi][bj]);
-          accI[bi][bj] = mma3(fa[bi], fi[bj], accI[bi][bj]);
-        }
-    } else {
-#pragma unroll
-      for (int s = 0; s < KS; ++s)
-#pragma unroll
-        for (int bi = 0; bi < NB; ++bi)
-#pragma unroll
-          for (int bj = 0; bj < NB; ++bj) {
-            if constexpr (MODE == F64) {
-              accR[bi][bj] = __builtin_amdgcn_mfma_f64_16x16x4f64(cur.a[bi][s], cur.br[bj][s], accR[bi][bj], 0, 0, 0);
-              accI[bi][bj] = __builtin_amdgcn_mfma_f64_16x16x4f64(cur.a[bi][s], cur.bi[bj][s], accI[bi][bj], 0, 0, 0);
-            } else {
-              accR[bi][bj] = __builtin_amdgcn_mfma_f32_16x16x4f32(cur.a[bi][s], cur.br[bj][s], accR[bi][bj], 0, 0, 0);
-              accI[bi][bj] = __builtin_amdgcn_mfma_f32_16x16x4f32(cur.a[bi][s], cur.bi[bj][s], accI[bi][bj], 0, 0, 0);
-            }
+          if constexpr (MODE == F64) {
+            accR[bi][bj] = __builtin_amdgcn_mfma_f64_16x16x4f64(cur.a[bi][s], cur.br[bj][s], accR[bi][bj], 0, 0, 0);
+            accI[bi][bj] = __builtin_amdgcn_mfma_f64_16x16x4f64(cur.a[bi][s], cur.bi[bj][s], accI[bi][bj], 0, 0, 0);
+          } else {
+            accR[bi][bj] = __builtin_amdgcn_mfma_f32_16x16x4f32(cur.a[bi][s], cur.br[bj][s], accR[bi][bj], 0, 0, 0);
+            accI[bi][bj] = __builtin_amdgcn_mfma_f32_16x16x4f32(cur.a[bi][s], cur.bi[bj][s], accI[bi][bj], 0, 0, 0);
           }
-    }
+        }
     if (c + CW < N) cur = nxt;
   }
 
@@ -225,6 +191,164 @@ __global__ __launch_bounds__(256, (NB <= 3 ? 2 : 1)) void exchange_kernel(const 
   }
 }
 
+// Split-BF16 x3 exchange on pre-split operands.  rot_chol is held as bf16
+// (hi, lo) planes and each walker's Green's function as (re_hi, re_lo, im_hi,
+// im_lo) planes, rows padded to 16*NB and columns to a multiple of 32 with
+// zeros, so every lane loads its 8 consecutive k as one 16-byte vector and no
+// bounds masks or conversions remain in the loop.  The three products
+// hi*hi, hi*lo, lo*hi are issued as three passes over all NB x NB blocks, so
+// consecutive MFMAs never depend on each other.
+template <int NB>
+__global__ __launch_bounds__(256, 2) void exchange_bf16_kernel(const v8bf* __restrict__ rcs, const v8bf* __restrict__ gs,
+                                                               int O, int Opad, int Npad, int Gn, int W, int nchunk,
+                                                               int cpg, long T8, double2* __restrict__ coul,
+                                                               double2* __restrict__ kpart) {
+  __shared__ double2 scr[4][16 * 17];
+  __shared__ double2 kred[4];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const long bid = blockIdx.x;
+  const long L = (bid & 7) * T8 + (bid >> 3);
+  const long per_group = (long)W * cpg;
+  const long grp = L / per_group;
+  const long rem = L - grp * per_group;
+  const int w = (int)(rem / cpg);
+  const int chunk = (int)(grp * cpg + rem % cpg);
+  if (chunk >= nchunk) return;  // uniform for the whole work-group
+  const int g = chunk * 4 + wave;
+  const bool valid = g < Gn;
+  const int gg = valid ? g : Gn - 1;
+
+  // planes in units of v8bf (8 bf16): a plane is Opad x Npad
+  const long plane = (long)Opad * Npad / 8;
+  const int rowv = Npad / 8;
+  const int r16 = lane & 15, q4 = lane >> 4;
+  const v8bf* Ah = rcs + (long)gg * 2 * plane + r16 * rowv + q4;
+  const v8bf* Al = Ah + plane;
+  const v8bf* Brh = gs + (long)w * 4 * plane + r16 * rowv + q4;
+  const v8bf* Brl = Brh + plane;
+  const v8bf* Bih = Brh + 2 * plane;
+  const v8bf* Bil = Brh + 3 * plane;
+
+  v4f accR[NB][NB], accI[NB][NB];
+#pragma unroll
+  for (int i = 0; i < NB; ++i)
+#pragma unroll
+    for (int j = 0; j < NB; ++j) {
+      accR[i][j] = v4f{0.f, 0.f, 0.f, 0.f};
+      accI[i][j] = v4f{0.f, 0.f, 0.f, 0.f};
+    }
+  for (int kv = 0; kv < rowv; kv += 4) {
+    v8bf ah[NB], br[NB], bi[NB];
+#pragma unroll
+    for (int b = 0; b < NB; ++b) {
+      const long o = (long)b * 16 * rowv + kv;
+      ah[b] = Ah[o];
+      br[b] = Brh[o];
+      bi[b] = Bih[o];
+    }
+#pragma unroll
+    for (int i = 0; i < NB; ++i)
+#pragma unroll
+      for (int j = 0; j < NB; ++j) {
+        accR[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], br[j], accR[i][j], 0, 0, 0);
+        accI[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bi[j], accI[i][j], 0, 0, 0);
+      }
+    v8bf al[NB];
+#pragma unroll
+    for (int b = 0; b < NB; ++b) al[b] = Al[(long)b * 16 * rowv + kv];
+#pragma unroll
+    for (int i = 0; i < NB; ++i)
+#pragma unroll
+      for (int j = 0; j < NB; ++j) {
+        accR[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[i], br[j], accR[i][j], 0, 0, 0);
+        accI[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[i], bi[j], accI[i][j], 0, 0, 0);
+      }
+#pragma unroll
+    for (int b = 0; b < NB; ++b) {
+      const long o = (long)b * 16 * rowv + kv;
+      br[b] = Brl[o];
+      bi[b] = Bil[o];
+    }
+#pragma unroll
+    for (int i = 0; i < NB; ++i)
+#pragma unroll
+      for (int j = 0; j < NB; ++j) {
+        accR[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], br[j], accR[i][j], 0, 0, 0);
+        accI[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bi[j], accI[i][j], 0, 0, 0);
+      }
+  }
+
+  // accumulator element v of lane l sits at (4(l/16) + v, l%16)
+  const int cc = lane & 15;
+  double kr = 0.0, ki = 0.0, cr = 0.0, ci = 0.0;
+  double2* S = scr[wave];
+#pragma unroll
+  for (int bi = 0; bi < NB; ++bi)
+#pragma unroll
+    for (int bj = bi; bj < NB; ++bj) {
+#pragma unroll
+      for (int v = 0; v < 4; ++v) S[(4 * q4 + v) * 17 + cc] = double2{(double)accR[bj][bi][v], (double)accI[bj][bi][v]};
+      __syncthreads();
+      const double wgt = (bi == bj) ? 1.0 : 2.0;
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int r = 4 * q4 + v;
+        const int gi = bi * 16 + r, gj = bj * 16 + cc;
+        if (gi < O && gj < O) {
+          const double xr = accR[bi][bj][v], xi = accI[bi][bj][v];
+          const double2 y = S[cc * 17 + r];
+          kr += wgt * (xr * y.x - xi * y.y);
+          ki += wgt * (xr * y.y + xi * y.x);
+          if (bi == bj && r == cc) {
+            cr += xr;
+            ci += xi;
+          }
+        }
+      }
+      __syncthreads();
+    }
+  kr = wave_sum(kr);
+  ki = wave_sum(ki);
+  cr = wave_sum(cr);
+  ci = wave_sum(ci);
+  if (lane == 0) {
+    if (valid) coul[(long)w * Gn + g] = double2{cr, ci};
+    kred[wave] = valid ? double2{kr, ki} : double2{0.0, 0.0};
+  }
+  __syncthreads();
+  if (tid == 0) {
+    double2 s = {0.0, 0.0};
+    for (int i = 0; i < 4; ++i) {
+      s.x += kred[i].x;
+      s.y += kred[i].y;
+    }
+    kpart[(long)w * nchunk + chunk] = s;
+  }
+}
+
+// out[r][plane][o][k] (o < Opad, k < Npad, zero padded) from src[r][o][k]
+// (o < O, k < N; real or interleaved complex double): planes hi, lo of
+// bf16(float(x)) and bf16(float(x) - hi), for the real then the imaginary part.
+__global__ void split_bf16_kernel(const double* __restrict__ src, int cplx, int O, int N, int Opad, int Npad,
+                                  long rows, __bf16* __restrict__ out) {
+  const long idx = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  const long per = (long)Opad * Npad;
+  if (idx >= rows * per) return;
+  const long r = idx / per;
+  const int o = (int)((idx % per) / Npad), k = (int)(idx % Npad);
+  const bool in = o < O && k < N;
+  const int np = cplx ? 4 : 2;
+  __bf16* base = out + r * np * per + (long)o * Npad + k;
+  for (int c = 0; c < (cplx ? 2 : 1); ++c) {
+    double x = 0.0;
+    if (in) x = cplx ? src[2 * ((r * O + o) * (long)N + k) + c] : src[(r * O + o) * (long)N + k];
+    const float xf = (float)x;
+    const __bf16 h = (__bf16)xf;
+    base[(2 * c) * per] = h;
+    base[(2 * c + 1) * per] = (__bf16)(xf - (float)h);
+  }
+}
+
 template <int MODE, int NB, int KS, typename TR>
 int launch_nb(const TR* rc, const double2* Gm, int O, int N, int Gn, int W, int nchunk, double2* coul, double2* kpart,
               hipStream_t s) {
@@ -266,13 +390,52 @@ int k_exchange(const double* rc, const double2* Gm, int O, int N, int Gn, int W,
   return launch_mode<F64, double>(rc, Gm, O, N, Gn, W, nchunk, coul, kpart, s);
 }
 
-int k_exchange_lp(const float* rc, const double2* Gm, int O, int N, int Gn, int W, bool bf16x3, double2* coul,
-                  double2* kpart, int* nchunk_out, hipStream_t s) {
+int k_exchange_lp(const float* rc, const double2* Gm, int O, int N, int Gn, int W, double2* coul, double2* kpart,
+                  int* nchunk_out, hipStream_t s) {
   if (O < 1 || O > 64) return fail(AFQ_E_UNSUPPORTED, "exchange kernel supports 1 <= nocc <= 64");
   const int nchunk = exchange_nchunk(Gn);
   *nchunk_out = nchunk;
-  if (bf16x3) return launch_mode<BF16X3, float>(rc, Gm, O, N, Gn, W, nchunk, coul, kpart, s);
   return launch_mode<F32, float>(rc, Gm, O, N, Gn, W, nchunk, coul, kpart, s);
+}
+
+int bf16_opad(int O) { return 16 * ((O + 15) / 16); }
+int bf16_npad(int N) { return 32 * ((N + 31) / 32); }
+
+int k_split_bf16(const double* src, bool cplx, int O, int N, long rows, uint16_t* out, hipStream_t s) {
+  const int Opad = bf16_opad(O), Npad = bf16_npad(N);
+  const long n = rows * Opad * (long)Npad;
+  if (n <= 0) return AFQ_OK;
+  hipLaunchKernelGGL(split_bf16_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, src, cplx ? 1 : 0, O, N,
+                     Opad, Npad, rows, reinterpret_cast<__bf16*>(out));
+  return launch_status("split_bf16_kernel");
+}
+
+int k_exchange_bf16(const uint16_t* rcs, const uint16_t* gs, int O, int N, int Gn, int W, double2* coul,
+                    double2* kpart, int* nchunk_out, hipStream_t s) {
+  if (O < 1 || O > 64) return fail(AFQ_E_UNSUPPORTED, "exchange kernel supports 1 <= nocc <= 64");
+  const int nchunk = exchange_nchunk(Gn);
+  *nchunk_out = nchunk;
+  const int Opad = bf16_opad(O), Npad = bf16_npad(N);
+  const long chunk_bytes = 4L * 2 * Opad * Npad * 2;
+  const int cpg = (int)std::max(1L, std::min((long)nchunk, (2560L * 1024) / std::max(1L, chunk_bytes)));
+  const long ngroups = (nchunk + cpg - 1) / cpg;
+  const long T = ngroups * cpg * (long)W;
+  const long T8 = (T + 7) / 8;
+  const long grid = T8 * 8;
+  if (grid > 0x7fffffffL) return fail(AFQ_E_UNSUPPORTED, "exchange grid too large");
+  const v8bf* a = reinterpret_cast<const v8bf*>(rcs);
+  const v8bf* b = reinterpret_cast<const v8bf*>(gs);
+#define EXB(NB)                                                                                                  \
+  hipLaunchKernelGGL(exchange_bf16_kernel<NB>, dim3((unsigned)grid), dim3(256), 0, s, a, b, O, Opad, Npad, Gn, W, \
+                     nchunk, cpg, T8, coul, kpart)
+  switch (Opad / 16) {
+    case 1: EXB(1); break;
+    case 2: EXB(2); break;
+    case 3: EXB(3); break;
+    default: EXB(4); break;
+  }
+#undef EXB
+  return launch_status("exchange_bf16_kernel");
 }
 
 }  // namespace afq

@@ -19,8 +19,15 @@ int k_ene1(const double2* G0, const double* rh0, int n0, const double2* G1, cons
 int exchange_nchunk(int Gn);
 int k_exchange(const double* rc, const double2* Gm, int O, int N, int Gn, int W, double2* coul, double2* kpart,
                int* nchunk_out, hipStream_t s);
-int k_exchange_lp(const float* rc, const double2* Gm, int O, int N, int Gn, int W, bool bf16x3, double2* coul,
-                  double2* kpart, int* nchunk_out, hipStream_t s);
+int k_exchange_lp(const float* rc, const double2* Gm, int O, int N, int Gn, int W, double2* coul, double2* kpart,
+                  int* nchunk_out, hipStream_t s);
+// Split-BF16 operands of the exchange: rows x (O, N) real (2 planes) or complex
+// (4 planes) doubles -> bf16 (hi, lo) planes of bf16_opad(O) x bf16_npad(N), zero padded.
+int bf16_opad(int O);
+int bf16_npad(int N);
+int k_split_bf16(const double* src, bool cplx, int O, int N, long rows, uint16_t* out, hipStream_t s);
+int k_exchange_bf16(const uint16_t* rcs, const uint16_t* gs, int O, int N, int Gn, int W, double2* coul,
+                    double2* kpart, int* nchunk_out, hipStream_t s);
 int k_to_f32(const double* src, float* dst, long n, hipStream_t s);
 int k_energy_finalize(const double2* ene1, const double2* c0, const double2* c1, const double2* k0, const double2* k1,
                       int Gn, int nch0, int nch1, double h0, double2* out, int W, hipStream_t s);

@@ -368,7 +368,9 @@ def main():
     c_cols = Oa
     per_launch = launch_flops(dominant, N, c_cols, occ, G, W)
     avg_s = dom_ms / max(dom_n, 1) / 1e3
-    peak = PEAKS["fp64"] if args.precision == "fp64" else PEAKS["fp32"]
+    # bf16x3 does three bf16 MFMA products per FP32-equivalent product: its
+    # ceiling is a third of the dense bf16 peak
+    peak = {"fp64": PEAKS["fp64"], "fp32": PEAKS["fp32"], "bf16x3": PEAKS["bf16"] / 3.0}[args.precision]
     roof = None
     if per_launch is not None and avg_s > 0:
         ach = per_launch / avg_s
