@@ -155,6 +155,10 @@ struct afq_job {
   // per spin and one walker batch of G (W, 4, Opad, Npad)
   uint16_t* rcs[2] = {nullptr, nullptr};
   uint16_t* gsplit = nullptr;
+  // split-BF16 x3 VHS operands: L^T (NNr, 2, Gpad) and the shifted fields
+  // (Wr, 4, Gpad); rows padded to 16, k to 32
+  uint16_t* lt_split = nullptr;
+  uint16_t* xs_split = nullptr;
   double* gemm_ws = nullptr;              // split-K workspace
   long gemm_ws_elems = 0;
 

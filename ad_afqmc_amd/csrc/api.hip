@@ -5,6 +5,7 @@
 #include <cstring>
 #include <string>
 
+#include "gemm_bf3.h"
 #include "kernels.h"
 #include "pipeline.h"
 
@@ -79,6 +80,8 @@ void release(afq_job* j) {
   fr(j->rcs[0]);
   if (!j->rhf) fr(j->rcs[1]);
   fr(j->gsplit);
+  fr(j->lt_split);
+  fr(j->xs_split);
   fr(j->dscratch);
   for (auto& p : j->prof) {
     for (auto& sp : p.spans) {
@@ -249,7 +252,31 @@ int trotprop_lp(afq_job* j, void* up, void* dn, const double2* xs, hipStream_t s
     g.ldc = NN;
     g.alpha = double2{0.0, std::sqrt(j->d.dt)};
     ProfScope ps(j, AFQ_K_VHS, st);
-    AFQ_TRY(gemm_lp(LpKind::VHS, j->bf16x3, g, st));
+    if (j->bf16x3) {
+      // on pre-split operands: x~ split per step, L^T split once in set_hamiltonian
+      const int gpad = bf16_npad(j->G);
+      AFQ_TRY(k_split_rows_bf16(reinterpret_cast<const double*>(xs), true, W, j->G, j->G, gpad, 4L * gpad, gpad,
+                                j->xs_split, st));
+      Bf3 v;
+      v.M = W;
+      v.N = (int)NN;
+      v.K = gpad;
+      v.a_cplx = true;
+      v.A = j->xs_split;
+      v.lda = 4L * gpad;
+      v.pA = gpad;
+      v.a_rows = bf16_opad(j->Wcap);
+      v.B = j->lt_split;
+      v.ldb = 2L * gpad;
+      v.pB = gpad;
+      v.b_rows = (int)(16 * ((NN + 15) / 16));
+      v.alpha = g.alpha;
+      v.C = j->vhs32;
+      v.ldc = NN;
+      AFQ_TRY(gemm_bf3(v, st));
+    } else {
+      AFQ_TRY(gemm_lp(LpKind::VHS, false, g, st));
+    }
   }
   for (int s = 0; s < j->nspin_prop; ++s) {
     double2* Wp = static_cast<double2*>(s == 0 ? up : dn);
@@ -598,21 +625,27 @@ int alloc_job(afq_job* j, bool dry, int64_t* total) {
   A_(j->xs, W * G);
   A_(j->terms, 2 * W);
   if (j->lp) {
-    // FP32 copies of L and rot_chol, complex64 VHS and Taylor terms; the
-    // complex128 Taylor buffers hold exp(-dt/2 h1) W and the running sum
-    A_(j->chol32, G * N * N);
+    // complex64 VHS and Taylor terms; the complex128 Taylor buffers hold
+    // exp(-dt/2 h1) W and the running sum.  FP32: FP32 copies of L and rot_chol.
+    // BF16x3: L^T and the shifted fields as split-BF16 planes for the VHS product,
+    // rot_chol as split planes (exchange) and FP32 (force bias), and one walker
+    // batch of split Green's functions.
+    if (j->bf16x3) {
+      const long np = bf16_npad(j->N), gp = bf16_npad(j->G);
+      A_(j->rcs[0], G * 2 * bf16_opad(j->nocc[0]) * np);
+      if (!j->rhf) A_(j->rcs[1], G * 2 * bf16_opad(j->nocc[1]) * np);
+      A_(j->gsplit, W * 4 * bf16_opad((int)std::max(j->nocc[0], j->nocc[1])) * np);
+      A_(j->lt_split, 16 * ((N * N + 15) / 16) * 2 * gp);
+      A_(j->xs_split, bf16_opad((int)W) * 4 * gp);
+    } else {
+      A_(j->chol32, G * N * N);
+    }
     A_(j->rot_chol32[0], G * j->nocc[0] * N);
     if (!j->rhf) A_(j->rot_chol32[1], G * j->nocc[1] * N);
     A_(j->vhs32, W * N * N);
     for (auto& t : j->t32) A_(t, W * N * wmax);
     A_(j->tbuf[0], W * N * wmax);
     A_(j->tbuf[2], W * N * wmax);
-    if (j->bf16x3) {
-      const long np = bf16_npad(j->N);
-      A_(j->rcs[0], G * 2 * bf16_opad(j->nocc[0]) * np);
-      if (!j->rhf) A_(j->rcs[1], G * 2 * bf16_opad(j->nocc[1]) * np);
-      A_(j->gsplit, W * 4 * bf16_opad((int)std::max(j->nocc[0], j->nocc[1])) * np);
-    }
   } else {
     A_(j->vhs, W * N * N);
     for (auto& t : j->tbuf) A_(t, W * N * wmax);
@@ -695,7 +728,12 @@ int afq_job_set_hamiltonian(afq_job_t j, double h0, const double* h1, const doub
   const long N = j->N;
   AFQ_HIP(hipMemcpyAsync(j->h1, h1, sizeof(double) * 2 * N * N, hipMemcpyDefault, S(stream)));
   AFQ_HIP(hipMemcpyAsync(j->chol, chol, sizeof(double) * j->G * N * N, hipMemcpyDefault, S(stream)));
-  if (j->lp) AFQ_TRY(k_to_f32(j->chol, j->chol32, (long)j->G * N * N, S(stream)));
+  if (j->bf16x3) {
+    const long gp = bf16_npad(j->G);
+    AFQ_TRY(k_split_chol_t_bf16(j->chol, j->G, N * N, (int)gp, 2 * gp, gp, j->lt_split, S(stream)));
+  } else if (j->lp) {
+    AFQ_TRY(k_to_f32(j->chol, j->chol32, (long)j->G * N * N, S(stream)));
+  }
   j->have_ham = true;
   j->have_meas = j->have_prop = false;
   return AFQ_OK;

@@ -1,0 +1,32 @@
+// Split-BF16 x3 VHS product on pre-split operands (gemm_bf3.hip).
+#pragma once
+#include "afq_internal.h"
+
+namespace afq {
+
+// C[b] (M x N, complex64) = alpha * A[b] (M x K, complex) . B[b]^T (N x K, real)
+// on bf16 (hi, lo) planes.  Offsets and strides count bf16 elements (operands)
+// or output elements.
+struct Bf3 {
+  int M = 0, N = 0, K = 0, batch = 1;  // K: padded depth, a multiple of 32
+  bool a_cplx = false, b_cplx = false;
+  const void* A = nullptr;
+  long lda = 0, pA = 0, sA = 0;  // row, plane and batch strides
+  int a_rows = 0;                // rows that may be read (>= M rounded up to 16, or clamped)
+  const void* B = nullptr;
+  long ldb = 0, pB = 0, sB = 0;
+  int b_rows = 0;
+  double2 alpha = {1.0, 0.0};
+  void* C = nullptr;
+  long ldc = 0, sC = 0;
+};
+int gemm_bf3(const Bf3& p, hipStream_t s);
+
+// Row split: dst[r*ldd + q*pd + k] for k < kpad from src rows of K doubles
+// (complex when cplx: four planes, else two), zero for k >= K.
+int k_split_rows_bf16(const double* src, bool cplx, long rows, int K, long lds, int kpad, long ldd, long pd,
+                      uint16_t* dst, hipStream_t s);
+// Cholesky vectors L (G x NN) -> dst[n*ldd + q*pd + g] for g < gpad (zero past G).
+int k_split_chol_t_bf16(const double* L, int G, long NN, int gpad, long ldd, long pd, uint16_t* dst, hipStream_t s);
+
+}  // namespace afq
