@@ -1020,29 +1020,26 @@ int afq_normal_fill(double* out, int64_t n, uint64_t seed, uint64_t counter, voi
   return k_normal_fill(out, n, seed, counter, S(stream));
 }
 
-int afq_block(afq_job_t j, void* up, void* dn, double* weights, void* overlaps, double* scalars, int n_prop_steps,
-              uint64_t seed, uint64_t* rng_counter, double* block_out, void* stream) {
-  AFQ_TRY(require(j, true, true));
-  AFQ_TRY(check_walkers(j, up, dn));
-  if (!weights || !overlaps || !scalars || !rng_counter || !block_out) return fail(AFQ_E_INVALID, "NULL argument");
-  if (n_prop_steps < 1) return fail(AFQ_E_INVALID, "n_prop_steps must be >= 1");
-  hipStream_t st = S(stream);
+}  // extern "C"
+
+namespace afq {
+
+int reserve_fields(afq_job* j, long need, hipStream_t st) {
+  if (need <= j->fields_cap) return AFQ_OK;
+  AFQ_HIP(hipStreamSynchronize(st));
+  if (j->fields) (void)hipFree(j->fields);
+  j->fields = nullptr;
+  j->fields_cap = 0;
+  AFQ_TRY(dalloc_n(&j->fields, (size_t)need));
+  j->fields_cap = need;
+  return AFQ_OK;
+}
+
+// Steps over the fields already in j->fields, then QR, overlap, energy and the
+// block bookkeeping (sampling.py:52-94).
+int block_body(afq_job* j, void* up, void* dn, double* weights, double2* ov, double* scalars, int n_prop_steps,
+               double* block_out, hipStream_t st) {
   const long per_step = (long)j->W * j->G;
-  const long need = per_step * n_prop_steps;
-  if (need > j->fields_cap) {
-    AFQ_HIP(hipStreamSynchronize(st));
-    if (j->fields) (void)hipFree(j->fields);
-    j->fields = nullptr;
-    j->fields_cap = 0;
-    AFQ_TRY(dalloc_n(&j->fields, (size_t)need));
-    j->fields_cap = need;
-  }
-  {
-    ProfScope ps(j, AFQ_K_RNG, st);
-    AFQ_TRY(k_normal_fill(j->fields, need, seed, *rng_counter, st));
-  }
-  *rng_counter += 1;
-  double2* ov = static_cast<double2*>(overlaps);
   for (int i = 0; i < n_prop_steps; ++i)
     AFQ_TRY(propagate(j, up, dn, j->fields + i * per_step, weights, ov, scalars, st));
   AFQ_TRY(orthonormalize(j, up, dn, st));
@@ -1051,6 +1048,72 @@ int afq_block(afq_job_t j, void* up, void* dn, double* weights, void* overlaps, 
   AFQ_TRY(energy_pipeline(j, up, dn, j->energies, st));
   ProfScope ps(j, AFQ_K_ENERGY_MISC, st);
   return k_block_finalize(j->energies, weights, scalars, std::sqrt(2.0 / j->d.dt), j->W, block_out, st);
+}
+
+// Comb resampling with a device-resident offset, then the overlaps of the new population.
+int sr_and_overlap(afq_job* j, void* up, void* dn, double* weights, const double* zeta, double2* ov, hipStream_t st) {
+  {
+    ProfScope ps(j, AFQ_K_SR, st);
+    AFQ_TRY(k_sr_index(weights, zeta, j->W, j->cum, j->sr_idx, st));
+    for (int s = 0; s < j->nspin_prop; ++s) {
+      double2* Wp = static_cast<double2*>(s == 0 ? up : dn);
+      const int c = j->restricted ? j->wcols : j->nocc[s];
+      const long per = (long)j->N * c;
+      AFQ_TRY(k_gather_walkers(Wp, j->wgather, j->sr_idx, per, j->W, st));
+      AFQ_HIP(hipMemcpyAsync(Wp, j->wgather, sizeof(double2) * per * j->W, hipMemcpyDeviceToDevice, st));
+    }
+  }
+  AFQ_TRY(green_pipeline(j, up, dn, false, false, st));
+  return overlap_from_dets(j, ov, st);
+}
+
+}  // namespace afq
+
+extern "C" {
+
+int afq_block(afq_job_t j, void* up, void* dn, double* weights, void* overlaps, double* scalars, int n_prop_steps,
+              uint64_t seed, uint64_t* rng_counter, double* block_out, void* stream) {
+  AFQ_TRY(require(j, true, true));
+  AFQ_TRY(check_walkers(j, up, dn));
+  if (!weights || !overlaps || !scalars || !rng_counter || !block_out) return fail(AFQ_E_INVALID, "NULL argument");
+  if (n_prop_steps < 1) return fail(AFQ_E_INVALID, "n_prop_steps must be >= 1");
+  hipStream_t st = S(stream);
+  const long need = (long)j->W * j->G * n_prop_steps;
+  AFQ_TRY(reserve_fields(j, need, st));
+  {
+    ProfScope ps(j, AFQ_K_RNG, st);
+    AFQ_TRY(k_normal_fill(j->fields, need, seed, *rng_counter, st));
+  }
+  *rng_counter += 1;
+  return block_body(j, up, dn, weights, static_cast<double2*>(overlaps), scalars, n_prop_steps, block_out, st);
+}
+
+int afq_job_reserve_fields(afq_job_t j, int32_t n_prop_steps, void* stream) {
+  if (!j) return fail(AFQ_E_INVALID, "job is NULL");
+  if (n_prop_steps < 1) return fail(AFQ_E_INVALID, "n_prop_steps must be >= 1");
+  return reserve_fields(j, (long)j->Wcap * j->G * n_prop_steps, S(stream));
+}
+
+int afq_sr_block(afq_job_t j, void* up, void* dn, double* weights, void* overlaps, double* scalars, int n_prop_steps,
+                 uint64_t seed, uint64_t* counter_dev, double* block_out, void* stream) {
+  AFQ_TRY(require(j, true, true));
+  AFQ_TRY(check_walkers(j, up, dn));
+  if (!weights || !overlaps || !scalars || !counter_dev || !block_out) return fail(AFQ_E_INVALID, "NULL argument");
+  if (n_prop_steps < 1) return fail(AFQ_E_INVALID, "n_prop_steps must be >= 1");
+  hipStream_t st = S(stream);
+  const long need = (long)j->W * j->G * n_prop_steps;
+  if (need > j->fields_cap)
+    return fail(AFQ_E_STATE, "afq_sr_block: call afq_job_reserve_fields for this n_prop_steps first");
+  {
+    ProfScope ps(j, AFQ_K_RNG, st);
+    AFQ_TRY(k_normal_fill_dev(j->fields, need, seed, counter_dev, st));
+  }
+  double2* ov = static_cast<double2*>(overlaps);
+  AFQ_TRY(block_body(j, up, dn, weights, ov, scalars, n_prop_steps, block_out, st));
+  double* zeta = j->dscratch + 4;
+  AFQ_TRY(k_sr_zeta(zeta, seed, counter_dev, st));
+  AFQ_TRY(sr_and_overlap(j, up, dn, weights, zeta, ov, st));
+  return k_counter_incr(counter_dev, st);
 }
 
 int afq_job_profile(afq_job_t j, int cls, int enable) {

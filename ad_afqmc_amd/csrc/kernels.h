@@ -41,5 +41,10 @@ int k_block_finalize(const double2* energies, const double* weights, double* sca
 int k_sr_index(double* weights, const double* zeta, int W, double* cum, int32_t* idx, hipStream_t s);
 int k_gather_walkers(const double2* src, double2* dst, const int32_t* idx, long per, int W, hipStream_t s);
 int k_normal_fill(double* out, long n, uint64_t seed, uint64_t counter, hipStream_t s);
+// The same stream with the counter read from device memory, the SR offset of
+// that counter, and the counter increment (graph-capturable block loop).
+int k_normal_fill_dev(double* out, long n, uint64_t seed, const uint64_t* counter_dev, hipStream_t s);
+int k_sr_zeta(double* zeta, uint64_t seed, const uint64_t* counter_dev, hipStream_t s);
+int k_counter_incr(uint64_t* counter_dev, hipStream_t s);
 
 }  // namespace afq

@@ -342,9 +342,11 @@ __device__ inline uint4 philox4x32_10(uint4 c, uint2 k) {
   return c;
 }
 
-__global__ void normal_fill_kernel(double* out, long n, uint64_t seed, uint64_t counter) {
+// counter_dev (when not null) supplies the stream counter from device memory.
+__global__ void normal_fill_kernel(double* out, long n, uint64_t seed, uint64_t counter, const uint64_t* counter_dev) {
   const long pair = blockIdx.x * (long)blockDim.x + threadIdx.x;
   if (2 * pair >= n) return;
+  if (counter_dev) counter = *counter_dev;
   const uint4 r = philox4x32_10(uint4{(uint32_t)pair, (uint32_t)((uint64_t)pair >> 32), (uint32_t)counter,
                                       (uint32_t)(counter >> 32)},
                                 uint2{(uint32_t)seed, (uint32_t)(seed >> 32)});
@@ -358,6 +360,22 @@ __global__ void normal_fill_kernel(double* out, long n, uint64_t seed, uint64_t 
   sincospi(2.0 * u2, &s, &c);
   out[2 * pair] = rad * c;
   if (2 * pair + 1 < n) out[2 * pair + 1] = rad * s;
+}
+
+// SR offset zeta ~ U[0, 1) of block `*counter_dev`: Philox on a key distinct from
+// the field stream's, then the counter advances to the next block.
+__global__ void sr_zeta_kernel(double* zeta, uint64_t seed, const uint64_t* counter_dev) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  const uint64_t c = *counter_dev;
+  const uint64_t key = seed ^ 0x9E3779B97F4A7C15ull;
+  const uint4 r = philox4x32_10(uint4{0xFFFFFFFFu, 0xFFFFFFFFu, (uint32_t)c, (uint32_t)(c >> 32)},
+                                uint2{(uint32_t)key, (uint32_t)(key >> 32)});
+  const uint64_t a = ((uint64_t)r.x << 21) ^ ((uint64_t)r.y >> 11);
+  *zeta = (double)(a & ((1ull << 53) - 1)) * (1.0 / 9007199254740992.0);
+}
+
+__global__ void counter_incr_kernel(uint64_t* counter_dev) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) *counter_dev += 1;
 }
 
 __global__ void to_f32_kernel(const double* src, float* dst, long n) {
@@ -449,8 +467,24 @@ int k_to_f32(const double* src, float* dst, long n, hipStream_t s) {
 int k_normal_fill(double* out, long n, uint64_t seed, uint64_t counter, hipStream_t s) {
   const long pairs = (n + 1) / 2;
   if (pairs <= 0) return AFQ_OK;
-  hipLaunchKernelGGL(normal_fill_kernel, dim3(blocks_for(pairs, 256)), dim3(256), 0, s, out, n, seed, counter);
+  hipLaunchKernelGGL(normal_fill_kernel, dim3(blocks_for(pairs, 256)), dim3(256), 0, s, out, n, seed, counter,
+                     (const uint64_t*)nullptr);
   return launch_status("normal_fill_kernel");
+}
+int k_normal_fill_dev(double* out, long n, uint64_t seed, const uint64_t* counter_dev, hipStream_t s) {
+  const long pairs = (n + 1) / 2;
+  if (pairs <= 0) return AFQ_OK;
+  hipLaunchKernelGGL(normal_fill_kernel, dim3(blocks_for(pairs, 256)), dim3(256), 0, s, out, n, seed, (uint64_t)0,
+                     counter_dev);
+  return launch_status("normal_fill_kernel");
+}
+int k_sr_zeta(double* zeta, uint64_t seed, const uint64_t* counter_dev, hipStream_t s) {
+  hipLaunchKernelGGL(sr_zeta_kernel, dim3(1), dim3(64), 0, s, zeta, seed, counter_dev);
+  return launch_status("sr_zeta_kernel");
+}
+int k_counter_incr(uint64_t* counter_dev, hipStream_t s) {
+  hipLaunchKernelGGL(counter_incr_kernel, dim3(1), dim3(64), 0, s, counter_dev);
+  return launch_status("counter_incr_kernel");
 }
 
 }  // namespace afq

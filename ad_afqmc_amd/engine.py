@@ -244,6 +244,33 @@ class EngineJob:
         )
         return out, int(ctr.value)
 
+    def reserve_fields(self, n_prop_steps: int):
+        """Allocates the field buffer of n_prop_steps steps (before graph capture)."""
+        nat.call("afq_job_reserve_fields", self._h, int(n_prop_steps), stream_ptr())
+
+    def sr_block(self, up, dn, weights, overlaps, scalars, n_prop_steps: int, seed: int, counter_dev: torch.Tensor,
+                 out: torch.Tensor):
+        """One block + comb resampling with the stream counter in device memory
+        (counter_dev: one int64 element, advanced on the device); capturable in a
+        HIP graph.  out (2,) receives {block_energy, block_weight}."""
+        if counter_dev.dtype != torch.int64 or counter_dev.numel() != 1 or not counter_dev.is_cuda:
+            raise ValueError("counter_dev must be a one-element int64 device tensor")
+        pu, pd = self._w(up, dn)
+        nat.call(
+            "afq_sr_block",
+            self._h,
+            pu,
+            pd,
+            ptr(weights),
+            ptr(overlaps),
+            ptr(scalars),
+            int(n_prop_steps),
+            C.c_uint64(seed),
+            ptr(counter_dev),
+            ptr(out),
+            stream_ptr(),
+        )
+
     def segment_energy_vjp(self, walkers, weights, e_estimate: float, fields, want_grad: bool = True):
         """Segment energy F over fields (n_blocks, n_steps, W, G) and, with
         want_grad, dF/dh1 (2,N,N) and dF/dchol (G,N*N) (afq_segment_energy_vjp).

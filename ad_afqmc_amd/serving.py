@@ -4,8 +4,9 @@ Requests (jobs of different N, O, G and walker counts) arrive over time.  A
 job is admitted as soon as a slot and its HBM footprint (as libafq.so would
 allocate it, `afq_job_footprint`) fit under the cap; admitted jobs keep their
 Hamiltonian, rotated Cholesky tensor and walkers resident and each runs its
-device blocks (afq_block: phaseless steps, QR, energy, then comb resampling)
-on its own HIP stream, so work of different jobs overlaps on the device.  A
+device blocks (afq_sr_block: phaseless steps, QR, energy, then comb
+resampling, replayed from a captured HIP graph) on its own HIP stream, so work
+of different jobs overlaps on the device.  A
 job holds at most one block in flight; the host issues the next block of every
 job whose previous block has finished, admits newly arrived jobs in between,
 and retires finished jobs (freeing their memory and slot) as soon as their
@@ -151,13 +152,20 @@ class Scheduler:
 
 
 class GpuExecutor:
-    """Runs each admitted job on its own HIP stream through libafq.so."""
+    """Runs each admitted job on its own HIP stream through libafq.so.
 
-    def __init__(self, n_prop_steps: int = 10):
+    A job's blocks are `afq_sr_block` calls (block + comb resampling, stream
+    counter in device memory).  With use_graphs, the first block runs eagerly
+    (it also warms up the library) and the second is captured into a HIP graph,
+    which every later block replays: one graph launch per block instead of a
+    few hundred kernel launches from the host."""
+
+    def __init__(self, n_prop_steps: int = 10, use_graphs: bool = True):
         import torch
 
         self.torch = torch
         self.n_prop_steps = int(n_prop_steps)
+        self.use_graphs = bool(use_graphs)
 
     def start(self, spec: JobSpec):
         torch = self.torch
@@ -180,33 +188,45 @@ class GpuExecutor:
             ham.pop("chol")  # the job keeps its own resident copy
             pd = prop.init_prop_data(trial, wave, ham, seed=spec.seed)
             job = engine_for(trial, wave, ham, "restricted", spec.n_walkers, prop)
+            job.reserve_fields(self.n_prop_steps)
             up = pd["walkers"].data.clone()
             weights = pd["weights"].clone()
             overlaps = job.overlap(up, None)
             e0 = pd["e_estimate"].reshape(())
             scalars = torch.stack([e0, e0.clone()]).contiguous()
             nblocks = max(1, -(-spec.n_steps // self.n_prop_steps))
-            zetas = torch.rand(nblocks, dtype=torch.float64, device="cuda",
-                               generator=torch.Generator(device="cuda").manual_seed(spec.seed + 17))
+            counter = torch.zeros(1, dtype=torch.int64, device="cuda")
+            out = torch.empty(2, dtype=torch.float64, device="cuda")
             ev = torch.cuda.Event()
             ev.record(stream)
         return dict(spec=spec, stream=stream, job=job, wave=wave, ham=ham, up=up, weights=weights,
-                    overlaps=overlaps, scalars=scalars, zetas=zetas, counter=0, issued=0, nblocks=nblocks,
-                    outs=[], event=ev)
+                    overlaps=overlaps, scalars=scalars, counter=counter, out=out, graph=None, issued=0,
+                    nblocks=nblocks, outs=[], event=ev)
 
     def ready(self, h) -> bool:
         return h["issued"] < h["nblocks"] and h["event"].query()
 
+    def _block(self, h):
+        h["job"].sr_block(h["up"], None, h["weights"], h["overlaps"], h["scalars"], self.n_prop_steps,
+                          h["spec"].seed, h["counter"], h["out"])
+
     def issue(self, h):
         torch = self.torch
-        with torch.cuda.stream(h["stream"]):
-            out, h["counter"] = h["job"].block(h["up"], None, h["weights"], h["overlaps"], h["scalars"],
-                                               self.n_prop_steps, h["spec"].seed, h["counter"])
-            i = h["issued"]
-            h["job"].stochastic_reconfiguration(h["up"], None, h["weights"], h["zetas"][i : i + 1])
-            h["outs"].append(out)
+        stream = h["stream"]
+        with torch.cuda.stream(stream):
+            if h["graph"] is not None:
+                h["graph"].replay()
+            elif self.use_graphs and h["issued"] == 1:
+                g = torch.cuda.CUDAGraph()
+                with torch.cuda.graph(g, stream=stream):
+                    self._block(h)  # recorded, not run
+                h["graph"] = g
+                g.replay()
+            else:
+                self._block(h)
+            h["outs"].append(h["out"].clone())
             h["event"] = torch.cuda.Event()
-            h["event"].record(h["stream"])
+            h["event"].record(stream)
         h["issued"] += 1
 
     def done(self, h) -> bool:
@@ -219,6 +239,7 @@ class GpuExecutor:
 
     def release(self, h):
         h["stream"].synchronize()
+        h["graph"] = None
         h["job"].close()
         h.clear()
 

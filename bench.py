@@ -262,7 +262,7 @@ def main():
     ap.add_argument("--breakdown", action="store_true", help="print per-class kernel times to stderr")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run 5 plain device copies of MIB MiB (to scale HBM counters) and exit")
-    ap.add_argument("--serving-jobs", type=int, default=16,
+    ap.add_argument("--serving-jobs", type=int, default=64,
                     help="jobs in the serving stream (Poisson arrivals at 4/s, N~U{30..250}, 300 steps); 0 skips it")
     ap.add_argument("--serving-precision", default="bf16x3")
     ap.add_argument("--precision-study", default="", metavar="WORKLOAD",

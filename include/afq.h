@@ -38,6 +38,7 @@
  *   afq_stochastic_reconfiguration -> stochastic_reconfiguration_local
  *                            (ad_afqmc/walkers.py:139-145, 244-250; ad_afqmc/sr.py:24-48)
  *   afq_block             -> sampler._block_scan (ad_afqmc/sampling.py:52-94) with on-device fields
+ *   afq_sr_block          -> sampler._sr_block_scan (ad_afqmc/sampling.py:146-168), graph-capturable
  *   afq_segment_energy_vjp -> jax.vjp of sampler.propagate_phaseless_ad_nosr_norot
  *                            (ad_afqmc/sampling.py:300-341; ad_afqmc/driver.py:865-876, 842-864)
  */
@@ -162,6 +163,23 @@ int afq_stochastic_reconfiguration(afq_job_t job, void* walkers_up, void* walker
 int afq_block(afq_job_t job, void* walkers_up, void* walkers_dn, double* weights, void* overlaps,
               double* scalars, int n_prop_steps, uint64_t seed, uint64_t* rng_counter, double* block_out,
               void* stream);
+
+/* One sampler block followed by comb resampling and the overlaps of the new
+ * population -- the reference's _sr_block_scan with one energy block
+ * (ad_afqmc/sampling.py:146-168) -- entirely on the device.  The stream
+ * counter lives in device memory: *counter_dev (uint64) keys this block's
+ * fields (the same Philox stream afq_block uses for that counter) and the SR
+ * offset zeta ~ U[0,1), and is incremented on the device at the end.  With
+ * the field buffer reserved (afq_job_reserve_fields) the call makes no host
+ * synchronisation and no allocation, so it can be captured once into a HIP
+ * graph and replayed.  block_out receives {block_energy, block_weight}
+ * (before resampling). */
+int afq_sr_block(afq_job_t job, void* walkers_up, void* walkers_dn, double* weights, void* overlaps,
+                 double* scalars, int n_prop_steps, uint64_t seed, uint64_t* counter_dev, double* block_out,
+                 void* stream);
+/* Allocates the field buffer for n_prop_steps steps of the job's walker
+ * capacity (synchronises the stream when it grows). */
+int afq_job_reserve_fields(afq_job_t job, int32_t n_prop_steps, void* stream);
 
 /* Fills out (n,) f64 with N(0,1) draws of the engine's field generator for
  * (seed, counter) -- the stream afq_block uses. */
