@@ -250,6 +250,133 @@ def _restart_paired(s, cfg, trial, wtype, n_blocks, n_eql, zetas):
     return out
 
 
+def syn_config(name):
+    from ad_afqmc_amd import synthetic as syn
+
+    return syn.CONFIGS[name]
+
+
+def make_system(name):
+    """The stated synthetic generator for workload `name` (seed 0)."""
+    from ad_afqmc_amd import synthetic as syn
+
+    cfg = syn.CONFIGS[name]
+    Oa = cfg["nocc_up"]
+    return syn.make_system(cfg["norb"], Oa, cfg["nchol"], nocc_dn=cfg.get("nocc_dn", Oa), uhf=cfg["uhf"], seed=0)
+
+
+def _precision_errors(name):
+    """Measured systematic energy errors of each mode on `name` (per block, restart-paired)."""
+    f = os.path.join(ROOT, "profiles", f"precision_study_{name}.json")
+    out = {}
+    if not os.path.exists(f):
+        return out
+    try:
+        d = json.load(open(f))
+    except Exception:
+        return out
+    for p in ("fp32", "bf16x3"):
+        rp = d.get(p + "_per_block", {}).get("restart_paired_block")
+        if rp:
+            out[p] = {"systematic_error_mha": rp["delta_mha"], "systematic_error_stderr_mha": rp["stderr_mha"],
+                      "error_source": f"profiles/precision_study_{name}.json (restart-paired blocks)"}
+    out["fp64"] = {"systematic_error_mha": 0.0}
+    return out
+
+
+def time_config(system, name, precision, steps, warmup, dominant_roofline=False, breakdown=False):
+    """Times `steps` device blocks (10 phaseless steps + QR + energy + comb
+    resampling each) of workload `name` in `precision`, after `warmup` blocks.
+    With dominant_roofline the warm-up times every kernel class and the timed
+    region brackets the dominant class with HIP events on the launch stream."""
+    import numpy as np
+    import torch
+
+    from ad_afqmc_amd import engine as E
+    from ad_afqmc_amd import hamiltonian as hm
+    from ad_afqmc_amd import propagation as pr
+    from ad_afqmc_amd import wavefunctions as wf
+    from ad_afqmc_amd.binding import engine_for
+
+    cfg = syn_config(name)
+    N, G, W = cfg["norb"], cfg["nchol"], cfg["n_walkers"]
+    uhf = cfg["uhf"]
+    Oa = cfg["nocc_up"]
+    Ob = cfg.get("nocc_dn", Oa)
+    nsteps, dt = cfg["n_prop_steps"], cfg["dt"]
+    wave, ham = system.wave_data(), system.ham_data()
+    trial = wf.uhf(N, (Oa, Ob)) if uhf else wf.rhf(N, (Oa, Oa))
+    wtype = "unrestricted" if uhf else "restricted"
+    prop = pr.propagator_afqmc(dt=dt, n_walkers=W, walker_type=wtype, precision=precision)
+    H = hm.hamiltonian(N)
+    ham = H.build_measurement_intermediates(ham, trial, wave)
+    ham = H.build_propagation_intermediates(ham, prop, trial, wave)
+    pd = prop.init_prop_data(trial, wave, ham, seed=0)
+    job = engine_for(trial, wave, ham, wtype, W, prop)
+    up, dn = (t.clone() if t is not None else None for t in pd["walkers"].tensors())
+    weights = pd["weights"].clone()
+    overlaps = job.overlap(up, dn)
+    scalars = torch.stack([pd["e_estimate"], pd["e_estimate"].clone()]).contiguous()
+    zetas = torch.rand(warmup + steps + 8, dtype=torch.float64, device=up.device)
+    counter = 0
+
+    def one_block(i):
+        nonlocal counter
+        out, counter = job.block(up, dn, weights, overlaps, scalars, nsteps, 1234, counter)
+        job.stochastic_reconfiguration(up, dn, weights, zetas[i : i + 1])
+        return out
+
+    classes = E.nat.KERNEL_CLASSES if dominant_roofline or breakdown else []
+    for c in classes:
+        job.profile(c, True)
+    for i in range(warmup):
+        one_block(i)
+    torch.cuda.synchronize()
+    bd = {c: job.profile_read(c) for c in classes}
+    for c in classes:
+        job.profile(c, False)
+    res = {"breakdown": {c: v[0] / max(warmup, 1) for c, v in bd.items()} or None, "roofline": None}
+    if breakdown:
+        tot = sum(v[0] for v in bd.values())
+        for c, (ms, n) in sorted(bd.items(), key=lambda kv: -kv[1][0]):
+            print(f"[bench] {c:14s} {ms / max(warmup, 1):9.3f} ms/block  {n // max(warmup, 1):5d} launches/block"
+                  f"  {100 * ms / max(tot, 1e-9):5.1f}%", file=sys.stderr)
+    dominant = max(bd, key=lambda c: bd[c][0]) if dominant_roofline else None
+    if dominant:
+        job.profile(dominant, True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    outs = [one_block(warmup + i) for i in range(steps)]
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    if dominant:
+        dom_ms, dom_n = job.profile_read(dominant)
+        job.profile(dominant, False)
+        per_launch = launch_flops(dominant, N, Oa, Oa, G, W)
+        avg_s = dom_ms / max(dom_n, 1) / 1e3
+        # bf16x3 does three bf16 MFMA products per FP32-equivalent product: its
+        # ceiling is a third of the dense bf16 peak
+        peak = {"fp64": PEAKS["fp64"], "fp32": PEAKS["fp32"], "bf16x3": PEAKS["bf16"] / 3.0}[precision]
+        if per_launch is not None and avg_s > 0:
+            ach = per_launch / avg_s
+            roof = {"kernel": dominant, "bound": "mfma", "achieved": ach / 1e12, "peak": peak / 1e12,
+                    "unit": "TFLOP/s", "frac": ach / peak, "traffic": None, "avg_launch_ms": avg_s * 1e3,
+                    "flops_per_launch": per_launch}
+            tfile = os.path.join(ROOT, "profiles", "traffic.json")
+            if os.path.exists(tfile):
+                try:
+                    t = json.load(open(tfile)).get(f"{name}:{precision}:{dominant}")
+                    if t is not None:
+                        roof["traffic"] = t
+                except Exception:
+                    pass
+            res["roofline"] = roof
+    res.update(value=W * nsteps * steps / elapsed, elapsed=elapsed, blocks=torch.stack(outs).cpu().numpy())
+    job.close()
+    del job, up, dn, weights, overlaps, scalars, pd, ham, wave
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__)
     ap.add_argument("--gpus", type=int, default=1)
@@ -270,6 +397,8 @@ def main():
     ap.add_argument("--study-blocks", type=int, default=50)
     ap.add_argument("--study-eql", type=int, default=10)
     ap.add_argument("--study-lcoef", type=float, default=0.3, help="Cholesky scale coefficient c in c/sqrt(N)")
+    ap.add_argument("--no-modes", action="store_true",
+                    help="skip the FP32 / bf16x3 runs of the workload and the N=300, G=2000 mode comparison")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this engine runs on one GPU; --gpus above 1 is not supported", file=sys.stderr)
@@ -293,106 +422,19 @@ def main():
         print(json.dumps({"copy_calibration_bytes_read": n * 8, "copy_calibration_bytes_written": n * 8}))
         return
 
-    from ad_afqmc_amd import engine as E
-    from ad_afqmc_amd import hamiltonian as hm
-    from ad_afqmc_amd import propagation as pr
-    from ad_afqmc_amd import synthetic as syn
-    from ad_afqmc_amd import wavefunctions as wf
-    from ad_afqmc_amd.binding import engine_for
-
-    cfg = syn.CONFIGS[args.config]
+    cfg = syn_config(args.config)
     N, G, W = cfg["norb"], cfg["nchol"], cfg["n_walkers"]
     uhf = cfg["uhf"]
     Oa = cfg["nocc_up"]
     Ob = cfg.get("nocc_dn", Oa)
     nsteps, dt = cfg["n_prop_steps"], cfg["dt"]
     print(f"[bench] building {args.config}", file=sys.stderr, flush=True)
-    s = syn.make_system(N, Oa, G, nocc_dn=Ob, uhf=uhf, seed=0)
-    wave, ham = s.wave_data(), s.ham_data()
-    trial = wf.uhf(N, (Oa, Ob)) if uhf else wf.rhf(N, (Oa, Oa))
-    wtype = "unrestricted" if uhf else "restricted"
-    prop = pr.propagator_afqmc(dt=dt, n_walkers=W, walker_type=wtype, precision=args.precision)
-    H = hm.hamiltonian(N)
-    ham = H.build_measurement_intermediates(ham, trial, wave)
-    ham = H.build_propagation_intermediates(ham, prop, trial, wave)
-    pd = prop.init_prop_data(trial, wave, ham, seed=0)
-    job = engine_for(trial, wave, ham, wtype, W, prop)
-    up, dn = (t.clone() if t is not None else None for t in pd["walkers"].tensors())
-    weights = pd["weights"].clone()
-    overlaps = job.overlap(up, dn)
-    scalars = torch.stack([pd["e_estimate"], pd["e_estimate"].clone()]).contiguous()
-    zetas = torch.rand(args.warmup + args.steps + 8, dtype=torch.float64, device=up.device)
-    counter = 0
-
-    def one_block(i):
-        nonlocal counter
-        out, counter = job.block(up, dn, weights, overlaps, scalars, nsteps, 1234, counter)
-        job.stochastic_reconfiguration(up, dn, weights, zetas[i : i + 1])
-        return out
-
-    # warmup with every kernel class timed, to find the dominant one
-    for c in E.nat.KERNEL_CLASSES:
-        job.profile(c, True)
-    for i in range(args.warmup):
-        one_block(i)
-    torch.cuda.synchronize()
-    breakdown = {c: job.profile_read(c) for c in E.nat.KERNEL_CLASSES}
-    for c in E.nat.KERNEL_CLASSES:
-        job.profile(c, False)
-    dominant = max(breakdown, key=lambda c: breakdown[c][0])
-    if args.breakdown:
-        tot = sum(v[0] for v in breakdown.values())
-        for c, (ms, n) in sorted(breakdown.items(), key=lambda kv: -kv[1][0]):
-            print(f"[bench] {c:14s} {ms / max(args.warmup, 1):9.3f} ms/block  {n // max(args.warmup, 1):5d} launches/block"
-                  f"  {100 * ms / max(tot, 1e-9):5.1f}%", file=sys.stderr)
-
-    # timed region: only the dominant class is bracketed by events
-    job.profile(dominant, True)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    outs = []
-    for i in range(args.steps):
-        outs.append(one_block(args.warmup + i))
-    torch.cuda.synchronize()
-    elapsed = time.perf_counter() - t0
-    dom_ms, dom_n = job.profile_read(dominant)
-    job.profile(dominant, False)
-    blocks = torch.stack(outs).cpu().numpy()
+    system = make_system(args.config)
+    r = time_config(system, args.config, args.precision, args.steps, args.warmup, dominant_roofline=True,
+                    breakdown=args.breakdown)
+    value, elapsed, blocks = r["value"], r["elapsed"], r["blocks"]
     finite = bool(np.all(np.isfinite(blocks)))
-
-    walker_steps = W * nsteps * args.steps
-    value = walker_steps / elapsed
-    one_spin = not uhf
-    f_total, _ = flops_per_walker_step(N, Oa, Ob, G, nsteps, one_spin)
-    occ = Oa
-    c_cols = Oa
-    per_launch = launch_flops(dominant, N, c_cols, occ, G, W)
-    avg_s = dom_ms / max(dom_n, 1) / 1e3
-    # bf16x3 does three bf16 MFMA products per FP32-equivalent product: its
-    # ceiling is a third of the dense bf16 peak
-    peak = {"fp64": PEAKS["fp64"], "fp32": PEAKS["fp32"], "bf16x3": PEAKS["bf16"] / 3.0}[args.precision]
-    roof = None
-    if per_launch is not None and avg_s > 0:
-        ach = per_launch / avg_s
-        roof = {
-            "kernel": dominant,
-            "bound": "mfma",
-            "achieved": ach / 1e12,
-            "peak": peak / 1e12,
-            "unit": "TFLOP/s",
-            "frac": ach / peak,
-            "traffic": None,
-            "avg_launch_ms": avg_s * 1e3,
-            "flops_per_launch": per_launch,
-        }
-        tfile = os.path.join(ROOT, "profiles", "traffic.json")
-        if os.path.exists(tfile):
-            try:
-                t = json.load(open(tfile)).get(f"{args.config}:{args.precision}:{dominant}")
-                if t is not None:
-                    roof["traffic"] = t
-            except Exception:
-                pass
+    f_total, _ = flops_per_walker_step(N, Oa, Ob, G, nsteps, not uhf)
     line = {
         "metric": "walker-steps/s at <=0.1 mHa vs FP64 oracle; p50/p99 job latency under load",
         "value": value,
@@ -420,19 +462,49 @@ def main():
         },
         "model_flops_per_walker_step": f_total,
         "achieved_tflops_whole_step": f_total * value / 1e12,
-        "roofline": roof,
-        "kernel_breakdown_ms_per_block": {c: v[0] / max(args.warmup, 1) for c, v in breakdown.items()},
+        "roofline": r["roofline"],
+        "kernel_breakdown_ms_per_block": r["breakdown"],
         # The stated generator kills the walker population (see synthetic.make_system);
         # the work per block is the same dense arithmetic either way.
         "population_finite": finite,
         "mean_block_energy": float(np.mean(blocks[:, 0])) if finite else None,
     }
+    del system
+    torch.cuda.empty_cache()
+    if not args.no_modes:
+        # the reduced-precision modes on the same workload, with their measured
+        # systematic energy error (restart-paired blocks, profiles/precision_study_*.json)
+        modes = {}
+        errs = _precision_errors(args.config)
+        system = make_system(args.config)
+        for p in ("fp64", "fp32", "bf16x3"):
+            if p == args.precision:
+                modes[p] = {"value": value, "ms_per_step": elapsed / args.steps * 1e3}
+            else:
+                m = time_config(system, args.config, p, max(3, args.steps // 2), 1)
+                modes[p] = {"value": m["value"], "ms_per_step": m["elapsed"] / max(3, args.steps // 2) * 1e3}
+            modes[p].update(errs.get(p, {}))
+        line["precision_modes"] = modes
+        del system
+        torch.cuda.empty_cache()
+        # the mixed-precision target is stated at N=300, G=2000 (UHF, 256 walkers)
+        big = "uhf_n300_o60_58_g2000_w256"
+        print(f"[bench] building {big}", file=sys.stderr, flush=True)
+        system = make_system(big)
+        ns = {}
+        for p in ("fp64", "fp32", "bf16x3"):
+            m = time_config(system, big, p, 2, 1)
+            ns[p] = {"value": m["value"], "ms_per_step": m["elapsed"] / 2 * 1e3}
+        errs = _precision_errors(big)
+        for p in ns:
+            ns[p].update(errs.get(p, {}))
+            ns[p]["speedup_vs_fp64"] = ns[p]["value"] / ns["fp64"]["value"]
+        line["uhf_n300_g2000_modes"] = ns
+        del system
+        torch.cuda.empty_cache()
     if args.serving_jobs > 0:
         from ad_afqmc_amd import serving
 
-        # release the benchmark job's memory before the stream starts
-        del job, up, dn, weights, overlaps, scalars, pd, ham, wave
-        wave = ham = None
         torch.cuda.empty_cache()
         print(f"[bench] serving stream: {args.serving_jobs} jobs", file=sys.stderr, flush=True)
         specs = serving.poisson_stream(args.serving_jobs, rate=4.0, seed=0, precision=args.serving_precision)
@@ -443,7 +515,7 @@ def main():
         torch.cuda.synchronize()
         summ = serving.summarize(res, time.perf_counter() - t0)
         summ.update(arrival_rate_per_s=4.0, precision=args.serving_precision, hbm_cap_gb=200,
-                    max_concurrent_jobs=sch.max_concurrent, peak_admitted_gb=sch.peak_used / 1e9,
+                    max_concurrent_jobs=sch.max_concurrent, peak_admitted_gb=sch.peak_used / 1e9, hip_graphs=True,
                     workload="N~U{30..250}, O=N/5, G=5N, W~U{64..512}, 300 steps, Poisson 4 jobs/s")
         line["serving"] = summ
     if not args.no_cpu_baseline:
