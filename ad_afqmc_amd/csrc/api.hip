@@ -195,9 +195,34 @@ int force_bias_gemm(afq_job* j, double2* out, hipStream_t st) {
     g.ws = j->gemm_ws;
     g.ws_elems = j->gemm_ws_elems;
     ProfScope ps(j, AFQ_K_FORCE_BIAS, st);
-    if (j->lp) {
+    if (j->bf16x3) {
+      // over the flattened, zero-padded (Opad, Npad) orbital pairs: split G_w, reuse the split rot_chol
+      const long P = (long)bf16_opad(O) * bf16_npad(j->N);
+      AFQ_TRY(k_split_bf16(reinterpret_cast<const double*>(j->green[s]), true, O, j->N, j->W, j->gsplit, st));
+      Bf3 b;
+      b.M = j->W;
+      b.N = j->G;
+      b.K = (int)P;
+      b.a_cplx = true;
+      b.A = j->gsplit;
+      b.lda = 4 * P;
+      b.pA = P;
+      b.a_rows = j->Wcap;
+      b.B = j->rcs[s];
+      b.ldb = 2 * P;
+      b.pB = P;
+      b.b_rows = j->G;
+      b.alpha = g.alpha;
+      b.beta = g.beta;
+      b.C = out;
+      b.ldc = j->G;
+      b.c_f64 = true;
+      b.ws = j->gemm_ws;
+      b.ws_elems = j->gemm_ws_elems;
+      AFQ_TRY(gemm_bf3(b, st));
+    } else if (j->lp) {
       g.B = j->rot_chol32[s];
-      AFQ_TRY(gemm_lp(LpKind::FORCE_BIAS, j->bf16x3, g, st));
+      AFQ_TRY(gemm_lp(LpKind::FORCE_BIAS, false, g, st));
     } else {
       AFQ_TRY(gemm_f64(g, st));
     }

@@ -1,12 +1,13 @@
-// Split-BF16 x3 VHS product on pre-split operands (gemm_bf3.hip).
+// Split-BF16 x3 products on pre-split operands (gemm_bf3.hip).
 #pragma once
 #include "afq_internal.h"
 
 namespace afq {
 
-// C[b] (M x N, complex64) = alpha * A[b] (M x K, complex) . B[b]^T (N x K, real)
-// on bf16 (hi, lo) planes.  Offsets and strides count bf16 elements (operands)
-// or output elements.
+// C[b] (M x N) = alpha * A[b] (M x K, complex) . B[b]^T (N x K, real) on bf16
+// (hi, lo) planes, into complex64 C, or (c_f64, batch 1) into complex128
+// C = alpha * product + beta * C through split-K partials in ws.  Offsets and
+// strides count bf16 elements (operands) or output elements.
 struct Bf3 {
   int M = 0, N = 0, K = 0, batch = 1;  // K: padded depth, a multiple of 32
   bool a_cplx = false, b_cplx = false;
@@ -17,8 +18,12 @@ struct Bf3 {
   long ldb = 0, pB = 0, sB = 0;
   int b_rows = 0;
   double2 alpha = {1.0, 0.0};
+  double2 beta = {0.0, 0.0};  // complex128 output only
   void* C = nullptr;
   long ldc = 0, sC = 0;
+  bool c_f64 = false;
+  void* ws = nullptr;  // doubles
+  long ws_elems = 0;
 };
 int gemm_bf3(const Bf3& p, hipStream_t s);
 

@@ -1,4 +1,4 @@
-// Split-BF16 x3 VHS product on pre-split operands.
+// Split-BF16 x3 products on pre-split operands (VHS and force bias).
 //
 //   C[b] (M x N) = alpha * A[b] (M x K) . B[b]^T (N x K)
 //
@@ -14,9 +14,10 @@
 // the 8 consecutive k (one 16-byte load) of row 16*block + l%16 at offset
 // 8*(l/16) of each 32-deep step, which is the MFMA's input layout.
 //
-// The output is complex64, C[b*sC + m*ldc + n] = alpha * acc (the VHS product,
-// complex shifted fields x real Cholesky vectors, feeding the complex64 Taylor
-// terms).
+// Outputs: complex64 C[b*sC + m*ldc + n] = alpha * acc (the VHS product,
+// feeding the complex64 Taylor terms), or complex128 C = alpha * acc + beta * C
+// through K slices written to a workspace and a fixed-order reduction (the
+// force bias: few output tiles, a long K).
 #include "afq_internal.h"
 #include "gemm_bf3.h"
 
@@ -34,7 +35,7 @@ __device__ inline v8bf neg8(v8bf x) {
 __device__ inline v4f mma(v8bf a, v8bf b, v4f c) { return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0); }
 template <bool AC, bool BC, int WM, int WN, int TM, int TN>
 __global__ __launch_bounds__(64 * WM * WN, 2) void gemm_bf3_kernel(Bf3 p, int tiles_m, int tiles_n, int xcd_remap,
-                                                                   long T8) {
+                                                                   long T8, int kchunk, double* ws) {
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wm = wave / WN, wn = wave % WN;
   long tile = blockIdx.x;
@@ -65,8 +66,9 @@ __global__ __launch_bounds__(64 * WM * WN, 2) void gemm_bf3_kernel(Bf3 p, int ti
       accR[i][j] = v4f{0.f, 0.f, 0.f, 0.f};
       accI[i][j] = v4f{0.f, 0.f, 0.f, 0.f};
     }
-  const int kvn = p.K / 8;
-  for (int kv = 0; kv < kvn; kv += 4) {
+  const int kz = blockIdx.z;
+  const int kv0 = kz * kchunk / 8, kvn = min(p.K, (kz + 1) * kchunk) / 8;
+  for (int kv = kv0; kv < kvn; kv += 4) {
     v8bf ar[TM], ai[TM], br[TN], bi[TN];
 #pragma unroll
     for (int i = 0; i < TM; ++i) {
@@ -134,6 +136,10 @@ __global__ __launch_bounds__(64 * WM * WN, 2) void gemm_bf3_kernel(Bf3 p, int ti
         const int n = n0 + j * 16 + r16;
         if (m >= p.M || n >= p.N) continue;
         const double xr = accR[i][j][v], xi = accI[i][j][v];
+        if (ws) {  // raw partial of K slice kz (complex128), reduced by the split-K kernel
+          reinterpret_cast<double2*>(ws)[((long)kz * p.M + m) * p.N + n] = double2{xr, xi};
+          continue;
+        }
         const double rr = p.alpha.x * xr - p.alpha.y * xi, ri = p.alpha.x * xi + p.alpha.y * xr;
         reinterpret_cast<float2*>(p.C)[b * p.sC + (long)m * p.ldc + n] = float2{(float)rr, (float)ri};
       }
@@ -148,9 +154,32 @@ int launch(const Bf3& p, hipStream_t s) {
   const long T8 = remap ? (tiles + 7) / 8 : 0;
   const long gx = remap ? T8 * 8 : tiles;
   if (gx > 0x7fffffffL || p.batch > 65535) return fail(AFQ_E_UNSUPPORTED, "gemm_bf3: grid too large");
-  hipLaunchKernelGGL((gemm_bf3_kernel<AC, BC, WM, WN, TM, TN>), dim3((unsigned)gx, p.batch), dim3(64 * WM * WN), 0,
-                     s, p, tm, tn, remap, T8);
-  return launch_status("gemm_bf3_kernel");
+  int ksplit = 1, kchunk = p.K;
+  double* ws = nullptr;
+  if (p.c_f64) {
+    // complex128 output goes through the fixed-order split-K reduction (which
+    // applies alpha and beta); split K until about 1024 work-groups are in flight
+    if (p.batch != 1 || !p.ws) return fail(AFQ_E_INVALID, "gemm_bf3: complex128 output needs batch 1 and a workspace");
+    ksplit = (int)std::max<long>(1, std::min<long>((1024 + tiles - 1) / tiles, p.K / 256));
+    while (ksplit > 1 && (long)ksplit * p.M * p.N * 2 > p.ws_elems) --ksplit;
+    if ((long)ksplit * p.M * p.N * 2 > p.ws_elems) return fail(AFQ_E_INVALID, "gemm_bf3: workspace too small");
+    kchunk = ((p.K + ksplit - 1) / ksplit + 31) / 32 * 32;
+    ksplit = (p.K + kchunk - 1) / kchunk;
+    ws = static_cast<double*>(p.ws);
+  }
+  hipLaunchKernelGGL((gemm_bf3_kernel<AC, BC, WM, WN, TM, TN>), dim3((unsigned)gx, p.batch, ksplit),
+                     dim3(64 * WM * WN), 0, s, p, tm, tn, remap, T8, kchunk, ws);
+  AFQ_TRY(launch_status("gemm_bf3_kernel"));
+  if (!ws) return AFQ_OK;
+  Gemm g;
+  g.M = p.M;
+  g.N = p.N;
+  g.batch = 1;
+  g.C = p.C;
+  g.ldc = p.ldc;
+  g.alpha = p.alpha;
+  g.beta = p.beta;
+  return gemm_splitk_reduce(g, ws, ksplit, true, s);
 }
 
 // ------------------------------------------------------------------ operand preparation
