@@ -378,6 +378,15 @@ int dispatch_shape(const Gemm& g, hipStream_t s) {
       return launch<BF, AC, BC, AT, BT, TA, TB, TCo, 2, 8, 2, 2>(g, s);
   if (g.N <= 16) return launch<BF, AC, BC, AT, BT, TA, TB, TCo, 4, 1, 1, 1>(g, s);
   if (g.N <= 32) return launch<BF, AC, BC, AT, BT, TA, TB, TCo, 4, 1, 1, 2>(g, s);
+  // walker-panel products (the Taylor terms, N = occupied orbitals in 33..64):
+  // more waves per work-group, each with fewer blocks, measured fastest at
+  // N = 40 and 60 (bf16x3: 4 x 2 waves of 1 x 2 blocks; FP32: 4 x 3 or 4 x 4
+  // waves of one block)
+  if (g.M > 64 && g.N <= 64) {
+    if constexpr (BF) return launch<BF, AC, BC, AT, BT, TA, TB, TCo, 4, 2, 1, 2>(g, s);
+    if (g.N <= 48) return launch<BF, AC, BC, AT, BT, TA, TB, TCo, 4, 3, 1, 1>(g, s);
+    return launch<BF, AC, BC, AT, BT, TA, TB, TCo, 4, 4, 1, 1>(g, s);
+  }
   if (g.N <= 48) return launch<BF, AC, BC, AT, BT, TA, TB, TCo, 4, 1, 1, 3>(g, s);
   if (g.N <= 64 && g.M > 64) return launch<BF, AC, BC, AT, BT, TA, TB, TCo, 4, 1, 1, 4>(g, s);
   return launch<BF, AC, BC, AT, BT, TA, TB, TCo, 2, 2, 2, 2>(g, s);
