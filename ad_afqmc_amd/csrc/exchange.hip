@@ -195,14 +195,22 @@ __global__ __launch_bounds__(256, (NB <= 3 ? 2 : 1)) void exchange_kernel(const 
 // (hi, lo) planes and each walker's Green's function as (re_hi, re_lo, im_hi,
 // im_lo) planes, rows padded to 16*NB and columns to a multiple of 32 with
 // zeros, so every lane loads its 8 consecutive k as one 16-byte vector and no
-// bounds masks or conversions remain in the loop.  The three products
-// hi*hi, hi*lo, lo*hi are issued as three passes over all NB x NB blocks, so
+// bounds masks or conversions remain in the loop.  The four waves of a
+// work-group share one walker: its Green's-function planes for the next 32-deep
+// step are staged once per work-group in LDS (double-buffered, one barrier per
+// step) instead of being read by every wave from L2; each wave reads its own
+// rot_chol fragments straight from global memory.  The three products hi*hi,
+// lo*hi, hi*lo are issued as three passes over all NB x NB blocks, so
 // consecutive MFMAs never depend on each other.
 template <int NB>
 __global__ __launch_bounds__(256, 2) void exchange_bf16_kernel(const v8bf* __restrict__ rcs, const v8bf* __restrict__ gs,
                                                                int O, int Opad, int Npad, int Gn, int W, int nchunk,
                                                                int cpg, long T8, double2* __restrict__ coul,
                                                                double2* __restrict__ kpart) {
+  constexpr int ROWS = NB * 16;
+  constexpr int PER = 4 * ROWS * 4;       // v8bf of one step's B stage: 4 planes x ROWS x 4 k-groups
+  constexpr int LD = (PER + 255) / 256;   // v8bf loaded per thread per step
+  __shared__ v8bf bsh[2][PER];
   __shared__ double2 scr[4][16 * 17];
   __shared__ double2 kred[4];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -224,10 +232,23 @@ __global__ __launch_bounds__(256, 2) void exchange_bf16_kernel(const v8bf* __res
   const int r16 = lane & 15, q4 = lane >> 4;
   const v8bf* Ah = rcs + (long)gg * 2 * plane + r16 * rowv + q4;
   const v8bf* Al = Ah + plane;
-  const v8bf* Brh = gs + (long)w * 4 * plane + r16 * rowv + q4;
-  const v8bf* Brl = Brh + plane;
-  const v8bf* Bih = Brh + 2 * plane;
-  const v8bf* Bil = Brh + 3 * plane;
+  const v8bf* Bw = gs + (long)w * 4 * plane;
+  // stage slot i of a step: plane i / (4 ROWS), row (i / 4) % ROWS, k-group i % 4
+  v8bf stage[LD];
+  auto fetch = [&](int kv) {
+#pragma unroll
+    for (int t = 0; t < LD; ++t) {
+      const int i = tid + t * 256;
+      if (i < PER) stage[t] = Bw[(long)(i / (4 * ROWS)) * plane + (long)((i / 4) % ROWS) * rowv + kv + (i & 3)];
+    }
+  };
+  auto store = [&](int buf) {
+#pragma unroll
+    for (int t = 0; t < LD; ++t) {
+      const int i = tid + t * 256;
+      if (i < PER) bsh[buf][i] = stage[t];
+    }
+  };
 
   v4f accR[NB][NB], accI[NB][NB];
 #pragma unroll
@@ -237,14 +258,20 @@ __global__ __launch_bounds__(256, 2) void exchange_bf16_kernel(const v8bf* __res
       accR[i][j] = v4f{0.f, 0.f, 0.f, 0.f};
       accI[i][j] = v4f{0.f, 0.f, 0.f, 0.f};
     }
+  fetch(0);
+  store(0);
+  __syncthreads();
+  int buf = 0;
   for (int kv = 0; kv < rowv; kv += 4) {
+    const bool more = kv + 4 < rowv;
+    if (more) fetch(kv + 4);
+    const v8bf* B = bsh[buf] + r16 * 4 + q4;  // + plane * 4 ROWS + 16 b * 4
     v8bf ah[NB], br[NB], bi[NB];
 #pragma unroll
     for (int b = 0; b < NB; ++b) {
-      const long o = (long)b * 16 * rowv + kv;
-      ah[b] = Ah[o];
-      br[b] = Brh[o];
-      bi[b] = Bih[o];
+      ah[b] = Ah[(long)b * 16 * rowv + kv];
+      br[b] = B[b * 64];
+      bi[b] = B[2 * 4 * ROWS + b * 64];
     }
 #pragma unroll
     for (int i = 0; i < NB; ++i)
@@ -265,9 +292,8 @@ __global__ __launch_bounds__(256, 2) void exchange_bf16_kernel(const v8bf* __res
       }
 #pragma unroll
     for (int b = 0; b < NB; ++b) {
-      const long o = (long)b * 16 * rowv + kv;
-      br[b] = Brl[o];
-      bi[b] = Bil[o];
+      br[b] = B[4 * ROWS + b * 64];
+      bi[b] = B[3 * 4 * ROWS + b * 64];
     }
 #pragma unroll
     for (int i = 0; i < NB; ++i)
@@ -276,6 +302,9 @@ __global__ __launch_bounds__(256, 2) void exchange_bf16_kernel(const v8bf* __res
         accR[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], br[j], accR[i][j], 0, 0, 0);
         accI[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bi[j], accI[i][j], 0, 0, 0);
       }
+    if (more) store(buf ^ 1);
+    __syncthreads();
+    buf ^= 1;
   }
 
   // accumulator element v of lane l sits at (4(l/16) + v, l%16)
