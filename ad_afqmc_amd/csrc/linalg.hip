@@ -21,21 +21,28 @@ __device__ inline double2 crecip(double2 a) {
 }
 
 // One 256-thread work-group per matrix; the matrix lives in dynamic LDS (n*n
-// complex, n <= 64).  Partial pivoting selects the first row of maximal
-// |re|+|im| in the pivot column (LAPACK izamax convention).  The in-place
-// Gauss-Jordan sweep leaves the inverse with its columns permuted by the row
-// interchanges, undone at the end.
+// complex, n <= 64).  Thread t owns column t % 64 for the rows t/64, t/64 + 4,
+// ..., so the elimination needs no index division and reads the pivot row and
+// factor column as LDS broadcasts.  Partial pivoting selects the first row of
+// maximal |re|+|im| in the pivot column (LAPACK izamax convention).  With an
+// inverse requested the sweep is Gauss-Jordan in place (the inverse ends with
+// its columns permuted by the row interchanges, undone at the end); for the
+// determinant alone only the trailing rows and columns are eliminated (the
+// pivots, hence the determinant, are the same).
 __global__ __launch_bounds__(256) void gj_det_inv_kernel(const double2* __restrict__ S, long strideS, int n,
                                                          double2* __restrict__ det, double2* __restrict__ inv,
                                                          long strideInv) {
   extern __shared__ double2 a[];
+  __shared__ double2 prow[64];
   __shared__ double2 fcol[64];
   __shared__ int piv[64];
   __shared__ int pk;
   const int tid = threadIdx.x;
+  const int j = tid & 63, rg = tid >> 6;
   const long b = blockIdx.x;
   const double2* src = S + b * strideS;
   const int nn = n * n;
+  const bool want_inv = inv != nullptr;
   for (int i = tid; i < nn; i += 256) a[i] = src[i];
   __syncthreads();
 
@@ -65,36 +72,47 @@ __global__ __launch_bounds__(256) void gj_det_inv_kernel(const double2* __restri
     }
     __syncthreads();
     const int p = pk;
-    if (p != k) {
-      for (int j = tid; j < n; j += 256) {
-        const double2 t = a[k * n + j];
-        a[k * n + j] = a[p * n + j];
-        a[p * n + j] = t;
-      }
-      __syncthreads();
-    }
-    const double2 d = a[k * n + k];
+    const double2 d = a[p * n + k];
+    const double2 dinv = crecip(d);
     if (tid == 0) {
       dprod = cmul(dprod, d);
       if (p != k) neg = !neg;
     }
-    const double2 dinv = crecip(d);
-    if (tid < n) fcol[tid] = a[tid * n + k];
+    // read: the pivot row (scaled) and the rows to swap (group 0), the factor
+    // column after the swap (group 1)
+    const int jlo = want_inv ? 0 : k;
+    double2 xp = {0.0, 0.0}, xk = {0.0, 0.0}, fc = {0.0, 0.0};
+    if (rg == 0 && j >= jlo && j < n) {
+      xp = a[p * n + j];
+      xk = a[k * n + j];
+    } else if (rg == 1 && j < n) {
+      fc = (j == k) ? d : a[(j == p ? k : j) * n + k];
+    }
     __syncthreads();
-    if (tid < n) a[k * n + tid] = (tid == k) ? dinv : cmul(a[k * n + tid], dinv);
+    if (rg == 0 && j >= jlo && j < n) {
+      const double2 r = (j == k) ? dinv : cmul(xp, dinv);
+      prow[j] = r;
+      a[k * n + j] = r;
+      if (p != k) a[p * n + j] = xk;
+    } else if (rg == 1 && j < n) {
+      fcol[j] = fc;
+    }
     __syncthreads();
-    for (int idx = tid; idx < nn; idx += 256) {
-      const int i = idx / n, j = idx - i * n;
-      if (i == k) continue;
-      const double2 f = fcol[i];
-      const double2 rk = a[k * n + j];
-      if (j == k) {
-        a[idx] = double2{-(f.x * rk.x - f.y * rk.y), -(f.x * rk.y + f.y * rk.x)};
-      } else {
-        double2 x = a[idx];
-        x.x -= f.x * rk.x - f.y * rk.y;
-        x.y -= f.x * rk.y + f.y * rk.x;
-        a[idx] = x;
+    if (j < n && j >= jlo) {
+      const double2 rk = prow[j];
+      const int i0 = want_inv ? rg : k + 1 + rg;
+      for (int i = i0; i < n; i += 4) {
+        if (i == k) continue;
+        const double2 f = fcol[i];
+        double2* x = &a[i * n + j];
+        if (j == k) {
+          *x = double2{-(f.x * rk.x - f.y * rk.y), -(f.x * rk.y + f.y * rk.x)};
+        } else {
+          double2 y = *x;
+          y.x -= f.x * rk.x - f.y * rk.y;
+          y.y -= f.x * rk.y + f.y * rk.x;
+          *x = y;
+        }
       }
     }
     __syncthreads();
