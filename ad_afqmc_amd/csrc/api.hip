@@ -257,6 +257,42 @@ int energy_pipeline(afq_job* j, const void* up, const void* dn, double2* out, hi
                            j->W, st);
 }
 
+// Column blocks of the propagation panel: spin s occupies columns
+// [off[s], off[s] + c[s]) of a (W, N, ctot) array.
+struct PanelCols {
+  int c[2] = {0, 0}, off[2] = {0, 0}, ctot = 0;
+  explicit PanelCols(const afq_job* j) {
+    for (int s = 0; s < j->nspin_prop; ++s) {
+      c[s] = j->restricted ? j->wcols : j->nocc[s];
+      off[s] = ctot;
+      ctot += c[s];
+    }
+  }
+};
+
+// dst[w] (N x c, row stride ldd, batch stride sd) = exp(-dt/2 h1)_s . src[w] (row stride lds, batch stride ss)
+int apply_exp_h1(afq_job* j, const double2* src, long lds, long ss, int s, double2* dst, long ldd, long sd,
+                 hipStream_t st) {
+  const int N = j->N;
+  Gemm g;
+  g.M = N;
+  g.N = (int)(j->restricted ? j->wcols : j->nocc[s]);
+  g.K = N;
+  g.batch = j->W;
+  g.A = j->exp_h1[j->restricted ? 0 : s];
+  g.lda = N;
+  g.sA = 0;
+  g.b_cplx = true;
+  g.B = src;
+  g.ldb = lds;
+  g.sB = ss;
+  g.C = dst;
+  g.ldc = ldd;
+  g.sC = sd;
+  ProfScope ps(j, AFQ_K_EXPH1, st);
+  return gemm_f64(g, st);
+}
+
 // Reduced precision: VHS and the Taylor terms in complex64 on FP32 / split-BF16
 // MFMA, exp(-dt/2 h1) applications and the running Taylor sum in complex128
 // (the reference's vhs_mixed_precision split, propagation.py:154-176, 196-203).
@@ -303,65 +339,49 @@ int trotprop_lp(afq_job* j, void* up, void* dn, const double2* xs, hipStream_t s
       AFQ_TRY(gemm_lp(LpKind::VHS, false, g, st));
     }
   }
-  for (int s = 0; s < j->nspin_prop; ++s) {
-    double2* Wp = static_cast<double2*>(s == 0 ? up : dn);
-    const int c = j->restricted ? j->wcols : j->nocc[s];
-    const long per = (long)N * c;
-    const double* E = j->exp_h1[j->restricted ? 0 : s];
-    auto apply_E = [&](const double2* src, double2* dst) {
-      Gemm g;
-      g.M = N;
-      g.N = c;
-      g.K = N;
-      g.batch = W;
-      g.A = E;
-      g.lda = N;
-      g.sA = 0;
-      g.b_cplx = true;
-      g.B = src;
-      g.ldb = c;
-      g.sB = per;
-      g.C = dst;
-      g.ldc = c;
-      g.sC = per;
-      ProfScope ps(j, AFQ_K_EXPH1, st);
-      return gemm_f64(g, st);
-    };
-    double2* t0 = j->tbuf[0];
-    double2* acc = j->tbuf[2];
-    AFQ_TRY(apply_E(Wp, t0));
-    AFQ_HIP(hipMemcpyAsync(acc, t0, sizeof(double2) * per * W, hipMemcpyDeviceToDevice, st));
-    int cur = -1;  // -1: previous term is t0 (complex128)
-    for (int n = 1; n < j->d.n_exp_terms; ++n) {
-      const int nxt = (cur == 0) ? 1 : 0;
-      Gemm g;
-      g.M = N;
-      g.N = c;
-      g.K = N;
-      g.batch = W;
-      g.a_cplx = true;
-      g.A = j->vhs32;
-      g.lda = N;
-      g.sA = NN;
-      g.b_cplx = true;
-      g.B = cur < 0 ? static_cast<const void*>(t0) : static_cast<const void*>(j->t32[cur]);
-      g.ldb = c;
-      g.sB = per;
-      g.C = j->t32[nxt];
-      g.ldc = c;
-      g.sC = per;
-      g.D = acc;
-      g.ldd = c;
-      g.sD = per;
-      g.alpha = double2{1.0 / n, 0.0};
-      {
-        ProfScope ps(j, AFQ_K_TAYLOR, st);
-        AFQ_TRY(gemm_lp(cur < 0 ? LpKind::TAYLOR_D : LpKind::TAYLOR_F, j->bf16x3, g, st));
-      }
-      cur = nxt;
+  // The walker buffers of both spins are propagated as the column blocks of
+  // one (W, N, c_up + c_dn) panel, so each Taylor term is one product that
+  // reads the shared VHS once.
+  PanelCols pc(j);
+  const long per = (long)N * pc.ctot;
+  double2* t0 = j->tbuf[0];
+  double2* acc = j->tbuf[2];
+  for (int s = 0; s < j->nspin_prop; ++s)
+    AFQ_TRY(apply_exp_h1(j, static_cast<const double2*>(s == 0 ? up : dn), pc.c[s], (long)N * pc.c[s], s,
+                         t0 + pc.off[s], pc.ctot, per, st));
+  AFQ_HIP(hipMemcpyAsync(acc, t0, sizeof(double2) * per * W, hipMemcpyDeviceToDevice, st));
+  int cur = -1;  // -1: previous term is t0 (complex128)
+  for (int n = 1; n < j->d.n_exp_terms; ++n) {
+    const int nxt = (cur == 0) ? 1 : 0;
+    Gemm g;
+    g.M = N;
+    g.N = pc.ctot;
+    g.K = N;
+    g.batch = W;
+    g.a_cplx = true;
+    g.A = j->vhs32;
+    g.lda = N;
+    g.sA = NN;
+    g.b_cplx = true;
+    g.B = cur < 0 ? static_cast<const void*>(t0) : static_cast<const void*>(j->t32[cur]);
+    g.ldb = pc.ctot;
+    g.sB = per;
+    g.C = j->t32[nxt];
+    g.ldc = pc.ctot;
+    g.sC = per;
+    g.D = acc;
+    g.ldd = pc.ctot;
+    g.sD = per;
+    g.alpha = double2{1.0 / n, 0.0};
+    {
+      ProfScope ps(j, AFQ_K_TAYLOR, st);
+      AFQ_TRY(gemm_lp(cur < 0 ? LpKind::TAYLOR_D : LpKind::TAYLOR_F, j->bf16x3, g, st));
     }
-    AFQ_TRY(apply_E(acc, Wp));
+    cur = nxt;
   }
+  for (int s = 0; s < j->nspin_prop; ++s)
+    AFQ_TRY(apply_exp_h1(j, acc + pc.off[s], pc.ctot, per, s, static_cast<double2*>(s == 0 ? up : dn), pc.c[s],
+                         (long)N * pc.c[s], st));
   return AFQ_OK;
 }
 
@@ -386,68 +406,49 @@ int trotprop(afq_job* j, void* up, void* dn, const double2* xs, hipStream_t st) 
     ProfScope ps(j, AFQ_K_VHS, st);
     AFQ_TRY(gemm_f64(g, st));
   }
-  for (int s = 0; s < j->nspin_prop; ++s) {
-    double2* Wp = static_cast<double2*>(s == 0 ? up : dn);
-    const int c = j->restricted ? j->wcols : j->nocc[s];
-    const long per = (long)N * c;
-    const double* E = j->exp_h1[j->restricted ? 0 : s];
-    auto apply_E = [&](const double2* src, double2* dst) {
-      Gemm g;
-      g.M = N;
-      g.N = c;
-      g.K = N;
-      g.batch = W;
-      g.A = E;
-      g.lda = N;
-      g.sA = 0;
-      g.b_cplx = true;
-      g.B = src;
-      g.ldb = c;
-      g.sB = per;
-      g.C = dst;
-      g.ldc = c;
-      g.sC = per;
-      ProfScope ps(j, AFQ_K_EXPH1, st);
-      return gemm_f64(g, st);
-    };
-    double2* t0 = j->tbuf[0];
-    double2* t1 = j->tbuf[1];
-    double2* acc = j->tbuf[2];
-    AFQ_TRY(apply_E(Wp, t0));
-    AFQ_HIP(hipMemcpyAsync(acc, t0, sizeof(double2) * per * W, hipMemcpyDeviceToDevice, st));
-    double2* src = t0;
-    double2* dst = t1;
-    for (int n = 1; n < j->d.n_exp_terms; ++n) {
-      Gemm g;
-      g.M = N;
-      g.N = c;
-      g.K = N;
-      g.batch = W;
-      g.a_cplx = true;
-      g.A = j->vhs;
-      g.lda = N;
-      g.sA = NN;
-      g.b_cplx = true;
-      g.B = src;
-      g.ldb = c;
-      g.sB = per;
-      g.C = dst;
-      g.ldc = c;
-      g.sC = per;
-      g.D = acc;
-      g.ldd = c;
-      g.sD = per;
-      g.alpha = double2{1.0 / n, 0.0};
-      {
-        ProfScope ps(j, AFQ_K_TAYLOR, st);
-        AFQ_TRY(gemm_f64(g, st));
-      }
-      double2* t = src;
-      src = dst;
-      dst = t;
+  PanelCols pc(j);
+  const long per = (long)N * pc.ctot;
+  double2* t0 = j->tbuf[0];
+  double2* t1 = j->tbuf[1];
+  double2* acc = j->tbuf[2];
+  for (int s = 0; s < j->nspin_prop; ++s)
+    AFQ_TRY(apply_exp_h1(j, static_cast<const double2*>(s == 0 ? up : dn), pc.c[s], (long)N * pc.c[s], s,
+                         t0 + pc.off[s], pc.ctot, per, st));
+  AFQ_HIP(hipMemcpyAsync(acc, t0, sizeof(double2) * per * W, hipMemcpyDeviceToDevice, st));
+  double2* src = t0;
+  double2* dst = t1;
+  for (int n = 1; n < j->d.n_exp_terms; ++n) {
+    Gemm g;
+    g.M = N;
+    g.N = pc.ctot;
+    g.K = N;
+    g.batch = W;
+    g.a_cplx = true;
+    g.A = j->vhs;
+    g.lda = N;
+    g.sA = NN;
+    g.b_cplx = true;
+    g.B = src;
+    g.ldb = pc.ctot;
+    g.sB = per;
+    g.C = dst;
+    g.ldc = pc.ctot;
+    g.sC = per;
+    g.D = acc;
+    g.ldd = pc.ctot;
+    g.sD = per;
+    g.alpha = double2{1.0 / n, 0.0};
+    {
+      ProfScope ps(j, AFQ_K_TAYLOR, st);
+      AFQ_TRY(gemm_f64(g, st));
     }
-    AFQ_TRY(apply_E(acc, Wp));
+    double2* t = src;
+    src = dst;
+    dst = t;
   }
+  for (int s = 0; s < j->nspin_prop; ++s)
+    AFQ_TRY(apply_exp_h1(j, acc + pc.off[s], pc.ctot, per, s, static_cast<double2*>(s == 0 ? up : dn), pc.c[s],
+                         (long)N * pc.c[s], st));
   return AFQ_OK;
 }
 
@@ -617,6 +618,8 @@ void init_job(afq_job* j, const afq_job_desc* d) {
 int alloc_job(afq_job* j, bool dry, int64_t* total) {
   const long N = j->N, G = j->G, W = j->W;
   const long wmax = std::max(j->wcols, std::max(j->nocc[0], j->nocc[1]));
+  // propagation panel: both spins' columns side by side for unrestricted walkers
+  const long wprop = j->restricted ? j->wcols : (long)j->nocc[0] + j->nocc[1];
   const int kch = j->kchunk_max;
   int rc = AFQ_OK;
   *total = 0;
@@ -668,12 +671,12 @@ int alloc_job(afq_job* j, bool dry, int64_t* total) {
     A_(j->rot_chol32[0], G * j->nocc[0] * N);
     if (!j->rhf) A_(j->rot_chol32[1], G * j->nocc[1] * N);
     A_(j->vhs32, W * N * N);
-    for (auto& t : j->t32) A_(t, W * N * wmax);
-    A_(j->tbuf[0], W * N * wmax);
-    A_(j->tbuf[2], W * N * wmax);
+    for (auto& t : j->t32) A_(t, W * N * wprop);
+    A_(j->tbuf[0], W * N * wprop);
+    A_(j->tbuf[2], W * N * wprop);
   } else {
     A_(j->vhs, W * N * N);
-    for (auto& t : j->tbuf) A_(t, W * N * wmax);
+    for (auto& t : j->tbuf) A_(t, W * N * wprop);
   }
   A_(j->ov_new, W);
   A_(j->coul, 2 * W * G);

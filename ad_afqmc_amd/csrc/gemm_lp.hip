@@ -387,6 +387,11 @@ int dispatch_shape(const Gemm& g, hipStream_t s) {
     if (g.N <= 48) return launch<BF, AC, BC, AT, BT, TA, TB, TCo, 4, 3, 1, 1>(g, s);
     return launch<BF, AC, BC, AT, BT, TA, TB, TCo, 4, 4, 1, 1>(g, s);
   }
+  // both spins' columns of unrestricted walkers (65..128 columns)
+  if (g.M > 64 && g.N <= 128 && g.batch > 1) {
+    if constexpr (BF) return launch<BF, AC, BC, AT, BT, TA, TB, TCo, 4, 4, 1, 2>(g, s);
+    return launch<BF, AC, BC, AT, BT, TA, TB, TCo, 4, 4, 1, 1>(g, s);
+  }
   if (g.N <= 48) return launch<BF, AC, BC, AT, BT, TA, TB, TCo, 4, 1, 1, 3>(g, s);
   if (g.N <= 64 && g.M > 64) return launch<BF, AC, BC, AT, BT, TA, TB, TCo, 4, 1, 1, 4>(g, s);
   return launch<BF, AC, BC, AT, BT, TA, TB, TCo, 2, 2, 2, 2>(g, s);
