@@ -84,6 +84,12 @@ int gemm_lp(LpKind kind, bool bf16x3, const Gemm& g, hipStream_t stream);
 int batched_det_inv(const double2* S, long strideS, int n, int count, double2* det, double2* inv, long strideInv,
                     hipStream_t stream);
 
+// Upper Cholesky factor R (S = R^H R, positive real diagonal, zero strict lower
+// triangle) of `count` Hermitian positive-definite n x n matrices, n <= 64.
+int batched_cholesky_upper(const double2* S, long strideS, int n, int count, double2* R, long strideR,
+                           hipStream_t stream);
+int k_conj(const double2* x, double2* y, long n, hipStream_t stream);
+
 // Classical Gram-Schmidt with re-orthogonalisation (CGS2) on `count` row-major
 // n_rows x n_cols complex panels (leading dimension ld, batch stride sW), in place.
 int batched_cgs2(double2* Wp, long ld, long sW, int n_rows, int n_cols, int count, hipStream_t stream);

@@ -467,12 +467,66 @@ int propagate(afq_job* j, void* up, void* dn, const double* fields, double* weig
   return k_weight_update(j->ov_new, overlaps, j->terms, weights, scalars, j->h0_prop, j->d.dt, j->W, st);
 }
 
+// Cholesky QR, twice (CholeskyQR2): S = W^H W on the matrix cores, S = R^H R
+// with a positive real diagonal, W <- W R^-1.  This is the QR factorisation
+// whose R has a positive real diagonal -- the one CGS2 computes -- and is
+// accurate for the well-conditioned panels of a block (re-orthonormalised
+// every block); the second pass restores orthogonality to rounding.
+int cholqr2_panel(afq_job* j, double2* Wp, int c, int s, hipStream_t st) {
+  const int N = j->N, W = j->W;
+  const long per = (long)N * c, cc = (long)c * c;
+  double2* tmp = j->wgather;
+  double2* S = j->S[s];
+  double2* Rinv = j->Sinv[s];
+  for (int pass = 0; pass < 2; ++pass) {
+    AFQ_TRY(k_conj(Wp, tmp, per * W, st));
+    Gemm g;
+    g.M = c;
+    g.N = c;
+    g.K = N;
+    g.batch = W;
+    g.a_cplx = true;
+    g.a_trans = true;
+    g.A = tmp;
+    g.lda = c;
+    g.sA = per;
+    g.b_cplx = true;
+    g.B = Wp;
+    g.ldb = c;
+    g.sB = per;
+    g.C = S;
+    g.ldc = c;
+    g.sC = cc;
+    AFQ_TRY(gemm_f64(g, st));
+    AFQ_TRY(batched_cholesky_upper(S, cc, c, W, S, cc, st));
+    AFQ_TRY(batched_det_inv(S, cc, c, W, j->det[s], Rinv, cc, st));
+    Gemm q;
+    q.M = N;
+    q.N = c;
+    q.K = c;
+    q.batch = W;
+    q.a_cplx = true;
+    q.A = Wp;
+    q.lda = c;
+    q.sA = per;
+    q.b_cplx = true;
+    q.B = Rinv;
+    q.ldb = c;
+    q.sB = cc;
+    q.C = tmp;
+    q.ldc = c;
+    q.sC = per;
+    AFQ_TRY(gemm_f64(q, st));
+    AFQ_HIP(hipMemcpyAsync(Wp, tmp, sizeof(double2) * per * W, hipMemcpyDeviceToDevice, st));
+  }
+  return AFQ_OK;
+}
+
 int orthonormalize(afq_job* j, void* up, void* dn, hipStream_t st) {
-  const int N = j->N;
   ProfScope ps(j, AFQ_K_QR, st);
-  if (j->restricted) return batched_cgs2(static_cast<double2*>(up), j->wcols, (long)N * j->wcols, N, j->wcols, j->W, st);
-  AFQ_TRY(batched_cgs2(static_cast<double2*>(up), j->nocc[0], (long)N * j->nocc[0], N, j->nocc[0], j->W, st));
-  return batched_cgs2(static_cast<double2*>(dn), j->nocc[1], (long)N * j->nocc[1], N, j->nocc[1], j->W, st);
+  if (j->restricted) return cholqr2_panel(j, static_cast<double2*>(up), j->wcols, 0, st);
+  AFQ_TRY(cholqr2_panel(j, static_cast<double2*>(up), j->nocc[0], 0, st));
+  return cholqr2_panel(j, static_cast<double2*>(dn), j->nocc[1], 1, st);
 }
 
 int require(const afq_job* j, bool meas, bool prop) {

@@ -2,8 +2,12 @@
 //   * Gauss-Jordan determinant + inverse of the O x O overlap matrix T^H W
 //     (replaces jnp.linalg.det / jnp.linalg.inv, ad_afqmc/wavefunctions.py:1066,
 //     1100, 1266-1268, 1306-1311)
-//   * CGS2 re-orthonormalisation of an N x O walker panel (replaces the batched
-//     Householder QR of ad_afqmc/linalg_utils.py:51-72; R is discarded there).
+//   * upper Cholesky factor of the O x O Gram matrix W^H W, the middle step of
+//     the Cholesky-QR re-orthonormalisation of the block loop (replaces the
+//     batched Householder QR of ad_afqmc/linalg_utils.py:51-72; R is
+//     discarded there)
+//   * CGS2 re-orthonormalisation of an N x O walker panel, kept for the
+//     job-free walker entry point, where no Gram or triangle workspace exists.
 #include "afq_internal.h"
 
 namespace afq {
@@ -195,6 +199,52 @@ __global__ __launch_bounds__(256) void cgs2_kernel(double2* __restrict__ Wp, lon
   }
 }
 
+// Upper Cholesky factor R (S = R^H R, positive real diagonal) of `count`
+// Hermitian positive-definite n x n matrices (n <= 64), one 256-thread
+// work-group per matrix in dynamic LDS; thread t owns column t % 64 for the
+// rows t/64, t/64 + 4, ...  The strict lower triangle of R is written as zero.
+__global__ __launch_bounds__(256) void cholesky_upper_kernel(const double2* __restrict__ S, long strideS, int n,
+                                                             double2* __restrict__ R, long strideR) {
+  extern __shared__ double2 a[];
+  const int tid = threadIdx.x;
+  const int j = tid & 63, rg = tid >> 6;
+  const long b = blockIdx.x;
+  const double2* src = S + b * strideS;
+  const int nn = n * n;
+  for (int i = tid; i < nn; i += 256) a[i] = src[i];
+  __syncthreads();
+  for (int k = 0; k < n; ++k) {
+    const double dinv = 1.0 / sqrt(a[k * n + k].x);
+    __syncthreads();
+    if (rg == 0 && j >= k && j < n) {
+      const double2 x = a[k * n + j];
+      a[k * n + j] = (j == k) ? double2{1.0 / dinv, 0.0} : double2{x.x * dinv, x.y * dinv};
+    }
+    __syncthreads();
+    if (j > k && j < n) {
+      const double2 rkj = a[k * n + j];
+      for (int i = k + 1 + rg; i <= j; i += 4) {
+        const double2 rki = a[k * n + i];  // S_ij -= conj(R_ki) R_kj
+        double2 x = a[i * n + j];
+        x.x -= rki.x * rkj.x + rki.y * rkj.y;
+        x.y -= rki.x * rkj.y - rki.y * rkj.x;
+        a[i * n + j] = x;
+      }
+    }
+    __syncthreads();
+  }
+  double2* dst = R + b * strideR;
+  for (int i = tid; i < nn; i += 256) {
+    const int r = i / n, c = i - r * n;
+    dst[i] = (c >= r) ? a[i] : double2{0.0, 0.0};
+  }
+}
+
+__global__ void conj_kernel(const double2* __restrict__ x, double2* __restrict__ y, long n) {
+  const long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (i < n) y[i] = double2{x[i].x, -x[i].y};
+}
+
 }  // namespace
 
 int batched_det_inv(const double2* S, long strideS, int n, int count, double2* det, double2* inv, long strideInv,
@@ -213,6 +263,30 @@ int batched_det_inv(const double2* S, long strideS, int n, int count, double2* d
   }
   hipLaunchKernelGGL(gj_det_inv_kernel, dim3(count), dim3(256), lds, stream, S, strideS, n, det, inv, strideInv);
   return launch_status("gj_det_inv_kernel");
+}
+
+int batched_cholesky_upper(const double2* S, long strideS, int n, int count, double2* R, long strideR,
+                           hipStream_t stream) {
+  if (n < 1 || n > 64) return fail(AFQ_E_UNSUPPORTED, "Cholesky kernel supports 1 <= n <= 64");
+  if (count <= 0) return AFQ_OK;
+  const size_t lds = sizeof(double2) * n * n;
+  if (lds > 65536) {
+    static bool raised = false;
+    if (!raised) {
+      hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(cholesky_upper_kernel),
+                                         hipFuncAttributeMaxDynamicSharedMemorySize, 96 * 1024);
+      if (e != hipSuccess) return fail(AFQ_E_HIP, std::string("Cholesky LDS attribute: ") + hipGetErrorString(e));
+      raised = true;
+    }
+  }
+  hipLaunchKernelGGL(cholesky_upper_kernel, dim3(count), dim3(256), lds, stream, S, strideS, n, R, strideR);
+  return launch_status("cholesky_upper_kernel");
+}
+
+int k_conj(const double2* x, double2* y, long n, hipStream_t stream) {
+  if (n <= 0) return AFQ_OK;
+  hipLaunchKernelGGL(conj_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, stream, x, y, n);
+  return launch_status("conj_kernel");
 }
 
 int batched_cgs2(double2* Wp, long ld, long sW, int n_rows, int n_cols, int count, hipStream_t stream) {

@@ -214,6 +214,37 @@ def test_orthonormalize_matches_oracle_up_to_phases():
     assert rel(e, otrial.calc_energy(qo, oham["h0"], oham["rot_h1"], oham["rot_chol"])) < TOL
 
 
+@pytest.mark.parametrize("case", [("uhf", "unrestricted", 30, 7, 5, 12, 5), ("rhf", "restricted", 70, 17, 17, 130, 9)])
+def test_block_qr_equals_gram_schmidt_and_oracle(case):
+    """The block loop's QR (Cholesky QR, twice) against Gram-Schmidt with
+    re-orthogonalisation (both give R with a positive real diagonal, so the
+    same Q) and against the oracle's Householder QR up to column signs, on
+    walkers after three propagation steps."""
+    from ad_afqmc_amd.binding import engine_for
+
+    kind, wtype, N, ou, od, G, W = case
+    sysm, trial, prop, ham, wave, walkers, otrial, oham, owalk, _ = _setup(*case)
+    job = engine_for(trial, wave, ham, wtype, W, prop)
+    up, dn = (t.clone() if t is not None else None for t in walkers.tensors())
+    weights = torch.ones(W, dtype=torch.float64, device="cuda")
+    ov = job.overlap(up, dn)
+    sc = torch.zeros(2, dtype=torch.float64, device="cuda")
+    rng = np.random.default_rng(3)
+    for _ in range(3):
+        job.propagate(up, dn, torch.as_tensor(rng.standard_normal((W, G)), device="cuda"), weights, ov, sc)
+    ref = [t.clone() for t in (up, dn) if t is not None]
+    gs = [wk.RHFWalkers(r).orthonormalize().data for r in ref] if wtype == "restricted" else \
+        wk.UHFWalkers(ref).orthonormalize().data
+    job.orthonormalize(up, dn)
+    for q, g, r in zip([t for t in (up, dn) if t is not None], gs, ref):
+        qn, gn, rn = npy(q), npy(g), npy(r)
+        assert np.allclose(np.einsum("wpi,wpj->wij", qn.conj(), qn), np.eye(qn.shape[2])[None], atol=1e-12)
+        assert rel(qn, gn) < 1e-10
+        qo = O.qr_restricted(rn)[0]
+        m = np.einsum("wpi,wpj->wij", qn.conj(), qo)
+        assert np.allclose(np.abs(m), np.eye(qn.shape[2])[None], atol=1e-10)
+
+
 def test_stochastic_reconfiguration_matches_oracle():
     rng = np.random.default_rng(7)
     W, N, Oo = 37, 8, 3
