@@ -58,6 +58,10 @@ struct Gemm {
   long ldc = 0, sC = 0;
   void* D = nullptr;
   long ldd = 0, sD = 0;
+  // reduced-precision products only: with E set, D = E + alpha*A.B (E read,
+  // complex double) instead of D += alpha*A.B; C may then be null
+  const void* E = nullptr;
+  long lde = 0, sE = 0;
   double2 alpha = {1.0, 0.0};
   double2 beta = {0.0, 0.0};
   // optional split-K workspace (doubles); used only for unbatched products
@@ -74,7 +78,8 @@ int gemm_splitk_reduce(const Gemm& g, const double* ws, int ksplit, bool cplx, h
 //   TAYLOR_D   A complex float (N x N) . B complex double (N x c)    -> C complex float, D complex double +=
 //   TAYLOR_F   A complex float (N x N) . B complex float (N x c)     -> C complex float, D complex double +=
 //   FORCE_BIAS A complex double (W x ON) . B float^T (G x ON)        -> C complex double
-enum class LpKind { VHS, TAYLOR_D, TAYLOR_F, FORCE_BIAS };
+//   ONE_BODY   A float (N x N) . B complex double (N x c)            -> D = E + A.B (FP32 MFMA always)
+enum class LpKind { VHS, TAYLOR_D, TAYLOR_F, FORCE_BIAS, ONE_BODY };
 int gemm_lp(LpKind kind, bool bf16x3, const Gemm& g, hipStream_t stream);
 
 // ---------------------------------------------------------------- small dense linear algebra
@@ -127,6 +132,7 @@ struct afq_job {
   double2* h0_prop = nullptr;   // device scalar
   double2 h0_prop_host = {0, 0};
   double* exp_h1[2] = {nullptr, nullptr}; // (N,N)
+  float* exp_h1m32[2] = {nullptr, nullptr}; // (N,N) exp_h1 - I in FP32, reduced-precision jobs
 
   // scratch (per job)
   double2* S[2] = {nullptr, nullptr};     // (W, nocc, nocc)

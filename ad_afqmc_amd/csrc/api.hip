@@ -72,6 +72,8 @@ void release(afq_job* j) {
   fr(j->tmp_nn);
   fr(j->gemm_ws);
   fr(j->chol32);
+  fr(j->exp_h1m32[0]);
+  fr(j->exp_h1m32[1]);
   fr(j->rot_chol32[0]);
   if (!j->rhf) fr(j->rot_chol32[1]);
   fr(j->vhs32);
@@ -111,37 +113,42 @@ int check_walkers(const afq_job* j, const void* up, const void* dn) {
   return AFQ_OK;
 }
 
-// S_s = T_s^H W_s ; det_s ; optionally S_s^-1 and G_s = (W_s S_s^-1)^T.
-int green_pipeline(afq_job* j, const void* up, const void* dn, bool want_inv, bool want_green, hipStream_t st) {
+// S_s = T_s^H W_s ; det_s ; optionally S_s^-1 and G_s = (W_s S_s^-1)^T.  With
+// have_inv the caller guarantees that S_s^-1 and det_s already belong to these
+// walkers (computed by the previous call on them), and only G_s is formed.
+int green_pipeline(afq_job* j, const void* up, const void* dn, bool want_inv, bool want_green, hipStream_t st,
+                   bool have_inv) {
   const int N = j->N, W = j->W;
   for (int s = 0; s < j->nspin_meas; ++s) {
     const int O = j->nocc[s];
     const double2* P;
     long ld;
     panel(j, s, up, dn, &P, &ld);
-    Gemm g;
-    g.M = O;
-    g.N = O;
-    g.K = N;
-    g.batch = W;
-    g.a_trans = true;
-    g.A = j->mo[s];
-    g.lda = O;
-    g.sA = 0;
-    g.b_cplx = true;
-    g.B = P;
-    g.ldb = ld;
-    g.sB = (long)N * ld;
-    g.C = j->S[s];
-    g.ldc = O;
-    g.sC = (long)O * O;
-    {
-      ProfScope ps(j, AFQ_K_OVLP_GEMM, st);
-      AFQ_TRY(gemm_f64(g, st));
-    }
-    {
-      ProfScope ps(j, AFQ_K_DETINV, st);
-      AFQ_TRY(batched_det_inv(j->S[s], (long)O * O, O, W, j->det[s], want_inv ? j->Sinv[s] : nullptr, (long)O * O, st));
+    if (!have_inv) {
+      Gemm g;
+      g.M = O;
+      g.N = O;
+      g.K = N;
+      g.batch = W;
+      g.a_trans = true;
+      g.A = j->mo[s];
+      g.lda = O;
+      g.sA = 0;
+      g.b_cplx = true;
+      g.B = P;
+      g.ldb = ld;
+      g.sB = (long)N * ld;
+      g.C = j->S[s];
+      g.ldc = O;
+      g.sC = (long)O * O;
+      {
+        ProfScope ps(j, AFQ_K_OVLP_GEMM, st);
+        AFQ_TRY(gemm_f64(g, st));
+      }
+      {
+        ProfScope ps(j, AFQ_K_DETINV, st);
+        AFQ_TRY(batched_det_inv(j->S[s], (long)O * O, O, W, j->det[s], want_inv ? j->Sinv[s] : nullptr, (long)O * O, st));
+      }
     }
     if (want_green) {
       ProfScope ps(j, AFQ_K_GREEN, st);
@@ -230,8 +237,8 @@ int force_bias_gemm(afq_job* j, double2* out, hipStream_t st) {
   return AFQ_OK;
 }
 
-int energy_pipeline(afq_job* j, const void* up, const void* dn, double2* out, hipStream_t st) {
-  AFQ_TRY(green_pipeline(j, up, dn, true, true, st));
+int energy_pipeline(afq_job* j, const void* up, const void* dn, double2* out, hipStream_t st, bool have_green) {
+  if (!have_green) AFQ_TRY(green_pipeline(j, up, dn, true, true, st));
   const bool two = !j->one_spin;
   {
     ProfScope ps(j, AFQ_K_ENERGY_MISC, st);
@@ -293,9 +300,38 @@ int apply_exp_h1(afq_job* j, const double2* src, long lds, long ss, int s, doubl
   return gemm_f64(g, st);
 }
 
+// dst = src + (exp(-dt/2 h1) - I) src with the product on FP32 MFMA: rounding
+// E - I (entries of order dt |h1|) to FP32 perturbs the effective h1 by a
+// relative 6e-8, and the identity part is carried exactly in complex128.
+int apply_exp_h1_lp(afq_job* j, const double2* src, long lds, long ss, int s, double2* dst, long ldd, long sd,
+                    hipStream_t st) {
+  const int N = j->N;
+  Gemm g;
+  g.M = N;
+  g.N = (int)(j->restricted ? j->wcols : j->nocc[s]);
+  g.K = N;
+  g.batch = j->W;
+  g.A = j->exp_h1m32[j->restricted ? 0 : s];
+  g.lda = N;
+  g.sA = 0;
+  g.b_cplx = true;
+  g.B = src;
+  g.ldb = lds;
+  g.sB = ss;
+  g.E = src;
+  g.lde = lds;
+  g.sE = ss;
+  g.D = dst;
+  g.ldd = ldd;
+  g.sD = sd;
+  ProfScope ps(j, AFQ_K_EXPH1, st);
+  return gemm_lp(LpKind::ONE_BODY, false, g, st);
+}
+
 // Reduced precision: VHS and the Taylor terms in complex64 on FP32 / split-BF16
-// MFMA, exp(-dt/2 h1) applications and the running Taylor sum in complex128
-// (the reference's vhs_mixed_precision split, propagation.py:154-176, 196-203).
+// MFMA, the running Taylor sum in complex128 (the reference's
+// vhs_mixed_precision split, propagation.py:154-176, 196-203), and the
+// exp(-dt/2 h1) applications as identity plus an FP32 product.
 int trotprop_lp(afq_job* j, void* up, void* dn, const double2* xs, hipStream_t st) {
   const int N = j->N, W = j->W;
   const long NN = (long)N * N;
@@ -347,9 +383,11 @@ int trotprop_lp(afq_job* j, void* up, void* dn, const double2* xs, hipStream_t s
   double2* t0 = j->tbuf[0];
   double2* acc = j->tbuf[2];
   for (int s = 0; s < j->nspin_prop; ++s)
-    AFQ_TRY(apply_exp_h1(j, static_cast<const double2*>(s == 0 ? up : dn), pc.c[s], (long)N * pc.c[s], s,
-                         t0 + pc.off[s], pc.ctot, per, st));
-  AFQ_HIP(hipMemcpyAsync(acc, t0, sizeof(double2) * per * W, hipMemcpyDeviceToDevice, st));
+    AFQ_TRY(apply_exp_h1_lp(j, static_cast<const double2*>(s == 0 ? up : dn), pc.c[s], (long)N * pc.c[s], s,
+                            t0 + pc.off[s], pc.ctot, per, st));
+  // the running sum starts as t0 + (first term), written by the first product
+  if (j->d.n_exp_terms < 2)
+    AFQ_HIP(hipMemcpyAsync(acc, t0, sizeof(double2) * per * W, hipMemcpyDeviceToDevice, st));
   int cur = -1;  // -1: previous term is t0 (complex128)
   for (int n = 1; n < j->d.n_exp_terms; ++n) {
     const int nxt = (cur == 0) ? 1 : 0;
@@ -372,6 +410,11 @@ int trotprop_lp(afq_job* j, void* up, void* dn, const double2* xs, hipStream_t s
     g.D = acc;
     g.ldd = pc.ctot;
     g.sD = per;
+    if (n == 1) {
+      g.E = t0;
+      g.lde = pc.ctot;
+      g.sE = per;
+    }
     g.alpha = double2{1.0 / n, 0.0};
     {
       ProfScope ps(j, AFQ_K_TAYLOR, st);
@@ -380,8 +423,8 @@ int trotprop_lp(afq_job* j, void* up, void* dn, const double2* xs, hipStream_t s
     cur = nxt;
   }
   for (int s = 0; s < j->nspin_prop; ++s)
-    AFQ_TRY(apply_exp_h1(j, acc + pc.off[s], pc.ctot, per, s, static_cast<double2*>(s == 0 ? up : dn), pc.c[s],
-                         (long)N * pc.c[s], st));
+    AFQ_TRY(apply_exp_h1_lp(j, acc + pc.off[s], pc.ctot, per, s, static_cast<double2*>(s == 0 ? up : dn), pc.c[s],
+                            (long)N * pc.c[s], st));
   return AFQ_OK;
 }
 
@@ -453,15 +496,15 @@ int trotprop(afq_job* j, void* up, void* dn, const double2* xs, hipStream_t st) 
 }
 
 int propagate(afq_job* j, void* up, void* dn, const double* fields, double* weights, double2* overlaps,
-              double* scalars, hipStream_t st) {
-  AFQ_TRY(green_pipeline(j, up, dn, true, true, st));
+              double* scalars, hipStream_t st, bool inv_ready, bool keep_inv) {
+  AFQ_TRY(green_pipeline(j, up, dn, true, true, st, inv_ready));
   AFQ_TRY(force_bias_gemm(j, j->fb, st));
   {
     ProfScope ps(j, AFQ_K_FIELD, st);
     AFQ_TRY(k_field_shift(j->fb, fields, j->mf_shifts, j->G, std::sqrt(j->d.dt), j->xs, j->terms, j->W, st));
   }
   AFQ_TRY(trotprop(j, up, dn, j->xs, st));
-  AFQ_TRY(green_pipeline(j, up, dn, false, false, st));
+  AFQ_TRY(green_pipeline(j, up, dn, keep_inv, false, st));
   AFQ_TRY(overlap_from_dets(j, j->ov_new, st));
   ProfScope ps(j, AFQ_K_WEIGHT, st);
   return k_weight_update(j->ov_new, overlaps, j->terms, weights, scalars, j->h0_prop, j->d.dt, j->W, st);
@@ -724,6 +767,8 @@ int alloc_job(afq_job* j, bool dry, int64_t* total) {
     }
     A_(j->rot_chol32[0], G * j->nocc[0] * N);
     if (!j->rhf) A_(j->rot_chol32[1], G * j->nocc[1] * N);
+    A_(j->exp_h1m32[0], N * N);
+    if (!j->restricted) A_(j->exp_h1m32[1], N * N);
     A_(j->vhs32, W * N * N);
     for (auto& t : j->t32) A_(t, W * N * wprop);
     A_(j->tbuf[0], W * N * wprop);
@@ -944,6 +989,7 @@ int afq_build_propagation_intermediates(afq_job_t j, void* stream) {
     // exp(A), A = -dt/2 h1_mod
     AFQ_TRY(k_scale(As, hmod, -0.5 * j->d.dt, N, st));
     AFQ_TRY(expm_device(As, N, j->exp_h1[s], j->tmp_nn + 5 * NN, j->dscratch, st));
+    if (j->lp) AFQ_TRY(k_minus_identity_f32(j->exp_h1[s], j->exp_h1m32[s], N, st));
   }
   AFQ_HIP(hipMemcpyAsync(&j->h0_prop_host, j->h0_prop, sizeof(double2), hipMemcpyDeviceToHost, st));
   AFQ_HIP(hipStreamSynchronize(st));
@@ -1004,7 +1050,10 @@ int afq_set_intermediate(afq_job_t j, int which, int spin, const void* src, void
       break;
     case AFQ_INT_MF_SHIFTS: j->have_mf = true; break;
     case AFQ_INT_H0_PROP: j->have_h0p = true; break;
-    case AFQ_INT_EXP_H1: j->have_exp[spin] = true; break;
+    case AFQ_INT_EXP_H1:
+      j->have_exp[spin] = true;
+      if (j->lp) AFQ_TRY(k_minus_identity_f32(j->exp_h1[spin], j->exp_h1m32[spin], (int)N, S(stream)));
+      break;
     default: break;
   }
   const int nsp = j->rhf ? 1 : 2;
@@ -1118,16 +1167,20 @@ int reserve_fields(afq_job* j, long need, hipStream_t st) {
 }
 
 // Steps over the fields already in j->fields, then QR, overlap, energy and the
-// block bookkeeping (sampling.py:52-94).
+// block bookkeeping (sampling.py:52-94).  The overlap matrices of the walkers a
+// step produces are inverted together with their determinants, so the next
+// step's force bias starts from that inverse instead of forming and factoring
+// T^H W again; after the QR one factorisation serves both the overlap and the
+// energy.
 int block_body(afq_job* j, void* up, void* dn, double* weights, double2* ov, double* scalars, int n_prop_steps,
                double* block_out, hipStream_t st) {
   const long per_step = (long)j->W * j->G;
   for (int i = 0; i < n_prop_steps; ++i)
-    AFQ_TRY(propagate(j, up, dn, j->fields + i * per_step, weights, ov, scalars, st));
+    AFQ_TRY(propagate(j, up, dn, j->fields + i * per_step, weights, ov, scalars, st, i > 0, i + 1 < n_prop_steps));
   AFQ_TRY(orthonormalize(j, up, dn, st));
-  AFQ_TRY(green_pipeline(j, up, dn, false, false, st));
+  AFQ_TRY(green_pipeline(j, up, dn, true, true, st));
   AFQ_TRY(overlap_from_dets(j, ov, st));
-  AFQ_TRY(energy_pipeline(j, up, dn, j->energies, st));
+  AFQ_TRY(energy_pipeline(j, up, dn, j->energies, st, true));
   ProfScope ps(j, AFQ_K_ENERGY_MISC, st);
   return k_block_finalize(j->energies, weights, scalars, std::sqrt(2.0 / j->d.dt), j->W, block_out, st);
 }

@@ -163,7 +163,8 @@ __device__ inline void store_b(double* Bs, const double (&r)[C::RB][4][2], int t
   }
 }
 
-// grid: x = tiles (XCD-remapped when xcd_remap), y = batch, z = K split.
+// grid: x = tiles (XCD-remapped when xcd_remap == 1), y = batch, z = K split;
+// xcd_remap == 2: x = tiles x batch rounded up to 8 entries, y = z = 1.
 template <bool AC, bool BC, bool AT, bool BT, int WM, int WN, int TM, int TN>
 __global__ __launch_bounds__(64 * WM * WN) void gemm_f64_kernel(Gemm g, int tiles_m, int tiles_n, int xcd_remap,
                                                                  long T8, int kchunk, double* ws) {
@@ -175,12 +176,22 @@ __global__ __launch_bounds__(64 * WM * WN) void gemm_f64_kernel(Gemm g, int tile
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wm = wave / WN, wn = wave % WN;
   long tile = blockIdx.x;
-  if (xcd_remap) tile = (tile & 7) * T8 + (tile >> 3);
+  long b = blockIdx.y;
+  if (xcd_remap == 1) {
+    tile = (tile & 7) * T8 + (tile >> 3);
+  } else if (xcd_remap == 2) {
+    // batched, 1-D grid: work-group L runs on XCD L % 8; that XCD takes the
+    // batch entries b = L % 8 (mod 8), all T8 tiles of one entry in a row, so
+    // an operand shared by the tiles of an entry is fetched into one L2
+    const long p = tile >> 3;
+    b = (tile & 7) + 8 * (p / T8);
+    tile = p % T8;
+    if (b >= g.batch) return;  // uniform per work-group
+  }
   const int mt = (int)(tile % tiles_m);
   const long ntl = tile / tiles_m;
   if (ntl >= tiles_n) return;  // uniform per work-group
   const int m0 = mt * C::BM, n0 = (int)ntl * C::BN;
-  const long b = blockIdx.y;
   const int kz = blockIdx.z;
   const int kbeg = kz * kchunk;
   const int kend = min(g.K, kbeg + kchunk);
@@ -255,7 +266,7 @@ __global__ __launch_bounds__(64 * WM * WN) void gemm_f64_kernel(Gemm g, int tile
   // ---- epilogue
   if (ws) {  // split-K partial: raw accumulators to ws[kz][b][m][n]
     const long plane = (long)g.M * g.N;
-    double* W = ws + ((long)kz * gridDim.y + b) * plane * (CC ? 2 : 1);
+    double* W = ws + ((long)kz * g.batch + b) * plane * (CC ? 2 : 1);
 #pragma unroll
     for (int i = 0; i < TM; ++i)
 #pragma unroll
@@ -371,11 +382,12 @@ int launch(const Gemm& g, hipStream_t s) {
     kchunk = ((g.K + ksplit - 1) / ksplit + BK - 1) / BK * BK;
     ksplit = (g.K + kchunk - 1) / kchunk;
   }
-  const int remap = (g.batch == 1 && tiles >= 64) ? 1 : 0;
-  const long T8 = remap ? (tiles + 7) / 8 : 0;
-  const long gx = remap ? T8 * 8 : tiles;
-  if (gx > 0x7fffffffL || g.batch > 65535 || ksplit > 65535) return fail(AFQ_E_UNSUPPORTED, "gemm_f64: grid too large");
-  dim3 grid((unsigned)gx, g.batch, ksplit);
+  const int remap = (g.batch == 1 && tiles >= 64) ? 1 : (g.batch > 1 && tiles > 1 && ksplit == 1) ? 2 : 0;
+  const long T8 = remap == 1 ? (tiles + 7) / 8 : remap == 2 ? tiles : 0;
+  const long gx = remap == 1 ? T8 * 8 : remap == 2 ? tiles * ((g.batch + 7) / 8 * 8) : tiles;
+  const unsigned gy = remap == 2 ? 1u : (unsigned)g.batch;
+  if (gx > 0x7fffffffL || gy > 65535u || ksplit > 65535) return fail(AFQ_E_UNSUPPORTED, "gemm_f64: grid too large");
+  dim3 grid((unsigned)gx, gy, ksplit);
   double* ws = ksplit > 1 ? static_cast<double*>(g.ws) : nullptr;
   hipLaunchKernelGGL((gemm_f64_kernel<AC, BC, AT, BT, WM, WN, TM, TN>), grid, dim3(C::NT), 0, s, g, tm, tn, remap, T8,
                      kchunk, ws);
@@ -424,6 +436,7 @@ int gemm_splitk_reduce(const Gemm& g, const double* ws, int ksplit, bool cplx, h
 int gemm_f64(const Gemm& g, hipStream_t s) {
   if (g.M <= 0 || g.N <= 0 || g.batch <= 0) return AFQ_OK;
   if (g.K <= 0) return fail(AFQ_E_INVALID, "gemm_f64: K must be positive");
+  if (g.E) return fail(AFQ_E_UNSUPPORTED, "gemm_f64: no separate addend");
   if (g.a_cplx) return g.b_cplx ? dispatch_t<true, true>(g, s) : dispatch_t<true, false>(g, s);
   return g.b_cplx ? dispatch_t<false, true>(g, s) : dispatch_t<false, false>(g, s);
 }

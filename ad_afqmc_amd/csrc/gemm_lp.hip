@@ -10,7 +10,8 @@
 //            a product is hi*hi + hi*lo + lo*hi on v_mfma_f32_16x16x32_bf16
 //            (lane l holds 8 consecutive k), i.e. ~16 significant bits per
 //            operand and FP32 accumulation.
-// Output C may be float2 (complex64) or double2; D (double2) += alpha*result.
+// Output C may be float2 (complex64) or double2; D (double2) += alpha*result, or
+// D = E + alpha*result with a separate complex-double addend E.
 //
 // MFMA f32 16x16 lane layout (gfx950): accumulator element v of lane l is
 // D[4*(l/16) + v][l%16]; f32 16x16x4 inputs A[l%16][l/16], B[l/16][l%16];
@@ -157,7 +158,8 @@ __device__ inline v4f mma3(const BfPair& a, const BfPair& b, v4f c) {
   return c;
 }
 
-// grid: x = tiles (XCD-remapped when xcd_remap), y = batch, z = K split.
+// grid: x = tiles (XCD-remapped when xcd_remap == 1), y = batch, z = K split;
+// xcd_remap == 2: x = tiles x batch rounded up to 8 entries, y = z = 1.
 template <bool BF, bool AC, bool BC, bool AT, bool BT, typename TA, typename TB, typename TCo, int WM, int WN, int TM,
           int TN>
 __global__ __launch_bounds__(64 * WM * WN) void gemm_lp_kernel(Gemm g, int tiles_m, int tiles_n, int xcd_remap,
@@ -170,12 +172,22 @@ __global__ __launch_bounds__(64 * WM * WN) void gemm_lp_kernel(Gemm g, int tiles
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wm = wave / WN, wn = wave % WN;
   long tile = blockIdx.x;
-  if (xcd_remap) tile = (tile & 7) * T8 + (tile >> 3);
+  long b = blockIdx.y;
+  if (xcd_remap == 1) {
+    tile = (tile & 7) * T8 + (tile >> 3);
+  } else if (xcd_remap == 2) {
+    // batched, 1-D grid: work-group L runs on XCD L % 8; that XCD takes the
+    // batch entries b = L % 8 (mod 8), all T8 tiles of one entry in a row, so
+    // an operand shared by the tiles of an entry is fetched into one L2
+    const long p = tile >> 3;
+    b = (tile & 7) + 8 * (p / T8);
+    tile = p % T8;
+    if (b >= g.batch) return;  // uniform per work-group
+  }
   const int mt = (int)(tile % tiles_m);
   const long ntl = tile / tiles_m;
   if (ntl >= tiles_n) return;
   const int m0 = mt * C::BM, n0 = (int)ntl * C::BN;
-  const long b = blockIdx.y;
   const int kz = blockIdx.z;
   const int kbeg = kz * kchunk;
   const int kend = min(g.K, kbeg + kchunk);
@@ -289,7 +301,7 @@ __global__ __launch_bounds__(64 * WM * WN) void gemm_lp_kernel(Gemm g, int tiles
 
   if (ws) {  // split-K partial (double) to ws[kz][b][m][n]
     const long plane = (long)g.M * g.N;
-    double* Wp = ws + ((long)kz * gridDim.y + b) * plane * (CC ? 2 : 1);
+    double* Wp = ws + ((long)kz * g.batch + b) * plane * (CC ? 2 : 1);
 #pragma unroll
     for (int i = 0; i < TM; ++i)
 #pragma unroll
@@ -308,8 +320,9 @@ __global__ __launch_bounds__(64 * WM * WN) void gemm_lp_kernel(Gemm g, int tiles
     return;
   }
   const bool has_beta = (g.beta.x != 0.0 || g.beta.y != 0.0);
-  TCo* Cp = static_cast<TCo*>(g.C) + b * g.sC;
+  TCo* Cp = g.C ? static_cast<TCo*>(g.C) + b * g.sC : nullptr;
   double2* Dp = g.D ? static_cast<double2*>(g.D) + b * g.sD : nullptr;
+  const double2* Ep = g.E ? static_cast<const double2*>(g.E) + b * g.sE : nullptr;
 #pragma unroll
   for (int i = 0; i < TM; ++i)
 #pragma unroll
@@ -321,18 +334,20 @@ __global__ __launch_bounds__(64 * WM * WN) void gemm_lp_kernel(Gemm g, int tiles
         if (m >= g.M || n >= g.N) continue;
         const long off = (long)m * g.ldc + n;
         double2 r = cmul(g.alpha, double2{(double)accR[i][j][v], (double)accI[i][j][v]});
-        if (has_beta) {
-          const double2 o = cmul(g.beta, double2{(double)Cp[off].x, (double)Cp[off].y});
-          r.x += o.x;
-          r.y += o.y;
+        if (Cp) {
+          if (has_beta) {
+            const double2 o = cmul(g.beta, double2{(double)Cp[off].x, (double)Cp[off].y});
+            r.x += o.x;
+            r.y += o.y;
+          }
+          if constexpr (sizeof(TCo) == 8)
+            Cp[off] = TCo{(float)r.x, (float)r.y};
+          else
+            Cp[off] = TCo{r.x, r.y};
         }
-        if constexpr (sizeof(TCo) == 8)
-          Cp[off] = TCo{(float)r.x, (float)r.y};
-        else
-          Cp[off] = TCo{r.x, r.y};
         if (Dp) {
           double2* Dc = Dp + (long)m * g.ldd + n;
-          double2 d = *Dc;
+          double2 d = Ep ? Ep[(long)m * g.lde + n] : *Dc;
           d.x += r.x;
           d.y += r.y;
           *Dc = d;
@@ -350,7 +365,7 @@ int launch(const Gemm& g, hipStream_t s) {
   int ksplit = 1;
   const long wgs = tiles * g.batch;
   // split-K partials are reduced into a double2 / double C only
-  if (sizeof(TCo) == 16 && g.ws && g.batch == 1 && wgs < 1024 && g.K >= 8 * BKL) {
+  if (sizeof(TCo) == 16 && g.ws && !g.E && g.batch == 1 && wgs < 1024 && g.K >= 8 * BKL) {
     ksplit = (int)std::min<long>((2048 + wgs - 1) / wgs, (long)g.K / (4 * BKL));
     while (ksplit > 1 && (long)ksplit * g.M * g.N * (CC ? 2 : 1) > g.ws_elems) --ksplit;
   }
@@ -359,13 +374,14 @@ int launch(const Gemm& g, hipStream_t s) {
     kchunk = ((g.K + ksplit - 1) / ksplit + BKL - 1) / BKL * BKL;
     ksplit = (g.K + kchunk - 1) / kchunk;
   }
-  const int remap = (g.batch == 1 && tiles >= 64) ? 1 : 0;
-  const long T8 = remap ? (tiles + 7) / 8 : 0;
-  const long gx = remap ? T8 * 8 : tiles;
-  if (gx > 0x7fffffffL || g.batch > 65535) return fail(AFQ_E_UNSUPPORTED, "gemm_lp: grid too large");
+  const int remap = (g.batch == 1 && tiles >= 64) ? 1 : (g.batch > 1 && tiles > 1 && ksplit == 1) ? 2 : 0;
+  const long T8 = remap == 1 ? (tiles + 7) / 8 : remap == 2 ? tiles : 0;
+  const long gx = remap == 1 ? T8 * 8 : remap == 2 ? tiles * ((g.batch + 7) / 8 * 8) : tiles;
+  const unsigned gy = remap == 2 ? 1u : (unsigned)g.batch;
+  if (gx > 0x7fffffffL || gy > 65535u) return fail(AFQ_E_UNSUPPORTED, "gemm_lp: grid too large");
   double* ws = ksplit > 1 ? static_cast<double*>(g.ws) : nullptr;
   hipLaunchKernelGGL((gemm_lp_kernel<BF, AC, BC, AT, BT, TA, TB, TCo, WM, WN, TM, TN>),
-                     dim3((unsigned)gx, g.batch, ksplit), dim3(C::NT), 0, s, g, tm, tn, remap, T8, kchunk, ws);
+                     dim3((unsigned)gx, gy, ksplit), dim3(C::NT), 0, s, g, tm, tn, remap, T8, kchunk, ws);
   AFQ_TRY(launch_status("gemm_lp_kernel"));
   if (ksplit > 1) return gemm_splitk_reduce(g, ws, ksplit, CC, s);
   return AFQ_OK;
@@ -408,6 +424,8 @@ int dispatch(LpKind kind, const Gemm& g, hipStream_t s) {
       return dispatch_shape<BF, true, true, false, false, float, float, float2>(g, s);
     case LpKind::FORCE_BIAS:  // complex double x real float (transposed) -> complex double
       return dispatch_shape<BF, true, false, false, true, double, float, double2>(g, s);
+    case LpKind::ONE_BODY:  // real float x complex double -> D = E + result (complex double)
+      return dispatch_shape<false, false, true, false, false, float, double, double2>(g, s);
   }
   return fail(AFQ_E_INVALID, "gemm_lp: unknown kind");
 }
@@ -417,6 +435,8 @@ int dispatch(LpKind kind, const Gemm& g, hipStream_t s) {
 int gemm_lp(LpKind kind, bool bf16x3, const Gemm& g, hipStream_t s) {
   if (g.M <= 0 || g.N <= 0 || g.batch <= 0) return AFQ_OK;
   if (g.K <= 0) return fail(AFQ_E_INVALID, "gemm_lp: K must be positive");
+  if (g.E && !g.D) return fail(AFQ_E_INVALID, "gemm_lp: an addend E needs an output D");
+  if (!g.C && !g.D) return fail(AFQ_E_INVALID, "gemm_lp: no output");
   return bf16x3 ? dispatch<true>(kind, g, s) : dispatch<false>(kind, g, s);
 }
 

@@ -29,6 +29,8 @@ int k_split_bf16(const double* src, bool cplx, int O, int N, long rows, uint16_t
 int k_exchange_bf16(const uint16_t* rcs, const uint16_t* gs, int O, int N, int Gn, int W, double2* coul,
                     double2* kpart, int* nchunk_out, hipStream_t s);
 int k_to_f32(const double* src, float* dst, long n, hipStream_t s);
+// dst = src - I (n x n), rounded to FP32 after the subtraction
+int k_minus_identity_f32(const double* src, float* dst, int n, hipStream_t s);
 int k_energy_finalize(const double2* ene1, const double2* c0, const double2* c1, const double2* k0, const double2* k1,
                       int Gn, int nch0, int nch1, double h0, double2* out, int W, hipStream_t s);
 

@@ -383,6 +383,14 @@ __global__ void to_f32_kernel(const double* src, float* dst, long n) {
   if (i < n) dst[i] = (float)src[i];
 }
 
+// dst = E - I in FP32 for an n x n row-major E (the identity is subtracted in FP64).
+__global__ void minus_identity_f32_kernel(const double* src, float* dst, int n) {
+  const long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (i >= (long)n * n) return;
+  const double v = src[i] - ((i / n) == (i % n) ? 1.0 : 0.0);
+  dst[i] = (float)v;
+}
+
 inline unsigned blocks_for(long n, int t) { return (unsigned)((n + t - 1) / t); }
 
 }  // namespace
@@ -463,6 +471,11 @@ int k_to_f32(const double* src, float* dst, long n, hipStream_t s) {
   if (n <= 0) return AFQ_OK;
   hipLaunchKernelGGL(to_f32_kernel, dim3(blocks_for(n, 256)), dim3(256), 0, s, src, dst, n);
   return launch_status("to_f32_kernel");
+}
+int k_minus_identity_f32(const double* src, float* dst, int n, hipStream_t s) {
+  if (n <= 0) return AFQ_OK;
+  hipLaunchKernelGGL(minus_identity_f32_kernel, dim3(blocks_for((long)n * n, 256)), dim3(256), 0, s, src, dst, n);
+  return launch_status("minus_identity_f32_kernel");
 }
 int k_normal_fill(double* out, long n, uint64_t seed, uint64_t counter, hipStream_t s) {
   const long pairs = (n + 1) / 2;
