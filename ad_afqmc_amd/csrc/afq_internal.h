@@ -88,6 +88,16 @@ int gemm_lp(LpKind kind, bool bf16x3, const Gemm& g, hipStream_t stream);
 // det (count,) and, if inv != nullptr, the inverse.  n <= 64.
 int batched_det_inv(const double2* S, long strideS, int n, int count, double2* det, double2* inv, long strideInv,
                     hipStream_t stream);
+// One launch over one or two sets of `count` matrices each (e.g. both spins).
+struct DetInvSet {
+  const double2* S;
+  long strideS;
+  int n;
+  double2* det;
+  double2* inv;  // nullptr: determinant only
+  long strideInv;
+};
+int batched_det_inv_sets(const DetInvSet* sets, int nsets, int count, hipStream_t stream);
 
 // Upper Cholesky factor R (S = R^H R, positive real diagonal, zero strict lower
 // triangle) of `count` Hermitian positive-definite n x n matrices, n <= 64.

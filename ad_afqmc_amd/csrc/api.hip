@@ -119,12 +119,13 @@ int check_walkers(const afq_job* j, const void* up, const void* dn) {
 int green_pipeline(afq_job* j, const void* up, const void* dn, bool want_inv, bool want_green, hipStream_t st,
                    bool have_inv) {
   const int N = j->N, W = j->W;
-  for (int s = 0; s < j->nspin_meas; ++s) {
-    const int O = j->nocc[s];
-    const double2* P;
-    long ld;
-    panel(j, s, up, dn, &P, &ld);
-    if (!have_inv) {
+  if (!have_inv) {
+    DetInvSet sets[2];
+    for (int s = 0; s < j->nspin_meas; ++s) {
+      const int O = j->nocc[s];
+      const double2* P;
+      long ld;
+      panel(j, s, up, dn, &P, &ld);
       Gemm g;
       g.M = O;
       g.N = O;
@@ -145,33 +146,37 @@ int green_pipeline(afq_job* j, const void* up, const void* dn, bool want_inv, bo
         ProfScope ps(j, AFQ_K_OVLP_GEMM, st);
         AFQ_TRY(gemm_f64(g, st));
       }
-      {
-        ProfScope ps(j, AFQ_K_DETINV, st);
-        AFQ_TRY(batched_det_inv(j->S[s], (long)O * O, O, W, j->det[s], want_inv ? j->Sinv[s] : nullptr, (long)O * O, st));
-      }
+      sets[s] = DetInvSet{j->S[s], (long)O * O, O, j->det[s], want_inv ? j->Sinv[s] : nullptr, (long)O * O};
     }
-    if (want_green) {
-      ProfScope ps(j, AFQ_K_GREEN, st);
-      Gemm h;
-      h.M = O;
-      h.N = N;
-      h.K = O;
-      h.batch = W;
-      h.a_cplx = true;
-      h.a_trans = true;
-      h.A = j->Sinv[s];
-      h.lda = O;
-      h.sA = (long)O * O;
-      h.b_cplx = true;
-      h.b_trans = true;
-      h.B = P;
-      h.ldb = ld;
-      h.sB = (long)N * ld;
-      h.C = j->green[s];
-      h.ldc = N;
-      h.sC = (long)O * N;
-      AFQ_TRY(gemm_f64(h, st));
-    }
+    ProfScope ps(j, AFQ_K_DETINV, st);
+    AFQ_TRY(batched_det_inv_sets(sets, j->nspin_meas, W, st));
+  }
+  if (!want_green) return AFQ_OK;
+  for (int s = 0; s < j->nspin_meas; ++s) {
+    const int O = j->nocc[s];
+    const double2* P;
+    long ld;
+    panel(j, s, up, dn, &P, &ld);
+    ProfScope ps(j, AFQ_K_GREEN, st);
+    Gemm h;
+    h.M = O;
+    h.N = N;
+    h.K = O;
+    h.batch = W;
+    h.a_cplx = true;
+    h.a_trans = true;
+    h.A = j->Sinv[s];
+    h.lda = O;
+    h.sA = (long)O * O;
+    h.b_cplx = true;
+    h.b_trans = true;
+    h.B = P;
+    h.ldb = ld;
+    h.sB = (long)N * ld;
+    h.C = j->green[s];
+    h.ldc = N;
+    h.sC = (long)O * N;
+    AFQ_TRY(gemm_f64(h, st));
   }
   return AFQ_OK;
 }

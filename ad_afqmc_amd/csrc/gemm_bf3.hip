@@ -1,6 +1,6 @@
 // Split-BF16 x3 products on pre-split operands (VHS and force bias).
 //
-//   C[b] (M x N) = alpha * A[b] (M x K) . B[b]^T (N x K)
+//   C[b] (M x N) = alpha * A[b] (M x K, complex) . B[b]^T (N x K, real)
 //
 // Both operands are stored as bf16 planes with k contiguous: a real operand as
 // (hi, lo) = (bf16(x), bf16(x - hi)), a complex one as (re_hi, re_lo, im_hi,
@@ -8,11 +8,7 @@
 // (same for B).  K is a multiple of 32 and zero-padded; rows past the valid
 // ones up to the next multiple of 16 exist and hold zeros, and reads of rows
 // beyond that are clamped (their outputs are discarded).  Each real product is
-// hi*hi + hi*lo + lo*hi on v_mfma_f32_16x16x32_bf16 with FP32 accumulation;
-// the passes run over all of a wave's TM x TN blocks so consecutive MFMAs are
-// independent.  Fragments are read straight from global memory: lane l reads
-// the 8 consecutive k (one 16-byte load) of row 16*block + l%16 at offset
-// 8*(l/16) of each 32-deep step, which is the MFMA's input layout.
+// hi*hi + lo*hi + hi*lo on v_mfma_f32_16x16x32_bf16 with FP32 accumulation.
 //
 // Outputs: complex64 C[b*sC + m*ldc + n] = alpha * acc (the VHS product,
 // feeding the complex64 Taylor terms), or complex128 C = alpha * acc + beta * C
@@ -22,20 +18,35 @@
 #include "gemm_bf3.h"
 
 typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
-typedef unsigned int v4u __attribute__((ext_vector_type(4)));
 
 namespace afq {
 namespace {
 
-__device__ inline v8bf neg8(v8bf x) {
-  v4u u = __builtin_bit_cast(v4u, x);
-  u ^= v4u{0x80008000u, 0x80008000u, 0x80008000u, 0x80008000u};
-  return __builtin_bit_cast(v8bf, u);
-}
 __device__ inline v4f mma(v8bf a, v8bf b, v4f c) { return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0); }
-template <bool AC, bool BC, int WM, int WN, int TM, int TN>
-__global__ __launch_bounds__(64 * WM * WN, 2) void gemm_bf3_kernel(Bf3 p, int tiles_m, int tiles_n, int xcd_remap,
-                                                                   long T8, int kchunk, double* ws) {
+
+// Complex A against real B, staged through LDS.  The complex rows of A act as real rows 2r (re)
+// and 2r + 1 (im), so a wave's 16 x 16 blocks cover re and im rows alike and
+// the accumulator rows 4(l/16) + v pair up as (re, im) of one output element.
+// Per 32-deep k step the work-group copies its A (BMR real rows) and B (BN
+// rows) hi/lo planes to LDS once (16-byte loads, rows padded to 144 bytes so
+// that the 16 rows of a fragment read fall in distinct banks), and the waves
+// read their fragments from there; two LDS stages, one barrier per step.
+// Per accumulator the products run in a fixed order (hi*hi, lo*hi, hi*lo per
+// step, steps ascending), so results do not depend on the tiling.
+constexpr int LROW = 72;  // bf16 per LDS row: hi (32), lo (32), pad (8)
+
+template <int BMR, int BN, int WM, int WN>
+__global__ __launch_bounds__(64 * WM * WN) void gemm_bf3_lds_kernel(Bf3 p, int tiles_m, int tiles_n, int xcd_remap,
+                                                                     long T8, int kchunk, double* ws) {
+  constexpr int NT = 64 * WM * WN;
+  constexpr int TM = BMR / (16 * WM), TN = BN / (16 * WN);
+  constexpr int CA = BMR * 8, CB = BN * 8;  // 16-byte chunks per stage
+  constexpr int RA = (CA + NT - 1) / NT, RB = (CB + NT - 1) / NT;
+  static_assert(TM * 16 * WM == BMR && TN * 16 * WN == BN, "tile");
+  extern __shared__ __align__(16) __bf16 lds[];
+  __bf16* As = lds;                      // [2][BMR][LROW]
+  __bf16* Bs = lds + 2 * BMR * LROW;     // [2][BN][LROW]
+
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wm = wave / WN, wn = wave % WN;
   long tile = blockIdx.x;
@@ -44,133 +55,136 @@ __global__ __launch_bounds__(64 * WM * WN, 2) void gemm_bf3_kernel(Bf3 p, int ti
   const long ntl = tile / tiles_m;
   if (ntl >= tiles_n) return;  // uniform per work-group
   const long b = blockIdx.y;
-  const int m0 = mt * 16 * TM * WM + wm * TM * 16, n0 = (int)ntl * 16 * TN * WN + wn * TN * 16;
-  const int r16 = lane & 15, q4 = lane >> 4;
+  const int m0r = mt * BMR;  // first real row
+  const int n0 = (int)ntl * BN;
   const __bf16* Ab = reinterpret_cast<const __bf16*>(p.A) + b * p.sA;
   const __bf16* Bb = reinterpret_cast<const __bf16*>(p.B) + b * p.sB;
-  const v8bf* pa[TM];
-  const v8bf* pb[TN];
-#pragma unroll
-  for (int i = 0; i < TM; ++i)
-    pa[i] = reinterpret_cast<const v8bf*>(Ab + (long)min(m0 + i * 16 + r16, p.a_rows - 1) * p.lda) + q4;
-#pragma unroll
-  for (int j = 0; j < TN; ++j)
-    pb[j] = reinterpret_cast<const v8bf*>(Bb + (long)min(n0 + j * 16 + r16, p.b_rows - 1) * p.ldb) + q4;
-  const long pA8 = p.pA / 8, pB8 = p.pB / 8;
 
-  v4f accR[TM][TN], accI[TM][TN];
+  // global source and LDS destination of each chunk this thread copies
+  const v8bf* ga[RA];
+  int la[RA];
 #pragma unroll
-  for (int i = 0; i < TM; ++i)
+  for (int i = 0; i < RA; ++i) {
+    const int c = tid + i * NT;
+    const int row = c >> 3, plane = (c >> 2) & 1, part = c & 3;
+    const int gr = m0r + row;
+    const int w = min(gr >> 1, p.a_rows - 1);
+    const int q = 2 * (gr & 1) + plane;  // re_hi, re_lo, im_hi, im_lo
+    ga[i] = reinterpret_cast<const v8bf*>(Ab + (long)w * p.lda + q * p.pA + part * 8);
+    la[i] = c < CA ? row * LROW + plane * 32 + part * 8 : -1;
+  }
+  const v8bf* gb[RB];
+  int lb[RB];
 #pragma unroll
-    for (int j = 0; j < TN; ++j) {
-      accR[i][j] = v4f{0.f, 0.f, 0.f, 0.f};
-      accI[i][j] = v4f{0.f, 0.f, 0.f, 0.f};
-    }
-  const int kz = blockIdx.z;
-  const int kv0 = kz * kchunk / 8, kvn = min(p.K, (kz + 1) * kchunk) / 8;
-  for (int kv = kv0; kv < kvn; kv += 4) {
-    v8bf ar[TM], ai[TM], br[TN], bi[TN];
-#pragma unroll
-    for (int i = 0; i < TM; ++i) {
-      ar[i] = pa[i][kv];
-      if (AC) ai[i] = pa[i][kv + 2 * pA8];
-    }
-#pragma unroll
-    for (int j = 0; j < TN; ++j) {
-      br[j] = pb[j][kv];
-      if (BC) bi[j] = pb[j][kv + 2 * pB8];
-    }
-    // hi x hi
-#pragma unroll
-    for (int i = 0; i < TM; ++i)
-#pragma unroll
-      for (int j = 0; j < TN; ++j) {
-        accR[i][j] = mma(ar[i], br[j], accR[i][j]);
-        if (AC && BC) accR[i][j] = mma(neg8(ai[i]), bi[j], accR[i][j]);
-        if (BC) accI[i][j] = mma(ar[i], bi[j], accI[i][j]);
-        if (AC) accI[i][j] = mma(ai[i], br[j], accI[i][j]);
-      }
-    // lo(A) x hi(B)
-    {
-      v8bf lr[TM], li[TM];
-#pragma unroll
-      for (int i = 0; i < TM; ++i) {
-        lr[i] = pa[i][kv + pA8];
-        if (AC) li[i] = pa[i][kv + 3 * pA8];
-      }
-#pragma unroll
-      for (int i = 0; i < TM; ++i)
-#pragma unroll
-        for (int j = 0; j < TN; ++j) {
-          accR[i][j] = mma(lr[i], br[j], accR[i][j]);
-          if (AC && BC) accR[i][j] = mma(neg8(li[i]), bi[j], accR[i][j]);
-          if (BC) accI[i][j] = mma(lr[i], bi[j], accI[i][j]);
-          if (AC) accI[i][j] = mma(li[i], br[j], accI[i][j]);
-        }
-    }
-    // hi(A) x lo(B)
-#pragma unroll
-    for (int j = 0; j < TN; ++j) {
-      br[j] = pb[j][kv + pB8];
-      if (BC) bi[j] = pb[j][kv + 3 * pB8];
-    }
-#pragma unroll
-    for (int i = 0; i < TM; ++i)
-#pragma unroll
-      for (int j = 0; j < TN; ++j) {
-        accR[i][j] = mma(ar[i], br[j], accR[i][j]);
-        if (AC && BC) accR[i][j] = mma(neg8(ai[i]), bi[j], accR[i][j]);
-        if (BC) accI[i][j] = mma(ar[i], bi[j], accI[i][j]);
-        if (AC) accI[i][j] = mma(ai[i], br[j], accI[i][j]);
-      }
+  for (int i = 0; i < RB; ++i) {
+    const int c = tid + i * NT;
+    const int row = c >> 3, plane = (c >> 2) & 1, part = c & 3;
+    const int gn = min(n0 + row, p.b_rows - 1);
+    gb[i] = reinterpret_cast<const v8bf*>(Bb + (long)gn * p.ldb + plane * p.pB + part * 8);
+    lb[i] = c < CB ? row * LROW + plane * 32 + part * 8 : -1;
   }
 
-  // accumulator element v of lane l is (4(l/16) + v, l%16)
+  v4f acc[TM][TN];
+#pragma unroll
+  for (int i = 0; i < TM; ++i)
+#pragma unroll
+    for (int j = 0; j < TN; ++j) acc[i][j] = v4f{0.f, 0.f, 0.f, 0.f};
+
+  const int kz = blockIdx.z;
+  const int ks0 = kz * kchunk / 32, nk = min(p.K, (kz + 1) * kchunk) / 32;
+  v8bf ra[RA], rb[RB];
+  auto fetch = [&](int ks) {
+    const int kv = ks * 4;  // in units of 8 bf16
+#pragma unroll
+    for (int i = 0; i < RA; ++i)
+      if (la[i] >= 0) ra[i] = ga[i][kv];
+#pragma unroll
+    for (int i = 0; i < RB; ++i)
+      if (lb[i] >= 0) rb[i] = gb[i][kv];
+  };
+  auto stash = [&](int st) {
+    __bf16* a = As + st * BMR * LROW;
+    __bf16* bb = Bs + st * BN * LROW;
+#pragma unroll
+    for (int i = 0; i < RA; ++i)
+      if (la[i] >= 0) *reinterpret_cast<v8bf*>(a + la[i]) = ra[i];
+#pragma unroll
+    for (int i = 0; i < RB; ++i)
+      if (lb[i] >= 0) *reinterpret_cast<v8bf*>(bb + lb[i]) = rb[i];
+  };
+
+  fetch(ks0);
+  stash(0);
+  __syncthreads();
+  const int r16 = lane & 15, q8 = 8 * (lane >> 4);
+  for (int ks = ks0; ks < nk; ++ks) {
+    const bool more = ks + 1 < nk;
+    if (more) fetch(ks + 1);
+    const int st = (ks - ks0) & 1;
+    const __bf16* a = As + st * BMR * LROW + (wm * TM * 16 + r16) * LROW + q8;
+    const __bf16* bb = Bs + st * BN * LROW + (wn * TN * 16 + r16) * LROW + q8;
+    v8bf ah[TM], al[TM], bh[TN], bl[TN];
+#pragma unroll
+    for (int i = 0; i < TM; ++i) {
+      ah[i] = *reinterpret_cast<const v8bf*>(a + i * 16 * LROW);
+      al[i] = *reinterpret_cast<const v8bf*>(a + i * 16 * LROW + 32);
+    }
+#pragma unroll
+    for (int j = 0; j < TN; ++j) {
+      bh[j] = *reinterpret_cast<const v8bf*>(bb + j * 16 * LROW);
+      bl[j] = *reinterpret_cast<const v8bf*>(bb + j * 16 * LROW + 32);
+    }
+#pragma unroll
+    for (int i = 0; i < TM; ++i)
+#pragma unroll
+      for (int j = 0; j < TN; ++j) acc[i][j] = mma(ah[i], bh[j], acc[i][j]);
+#pragma unroll
+    for (int i = 0; i < TM; ++i)
+#pragma unroll
+      for (int j = 0; j < TN; ++j) acc[i][j] = mma(al[i], bh[j], acc[i][j]);
+#pragma unroll
+    for (int i = 0; i < TM; ++i)
+#pragma unroll
+      for (int j = 0; j < TN; ++j) acc[i][j] = mma(ah[i], bl[j], acc[i][j]);
+    if (more) stash(st ^ 1);
+    __syncthreads();
+  }
+
+  // accumulator element v of lane l is real row 4(l/16) + v of the block:
+  // v = 0, 1 are (re, im) of one output row, v = 2, 3 of the next
+  float2* C = reinterpret_cast<float2*>(p.C) + b * p.sC;
 #pragma unroll
   for (int i = 0; i < TM; ++i)
 #pragma unroll
     for (int j = 0; j < TN; ++j)
 #pragma unroll
-      for (int v = 0; v < 4; ++v) {
-        const int m = m0 + i * 16 + 4 * q4 + v;
-        const int n = n0 + j * 16 + r16;
+      for (int h = 0; h < 2; ++h) {
+        const int rr = m0r + (wm * TM + i) * 16 + q8 / 2 + 2 * h;
+        const int m = rr >> 1;
+        const int n = n0 + (wn * TN + j) * 16 + r16;
         if (m >= p.M || n >= p.N) continue;
-        const double xr = accR[i][j][v], xi = accI[i][j][v];
+        const double xr = acc[i][j][2 * h], xi = acc[i][j][2 * h + 1];
         if (ws) {  // raw partial of K slice kz (complex128), reduced by the split-K kernel
           reinterpret_cast<double2*>(ws)[((long)kz * p.M + m) * p.N + n] = double2{xr, xi};
           continue;
         }
-        const double rr = p.alpha.x * xr - p.alpha.y * xi, ri = p.alpha.x * xi + p.alpha.y * xr;
-        reinterpret_cast<float2*>(p.C)[b * p.sC + (long)m * p.ldc + n] = float2{(float)rr, (float)ri};
+        const double orr = p.alpha.x * xr - p.alpha.y * xi, oi = p.alpha.x * xi + p.alpha.y * xr;
+        C[(long)m * p.ldc + n] = float2{(float)orr, (float)oi};
       }
 }
 
-template <bool AC, bool BC, int WM, int WN, int TM, int TN>
-int launch(const Bf3& p, hipStream_t s) {
-  const int BM = 16 * TM * WM, BN = 16 * TN * WN;
-  const int tm = (p.M + BM - 1) / BM, tn = (p.N + BN - 1) / BN;
-  const long tiles = (long)tm * tn;
-  const int remap = (p.batch == 1 && tiles >= 64) ? 1 : 0;
-  const long T8 = remap ? (tiles + 7) / 8 : 0;
-  const long gx = remap ? T8 * 8 : tiles;
-  if (gx > 0x7fffffffL || p.batch > 65535) return fail(AFQ_E_UNSUPPORTED, "gemm_bf3: grid too large");
-  int ksplit = 1, kchunk = p.K;
-  double* ws = nullptr;
-  if (p.c_f64) {
-    // complex128 output goes through the fixed-order split-K reduction (which
-    // applies alpha and beta); split K until about 1024 work-groups are in flight
-    if (p.batch != 1 || !p.ws) return fail(AFQ_E_INVALID, "gemm_bf3: complex128 output needs batch 1 and a workspace");
-    ksplit = (int)std::max<long>(1, std::min<long>((1024 + tiles - 1) / tiles, p.K / 256));
-    while (ksplit > 1 && (long)ksplit * p.M * p.N * 2 > p.ws_elems) --ksplit;
-    if ((long)ksplit * p.M * p.N * 2 > p.ws_elems) return fail(AFQ_E_INVALID, "gemm_bf3: workspace too small");
-    kchunk = ((p.K + ksplit - 1) / ksplit + 31) / 32 * 32;
-    ksplit = (p.K + kchunk - 1) / kchunk;
-    ws = static_cast<double*>(p.ws);
-  }
-  hipLaunchKernelGGL((gemm_bf3_kernel<AC, BC, WM, WN, TM, TN>), dim3((unsigned)gx, p.batch, ksplit),
-                     dim3(64 * WM * WN), 0, s, p, tm, tn, remap, T8, kchunk, ws);
-  AFQ_TRY(launch_status("gemm_bf3_kernel"));
-  if (!ws) return AFQ_OK;
+// K slices for a complex128 output: split until about 1024 work-groups are in
+// flight; the fixed-order reduction applies alpha and beta.
+int bf3_ksplit(const Bf3& p, long tiles, int* kchunk) {
+  if (p.batch != 1 || !p.ws) return fail(AFQ_E_INVALID, "gemm_bf3: complex128 output needs batch 1 and a workspace");
+  int ksplit = (int)std::max<long>(1, std::min<long>((1024 + tiles - 1) / tiles, p.K / 256));
+  while (ksplit > 1 && (long)ksplit * p.M * p.N * 2 > p.ws_elems) --ksplit;
+  if ((long)ksplit * p.M * p.N * 2 > p.ws_elems) return fail(AFQ_E_INVALID, "gemm_bf3: workspace too small");
+  *kchunk = ((p.K + ksplit - 1) / ksplit + 31) / 32 * 32;
+  return (p.K + *kchunk - 1) / *kchunk;
+}
+
+int bf3_reduce(const Bf3& p, double* ws, int ksplit, hipStream_t s) {
   Gemm g;
   g.M = p.M;
   g.N = p.N;
@@ -180,6 +194,35 @@ int launch(const Bf3& p, hipStream_t s) {
   g.alpha = p.alpha;
   g.beta = p.beta;
   return gemm_splitk_reduce(g, ws, ksplit, true, s);
+}
+
+template <int BMR, int BN, int WM, int WN>
+int launch_lds(const Bf3& p, hipStream_t s) {
+  const int tm = (2 * p.M + BMR - 1) / BMR, tn = (p.N + BN - 1) / BN;
+  const long tiles = (long)tm * tn;
+  const int remap = (p.batch == 1 && tiles >= 64) ? 1 : 0;
+  const long T8 = remap ? (tiles + 7) / 8 : 0;
+  const long gx = remap ? T8 * 8 : tiles;
+  if (gx > 0x7fffffffL || p.batch > 65535) return fail(AFQ_E_UNSUPPORTED, "gemm_bf3: grid too large");
+  int ksplit = 1, kchunk = p.K;
+  double* ws = nullptr;
+  if (p.c_f64) {
+    ksplit = bf3_ksplit(p, tiles, &kchunk);
+    if (ksplit < 1) return ksplit;
+    ws = static_cast<double*>(p.ws);
+  }
+  const size_t lds = sizeof(__bf16) * 2 * (BMR + BN) * LROW;
+  static bool raised = false;
+  if (!raised && lds > 65536) {
+    hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(gemm_bf3_lds_kernel<BMR, BN, WM, WN>),
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (e != hipSuccess) return fail(AFQ_E_HIP, std::string("gemm_bf3 LDS attribute: ") + hipGetErrorString(e));
+    raised = true;
+  }
+  hipLaunchKernelGGL((gemm_bf3_lds_kernel<BMR, BN, WM, WN>), dim3((unsigned)gx, p.batch, ksplit), dim3(64 * WM * WN),
+                     lds, s, p, tm, tn, remap, T8, kchunk, ws);
+  AFQ_TRY(launch_status("gemm_bf3_lds_kernel"));
+  return ws ? bf3_reduce(p, ws, ksplit, s) : AFQ_OK;
 }
 
 // ------------------------------------------------------------------ operand preparation
@@ -246,9 +289,12 @@ int gemm_bf3(const Bf3& p, hipStream_t s) {
   if (p.K <= 0 || p.K % 32 != 0) return fail(AFQ_E_INVALID, "gemm_bf3: K must be a positive multiple of 32");
   if (p.a_rows < 1 || p.b_rows < 1) return fail(AFQ_E_INVALID, "gemm_bf3: operand row counts must be positive");
   if (!p.a_cplx || p.b_cplx) return fail(AFQ_E_INVALID, "gemm_bf3: complex A and real B expected");
-  // 2 waves of 4 x 4 blocks (64 x 128 tiles): 1.06 ms vs 1.28 ms for 2 x 2 waves of
-  // 2 x 4 blocks on the VHS product at W=256, G=2000, N=300
-  return launch<true, false, 1, 2, 4, 4>(p, s);
+  // 128 real rows (64 complex) x 128 columns per work-group, 2 x 4 waves of
+  // 4 x 2 blocks: measured fastest of the LDS tilings on the VHS product
+  // (W=256, N=300, G=2000: 0.63 ms; W=512, N=200, G=1200: 0.31 ms) and the force
+  // bias (0.17 and 0.14 ms), against 1.12 / 0.56 / 0.29 / 0.25 ms for fragments
+  // read straight from global memory
+  return launch_lds<128, 128, 2, 4>(p, s);
 }
 
 int k_split_rows_bf16(const double* src, bool cplx, long rows, int K, long lds, int kpad, long ldd, long pd,

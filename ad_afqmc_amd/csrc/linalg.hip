@@ -33,18 +33,22 @@ __device__ inline double2 crecip(double2 a) {
 // its columns permuted by the row interchanges, undone at the end); for the
 // determinant alone only the trailing rows and columns are eliminated (the
 // pivots, hence the determinant, are the same).
-__global__ __launch_bounds__(256) void gj_det_inv_kernel(const double2* __restrict__ S, long strideS, int n,
-                                                         double2* __restrict__ det, double2* __restrict__ inv,
-                                                         long strideInv) {
+// blockIdx.y selects one of up to two matrix sets (the two spins' overlap
+// matrices factor in one launch, so a CU holds two work-groups at W = 256).
+__global__ __launch_bounds__(256) void gj_det_inv_kernel(DetInvSet s0, DetInvSet s1) {
   extern __shared__ double2 a[];
   __shared__ double2 prow[64];
   __shared__ double2 fcol[64];
   __shared__ int piv[64];
   __shared__ int pk;
+  const DetInvSet& set = blockIdx.y == 0 ? s0 : s1;
+  const int n = set.n;
+  double2* __restrict__ det = set.det;
+  double2* __restrict__ inv = set.inv;
   const int tid = threadIdx.x;
   const int j = tid & 63, rg = tid >> 6;
   const long b = blockIdx.x;
-  const double2* src = S + b * strideS;
+  const double2* src = set.S + b * set.strideS;
   const int nn = n * n;
   const bool want_inv = inv != nullptr;
   for (int i = tid; i < nn; i += 256) a[i] = src[i];
@@ -136,7 +140,7 @@ __global__ __launch_bounds__(256) void gj_det_inv_kernel(const double2* __restri
       }
     }
     __syncthreads();
-    double2* dst = inv + b * strideInv;
+    double2* dst = inv + b * set.strideInv;
     for (int i = tid; i < nn; i += 256) dst[i] = a[i];
   }
 }
@@ -247,11 +251,15 @@ __global__ void conj_kernel(const double2* __restrict__ x, double2* __restrict__
 
 }  // namespace
 
-int batched_det_inv(const double2* S, long strideS, int n, int count, double2* det, double2* inv, long strideInv,
-                    hipStream_t stream) {
-  if (n < 1 || n > 64) return fail(AFQ_E_UNSUPPORTED, "det/inverse kernel supports 1 <= nocc <= 64");
+int batched_det_inv_sets(const DetInvSet* sets, int nsets, int count, hipStream_t stream) {
+  if (nsets < 1 || nsets > 2) return fail(AFQ_E_INVALID, "det/inverse: one or two matrix sets");
+  int nmax = 0;
+  for (int i = 0; i < nsets; ++i) {
+    if (sets[i].n < 1 || sets[i].n > 64) return fail(AFQ_E_UNSUPPORTED, "det/inverse kernel supports 1 <= nocc <= 64");
+    nmax = std::max(nmax, sets[i].n);
+  }
   if (count <= 0) return AFQ_OK;
-  const size_t lds = sizeof(double2) * n * n;
+  const size_t lds = sizeof(double2) * nmax * nmax;
   if (lds + 2048 > 65536) {
     static bool raised = false;
     if (!raised) {
@@ -261,8 +269,15 @@ int batched_det_inv(const double2* S, long strideS, int n, int count, double2* d
       raised = true;
     }
   }
-  hipLaunchKernelGGL(gj_det_inv_kernel, dim3(count), dim3(256), lds, stream, S, strideS, n, det, inv, strideInv);
+  hipLaunchKernelGGL(gj_det_inv_kernel, dim3(count, nsets), dim3(256), lds, stream, sets[0],
+                     sets[nsets > 1 ? 1 : 0]);
   return launch_status("gj_det_inv_kernel");
+}
+
+int batched_det_inv(const double2* S, long strideS, int n, int count, double2* det, double2* inv, long strideInv,
+                    hipStream_t stream) {
+  const DetInvSet set{S, strideS, n, det, inv, strideInv};
+  return batched_det_inv_sets(&set, 1, count, stream);
 }
 
 int batched_cholesky_upper(const double2* S, long strideS, int n, int count, double2* R, long strideR,
