@@ -47,6 +47,24 @@ inline int launch_status(const char* what) {
 //   B element (k,n) at B[k*ldb+n]        (b_trans: B[n*ldb+k])
 //   C element (m,n) at C[m*ldc+n]
 // Leading dimensions and batch strides count elements (real or complex).
+// Symmetric Cholesky vectors, block-packed lower triangle.
+// Column p of the packed layout is block b = p / 256, entry (r, c) = ((p % 256) / 16,
+// p % 16) of the 16 x 16 orbital block (ib, jb), ib >= jb, b = ib (ib + 1) / 2 + jb.
+__device__ inline void sym_block(int b, int* ib, int* jb) {
+  int i = (int)((sqrtf(8.0f * b + 1.0f) - 1.0f) * 0.5f);
+  while ((i + 1) * (i + 2) / 2 <= b) ++i;
+  while (i * (i + 1) / 2 > b) --i;
+  *ib = i;
+  *jb = b - i * (i + 1) / 2;
+}
+// source column i*N + j of packed column p, or -1 for padding
+__device__ inline long sym_src(long p, int N) {
+  int ib, jb;
+  sym_block((int)(p >> 8), &ib, &jb);
+  const int i = 16 * ib + (int)((p >> 4) & 15), j = 16 * jb + (int)(p & 15);
+  return (i < N && j < N) ? (long)i * N + j : -1;
+}
+
 struct Gemm {
   int M = 0, N = 0, K = 0, batch = 1;
   bool a_cplx = false, b_cplx = false, a_trans = false, b_trans = false;
@@ -181,6 +199,14 @@ struct afq_job {
   // (Wr, 4, Gpad); rows padded to 16, k to 32
   uint16_t* lt_split = nullptr;
   uint16_t* xs_split = nullptr;
+  // symmetric Cholesky vectors: the VHS product over block-packed columns
+  // (sym_packed_cols), used when set_hamiltonian finds L_g = L_g^T; npk = 0 when
+  // packing would not save work at this N
+  long npk = 0;
+  bool sym_chol = false;
+  double* cholp = nullptr;                // (G, npk) FP64 packed L
+  double2* vhsp = nullptr;                // (W, npk) FP64 packed VHS
+  float2* vhsp32 = nullptr;               // (W, npk) reduced-precision packed VHS
   double* gemm_ws = nullptr;              // split-K workspace
   long gemm_ws_elems = 0;
 

@@ -72,6 +72,9 @@ void release(afq_job* j) {
   fr(j->tmp_nn);
   fr(j->gemm_ws);
   fr(j->chol32);
+  fr(j->cholp);
+  fr(j->vhsp);
+  fr(j->vhsp32);
   fr(j->exp_h1m32[0]);
   fr(j->exp_h1m32[1]);
   fr(j->rot_chol32[0]);
@@ -341,17 +344,21 @@ int trotprop_lp(afq_job* j, void* up, void* dn, const double2* xs, hipStream_t s
   const int N = j->N, W = j->W;
   const long NN = (long)N * N;
   {
+    // symmetric L: packed columns into vhsp32, then mirrored into vhs32
+    const bool sym = j->sym_chol;
+    const long ncols = sym ? j->npk : NN;
+    float2* out = sym ? j->vhsp32 : j->vhs32;
     Gemm g;
     g.M = W;
-    g.N = (int)NN;
+    g.N = (int)ncols;
     g.K = j->G;
     g.a_cplx = true;
     g.A = xs;
     g.lda = j->G;
     g.B = j->chol32;
-    g.ldb = NN;
-    g.C = j->vhs32;
-    g.ldc = NN;
+    g.ldb = ncols;
+    g.C = out;
+    g.ldc = ncols;
     g.alpha = double2{0.0, std::sqrt(j->d.dt)};
     ProfScope ps(j, AFQ_K_VHS, st);
     if (j->bf16x3) {
@@ -361,7 +368,7 @@ int trotprop_lp(afq_job* j, void* up, void* dn, const double2* xs, hipStream_t s
                                 j->xs_split, st));
       Bf3 v;
       v.M = W;
-      v.N = (int)NN;
+      v.N = (int)ncols;
       v.K = gpad;
       v.a_cplx = true;
       v.A = j->xs_split;
@@ -371,14 +378,15 @@ int trotprop_lp(afq_job* j, void* up, void* dn, const double2* xs, hipStream_t s
       v.B = j->lt_split;
       v.ldb = 2L * gpad;
       v.pB = gpad;
-      v.b_rows = (int)(16 * ((NN + 15) / 16));
+      v.b_rows = (int)(16 * ((ncols + 15) / 16));
       v.alpha = g.alpha;
-      v.C = j->vhs32;
-      v.ldc = NN;
+      v.C = out;
+      v.ldc = ncols;
       AFQ_TRY(gemm_bf3(v, st));
     } else {
       AFQ_TRY(gemm_lp(LpKind::VHS, false, g, st));
     }
+    if (sym) AFQ_TRY(k_sym_unpack(j->vhsp32, false, N, W, j->vhs32, st));
   }
   // The walker buffers of both spins are propagated as the column blocks of
   // one (W, N, c_up + c_dn) panel, so each Taylor term is one product that
@@ -439,20 +447,23 @@ int trotprop(afq_job* j, void* up, void* dn, const double2* xs, hipStream_t st) 
   const int N = j->N, W = j->W;
   const long NN = (long)N * N;
   {
+    const bool sym = j->sym_chol;
+    const long ncols = sym ? j->npk : NN;
     Gemm g;
     g.M = W;
-    g.N = (int)NN;
+    g.N = (int)ncols;
     g.K = j->G;
     g.a_cplx = true;
     g.A = xs;
     g.lda = j->G;
-    g.B = j->chol;
-    g.ldb = NN;
-    g.C = j->vhs;
-    g.ldc = NN;
+    g.B = sym ? j->cholp : j->chol;
+    g.ldb = ncols;
+    g.C = sym ? static_cast<void*>(j->vhsp) : static_cast<void*>(j->vhs);
+    g.ldc = ncols;
     g.alpha = double2{0.0, std::sqrt(j->d.dt)};
     ProfScope ps(j, AFQ_K_VHS, st);
     AFQ_TRY(gemm_f64(g, st));
+    if (sym) AFQ_TRY(k_sym_unpack(j->vhsp, true, N, W, j->vhs, st));
   }
   PanelCols pc(j);
   const long per = (long)N * pc.ctot;
@@ -754,6 +765,20 @@ int alloc_job(afq_job* j, bool dry, int64_t* total) {
   A_(j->fb, W * G);
   A_(j->xs, W * G);
   A_(j->terms, 2 * W);
+  // packed symmetric VHS product (its L copy lives in chol32 / lt_split in
+  // reduced precision, which have at least npk columns)
+  {
+    const long npk = sym_packed_cols((int)N);
+    j->npk = (npk * 20 <= N * N * 17) ? npk : 0;
+  }
+  if (j->npk > 0) {
+    if (j->lp) {
+      A_(j->vhsp32, W * j->npk);
+    } else {
+      A_(j->cholp, G * j->npk);
+      A_(j->vhsp, W * j->npk);
+    }
+  }
   if (j->lp) {
     // complex64 VHS and Taylor terms; the complex128 Taylor buffers hold
     // exp(-dt/2 h1) W and the running sum.  FP32: FP32 copies of L and rot_chol.
@@ -860,11 +885,32 @@ int afq_job_set_hamiltonian(afq_job_t j, double h0, const double* h1, const doub
   const long N = j->N;
   AFQ_HIP(hipMemcpyAsync(j->h1, h1, sizeof(double) * 2 * N * N, hipMemcpyDefault, S(stream)));
   AFQ_HIP(hipMemcpyAsync(j->chol, chol, sizeof(double) * j->G * N * N, hipMemcpyDefault, S(stream)));
+  // Symmetric L_g (to 1e-12 of its largest entry): the VHS product covers the
+  // lower block triangle only and mirrors it.
+  j->sym_chol = false;
+  if (j->npk > 0) {
+    unsigned long long* dev = reinterpret_cast<unsigned long long*>(j->dscratch + 6);
+    AFQ_TRY(k_sym_check(j->chol, j->G, (int)N, dev, S(stream)));
+    unsigned long long host[2] = {0, 0};
+    AFQ_HIP(hipMemcpyAsync(host, dev, sizeof(host), hipMemcpyDeviceToHost, S(stream)));
+    AFQ_HIP(hipStreamSynchronize(S(stream)));
+    double dmax, amax;
+    std::memcpy(&dmax, &host[0], sizeof(double));
+    std::memcpy(&amax, &host[1], sizeof(double));
+    j->sym_chol = dmax <= 1e-12 * amax;
+  }
+  const long ncols = j->sym_chol ? j->npk : N * N;
   if (j->bf16x3) {
     const long gp = bf16_npad(j->G);
-    AFQ_TRY(k_split_chol_t_bf16(j->chol, j->G, N * N, (int)gp, 2 * gp, gp, j->lt_split, S(stream)));
+    AFQ_TRY(k_split_chol_t_bf16(j->chol, j->G, N * N, ncols, j->sym_chol ? (int)N : 0, (int)gp, 2 * gp, gp,
+                                j->lt_split, S(stream)));
   } else if (j->lp) {
-    AFQ_TRY(k_to_f32(j->chol, j->chol32, (long)j->G * N * N, S(stream)));
+    if (j->sym_chol)
+      AFQ_TRY(k_sym_pack(j->chol, j->G, (int)N, nullptr, j->chol32, S(stream)));
+    else
+      AFQ_TRY(k_to_f32(j->chol, j->chol32, (long)j->G * N * N, S(stream)));
+  } else if (j->sym_chol) {
+    AFQ_TRY(k_sym_pack(j->chol, j->G, (int)N, j->cholp, nullptr, S(stream)));
   }
   j->have_ham = true;
   j->have_meas = j->have_prop = false;

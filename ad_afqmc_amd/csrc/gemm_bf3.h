@@ -31,7 +31,10 @@ int gemm_bf3(const Bf3& p, hipStream_t s);
 // (complex when cplx: four planes, else two), zero for k >= K.
 int k_split_rows_bf16(const double* src, bool cplx, long rows, int K, long lds, int kpad, long ldd, long pd,
                       uint16_t* dst, hipStream_t s);
-// Cholesky vectors L (G x NN) -> dst[n*ldd + q*pd + g] for g < gpad (zero past G).
-int k_split_chol_t_bf16(const double* L, int G, long NN, int gpad, long ldd, long pd, uint16_t* dst, hipStream_t s);
+// Cholesky vectors L (G x NN) -> dst[n*ldd + q*pd + g] for g < gpad (zero past G)
+// and rows n < ncols: column n of L, or with sym_N > 0 packed column n of a
+// symmetric L (sym_packed_cols layout).
+int k_split_chol_t_bf16(const double* L, int G, long NN, long ncols, int sym_N, int gpad, long ldd, long pd,
+                        uint16_t* dst, hipStream_t s);
 
 }  // namespace afq

@@ -256,23 +256,28 @@ __global__ void split_rows_kernel(const double* __restrict__ src, int cplx, long
   }
 }
 
-// L (G x NN, row-major doubles) -> Lt[n][q][g] (bf16 hi/lo planes, g < gpad, zero for g >= G)
-__global__ void split_chol_t_kernel(const double* __restrict__ L, int G, long NN, int gpad, long ldd, long pd,
-                                    __bf16* __restrict__ dst) {
+// L (G x NN, row-major doubles) -> Lt[n][q][g] (bf16 hi/lo planes, g < gpad,
+// zero for g >= G) for output rows n < ncols; with sym_N > 0 output row n is
+// packed column n of a symmetric L (sym_src), else column n itself.
+__global__ void split_chol_t_kernel(const double* __restrict__ L, int G, long NN, long ncols, int sym_N, int gpad,
+                                    long ldd, long pd, __bf16* __restrict__ dst) {
   __shared__ double t[32][33];
   const long n0 = (long)blockIdx.x * 32;
   const int g0 = blockIdx.y * 32;
   const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
-  for (int y = ty; y < 32; y += 8) {
-    const int g = g0 + y;
+  {
     const long n = n0 + tx;
-    t[y][tx] = (g < G && n < NN) ? L[(long)g * NN + n] : 0.0;
+    const long src = n >= ncols ? -1 : sym_N > 0 ? sym_src(n, sym_N) : n;
+    for (int y = ty; y < 32; y += 8) {
+      const int g = g0 + y;
+      t[y][tx] = (g < G && src >= 0) ? L[(long)g * NN + src] : 0.0;
+    }
   }
   __syncthreads();
   for (int y = ty; y < 32; y += 8) {
     const long n = n0 + y;
     const int g = g0 + tx;
-    if (n < NN && g < gpad) {
+    if (n < ncols && g < gpad) {
       const float x = (float)t[tx][y];
       const __bf16 h = (__bf16)x;
       __bf16* d = dst + n * ldd + g;
@@ -306,11 +311,12 @@ int k_split_rows_bf16(const double* src, bool cplx, long rows, int K, long lds, 
   return launch_status("split_rows_kernel");
 }
 
-int k_split_chol_t_bf16(const double* L, int G, long NN, int gpad, long ldd, long pd, uint16_t* dst, hipStream_t s) {
-  const long nb = (NN + 31) / 32;
+int k_split_chol_t_bf16(const double* L, int G, long NN, long ncols, int sym_N, int gpad, long ldd, long pd,
+                        uint16_t* dst, hipStream_t s) {
+  const long nb = (ncols + 31) / 32;
   if (nb > 0x7fffffffL) return fail(AFQ_E_UNSUPPORTED, "split_chol_t: too large");
-  hipLaunchKernelGGL(split_chol_t_kernel, dim3((unsigned)nb, (gpad + 31) / 32), dim3(256), 0, s, L, G, NN, gpad, ldd,
-                     pd, reinterpret_cast<__bf16*>(dst));
+  hipLaunchKernelGGL(split_chol_t_kernel, dim3((unsigned)nb, (gpad + 31) / 32), dim3(256), 0, s, L, G, NN, ncols,
+                     sym_N, gpad, ldd, pd, reinterpret_cast<__bf16*>(dst));
   return launch_status("split_chol_t_kernel");
 }
 

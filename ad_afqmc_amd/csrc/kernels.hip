@@ -391,6 +391,61 @@ __global__ void minus_identity_f32_kernel(const double* src, float* dst, int n) 
   dst[i] = (float)v;
 }
 
+// out[0] = max |L_g,ij - L_g,ji|, out[1] = max |L_g,ij| as IEEE bit patterns
+// (non-negative doubles order like their unsigned bits); out must start at zero.
+__global__ void sym_check_kernel(const double* __restrict__ L, long total, int N, unsigned long long* out) {
+  const long NN = (long)N * N;
+  double dmax = 0.0, amax = 0.0;
+  for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < total; t += (long)gridDim.x * blockDim.x) {
+    const long g = t / NN, q = t - g * NN;
+    const int i = (int)(q / N), j = (int)(q - (long)i * N);
+    const double a = L[t];
+    amax = fmax(amax, fabs(a));
+    if (j > i) dmax = fmax(dmax, fabs(a - L[g * NN + (long)j * N + i]));
+  }
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    dmax = fmax(dmax, __shfl_xor(dmax, off));
+    amax = fmax(amax, __shfl_xor(amax, off));
+  }
+  if ((threadIdx.x & 63) == 0) {
+    atomicMax(&out[0], (unsigned long long)__double_as_longlong(dmax));
+    atomicMax(&out[1], (unsigned long long)__double_as_longlong(amax));
+  }
+}
+
+template <typename T>
+__global__ void sym_pack_kernel(const double* __restrict__ L, int N, long npk, long total, T* __restrict__ out) {
+  const long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (t >= total) return;
+  const long g = t / npk, p = t - g * npk;
+  const long src = sym_src(p, N);
+  out[t] = src >= 0 ? (T)L[g * (long)N * N + src] : (T)0;
+}
+
+// One work-group per (block, walker): writes the block at (ib, jb) and, for
+// ib != jb, its transpose at (jb, ib) through an LDS tile so both writes are
+// 16-element rows.
+template <typename T2>
+__global__ __launch_bounds__(256) void sym_unpack_kernel(const T2* __restrict__ Cp, long npk, int N,
+                                                         T2* __restrict__ V) {
+  __shared__ T2 tileb[16][17];
+  const int b = blockIdx.x, t = threadIdx.x;
+  const long w = blockIdx.y;
+  int ib, jb;
+  sym_block(b, &ib, &jb);
+  const long NN = (long)N * N;
+  const T2 v = Cp[w * npk + (long)b * 256 + t];
+  const int r = t >> 4, c = t & 15;
+  const int i = 16 * ib + r, j = 16 * jb + c;
+  if (i < N && j < N) V[w * NN + (long)i * N + j] = v;
+  if (ib == jb) return;  // uniform per work-group
+  tileb[r][c] = v;
+  __syncthreads();
+  const int i2 = 16 * jb + r, j2 = 16 * ib + c;
+  if (i2 < N && j2 < N) V[w * NN + (long)i2 * N + j2] = tileb[c][r];
+}
+
 inline unsigned blocks_for(long n, int t) { return (unsigned)((n + t - 1) / t); }
 
 }  // namespace
@@ -471,6 +526,39 @@ int k_to_f32(const double* src, float* dst, long n, hipStream_t s) {
   if (n <= 0) return AFQ_OK;
   hipLaunchKernelGGL(to_f32_kernel, dim3(blocks_for(n, 256)), dim3(256), 0, s, src, dst, n);
   return launch_status("to_f32_kernel");
+}
+long sym_packed_cols(int N) {
+  const long nb = (N + 15) / 16;
+  return nb * (nb + 1) / 2 * 256;
+}
+int k_sym_check(const double* L, int G, int N, unsigned long long* out2, hipStream_t s) {
+  const long total = (long)G * N * N;
+  if (total <= 0) return AFQ_OK;
+  AFQ_HIP(hipMemsetAsync(out2, 0, 2 * sizeof(unsigned long long), s));
+  const unsigned blocks = (unsigned)std::min<long>(4096, (total + 255) / 256);
+  hipLaunchKernelGGL(sym_check_kernel, dim3(blocks), dim3(256), 0, s, L, total, N, out2);
+  return launch_status("sym_check_kernel");
+}
+int k_sym_pack(const double* L, int G, int N, double* out64, float* out32, hipStream_t s) {
+  const long npk = sym_packed_cols(N), total = (long)G * npk;
+  if (out64)
+    hipLaunchKernelGGL((sym_pack_kernel<double>), dim3(blocks_for(total, 256)), dim3(256), 0, s, L, N, npk, total,
+                       out64);
+  else
+    hipLaunchKernelGGL((sym_pack_kernel<float>), dim3(blocks_for(total, 256)), dim3(256), 0, s, L, N, npk, total,
+                       out32);
+  return launch_status("sym_pack_kernel");
+}
+int k_sym_unpack(const void* Cp, bool c64, int N, int W, void* V, hipStream_t s) {
+  const long npk = sym_packed_cols(N);
+  const dim3 grid((unsigned)(npk / 256), W);
+  if (c64)
+    hipLaunchKernelGGL((sym_unpack_kernel<double2>), grid, dim3(256), 0, s, static_cast<const double2*>(Cp), npk, N,
+                       static_cast<double2*>(V));
+  else
+    hipLaunchKernelGGL((sym_unpack_kernel<float2>), grid, dim3(256), 0, s, static_cast<const float2*>(Cp), npk, N,
+                       static_cast<float2*>(V));
+  return launch_status("sym_unpack_kernel");
 }
 int k_minus_identity_f32(const double* src, float* dst, int n, hipStream_t s) {
   if (n <= 0) return AFQ_OK;

@@ -87,10 +87,14 @@ CASES = [
 ]
 
 
-def _setup(trial_kind, wtype, N, ou, od, G, W, seed=0, dt=0.01):
+def _setup(trial_kind, wtype, N, ou, od, G, W, seed=0, dt=0.01, chol_asym=0.0):
     # Cholesky scale ~ 1/sqrt(N) keeps the walker population alive at this dt
     sysm = syn.make_system(N, ou, G, nocc_dn=od, uhf=(trial_kind == "uhf"), seed=seed,
                            l_scale=min(1.0, 2.0 / math.sqrt(N)))
+    if chol_asym:
+        # an antisymmetric part above the engine's symmetry threshold (1e-12 of max |L|)
+        a = np.random.default_rng(seed + 9).standard_normal((G, N, N))
+        sysm.chol += chol_asym * (a - a.transpose(0, 2, 1)).reshape(G, N * N)
     wave = sysm.wave_data()
     ham = sysm.ham_data()
     trial = wf.rhf(N, (ou, ou)) if trial_kind == "rhf" else wf.uhf(N, (ou, od))
@@ -165,6 +169,31 @@ def test_propagate_constrained_steps(case):
         assert rel(npy(pd["weights"]), st["weights"]) < TOL, step
         assert np.count_nonzero(st["weights"]) >= W // 2, step  # the step is not trivially clipped
         assert abs(pd["pop_control_ene_shift"].item() - st["pop_control_ene_shift"]) < TOL * max(1, abs(e_est))
+
+
+# N = 70: the VHS product runs over the packed lower block triangle when L_g is
+# symmetric (exactly so here) and over all N^2 columns when it is not.
+@pytest.mark.parametrize("asym", [0.0, 1e-9], ids=["symmetric-L", "asymmetric-L"])
+@pytest.mark.parametrize("case", [("rhf", "restricted", 70, 17, 17, 130, 9), ("uhf", "unrestricted", 70, 21, 18, 97, 7)],
+                         ids=["rhf-N70", "uhf-N70"])
+def test_vhs_symmetric_and_general_cholesky(case, asym):
+    sysm, trial, prop, ham, wave, walkers, otrial, oham, owalk, _ = _setup(*case, chol_asym=asym)
+    W, G = case[6], case[5]
+    rng = np.random.default_rng(5)
+    pd = prop.init_prop_data(trial, wave, ham, seed=3, init_walkers=walkers)
+    e_est = float(pd["e_estimate"].item())
+    st = dict(walkers=owalk, weights=np.ones(W), overlaps=otrial.calc_overlap(owalk), e_estimate=e_est,
+              pop_control_ene_shift=e_est)
+    for step in range(2):
+        fields = rng.standard_normal((W, G))
+        pd = prop.propagate_constrained(trial, ham, pd, fields, wave)
+        st = O.propagate_constrained(otrial, st, fields, oham, prop.dt, prop.n_exp_terms)
+        wa = npy(pd["walkers"].data)
+        for s in range(1 if case[1] == "restricted" else 2):
+            got = wa if case[1] == "restricted" else wa[s]
+            ref = st["walkers"] if case[1] == "restricted" else st["walkers"][s]
+            assert rel(got, ref) < TOL, (step, s)
+        assert rel(npy(pd["weights"]), st["weights"]) < TOL, step
 
 
 def test_trotprop_matches_oracle():
