@@ -48,22 +48,53 @@ def flops_per_walker_step(N, Oa, Ob, G, m, one_spin, K=5):
     return sum(f.values()), f
 
 
-def launch_flops(cls, N, c_cols, occ, G, W):
-    """Algorithmic flops of ONE launch of a kernel class (per spin buffer)."""
+def sym_packed_cols(N):
+    """Columns of the VHS product when the Cholesky vectors are symmetric: the
+    16 x 16 orbital blocks on and below the diagonal."""
+    nb = (N + 15) // 16
+    return nb * (nb + 1) // 2 * 256
+
+
+def vhs_columns(chol, N):
+    """Columns the engine's VHS product covers for these Cholesky vectors (it
+    packs the lower block triangle when max |L - L^T| <= 1e-12 max |L| and that
+    saves at least 15% of the N^2 columns)."""
+    import numpy as np
+
+    npk = sym_packed_cols(N)
+    if 20 * npk > 17 * N * N:
+        return N * N
+    dmax = amax = 0.0
+    for g0 in range(0, chol.shape[0], 64):
+        L = chol[g0 : g0 + 64].reshape(-1, N, N)
+        dmax = max(dmax, float(np.max(np.abs(L - L.transpose(0, 2, 1)))))
+        amax = max(amax, float(np.max(np.abs(L))))
+    return npk if dmax <= 1e-12 * amax else N * N
+
+
+def launch_flops(cls, N, Oa, Ob, G, W, unrestricted=False, vhs_cols=None):
+    """Flops of one launch of a kernel class, as the kernel computes them,
+    averaged over the launches of a block: the VHS product covers `vhs_cols`
+    columns (N^2 by default); a Taylor launch covers the walker panel (both
+    spins' columns side by side for unrestricted walkers); the one-body,
+    exchange, force-bias, Green's-function and overlap launches alternate
+    between the spins of an unrestricted or UHF case."""
+    occ = (Oa, Ob) if (unrestricted or Oa != Ob) else (Oa,)
+    mean = lambda f: sum(f(o) for o in occ) / len(occ)
     if cls == "vhs":
-        return 4.0 * G * N * N * W
+        return 4.0 * G * (N * N if vhs_cols is None else vhs_cols) * W
     if cls == "taylor":
-        return 8.0 * N * N * c_cols * W
+        return 8.0 * N * N * ((Oa + Ob) if unrestricted else Oa) * W
     if cls == "exp_h1":
-        return 4.0 * N * N * c_cols * W
+        return mean(lambda o: 4.0 * N * N * o * W) if unrestricted else 4.0 * N * N * Oa * W
     if cls == "exchange":
-        return 4.0 * G * occ * occ * N * W
+        return mean(lambda o: 4.0 * G * o * o * N * W)
     if cls == "force_bias":
-        return 4.0 * G * occ * N * W
+        return mean(lambda o: 4.0 * G * o * N * W)
     if cls == "green":
-        return 8.0 * occ * occ * N * W
+        return mean(lambda o: 8.0 * o * o * N * W)
     if cls == "overlap_gemm":
-        return 4.0 * occ * occ * N * W
+        return mean(lambda o: 4.0 * o * o * N * W)
     return None
 
 
@@ -352,7 +383,8 @@ def time_config(system, name, precision, steps, warmup, dominant_roofline=False,
     if dominant:
         dom_ms, dom_n = job.profile_read(dominant)
         job.profile(dominant, False)
-        per_launch = launch_flops(dominant, N, Oa, Oa, G, W)
+        per_launch = launch_flops(dominant, N, Oa, Ob, G, W, unrestricted=uhf,
+                                  vhs_cols=vhs_columns(system.chol, N) if dominant == "vhs" else None)
         avg_s = dom_ms / max(dom_n, 1) / 1e3
         # bf16x3 does three bf16 MFMA products per FP32-equivalent product: its
         # ceiling is a third of the dense bf16 peak
