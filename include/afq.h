@@ -107,7 +107,10 @@ int afq_job_set_batch(afq_job_t job, int32_t n_walkers);
 
 /* Sets h0 and copies h1 (2,N,N) and chol (G,N*N) (device or host pointers)
  * into resident HBM.  h1/chol may be NULL to set h0 only (measurement-only use
- * with intermediates supplied by afq_set_intermediate). */
+ * with intermediates supplied by afq_set_intermediate).  Checks whether every
+ * L_g is symmetric (to 1e-12 of max |L|; synchronises the stream once); if so,
+ * the VHS product of the propagation covers the lower block triangle and
+ * mirrors it. */
 int afq_job_set_hamiltonian(afq_job_t job, double h0, const double* h1, const double* chol, void* stream);
 /* Trial orbitals mo_up (N,nelec_up), mo_dn (N,nelec_dn; NULL for RHF) and the
  * trial 1-RDM rdm1 (2,N,N), device or host pointers. */
@@ -146,7 +149,8 @@ int afq_propagate_constrained(afq_job_t job, void* walkers_up, void* walkers_dn,
                               double* weights, void* overlaps, double* scalars, void* force_bias_out,
                               void* stream);
 
-/* QR re-orthonormalisation in place (R discarded). */
+/* QR re-orthonormalisation in place (R discarded): Cholesky QR twice, the
+ * factorisation whose R has a positive real diagonal. */
 int afq_orthonormalize(afq_job_t job, void* walkers_up, void* walkers_dn, void* stream);
 
 /* Comb resampling by |weight| with offset zeta (device f64 scalar); permutes
