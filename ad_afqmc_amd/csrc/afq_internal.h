@@ -65,11 +65,22 @@ __device__ inline long sym_src(long p, int N) {
   return (i < N && j < N) ? (long)i * N + j : -1;
 }
 
+// Offset of entry (i, k) of a symmetric matrix stored as its packed lower
+// block triangle (entries of upper blocks are read from the mirrored block).
+__device__ inline long sym_off(int i, int k) {
+  const int ib = i >> 4, kb = k >> 4;
+  return ib >= kb ? ((long)(ib * (ib + 1) / 2 + kb) << 8) + ((i & 15) << 4) + (k & 15)
+                  : ((long)(kb * (kb + 1) / 2 + ib) << 8) + ((k & 15) << 4) + (i & 15);
+}
+
 struct Gemm {
   int M = 0, N = 0, K = 0, batch = 1;
   bool a_cplx = false, b_cplx = false, a_trans = false, b_trans = false;
   const void* A = nullptr;
   long lda = 0, sA = 0;
+  // a_sym > 0: A is a symmetric a_sym x a_sym matrix held as its packed lower
+  // block triangle (sym_packed_cols layout, lda unused, batch stride sA); !a_trans
+  int a_sym = 0;
   const void* B = nullptr;
   long ldb = 0, sB = 0;
   void* C = nullptr;

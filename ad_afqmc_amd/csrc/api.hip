@@ -344,7 +344,7 @@ int trotprop_lp(afq_job* j, void* up, void* dn, const double2* xs, hipStream_t s
   const int N = j->N, W = j->W;
   const long NN = (long)N * N;
   {
-    // symmetric L: packed columns into vhsp32, then mirrored into vhs32
+    // symmetric L: packed columns into vhsp32, which the Taylor products read
     const bool sym = j->sym_chol;
     const long ncols = sym ? j->npk : NN;
     float2* out = sym ? j->vhsp32 : j->vhs32;
@@ -386,7 +386,6 @@ int trotprop_lp(afq_job* j, void* up, void* dn, const double2* xs, hipStream_t s
     } else {
       AFQ_TRY(gemm_lp(LpKind::VHS, false, g, st));
     }
-    if (sym) AFQ_TRY(k_sym_unpack(j->vhsp32, false, N, W, j->vhs32, st));
   }
   // The walker buffers of both spins are propagated as the column blocks of
   // one (W, N, c_up + c_dn) panel, so each Taylor term is one product that
@@ -410,9 +409,10 @@ int trotprop_lp(afq_job* j, void* up, void* dn, const double2* xs, hipStream_t s
     g.K = N;
     g.batch = W;
     g.a_cplx = true;
-    g.A = j->vhs32;
+    g.A = j->sym_chol ? j->vhsp32 : j->vhs32;
     g.lda = N;
-    g.sA = NN;
+    g.sA = j->sym_chol ? j->npk : NN;
+    g.a_sym = j->sym_chol ? N : 0;
     g.b_cplx = true;
     g.B = cur < 0 ? static_cast<const void*>(t0) : static_cast<const void*>(j->t32[cur]);
     g.ldb = pc.ctot;
@@ -463,7 +463,6 @@ int trotprop(afq_job* j, void* up, void* dn, const double2* xs, hipStream_t st) 
     g.alpha = double2{0.0, std::sqrt(j->d.dt)};
     ProfScope ps(j, AFQ_K_VHS, st);
     AFQ_TRY(gemm_f64(g, st));
-    if (sym) AFQ_TRY(k_sym_unpack(j->vhsp, true, N, W, j->vhs, st));
   }
   PanelCols pc(j);
   const long per = (long)N * pc.ctot;
@@ -483,9 +482,11 @@ int trotprop(afq_job* j, void* up, void* dn, const double2* xs, hipStream_t st) 
     g.K = N;
     g.batch = W;
     g.a_cplx = true;
-    g.A = j->vhs;
+    // symmetric L: the Taylor products read the packed VHS directly
+    g.A = j->sym_chol ? static_cast<const void*>(j->vhsp) : static_cast<const void*>(j->vhs);
     g.lda = N;
-    g.sA = NN;
+    g.sA = j->sym_chol ? j->npk : NN;
+    g.a_sym = j->sym_chol ? N : 0;
     g.b_cplx = true;
     g.B = src;
     g.ldb = pc.ctot;

@@ -45,7 +45,7 @@ struct Cfg {
 // Loads quad q of the A tile (rows m0.., k0..) into registers.
 template <class C, bool AC, bool AT>
 __device__ inline void fetch_a(double (&r)[C::RA][4][2], const double* A, long lda, int M, int K, int m0, int k0,
-                               int tid) {
+                               int tid, int asym) {
 #pragma unroll
   for (int i = 0; i < C::RA; ++i) {
     const int q = tid + i * C::NT;
@@ -67,7 +67,7 @@ __device__ inline void fetch_a(double (&r)[C::RA][4][2], const double* A, long l
     for (int t = 0; t < 4; ++t) {
       const int gm = m0 + m + (AT ? t : 0), gk = k0 + k + (AT ? 0 : t);
       if (gm < M && gk < K) {
-        const long off = AT ? (long)gk * lda + gm : (long)gm * lda + gk;
+        const long off = AT ? (long)gk * lda + gm : asym ? sym_off(gm, gk) : (long)gm * lda + gk;
         if (AC) {
           const double2 v = reinterpret_cast<const double2*>(A)[off];
           r[i][t][0] = v.x;
@@ -209,7 +209,7 @@ __global__ __launch_bounds__(64 * WM * WN) void gemm_f64_kernel(Gemm g, int tile
 
   double ra[C::RA][4][2], rb[C::RB][4][2];
   // K is processed as [kbeg, kend); the fetch helpers clip against kend.
-  fetch_a<C, AC, AT>(ra, A, g.lda, g.M, kend, m0, kbeg, tid);
+  fetch_a<C, AC, AT>(ra, A, g.lda, g.M, kend, m0, kbeg, tid, g.a_sym);
   fetch_b<C, BC, BT>(rb, B, g.ldb, g.N, kend, n0, kbeg, tid);
   store_a<C, AC, AT>(As[0], ra, tid);
   store_b<C, BC, BT>(Bs[0], rb, tid);
@@ -218,7 +218,7 @@ __global__ __launch_bounds__(64 * WM * WN) void gemm_f64_kernel(Gemm g, int tile
   for (int k0 = kbeg; k0 < kend; k0 += BK) {
     const bool more = k0 + BK < kend;
     if (more) {
-      fetch_a<C, AC, AT>(ra, A, g.lda, g.M, kend, m0, k0 + BK, tid);
+      fetch_a<C, AC, AT>(ra, A, g.lda, g.M, kend, m0, k0 + BK, tid, g.a_sym);
       fetch_b<C, BC, BT>(rb, B, g.ldb, g.N, kend, n0, k0 + BK, tid);
     }
     const double* as = As[buf];
@@ -437,6 +437,8 @@ int gemm_f64(const Gemm& g, hipStream_t s) {
   if (g.M <= 0 || g.N <= 0 || g.batch <= 0) return AFQ_OK;
   if (g.K <= 0) return fail(AFQ_E_INVALID, "gemm_f64: K must be positive");
   if (g.E) return fail(AFQ_E_UNSUPPORTED, "gemm_f64: no separate addend");
+  if (g.a_sym && (g.a_trans || g.a_sym != g.M || g.a_sym != g.K))
+    return fail(AFQ_E_INVALID, "gemm_f64: a packed symmetric A is square and not transposed");
   if (g.a_cplx) return g.b_cplx ? dispatch_t<true, true>(g, s) : dispatch_t<true, false>(g, s);
   return g.b_cplx ? dispatch_t<false, true>(g, s) : dispatch_t<false, false>(g, s);
 }

@@ -57,7 +57,8 @@ struct CfgL {
 };
 
 template <class C, bool AC, bool AT, typename TA>
-__device__ inline void fetch_a(float (&r)[C::RA][4][2], const TA* A, long lda, int M, int K, int m0, int k0, int tid) {
+__device__ inline void fetch_a(float (&r)[C::RA][4][2], const TA* A, long lda, int M, int K, int m0, int k0, int tid,
+                               int asym) {
 #pragma unroll
   for (int i = 0; i < C::RA; ++i) {
     const int q = tid + i * C::NT;
@@ -70,7 +71,7 @@ __device__ inline void fetch_a(float (&r)[C::RA][4][2], const TA* A, long lda, i
     for (int t = 0; t < 4; ++t) {
       const int gm = m0 + m + (AT ? t : 0), gk = k0 + k + (AT ? 0 : t);
       if (gm < M && gk < K) {
-        const float2 v = ld2(A, AT ? (long)gk * lda + gm : (long)gm * lda + gk, AC);
+        const float2 v = ld2(A, AT ? (long)gk * lda + gm : asym ? sym_off(gm, gk) : (long)gm * lda + gk, AC);
         r[i][t][0] = v.x;
         r[i][t][1] = v.y;
       }
@@ -204,7 +205,7 @@ __global__ __launch_bounds__(64 * WM * WN) void gemm_lp_kernel(Gemm g, int tiles
     }
 
   float ra[C::RA][4][2], rb[C::RB][4][2];
-  fetch_a<C, AC, AT, TA>(ra, A, g.lda, g.M, kend, m0, kbeg, tid);
+  fetch_a<C, AC, AT, TA>(ra, A, g.lda, g.M, kend, m0, kbeg, tid, g.a_sym);
   fetch_b<C, BC, BT, TB>(rb, B, g.ldb, g.N, kend, n0, kbeg, tid);
   store_a<C, AC, AT>(As[0], ra, tid);
   store_b<C, BC, BT>(Bs[0], rb, tid);
@@ -214,7 +215,7 @@ __global__ __launch_bounds__(64 * WM * WN) void gemm_lp_kernel(Gemm g, int tiles
   for (int k0 = kbeg; k0 < kend; k0 += BKL) {
     const bool more = k0 + BKL < kend;
     if (more) {
-      fetch_a<C, AC, AT, TA>(ra, A, g.lda, g.M, kend, m0, k0 + BKL, tid);
+      fetch_a<C, AC, AT, TA>(ra, A, g.lda, g.M, kend, m0, k0 + BKL, tid, g.a_sym);
       fetch_b<C, BC, BT, TB>(rb, B, g.ldb, g.N, kend, n0, k0 + BKL, tid);
     }
     const float* as = As[buf];
@@ -436,6 +437,8 @@ int gemm_lp(LpKind kind, bool bf16x3, const Gemm& g, hipStream_t s) {
   if (g.M <= 0 || g.N <= 0 || g.batch <= 0) return AFQ_OK;
   if (g.K <= 0) return fail(AFQ_E_INVALID, "gemm_lp: K must be positive");
   if (g.E && !g.D) return fail(AFQ_E_INVALID, "gemm_lp: an addend E needs an output D");
+  if (g.a_sym && (g.a_trans || g.a_sym != g.M || g.a_sym != g.K))
+    return fail(AFQ_E_INVALID, "gemm_lp: a packed symmetric A is square and not transposed");
   if (!g.C && !g.D) return fail(AFQ_E_INVALID, "gemm_lp: no output");
   return bf16x3 ? dispatch<true>(kind, g, s) : dispatch<false>(kind, g, s);
 }

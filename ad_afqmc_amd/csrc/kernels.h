@@ -30,7 +30,8 @@ int k_exchange_bf16(const uint16_t* rcs, const uint16_t* gs, int O, int N, int G
                     double2* kpart, int* nchunk_out, hipStream_t s);
 int k_to_f32(const double* src, float* dst, long n, hipStream_t s);
 // Symmetric Cholesky vectors L_g = L_g^T: the VHS product runs over the 16 x 16
-// orbital blocks (ib, jb) with ib >= jb only.  Packed column p is entry
+// orbital blocks (ib, jb) with ib >= jb only, and the Taylor products read that
+// packed VHS (Gemm::a_sym).  Packed column p is entry
 // ((p % 256) / 16, p % 16) of block p / 256 = ib (ib + 1) / 2 + jb (zero where
 // the block overhangs N).
 long sym_packed_cols(int N);
@@ -38,8 +39,6 @@ long sym_packed_cols(int N);
 int k_sym_check(const double* L, int G, int N, unsigned long long* out2, hipStream_t s);
 // (G, N*N) -> (G, sym_packed_cols(N)) as doubles (out64) or floats (out32)
 int k_sym_pack(const double* L, int G, int N, double* out64, float* out32, hipStream_t s);
-// packed (W, npk) -> full symmetric (W, N, N); complex128 (c64) or complex64
-int k_sym_unpack(const void* Cp, bool c64, int N, int W, void* V, hipStream_t s);
 // dst = src - I (n x n), rounded to FP32 after the subtraction
 int k_minus_identity_f32(const double* src, float* dst, int n, hipStream_t s);
 int k_energy_finalize(const double2* ene1, const double2* c0, const double2* c1, const double2* k0, const double2* k1,

@@ -423,29 +423,6 @@ __global__ void sym_pack_kernel(const double* __restrict__ L, int N, long npk, l
   out[t] = src >= 0 ? (T)L[g * (long)N * N + src] : (T)0;
 }
 
-// One work-group per (block, walker): writes the block at (ib, jb) and, for
-// ib != jb, its transpose at (jb, ib) through an LDS tile so both writes are
-// 16-element rows.
-template <typename T2>
-__global__ __launch_bounds__(256) void sym_unpack_kernel(const T2* __restrict__ Cp, long npk, int N,
-                                                         T2* __restrict__ V) {
-  __shared__ T2 tileb[16][17];
-  const int b = blockIdx.x, t = threadIdx.x;
-  const long w = blockIdx.y;
-  int ib, jb;
-  sym_block(b, &ib, &jb);
-  const long NN = (long)N * N;
-  const T2 v = Cp[w * npk + (long)b * 256 + t];
-  const int r = t >> 4, c = t & 15;
-  const int i = 16 * ib + r, j = 16 * jb + c;
-  if (i < N && j < N) V[w * NN + (long)i * N + j] = v;
-  if (ib == jb) return;  // uniform per work-group
-  tileb[r][c] = v;
-  __syncthreads();
-  const int i2 = 16 * jb + r, j2 = 16 * ib + c;
-  if (i2 < N && j2 < N) V[w * NN + (long)i2 * N + j2] = tileb[c][r];
-}
-
 inline unsigned blocks_for(long n, int t) { return (unsigned)((n + t - 1) / t); }
 
 }  // namespace
@@ -548,17 +525,6 @@ int k_sym_pack(const double* L, int G, int N, double* out64, float* out32, hipSt
     hipLaunchKernelGGL((sym_pack_kernel<float>), dim3(blocks_for(total, 256)), dim3(256), 0, s, L, N, npk, total,
                        out32);
   return launch_status("sym_pack_kernel");
-}
-int k_sym_unpack(const void* Cp, bool c64, int N, int W, void* V, hipStream_t s) {
-  const long npk = sym_packed_cols(N);
-  const dim3 grid((unsigned)(npk / 256), W);
-  if (c64)
-    hipLaunchKernelGGL((sym_unpack_kernel<double2>), grid, dim3(256), 0, s, static_cast<const double2*>(Cp), npk, N,
-                       static_cast<double2*>(V));
-  else
-    hipLaunchKernelGGL((sym_unpack_kernel<float2>), grid, dim3(256), 0, s, static_cast<const float2*>(Cp), npk, N,
-                       static_cast<float2*>(V));
-  return launch_status("sym_unpack_kernel");
 }
 int k_minus_identity_f32(const double* src, float* dst, int n, hipStream_t s) {
   if (n <= 0) return AFQ_OK;
