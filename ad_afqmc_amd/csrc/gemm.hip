@@ -396,6 +396,140 @@ int launch(const Gemm& g, hipStream_t s) {
   return AFQ_OK;
 }
 
+// Walker-panel products (A: N x N per walker or shared, B: N x c walker
+// columns, c <= 16 TN per column tile): each wave owns 16 rows of A and all
+// TN column blocks, so A is never shared between waves and goes straight from
+// global memory to registers in MFMA layout, one slab ahead; only the B panel
+// is staged in LDS (two buffers, one barrier per slab).  Within a 16-deep slab
+// lane (r, q) holds A[r][k0 + 4q + s] for MFMA step s, so each lane reads 4
+// consecutive k of its row, and B rows are read at the same permuted k.  With
+// A out of LDS a work-group needs a third of the LDS of the staged tiles,
+// which doubles the work-groups (and loads in flight) per CU.
+// grid: one dimension, tiles x batch rounded up to 8 entries, walkers mapped
+// to XCDs as in gemm_f64_kernel's batched order.
+template <bool AC, int NW, int TN>
+__global__ __launch_bounds__(64 * NW) void panel_f64_kernel(Gemm g, int tiles_m, int tiles_n) {
+  using C = Cfg<AC, true, false, false, NW, 1, 1, TN>;
+  __shared__ double Bs[2][2 * BK * C::LDB];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const long T = (long)tiles_m * tiles_n;
+  const long L = blockIdx.x;
+  const long p = L >> 3;
+  const long b = (L & 7) + 8 * (p / T);
+  if (b >= g.batch) return;  // uniform per work-group
+  const long tile = p % T;
+  const int mt = (int)(tile % tiles_m), nt = (int)(tile / tiles_m);
+  const int m0 = mt * 16 * NW, n0 = nt * C::BN;
+  const int r = lane & 15, q = lane >> 4;
+  const int row = m0 + wave * 16 + r;
+  const bool rok = row < g.M;
+  // a wave whose 16 rows all lie past M only helps stage B
+  const bool active = m0 + wave * 16 < g.M;
+  const double* A = static_cast<const double*>(g.A) + b * g.sA * (AC ? 2 : 1);
+  const double* B = static_cast<const double*>(g.B) + b * g.sB * 2;
+
+  auto load_a = [&](double (&x)[4][2], int k0) {
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+      const int k = k0 + 4 * q + t;
+      x[t][0] = x[t][1] = 0.0;
+      if (rok && k < g.K) {
+        const long off = g.a_sym ? sym_off(row, k) : (long)row * g.lda + k;
+        if (AC) {
+          const double2 v = reinterpret_cast<const double2*>(A)[off];
+          x[t][0] = v.x;
+          x[t][1] = v.y;
+        } else {
+          x[t][0] = A[off];
+        }
+      }
+    }
+  };
+
+  v4d accR[TN], accI[TN];
+#pragma unroll
+  for (int j = 0; j < TN; ++j) {
+    accR[j] = v4d{0.0, 0.0, 0.0, 0.0};
+    accI[j] = v4d{0.0, 0.0, 0.0, 0.0};
+  }
+  double ac[4][2], an[4][2], rb[C::RB][4][2];
+  if (active) load_a(ac, 0);
+  fetch_b<C, true, false>(rb, B, g.ldb, g.N, g.K, n0, 0, tid);
+  store_b<C, true, false>(Bs[0], rb, tid);
+  __syncthreads();
+  int buf = 0;
+  for (int k0 = 0; k0 < g.K; k0 += BK) {
+    const bool more = k0 + BK < g.K;
+    if (more) {
+      if (active) load_a(an, k0 + BK);
+      fetch_b<C, true, false>(rb, B, g.ldb, g.N, g.K, n0, k0 + BK, tid);
+    }
+    const double* bs = Bs[buf];
+#pragma unroll
+    for (int s = 0; s < 4 && active; ++s) {
+      const int kr = 4 * q + s;
+#pragma unroll
+      for (int j = 0; j < TN; ++j) {
+        const int col = j * 16 + r;
+        const double br = bs[kr * C::LDB + col], bi = bs[BK * C::LDB + kr * C::LDB + col];
+        accR[j] = __builtin_amdgcn_mfma_f64_16x16x4f64(ac[s][0], br, accR[j], 0, 0, 0);
+        accI[j] = __builtin_amdgcn_mfma_f64_16x16x4f64(ac[s][0], bi, accI[j], 0, 0, 0);
+        if (AC) {
+          accR[j] = __builtin_amdgcn_mfma_f64_16x16x4f64(-ac[s][1], bi, accR[j], 0, 0, 0);
+          accI[j] = __builtin_amdgcn_mfma_f64_16x16x4f64(ac[s][1], br, accI[j], 0, 0, 0);
+        }
+      }
+    }
+    if (more) {
+      store_b<C, true, false>(Bs[buf ^ 1], rb, tid);
+#pragma unroll
+      for (int t = 0; t < 4; ++t) {
+        ac[t][0] = an[t][0];
+        ac[t][1] = an[t][1];
+      }
+    }
+    __syncthreads();
+    buf ^= 1;
+  }
+
+  if (!active) return;
+  const bool has_beta = (g.beta.x != 0.0 || g.beta.y != 0.0);
+  double2* Cp = reinterpret_cast<double2*>(static_cast<double*>(g.C) + b * g.sC * 2);
+  double2* Dp = g.D ? reinterpret_cast<double2*>(static_cast<double*>(g.D) + b * g.sD * 2) : nullptr;
+#pragma unroll
+  for (int j = 0; j < TN; ++j)
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const int m = m0 + wave * 16 + q + 4 * v;
+      const int n = n0 + j * 16 + r;
+      if (m >= g.M || n >= g.N) continue;
+      const long off = (long)m * g.ldc + n;
+      double2 x = cmul(g.alpha, double2{accR[j][v], accI[j][v]});
+      if (has_beta) {
+        const double2 o = cmul(g.beta, Cp[off]);
+        x.x += o.x;
+        x.y += o.y;
+      }
+      Cp[off] = x;
+      if (Dp) {
+        double2* Dc = Dp + (long)m * g.ldd + n;
+        double2 d = *Dc;
+        d.x += x.x;
+        d.y += x.y;
+        *Dc = d;
+      }
+    }
+}
+
+template <bool AC, int NW, int TN>
+int launch_panel(const Gemm& g, hipStream_t s) {
+  const int tm = (g.M + 16 * NW - 1) / (16 * NW), tn = (g.N + 16 * TN - 1) / (16 * TN);
+  const long gx = (long)tm * tn * ((g.batch + 7) / 8 * 8);
+  if (gx > 0x7fffffffL) return fail(AFQ_E_UNSUPPORTED, "gemm_f64: grid too large");
+  hipLaunchKernelGGL((panel_f64_kernel<AC, NW, TN>), dim3((unsigned)gx), dim3(64 * NW), 0, s, g, tm, tn);
+  return launch_status("panel_f64_kernel");
+}
+
 template <bool AC, bool BC, bool AT, bool BT>
 int dispatch_shape(const Gemm& g, hipStream_t s) {
   // Large unbatched products with a wide N (the VHS product, N = norb^2): 16 waves
@@ -404,6 +538,13 @@ int dispatch_shape(const Gemm& g, hipStream_t s) {
   // at W=512, G=1200, N=200).
   if constexpr (!BC && !BT)  // (a complex B tile of this width would not fit in LDS)
     if (g.batch == 1 && g.N >= 4096 && g.M >= 64 && g.K >= 64) return launch<AC, BC, AT, BT, 2, 8, 2, 2>(g, s);
+  // walker panels of 33..48 columns against a complex panel (the Taylor terms and
+  // exp(-dt/2 h1) at O = 40): A straight to registers, 8 waves of 16 rows
+  // (0.243 vs 0.266 ms per Taylor term, 0.108 vs 0.130 ms per exp_h1 product at
+  // N=200, 512 walkers); wider panels stay on the LDS-staged tiles, which were
+  // faster at 118 columns
+  if constexpr (!AT && !BT && BC)
+    if (g.batch > 1 && g.N > 32 && g.N <= 48 && g.M > 64 && !g.ws) return launch_panel<AC, 8, 3>(g, s);
   // panel tiles for N <= 64 (walker-column products)
   if (g.N <= 16) return launch<AC, BC, AT, BT, 4, 1, 1, 1>(g, s);
   if (g.N <= 32) return launch<AC, BC, AT, BT, 4, 1, 1, 2>(g, s);
