@@ -84,6 +84,8 @@ CASES = [
     ("uhf", "restricted", 11, 4, 2, 9, 5),
     ("rhf", "restricted", 70, 17, 17, 130, 9),
     ("uhf", "unrestricted", 67, 21, 18, 97, 7),
+    # 40 walker columns: the register-panel Taylor and exp(-dt/2 h1) products
+    ("rhf", "restricted", 96, 40, 40, 150, 8),
 ]
 
 
@@ -324,6 +326,7 @@ LP_CASES = [
     ("rhf", "restricted", 12, 4, 4, 10, 6),
     ("uhf", "unrestricted", 67, 21, 18, 97, 7),
     ("rhf", "restricted", 70, 17, 17, 130, 9),
+    ("rhf", "restricted", 96, 40, 40, 150, 8),
 ]
 # relative tolerances vs the FP64 oracle for one measurement / one step
 LP_TOL = {"fp32": 2e-5, "bf16x3": 5e-4}
