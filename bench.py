@@ -48,6 +48,25 @@ def flops_per_walker_step(N, Oa, Ob, G, m, one_spin, K=5):
     return sum(f.values()), f
 
 
+def whole_step_roofline(N, Oa, Ob, G, m, one_spin, precision, value):
+    """Whole-step roofline: achieved = F_total x walker-steps/s; attainable =
+    sum_k F_k / sum_k (F_k / P_k) with P_k the peak of the precision contraction
+    k runs in (bf16x3 counted at a third of the bf16 peak).  In the reduced modes
+    the VHS, Taylor, force-bias and energy contractions run reduced; exp(-dt/2
+    h1) runs on FP32 (on FP64 at 33..48 walker columns); the Green's function,
+    overlap and QR stay FP64."""
+    f_total, f = flops_per_walker_step(N, Oa, Ob, G, m, one_spin)
+    red = {"fp64": PEAKS["fp64"], "fp32": PEAKS["fp32"], "bf16x3": PEAKS["bf16"] / 3.0}[precision]
+    cols = Oa if one_spin else Oa  # exp(-dt/2 h1) runs per spin buffer
+    peaks = {"vhs": red, "taylor": red, "force_bias": red, "energy": red,
+             "exp_h1": PEAKS["fp64"] if precision == "fp64" or 32 < cols <= 48 else PEAKS["fp32"],
+             "green_fb": PEAKS["fp64"], "overlap": PEAKS["fp64"], "qr": PEAKS["fp64"]}
+    attainable = f_total / sum(fk / peaks[k] for k, fk in f.items())
+    achieved = f_total * value
+    return {"achieved": achieved / 1e12, "attainable": attainable / 1e12, "unit": "TFLOP/s",
+            "frac": achieved / attainable, "flops_per_walker_step": f_total, "measurement_interval": m}
+
+
 def sym_packed_cols(N):
     """Columns of the VHS product when the Cholesky vectors are symmetric: the
     16 x 16 orbital blocks on and below the diagonal."""
@@ -98,20 +117,16 @@ def launch_flops(cls, N, Oa, Ob, G, W, unrestricted=False, vhs_cols=None):
     return None
 
 
-def cpu_baseline(cfg, sample_walkers=8, seed=0):
-    """The FP64 oracle timed on one host core on a bounded sample: a few walkers
-    of the same workload through one block (10 steps + QR + energy)."""
+def _oracle_block(cfg, W, seed):
+    """Set up the FP64 oracle on the workload and return a closure that runs one
+    block (n_prop_steps steps + QR + energy) of W walkers, and the step count."""
     import numpy as np
 
-    try:
-        from threadpoolctl import threadpool_limits
-    except Exception:  # pragma: no cover
-        threadpool_limits = None
     from ad_afqmc_amd import synthetic as syn
     from oracle import afqmc_oracle as O
 
-    s = syn.make_system(cfg["norb"], cfg["nocc_up"], cfg["nchol"], seed=seed)
-    dt, nsteps, W = cfg["dt"], cfg["n_prop_steps"], sample_walkers
+    s = syn.make_system(cfg["norb"], cfg["nocc_up"], cfg["nchol"], seed=0)
+    dt, nsteps = cfg["dt"], cfg["n_prop_steps"]
     h1s, rh, rc = O.rhf_measurement_intermediates(s.h1, s.chol, s.mo_up)
     pi = O.propagation_intermediates(s.h0, h1s, s.chol, s.rdm1(), dt)
     trial = O.RHFTrial(s.norb, s.nelec, s.mo_up)
@@ -121,16 +136,35 @@ def cpu_baseline(cfg, sample_walkers=8, seed=0):
     e0 = float(np.mean(trial.calc_energy(w0, s.h0, rh, rc).real))
     state = dict(walkers=w0, weights=np.ones(W), overlaps=trial.calc_overlap(w0), e_estimate=e0,
                  pop_control_ene_shift=e0)
-    rng = np.random.default_rng(seed)
-    fields = rng.standard_normal((nsteps, W, s.nchol))
-    ctx = threadpool_limits(limits=1) if threadpool_limits else None
+    fields = np.random.default_rng(seed).standard_normal((nsteps, W, s.nchol))
+    return (lambda: O.block_scan(trial, state, fields, ham, dt)), nsteps
+
+
+def _one_thread():
+    """Limits BLAS to one thread; numpy is imported first so that its BLAS is
+    loaded when the limit is applied (a fresh worker process has not loaded it)."""
     try:
+        import numpy  # noqa: F401
+        from threadpoolctl import threadpool_limits
+
+        return threadpool_limits(limits=1)
+    except Exception:  # pragma: no cover
+        return None
+
+
+def cpu_baseline(cfg, sample_walkers=8, seed=0):
+    """The FP64 oracle timed on one host core on a bounded sample: a few walkers
+    of the same workload through one block (10 steps + QR + energy)."""
+    limits = _one_thread()
+    try:
+        run, nsteps = _oracle_block(cfg, sample_walkers, seed)
         t0 = time.perf_counter()
-        state, be, bw = O.block_scan(trial, state, fields, ham, dt)
+        run()
         t = time.perf_counter() - t0
     finally:
-        if ctx is not None:
-            ctx.unregister() if hasattr(ctx, "unregister") else None
+        if limits is not None:
+            limits.restore_original_limits()
+    W = sample_walkers
     return {
         "value": W * nsteps / t,
         "unit": "walker-steps/s",
@@ -139,6 +173,66 @@ def cpu_baseline(cfg, sample_walkers=8, seed=0):
         "sample": f"{W} walkers x 1 block ({nsteps} steps + QR + energy) of the same workload, FP64 NumPy oracle, "
         f"1 BLAS thread, {t:.1f} s",
     }
+
+
+def _cpu_worker(cfg, W, seed, barrier, queue):
+    _limits = _one_thread()
+    run, nsteps = _oracle_block(cfg, W, seed)
+    barrier.wait(timeout=900)
+    t0 = time.perf_counter()
+    run()
+    queue.put((W * nsteps, t0, time.perf_counter()))
+
+
+def cpu_baseline_parallel(cfg, sample_walkers=8, procs=16):
+    """The same oracle sample in `procs` walker-sharded processes of one BLAS
+    thread each (the reference's MPI walker parallelism with seed + rank):
+    aggregate walker-steps over the span from the first start to the last end.
+    Worker processes are spawned fresh, before the caller touches the GPU."""
+    import multiprocessing as mp
+
+    ctx = mp.get_context("spawn")
+    barrier, queue = ctx.Barrier(procs), ctx.Queue()
+    ps = [ctx.Process(target=_cpu_worker, args=(cfg, sample_walkers, 1 + r, barrier, queue)) for r in range(procs)]
+    for p_ in ps:
+        p_.start()
+    res = []
+    deadline = time.time() + 1200
+    try:
+        for _ in ps:
+            res.append(queue.get(timeout=max(1.0, deadline - time.time())))
+    except Exception:  # a worker failed: report no parallel baseline rather than wait
+        pass
+    finally:
+        for p_ in ps:
+            p_.join(timeout=10)
+            if p_.is_alive():
+                p_.terminate()
+    if len(res) < procs:
+        return None
+    work = sum(r[0] for r in res)
+    span = max(r[2] for r in res) - min(r[1] for r in res)
+    return {
+        "value": work / span,
+        "unit": "walker-steps/s",
+        "cores": procs,
+        "kind": "port",
+        "sample": f"{procs} processes x {sample_walkers} walkers x 1 block of the same workload, FP64 NumPy oracle, "
+        f"1 BLAS thread each, {span:.1f} s span",
+    }
+
+
+def host_description():
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"cpu_count": os.cpu_count(), "cpu_model": model}
 
 
 def precision_study(cfg_name: str, n_blocks: int, n_eql: int, l_coef: float = 0.3):
@@ -315,7 +409,7 @@ def _precision_errors(name):
     return out
 
 
-def time_config(system, name, precision, steps, warmup, dominant_roofline=False, breakdown=False):
+def time_config(system, name, precision, steps, warmup, dominant_roofline=False, breakdown=False, nsteps=None):
     """Times `steps` device blocks (10 phaseless steps + QR + energy + comb
     resampling each) of workload `name` in `precision`, after `warmup` blocks.
     With dominant_roofline the warm-up times every kernel class and the timed
@@ -334,7 +428,7 @@ def time_config(system, name, precision, steps, warmup, dominant_roofline=False,
     uhf = cfg["uhf"]
     Oa = cfg["nocc_up"]
     Ob = cfg.get("nocc_dn", Oa)
-    nsteps, dt = cfg["n_prop_steps"], cfg["dt"]
+    nsteps, dt = nsteps or cfg["n_prop_steps"], cfg["dt"]
     wave, ham = system.wave_data(), system.ham_data()
     trial = wf.uhf(N, (Oa, Ob)) if uhf else wf.rhf(N, (Oa, Oa))
     wtype = "unrestricted" if uhf else "restricted"
@@ -418,6 +512,8 @@ def main():
     ap.add_argument("--precision", default="fp64")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-sample-walkers", type=int, default=8)
+    ap.add_argument("--cpu-procs", type=int, default=16,
+                    help="walker-sharded oracle processes for the all-cores CPU baseline (1: skip it)")
     ap.add_argument("--breakdown", action="store_true", help="print per-class kernel times to stderr")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run 5 plain device copies of MIB MiB (to scale HBM counters) and exit")
@@ -460,6 +556,13 @@ def main():
     Oa = cfg["nocc_up"]
     Ob = cfg.get("nocc_dn", Oa)
     nsteps, dt = cfg["n_prop_steps"], cfg["dt"]
+    cpu = cpu_all = None
+    if not args.no_cpu_baseline and not uhf:
+        # before anything touches the GPU: the parallel baseline spawns its workers
+        print("[bench] cpu baseline (oracle sample)", file=sys.stderr, flush=True)
+        cpu = cpu_baseline(cfg, args.cpu_sample_walkers)
+        if args.cpu_procs > 1:
+            cpu_all = cpu_baseline_parallel(cfg, args.cpu_sample_walkers, min(args.cpu_procs, os.cpu_count() or 1))
     print(f"[bench] building {args.config}", file=sys.stderr, flush=True)
     system = make_system(args.config)
     r = time_config(system, args.config, args.precision, args.steps, args.warmup, dominant_roofline=True,
@@ -495,6 +598,7 @@ def main():
         "model_flops_per_walker_step": f_total,
         "achieved_tflops_whole_step": f_total * value / 1e12,
         "roofline": r["roofline"],
+        "whole_step_roofline": whole_step_roofline(N, Oa, Ob, G, nsteps, not uhf, args.precision, value),
         "kernel_breakdown_ms_per_block": r["breakdown"],
         # The stated generator kills the walker population (see synthetic.make_system);
         # the work per block is the same dense arithmetic either way.
@@ -516,6 +620,8 @@ def main():
                 m = time_config(system, args.config, p, max(3, args.steps // 2), 1)
                 modes[p] = {"value": m["value"], "ms_per_step": m["elapsed"] / max(3, args.steps // 2) * 1e3}
             modes[p].update(errs.get(p, {}))
+            modes[p]["whole_step_roofline"] = whole_step_roofline(N, Oa, Ob, G, nsteps, not uhf, p,
+                                                                  modes[p]["value"])
         line["precision_modes"] = modes
         del system
         torch.cuda.empty_cache()
@@ -523,15 +629,22 @@ def main():
         big = "uhf_n300_o60_58_g2000_w256"
         print(f"[bench] building {big}", file=sys.stderr, flush=True)
         system = make_system(big)
-        ns = {}
-        for p in ("fp64", "fp32", "bf16x3"):
-            m = time_config(system, big, p, 2, 1)
-            ns[p] = {"value": m["value"], "ms_per_step": m["elapsed"] / 2 * 1e3}
+        bc = syn_config(big)
         errs = _precision_errors(big)
-        for p in ns:
-            ns[p].update(errs.get(p, {}))
-            ns[p]["speedup_vs_fp64"] = ns[p]["value"] / ns["fp64"]["value"]
-        line["uhf_n300_g2000_modes"] = ns
+        # measurement every 10 steps (as the RHF workload) and every 50 (the
+        # reference's default interval)
+        for m_int, key in ((10, "uhf_n300_g2000_modes"), (50, "uhf_n300_g2000_modes_m50")):
+            ns = {}
+            for p in ("fp64", "fp32", "bf16x3"):
+                m = time_config(system, big, p, 2, 1, nsteps=m_int)
+                ns[p] = {"value": m["value"], "ms_per_step": m["elapsed"] / 2 * 1e3, "steps_per_block": m_int}
+            for p in ns:
+                if m_int == bc["n_prop_steps"]:
+                    ns[p].update(errs.get(p, {}))
+                ns[p]["speedup_vs_fp64"] = ns[p]["value"] / ns["fp64"]["value"]
+                ns[p]["whole_step_roofline"] = whole_step_roofline(bc["norb"], bc["nocc_up"], bc["nocc_dn"],
+                                                                   bc["nchol"], m_int, False, p, ns[p]["value"])
+            line[key] = ns
         del system
         torch.cuda.empty_cache()
     if args.serving_jobs > 0:
@@ -550,9 +663,9 @@ def main():
                     max_concurrent_jobs=sch.max_concurrent, peak_admitted_gb=sch.peak_used / 1e9, hip_graphs=True,
                     workload="N~U{30..250}, O=N/5, G=5N, W~U{64..512}, 300 steps, Poisson 4 jobs/s")
         line["serving"] = summ
-    if not args.no_cpu_baseline:
-        print("[bench] cpu baseline (oracle sample)", file=sys.stderr, flush=True)
-        line["cpu_baseline"] = cpu_baseline(cfg, args.cpu_sample_walkers) if not uhf else None
+    line["cpu_baseline"] = cpu
+    line["cpu_baseline_all_cores"] = cpu_all
+    line["host"] = host_description()
     print(json.dumps(line), flush=True)
 
 
