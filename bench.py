@@ -488,12 +488,16 @@ def time_config(system, name, precision, steps, warmup, dominant_roofline=False,
             roof = {"kernel": dominant, "bound": "mfma", "achieved": ach / 1e12, "peak": peak / 1e12,
                     "unit": "TFLOP/s", "frac": ach / peak, "traffic": None, "avg_launch_ms": avg_s * 1e3,
                     "flops_per_launch": per_launch}
-            tfile = os.path.join(ROOT, "profiles", "traffic.json")
-            if os.path.exists(tfile):
+            # HBM bytes per launch and the MFMA pipe's busy fraction, both from
+            # rocprofv3 counter runs of this kernel on this workload
+            for fname, field in (("traffic.json", "traffic"), ("mfma_busy.json", "mfma_busy")):
+                tfile = os.path.join(ROOT, "profiles", fname)
+                if not os.path.exists(tfile):
+                    continue
                 try:
                     t = json.load(open(tfile)).get(f"{name}:{precision}:{dominant}")
                     if t is not None:
-                        roof["traffic"] = t
+                        roof[field] = t
                 except Exception:
                     pass
             res["roofline"] = roof
