@@ -191,6 +191,176 @@ __global__ __launch_bounds__(256, (NB <= 3 ? 2 : 1)) void exchange_kernel(const 
   }
 }
 
+// FP64 exchange for 48 < O <= 64: a 64 x 64 complex f-block takes 256
+// accumulator registers, which leaves one wave per SIMD.  Here two waves share
+// one (walker, gamma): wave half h holds the block rows 2h, 2h + 1 (2 x 4 blocks,
+// 128 registers), so two waves fit per SIMD.  The eight waves of a work-group
+// take four consecutive gamma of one walker, as in exchange_kernel.  For
+// sum_ij f_ij f_ji, a block whose transposed partner lies in the other half is
+// exchanged through LDS (each wave publishes its four cross blocks), and the
+// pairs inside a half are matched through a per-wave LDS tile afterwards.
+template <int KS>
+__global__ __launch_bounds__(512, 1) void exchange_split_kernel(const double* __restrict__ rc,
+                                                                const double2* __restrict__ Gm, int O, int N, int Gn,
+                                                                int W, int nchunk, int cpg, long T8,
+                                                                double2* __restrict__ coul, double2* __restrict__ kpart) {
+  constexpr int TS = 16 * 17;  // one padded 16 x 16 tile
+  extern __shared__ double2 xsh[];  // [8 waves][4 tiles][TS]
+  __shared__ double2 kred[8], cred[8];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int gi = wave >> 1, h = wave & 1;
+  const long bid = blockIdx.x;
+  const long L = (bid & 7) * T8 + (bid >> 3);
+  const long per_group = (long)W * cpg;
+  const long grp = L / per_group;
+  const long rem = L - grp * per_group;
+  const int w = (int)(rem / cpg);
+  const int chunk = (int)(grp * cpg + rem % cpg);
+  if (chunk >= nchunk) return;  // uniform for the whole work-group
+  const int g = chunk * 4 + gi;
+  const bool valid = g < Gn;
+  const int gg = valid ? g : Gn - 1;
+  const double* A = rc + (long)gg * O * N;
+  const double2* B = Gm + (long)w * O * N;
+  const int r16 = lane & 15, q4 = lane >> 4;
+
+  long ra[2], rb[4];
+  bool oka[2], okb[4];
+#pragma unroll
+  for (int i = 0; i < 2; ++i) {
+    const int row = 16 * (2 * h + i) + r16;
+    oka[i] = row < O;
+    ra[i] = (long)min(row, O - 1) * N;
+  }
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    const int row = 16 * j + r16;
+    okb[j] = row < O;
+    rb[j] = (long)min(row, O - 1) * N;
+  }
+  v4d accR[2][4], accI[2][4];
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) accR[i][j] = accI[i][j] = v4d{0.0, 0.0, 0.0, 0.0};
+
+  struct Fr {
+    double a[2][KS], br[4][KS], bi[4][KS];
+  };
+  auto load = [&](Fr& F, int c) {
+    const int kb = c + KS * q4;
+#pragma unroll
+    for (int s = 0; s < KS; ++s) {
+      const int p = kb + s;
+      const bool pk = p < N;
+      const int pc = pk ? p : N - 1;
+#pragma unroll
+      for (int i = 0; i < 2; ++i) {
+        const double av = A[ra[i] + pc];
+        F.a[i][s] = (pk && oka[i]) ? av : 0.0;
+      }
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const double2 bv = B[rb[j] + pc];
+        F.br[j][s] = (pk && okb[j]) ? bv.x : 0.0;
+        F.bi[j][s] = (pk && okb[j]) ? bv.y : 0.0;
+      }
+    }
+  };
+  constexpr int CW = 4 * KS;
+  Fr cur, nxt;
+  load(cur, 0);
+  for (int c = 0; c < N; c += CW) {
+    if (c + CW < N) load(nxt, c + CW);
+#pragma unroll
+    for (int s = 0; s < KS; ++s)
+#pragma unroll
+      for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+          accR[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(cur.a[i][s], cur.br[j][s], accR[i][j], 0, 0, 0);
+          accI[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(cur.a[i][s], cur.bi[j][s], accI[i][j], 0, 0, 0);
+        }
+    if (c + CW < N) cur = nxt;
+  }
+
+  // element v of lane l is (row, col) = (l/16 + 4v, l%16) of its 16 x 16 block
+  const int cc = lane & 15;
+  double kr = 0.0, ki = 0.0, cr = 0.0, ci = 0.0;
+  double2* mine = xsh + (long)wave * 4 * TS;
+  const double2* other = xsh + (long)(wave ^ 1) * 4 * TS;
+  const int ho = 1 - h;
+  // cross blocks (2h + i, 2ho + jj): publish, then pair with the partner's (2ho + jj, 2h + i)
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int jj = 0; jj < 2; ++jj)
+#pragma unroll
+      for (int v = 0; v < 4; ++v)
+        mine[(i * 2 + jj) * TS + (q4 + 4 * v) * 17 + cc] =
+            double2{accR[i][2 * ho + jj][v], accI[i][2 * ho + jj][v]};
+  __syncthreads();
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int jj = 0; jj < 2; ++jj)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int r = q4 + 4 * v;
+        const double xr = accR[i][2 * ho + jj][v], xi = accI[i][2 * ho + jj][v];
+        const double2 y = other[(jj * 2 + i) * TS + cc * 17 + r];
+        kr += xr * y.x - xi * y.y;
+        ki += xr * y.y + xi * y.x;
+      }
+  __syncthreads();
+  // pairs inside the half: blocks (2h + i, 2h + j), i <= j, through the first tile
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = i; j < 2; ++j) {
+#pragma unroll
+      for (int v = 0; v < 4; ++v)
+        mine[(q4 + 4 * v) * 17 + cc] = double2{accR[j][2 * h + i][v], accI[j][2 * h + i][v]};
+      __syncthreads();
+      const double wgt = (i == j) ? 1.0 : 2.0;
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int r = q4 + 4 * v;
+        const double xr = accR[i][2 * h + j][v], xi = accI[i][2 * h + j][v];
+        const double2 y = mine[cc * 17 + r];
+        kr += wgt * (xr * y.x - xi * y.y);
+        ki += wgt * (xr * y.y + xi * y.x);
+        if (i == j && r == cc) {
+          cr += xr;
+          ci += xi;
+        }
+      }
+      __syncthreads();
+    }
+  kr = wave_sum(kr);
+  ki = wave_sum(ki);
+  cr = wave_sum(cr);
+  ci = wave_sum(ci);
+  if (lane == 0) {
+    kred[wave] = valid ? double2{kr, ki} : double2{0.0, 0.0};
+    cred[wave] = double2{cr, ci};
+  }
+  __syncthreads();
+  if (tid < 4) {
+    const int gq = chunk * 4 + tid;
+    if (gq < Gn)
+      coul[(long)w * Gn + gq] = double2{cred[2 * tid].x + cred[2 * tid + 1].x, cred[2 * tid].y + cred[2 * tid + 1].y};
+  }
+  if (tid == 0) {
+    double2 sum = {0.0, 0.0};
+    for (int q = 0; q < 8; ++q) {
+      sum.x += kred[q].x;
+      sum.y += kred[q].y;
+    }
+    kpart[(long)w * nchunk + chunk] = sum;
+  }
+}
+
 // Split-BF16 x3 exchange on pre-split operands.  rot_chol is held as bf16
 // (hi, lo) planes and each walker's Green's function as (re_hi, re_lo, im_hi,
 // im_lo) planes, rows padded to 16*NB and columns to a multiple of 32 with
@@ -394,6 +564,29 @@ int launch_nb(const TR* rc, const double2* Gm, int O, int N, int Gn, int W, int 
   return launch_status("exchange_kernel");
 }
 
+template <int KS>
+int launch_split(const double* rc, const double2* Gm, int O, int N, int Gn, int W, int nchunk, double2* coul,
+                 double2* kpart, hipStream_t s) {
+  const long chunk_bytes = 4L * O * N * (long)sizeof(double);
+  const int cpg = (int)std::max(1L, std::min((long)nchunk, (2560L * 1024) / std::max(1L, chunk_bytes)));
+  const long ngroups = (nchunk + cpg - 1) / cpg;
+  const long T = ngroups * cpg * (long)W;
+  const long T8 = (T + 7) / 8;
+  const long grid = T8 * 8;
+  if (grid > 0x7fffffffL) return fail(AFQ_E_UNSUPPORTED, "exchange grid too large");
+  const size_t lds = sizeof(double2) * 8 * 4 * 16 * 17;
+  static bool raised = false;
+  if (!raised) {
+    hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(exchange_split_kernel<KS>),
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (e != hipSuccess) return fail(AFQ_E_HIP, std::string("exchange LDS attribute: ") + hipGetErrorString(e));
+    raised = true;
+  }
+  hipLaunchKernelGGL((exchange_split_kernel<KS>), dim3((unsigned)grid), dim3(512), lds, s, rc, Gm, O, N, Gn, W,
+                     nchunk, cpg, T8, coul, kpart);
+  return launch_status("exchange_split_kernel");
+}
+
 template <int MODE, typename TR>
 int launch_mode(const TR* rc, const double2* Gm, int O, int N, int Gn, int W, int nchunk, double2* coul,
                 double2* kpart, hipStream_t s) {
@@ -416,6 +609,8 @@ int k_exchange(const double* rc, const double2* Gm, int O, int N, int Gn, int W,
   if (O < 1 || O > 64) return fail(AFQ_E_UNSUPPORTED, "exchange kernel supports 1 <= nocc <= 64");
   const int nchunk = exchange_nchunk(Gn);
   *nchunk_out = nchunk;
+  // 48 < O <= 64: two waves per f-block (the 64 x 64 block alone would take 256 registers)
+  if (O > 48) return launch_split<2>(rc, Gm, O, N, Gn, W, nchunk, coul, kpart, s);
   return launch_mode<F64, double>(rc, Gm, O, N, Gn, W, nchunk, coul, kpart, s);
 }
 

@@ -86,6 +86,8 @@ CASES = [
     ("uhf", "unrestricted", 67, 21, 18, 97, 7),
     # 40 walker columns: the register-panel Taylor and exp(-dt/2 h1) products
     ("rhf", "restricted", 96, 40, 40, 150, 8),
+    # 48 < O <= 64: the two-wave FP64 exchange
+    ("uhf", "unrestricted", 72, 60, 52, 40, 4),
 ]
 
 
