@@ -61,8 +61,10 @@ __device__ inline void load_frag(Frag<MODE, NB, KS>& F, const TR* __restrict__ A
   const int kb = c + KE * (lane >> 4);
   const int r16 = lane & 15;
   const bool full = (kb + KE - 1 < N);
-  // Branch-free: rows >= O and columns >= N are clamped to valid addresses
-  // and their values masked to zero.
+  // Rows >= O and columns >= N are clamped to valid addresses.  F64 keeps the
+  // raw values (mask_frag zeroes A there before use, so that no value is
+  // consumed until the MFMAs of the current chunk have been issued); F32
+  // masks and converts here.
 #pragma unroll
   for (int bi = 0; bi < NB; ++bi) {
     const int row = bi * 16 + r16;
@@ -70,13 +72,36 @@ __device__ inline void load_frag(Frag<MODE, NB, KS>& F, const TR* __restrict__ A
     const long roff = (long)(rok ? row : O - 1) * N;
 #pragma unroll
     for (int s = 0; s < KE; ++s) {
-      const bool ok = rok && (full || kb + s < N);
       const int col = full ? kb + s : min(kb + s, N - 1);
       const double2 v = B[roff + col];
       const TR a = A[roff + col];
-      F.a[bi][s] = ok ? (R)a : (R)0;
-      F.br[bi][s] = ok ? (R)v.x : (R)0;
-      F.bi[bi][s] = ok ? (R)v.y : (R)0;
+      if constexpr (MODE == F64) {
+        F.a[bi][s] = a;
+        F.br[bi][s] = v.x;
+        F.bi[bi][s] = v.y;
+      } else {
+        const bool ok = rok && (full || kb + s < N);
+        F.a[bi][s] = ok ? (R)a : (R)0;
+        F.br[bi][s] = ok ? (R)v.x : (R)0;
+        F.bi[bi][s] = ok ? (R)v.y : (R)0;
+      }
+    }
+  }
+}
+
+// F64: zero the A (rot_chol) entries of rows >= O and columns >= N.  That
+// alone makes every padded product vanish: f_ij = 0 for i >= O, so f_ij f_ji
+// and f_ii vanish whenever i or j >= O; the clamped B values stay finite.
+template <int MODE, int NB, int KS>
+__device__ inline void mask_frag(Frag<MODE, NB, KS>& F, int O, int N, int c, int lane) {
+  if constexpr (MODE == F64) {
+    constexpr int KE = Frag<MODE, NB, KS>::KE;
+    const int kb = c + KE * (lane >> 4);
+#pragma unroll
+    for (int bi = 0; bi < NB; ++bi) {
+      const bool rok = bi * 16 + (lane & 15) < O;
+#pragma unroll
+      for (int s = 0; s < KE; ++s) F.a[bi][s] = (rok && kb + s < N) ? F.a[bi][s] : 0.0;
     }
   }
 }
@@ -120,6 +145,7 @@ __global__ __launch_bounds__(256, (NB <= 3 ? 2 : 1)) void exchange_kernel(const 
   constexpr int CW = 4 * Fr::KE;
   Fr cur, nxt;
   load_frag<MODE, NB, KS, TR>(cur, A, B, O, N, 0, lane);
+  mask_frag<MODE, NB, KS>(cur, O, N, 0, lane);
   for (int c = 0; c < N; c += CW) {
     if (c + CW < N) load_frag<MODE, NB, KS, TR>(nxt, A, B, O, N, c + CW, lane);
 #pragma unroll
@@ -136,7 +162,10 @@ __global__ __launch_bounds__(256, (NB <= 3 ? 2 : 1)) void exchange_kernel(const 
             accI[bi][bj] = __builtin_amdgcn_mfma_f32_16x16x4f32(cur.a[bi][s], cur.bi[bj][s], accI[bi][bj], 0, 0, 0);
           }
         }
-    if (c + CW < N) cur = nxt;
+    if (c + CW < N) {
+      cur = nxt;
+      mask_frag<MODE, NB, KS>(cur, O, N, c + CW, lane);
+    }
   }
 
   // element v of lane l sits at (row, col) = (l/16 + 4v, l%16) for f64 and
@@ -225,7 +254,7 @@ __global__ __launch_bounds__(512, 1) void exchange_split_kernel(const double* __
   const int r16 = lane & 15, q4 = lane >> 4;
 
   long ra[2], rb[4];
-  bool oka[2], okb[4];
+  bool oka[2];
 #pragma unroll
   for (int i = 0; i < 2; ++i) {
     const int row = 16 * (2 * h + i) + r16;
@@ -234,9 +263,7 @@ __global__ __launch_bounds__(512, 1) void exchange_split_kernel(const double* __
   }
 #pragma unroll
   for (int j = 0; j < 4; ++j) {
-    const int row = 16 * j + r16;
-    okb[j] = row < O;
-    rb[j] = (long)min(row, O - 1) * N;
+    rb[j] = (long)min(16 * j + r16, O - 1) * N;
   }
   v4d accR[2][4], accI[2][4];
 #pragma unroll
@@ -247,29 +274,35 @@ __global__ __launch_bounds__(512, 1) void exchange_split_kernel(const double* __
   struct Fr {
     double a[2][KS], br[4][KS], bi[4][KS];
   };
+  // raw values at clamped addresses; mask() zeroes A at rows >= O and
+  // columns >= N once the chunk is about to be used, which makes every padded
+  // product vanish (f_ij = 0 for i >= O) while the clamped B values stay finite
   auto load = [&](Fr& F, int c) {
     const int kb = c + KS * q4;
 #pragma unroll
     for (int s = 0; s < KS; ++s) {
-      const int p = kb + s;
-      const bool pk = p < N;
-      const int pc = pk ? p : N - 1;
+      const int pc = min(kb + s, N - 1);
 #pragma unroll
-      for (int i = 0; i < 2; ++i) {
-        const double av = A[ra[i] + pc];
-        F.a[i][s] = (pk && oka[i]) ? av : 0.0;
-      }
+      for (int i = 0; i < 2; ++i) F.a[i][s] = A[ra[i] + pc];
 #pragma unroll
       for (int j = 0; j < 4; ++j) {
         const double2 bv = B[rb[j] + pc];
-        F.br[j][s] = (pk && okb[j]) ? bv.x : 0.0;
-        F.bi[j][s] = (pk && okb[j]) ? bv.y : 0.0;
+        F.br[j][s] = bv.x;
+        F.bi[j][s] = bv.y;
       }
     }
+  };
+  auto mask = [&](Fr& F, int c) {
+    const int kb = c + KS * q4;
+#pragma unroll
+    for (int s = 0; s < KS; ++s)
+#pragma unroll
+      for (int i = 0; i < 2; ++i) F.a[i][s] = (oka[i] && kb + s < N) ? F.a[i][s] : 0.0;
   };
   constexpr int CW = 4 * KS;
   Fr cur, nxt;
   load(cur, 0);
+  mask(cur, 0);
   for (int c = 0; c < N; c += CW) {
     if (c + CW < N) load(nxt, c + CW);
 #pragma unroll
@@ -281,7 +314,10 @@ __global__ __launch_bounds__(512, 1) void exchange_split_kernel(const double* __
           accR[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(cur.a[i][s], cur.br[j][s], accR[i][j], 0, 0, 0);
           accI[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(cur.a[i][s], cur.bi[j][s], accI[i][j], 0, 0, 0);
         }
-    if (c + CW < N) cur = nxt;
+    if (c + CW < N) {
+      cur = nxt;
+      mask(cur, c + CW);
+    }
   }
 
   // element v of lane l is (row, col) = (l/16 + 4v, l%16) of its 16 x 16 block
@@ -428,6 +464,17 @@ __global__ __launch_bounds__(256, 2) void exchange_bf16_kernel(const v8bf* __res
       accR[i][j] = v4f{0.f, 0.f, 0.f, 0.f};
       accI[i][j] = v4f{0.f, 0.f, 0.f, 0.f};
     }
+  // This wave's rot_chol fragments go straight to registers.  With NB = 4
+  // (two waves per SIMD whatever the register count) the hi ones are loaded
+  // one step ahead and the lo ones at the start of their step, so that their
+  // loads overlap the hi x hi pass; smaller NB keep to the registers that
+  // leave three or more waves per SIMD: both are loaded in their step.
+  constexpr bool PF = NB == 4;
+  v8bf ah[NB], al[NB], nh[NB];
+  if (PF) {
+#pragma unroll
+    for (int b = 0; b < NB; ++b) ah[b] = Ah[(long)b * 16 * rowv];
+  }
   fetch(0);
   store(0);
   __syncthreads();
@@ -435,11 +482,22 @@ __global__ __launch_bounds__(256, 2) void exchange_bf16_kernel(const v8bf* __res
   for (int kv = 0; kv < rowv; kv += 4) {
     const bool more = kv + 4 < rowv;
     if (more) fetch(kv + 4);
+    if (PF && more) {
+#pragma unroll
+      for (int b = 0; b < NB; ++b) nh[b] = Ah[(long)b * 16 * rowv + kv + 4];
+    }
+    if (!PF) {
+#pragma unroll
+      for (int b = 0; b < NB; ++b) ah[b] = Ah[(long)b * 16 * rowv + kv];
+    }
+    if (PF) {
+#pragma unroll
+      for (int b = 0; b < NB; ++b) al[b] = Al[(long)b * 16 * rowv + kv];
+    }
     const v8bf* B = bsh[buf] + r16 * 4 + q4;  // + plane * 4 ROWS + 16 b * 4
-    v8bf ah[NB], br[NB], bi[NB];
+    v8bf br[NB], bi[NB];
 #pragma unroll
     for (int b = 0; b < NB; ++b) {
-      ah[b] = Ah[(long)b * 16 * rowv + kv];
       br[b] = B[b * 64];
       bi[b] = B[2 * 4 * ROWS + b * 64];
     }
@@ -450,9 +508,10 @@ __global__ __launch_bounds__(256, 2) void exchange_bf16_kernel(const v8bf* __res
         accR[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], br[j], accR[i][j], 0, 0, 0);
         accI[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bi[j], accI[i][j], 0, 0, 0);
       }
-    v8bf al[NB];
+    if (!PF) {
 #pragma unroll
-    for (int b = 0; b < NB; ++b) al[b] = Al[(long)b * 16 * rowv + kv];
+      for (int b = 0; b < NB; ++b) al[b] = Al[(long)b * 16 * rowv + kv];
+    }
 #pragma unroll
     for (int i = 0; i < NB; ++i)
 #pragma unroll
@@ -472,7 +531,13 @@ __global__ __launch_bounds__(256, 2) void exchange_bf16_kernel(const v8bf* __res
         accR[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], br[j], accR[i][j], 0, 0, 0);
         accI[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bi[j], accI[i][j], 0, 0, 0);
       }
-    if (more) store(buf ^ 1);
+    if (more) {
+      store(buf ^ 1);
+      if (PF) {
+#pragma unroll
+        for (int b = 0; b < NB; ++b) ah[b] = nh[b];
+      }
+    }
     __syncthreads();
     buf ^= 1;
   }

@@ -399,7 +399,7 @@ int launch(const Gemm& g, hipStream_t s) {
 // with k contiguous, so a B fragment is one 16-byte LDS read.
 constexpr int PROW = 40;  // bf16 per LDS row of a B plane: 32 k + 8 pad (80 bytes: a fragment's 16 rows hit distinct banks)
 
-template <int NW, int TN, typename TB>
+template <int NW, int TN, typename TB, bool ASYM>
 __global__ __launch_bounds__(64 * NW) void panel_bf3_kernel(Gemm g, int tiles_m, int tiles_n) {
   constexpr int BN = 16 * TN, PLANE = BN * PROW;
   constexpr int NE = 32 * BN, RE = (NE + 64 * NW - 1) / (64 * NW);  // B elements per slab
@@ -419,55 +419,64 @@ __global__ __launch_bounds__(64 * NW) void panel_bf3_kernel(Gemm g, int tiles_m,
   const bool active = m0 + wave * 16 < g.M;
   const float2* A = static_cast<const float2*>(g.A) + b * g.sA;
   const TB* B = static_cast<const TB*>(g.B) + b * g.sB * 2;
+  const int rowc = min(row, g.M - 1);
 
-  auto load_a = [&](float (&re)[8], float (&im)[8], int k0) {
+  // A is read at clamped (always valid) addresses by every wave and masked by
+  // mask_a only when the slab is about to be used, so that no loaded value is
+  // consumed inside a divergent branch (which makes the compiler wait for each
+  // load there and serialises the slab's loads ahead of the MFMAs).
+  auto load_a = [&](float2 (&x)[8], int k0) {
 #pragma unroll
     for (int t = 0; t < 8; ++t) {
-      const int k = k0 + 8 * q + t;
-      re[t] = im[t] = 0.f;
-      if (rok && k < g.K) {
-        const float2 v = A[g.a_sym ? sym_off(row, k) : (long)row * g.lda + k];
-        re[t] = v.x;
-        im[t] = v.y;
-      }
+      const int k = min(k0 + 8 * q + t, g.K - 1);
+      x[t] = A[ASYM ? sym_off(rowc, k) : (long)rowc * g.lda + k];
     }
   };
-  auto split8 = [](const float (&x)[8], v8bf& hi, v8bf& lo, bool neg) {
+  auto mask_a = [&](float2 (&x)[8], int k0) {
 #pragma unroll
     for (int t = 0; t < 8; ++t) {
-      const float y = neg ? -x[t] : x[t];
+      const bool ok = rok && k0 + 8 * q + t < g.K;
+      x[t].x = ok ? x[t].x : 0.f;
+      x[t].y = ok ? x[t].y : 0.f;
+    }
+  };
+  // hi / lo bf16 of the real (im = false) or imaginary parts, negated on request
+  auto split8 = [](const float2 (&x)[8], bool im, v8bf& hi, v8bf& lo, bool neg) {
+#pragma unroll
+    for (int t = 0; t < 8; ++t) {
+      const float v = im ? x[t].y : x[t].x;
+      const float y = neg ? -v : v;
       const __bf16 h = (__bf16)y;
       hi[t] = h;
       lo[t] = (__bf16)(y - (float)h);
     }
   };
-  // B slab rows k0 .. k0+31: element e = (k, n), n fastest
-  auto load_b = [&](float (&x)[RE][2], int k0) {
+  // B slab rows k0 .. k0+31: element e = (k, n), n fastest; read raw at
+  // clamped addresses, masked, rounded and split when stashed
+  auto load_b = [&](TB (&x)[RE][2], int k0) {
 #pragma unroll
     for (int i = 0; i < RE; ++i) {
-      const int e = tid + i * 64 * NW;
-      const int kk = e / BN, nn = e % BN;
-      const int gk = k0 + kk, gn = n0 + nn;
-      x[i][0] = x[i][1] = 0.f;
-      if (e < NE && gk < g.K && gn < g.N) {
-        const long off = ((long)gk * g.ldb + gn) * 2;
-        x[i][0] = (float)B[off];
-        x[i][1] = (float)B[off + 1];
-      }
+      const int e = min(tid + i * 64 * NW, NE - 1);
+      const int gk = min(k0 + e / BN, g.K - 1), gn = min(n0 + e % BN, g.N - 1);
+      const long off = ((long)gk * g.ldb + gn) * 2;
+      x[i][0] = B[off];
+      x[i][1] = B[off + 1];
     }
   };
-  auto stash_b = [&](__bf16* bs, const float (&x)[RE][2]) {
+  auto stash_b = [&](__bf16* bs, const TB (&x)[RE][2], int k0) {
 #pragma unroll
     for (int i = 0; i < RE; ++i) {
       const int e = tid + i * 64 * NW;
       if (e >= NE) continue;
       const int kk = e / BN, nn = e % BN;
+      const bool ok = k0 + kk < g.K && n0 + nn < g.N;
       const int o = nn * PROW + kk;
 #pragma unroll
       for (int c = 0; c < 2; ++c) {
-        const __bf16 h = (__bf16)x[i][c];
+        const float y = ok ? (float)x[i][c] : 0.f;
+        const __bf16 h = (__bf16)y;
         bs[(2 * c) * PLANE + o] = h;
-        bs[(2 * c + 1) * PLANE + o] = (__bf16)(x[i][c] - (float)h);
+        bs[(2 * c + 1) * PLANE + o] = (__bf16)(y - (float)h);
       }
     }
   };
@@ -478,23 +487,28 @@ __global__ __launch_bounds__(64 * NW) void panel_bf3_kernel(Gemm g, int tiles_m,
     accR[j] = v4f{0.f, 0.f, 0.f, 0.f};
     accI[j] = v4f{0.f, 0.f, 0.f, 0.f};
   }
-  float are[8], aim[8], nre[8], nim[8], rb[RE][2];
-  if (active) load_a(are, aim, 0);
+  float2 ac[8], an[8];
+  TB rb[RE][2];
+  load_a(ac, 0);
+  mask_a(ac, 0);
   load_b(rb, 0);
-  stash_b(Bs[0], rb);
+  stash_b(Bs[0], rb, 0);
   __syncthreads();
   int buf = 0;
   for (int k0 = 0; k0 < g.K; k0 += 32) {
     const bool more = k0 + 32 < g.K;
     if (more) {
-      if (active) load_a(nre, nim, k0 + 32);
+      load_a(an, k0 + 32);
       load_b(rb, k0 + 32);
     }
-    if (active) {
+    // every wave multiplies (a wave past M has all-zero A and stores nothing):
+    // a divergent branch here would make the compiler settle the prefetched
+    // A registers, and so wait for their loads, before the MFMAs
+    {
       v8bf rh, rl, ih, il, nih, nil;
-      split8(are, rh, rl, false);
-      split8(aim, ih, il, false);
-      split8(aim, nih, nil, true);
+      split8(ac, false, rh, rl, false);
+      split8(ac, true, ih, il, false);
+      split8(ac, true, nih, nil, true);
       const __bf16* bs = Bs[buf];
 #pragma unroll
       for (int j = 0; j < TN; ++j) {
@@ -519,12 +533,10 @@ __global__ __launch_bounds__(64 * NW) void panel_bf3_kernel(Gemm g, int tiles_m,
       }
     }
     if (more) {
-      stash_b(Bs[buf ^ 1], rb);
+      stash_b(Bs[buf ^ 1], rb, k0 + 32);
 #pragma unroll
-      for (int t = 0; t < 8; ++t) {
-        are[t] = nre[t];
-        aim[t] = nim[t];
-      }
+      for (int t = 0; t < 8; ++t) ac[t] = an[t];
+      mask_a(ac, k0 + 32);
     }
     __syncthreads();
     buf ^= 1;
@@ -559,7 +571,10 @@ int launch_panel_bf3(const Gemm& g, hipStream_t s) {
   const int tm = (g.M + 16 * NW - 1) / (16 * NW), tn = (g.N + 16 * TN - 1) / (16 * TN);
   const long gx = (long)tm * tn * ((g.batch + 7) / 8 * 8);
   if (gx > 0x7fffffffL) return fail(AFQ_E_UNSUPPORTED, "gemm_lp: grid too large");
-  hipLaunchKernelGGL((panel_bf3_kernel<NW, TN, TB>), dim3((unsigned)gx), dim3(64 * NW), 0, s, g, tm, tn);
+  if (g.a_sym)
+    hipLaunchKernelGGL((panel_bf3_kernel<NW, TN, TB, true>), dim3((unsigned)gx), dim3(64 * NW), 0, s, g, tm, tn);
+  else
+    hipLaunchKernelGGL((panel_bf3_kernel<NW, TN, TB, false>), dim3((unsigned)gx), dim3(64 * NW), 0, s, g, tm, tn);
   return launch_status("panel_bf3_kernel");
 }
 
