@@ -613,10 +613,14 @@ int dispatch(LpKind kind, const Gemm& g, hipStream_t s) {
       return dispatch_shape<BF, true, true, false, false, float, double, float2>(g, s);
     case LpKind::TAYLOR_F:  // complex float x complex float -> complex float (+ D)
       // bf16x3 at 33..48 columns: the register-panel kernel with 16 waves (0.120 vs
-      // 0.143 ms per term at N=200, 40 columns, 512 walkers)
+      // 0.143 ms per term at N=200, 40 columns, 512 walkers); at 65..128 columns
+      // (both spins of unrestricted walkers) with 8 waves of 8 column blocks
+      // (0.231 vs 0.273 ms per term at N=300, 118 columns, 256 walkers, packed A)
       if constexpr (BF)
-        if (g.batch > 1 && g.N > 32 && g.N <= 48 && g.M > 64 && !g.ws && !g.a_trans && !g.b_trans)
-          return launch_panel_bf3<16, 3, float>(g, s);
+        if (g.batch > 1 && g.M > 64 && !g.ws && !g.a_trans && !g.b_trans) {
+          if (g.N > 32 && g.N <= 48) return launch_panel_bf3<16, 3, float>(g, s);
+          if (g.N > 64 && g.N <= 128) return launch_panel_bf3<8, 8, float>(g, s);
+        }
       return dispatch_shape<BF, true, true, false, false, float, float, float2>(g, s);
     case LpKind::FORCE_BIAS:  // complex double x real float (transposed) -> complex double
       return dispatch_shape<BF, true, false, false, true, double, float, double2>(g, s);

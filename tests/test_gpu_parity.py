@@ -329,6 +329,8 @@ LP_CASES = [
     ("uhf", "unrestricted", 67, 21, 18, 97, 7),
     ("rhf", "restricted", 70, 17, 17, 130, 9),
     ("rhf", "restricted", 96, 40, 40, 150, 8),
+    # 112 walker columns (both spins) on the bf16x3 register-panel Taylor kernel; O > 48 exchanges
+    ("uhf", "unrestricted", 72, 60, 52, 40, 4),
 ]
 # relative tolerances vs the FP64 oracle for one measurement / one step
 LP_TOL = {"fp32": 2e-5, "bf16x3": 5e-4}
