@@ -610,6 +610,14 @@ int dispatch(LpKind kind, const Gemm& g, hipStream_t s) {
     case LpKind::VHS:  // complex double x real float -> complex float
       return dispatch_shape<BF, true, false, false, false, double, float, float2>(g, s);
     case LpKind::TAYLOR_D:  // complex float x complex double -> complex float (+ D)
+      // bf16x3: the register-panel kernel at 33..48 columns (0.081 vs 0.149 ms per
+      // first term at N=200, 40 columns, 512 walkers, packed A) and at 65..128
+      // columns (0.224 vs 0.329 ms at N=300, 118 columns, 256 walkers)
+      if constexpr (BF)
+        if (g.batch > 1 && g.M > 64 && !g.ws && !g.a_trans && !g.b_trans) {
+          if (g.N > 32 && g.N <= 48) return launch_panel_bf3<16, 3, double>(g, s);
+          if (g.N > 64 && g.N <= 128) return launch_panel_bf3<8, 8, double>(g, s);
+        }
       return dispatch_shape<BF, true, true, false, false, float, double, float2>(g, s);
     case LpKind::TAYLOR_F:  // complex float x complex float -> complex float (+ D)
       // bf16x3 at 33..48 columns: the register-panel kernel with 16 waves (0.120 vs
