@@ -578,6 +578,149 @@ int launch_panel_bf3(const Gemm& g, hipStream_t s) {
   return launch_status("panel_bf3_kernel");
 }
 
+// FP32 walker-panel product with a real A (the one-body propagator's E - I,
+// N x N, one per batch entry or shared with sA = 0) and a complex double panel
+// B (N x c, c <= 16 TN): complex double D = E + alpha A.B (D += alpha A.B
+// without E), complex float C = alpha A.B when given.  Each of the NW waves
+// owns 16 rows of A and all TN column blocks.  In a 32-deep slab, step s of
+// the 16x16x4 f32 MFMA takes k = 8q + s from lane (r, q), so a lane reads the 8
+// consecutive k of its row straight from global memory; the B slab is staged
+// once per work-group as two FP32 planes (re, im) with k contiguous, and a
+// lane's 8 values per plane are two 16-byte LDS reads.  As in
+// panel_bf3_kernel, loads go to clamped addresses and are masked when used.
+constexpr int FROW = 36;  // floats per LDS row of a B plane: 32 k + 4 pad (144 bytes)
+
+template <int NW, int TN>
+__global__ __launch_bounds__(64 * NW) void panel_f32_kernel(Gemm g, int tiles_m, int tiles_n) {
+  constexpr int BN = 16 * TN, PLANE = BN * FROW;
+  constexpr int NE = 32 * BN, RE = (NE + 64 * NW - 1) / (64 * NW);  // B elements per slab
+  __shared__ __align__(16) float Bs[2][2 * PLANE];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const long T = (long)tiles_m * tiles_n;
+  const long L = blockIdx.x;
+  const long p = L >> 3;
+  const long b = (L & 7) + 8 * (p / T);
+  if (b >= g.batch) return;  // uniform per work-group
+  const long tile = p % T;
+  const int mt = (int)(tile % tiles_m), nt = (int)(tile / tiles_m);
+  const int m0 = mt * 16 * NW, n0 = nt * BN;
+  const int r = lane & 15, q = lane >> 4;
+  const int row = m0 + wave * 16 + r;
+  const bool rok = row < g.M;
+  const bool active = m0 + wave * 16 < g.M;
+  const float* A = static_cast<const float*>(g.A) + b * g.sA;
+  const double* B = static_cast<const double*>(g.B) + b * g.sB * 2;
+  const long arow = (long)min(row, g.M - 1) * g.lda;
+
+  auto load_a = [&](float (&x)[8], int k0) {
+#pragma unroll
+    for (int t = 0; t < 8; ++t) x[t] = A[arow + min(k0 + 8 * q + t, g.K - 1)];
+  };
+  auto mask_a = [&](float (&x)[8], int k0) {
+#pragma unroll
+    for (int t = 0; t < 8; ++t) x[t] = (rok && k0 + 8 * q + t < g.K) ? x[t] : 0.f;
+  };
+  // B slab rows k0 .. k0+31: element e = (k, n), n fastest
+  auto load_b = [&](double (&x)[RE][2], int k0) {
+#pragma unroll
+    for (int i = 0; i < RE; ++i) {
+      const int e = min(tid + i * 64 * NW, NE - 1);
+      const int gk = min(k0 + e / BN, g.K - 1), gn = min(n0 + e % BN, g.N - 1);
+      const long off = ((long)gk * g.ldb + gn) * 2;
+      x[i][0] = B[off];
+      x[i][1] = B[off + 1];
+    }
+  };
+  auto stash_b = [&](float* bs, const double (&x)[RE][2], int k0) {
+#pragma unroll
+    for (int i = 0; i < RE; ++i) {
+      const int e = tid + i * 64 * NW;
+      if (e >= NE) continue;
+      const int kk = e / BN, nn = e % BN;
+      const bool ok = k0 + kk < g.K && n0 + nn < g.N;
+      bs[nn * FROW + kk] = ok ? (float)x[i][0] : 0.f;
+      bs[PLANE + nn * FROW + kk] = ok ? (float)x[i][1] : 0.f;
+    }
+  };
+
+  v4f accR[TN], accI[TN];
+#pragma unroll
+  for (int j = 0; j < TN; ++j) {
+    accR[j] = v4f{0.f, 0.f, 0.f, 0.f};
+    accI[j] = v4f{0.f, 0.f, 0.f, 0.f};
+  }
+  float ac[8], an[8];
+  double rb[RE][2];
+  load_a(ac, 0);
+  mask_a(ac, 0);
+  load_b(rb, 0);
+  stash_b(Bs[0], rb, 0);
+  __syncthreads();
+  int buf = 0;
+  for (int k0 = 0; k0 < g.K; k0 += 32) {
+    const bool more = k0 + 32 < g.K;
+    if (more) {
+      load_a(an, k0 + 32);
+      load_b(rb, k0 + 32);
+    }
+    // every wave multiplies (a wave past M has all-zero A and stores nothing)
+    const float* bs = Bs[buf];
+#pragma unroll
+    for (int j = 0; j < TN; ++j) {
+      const float* pr = bs + (j * 16 + r) * FROW + 8 * q;
+      const float4 r0 = *reinterpret_cast<const float4*>(pr), r1 = *reinterpret_cast<const float4*>(pr + 4);
+      const float4 i0 = *reinterpret_cast<const float4*>(pr + PLANE), i1 = *reinterpret_cast<const float4*>(pr + PLANE + 4);
+      const float br[8] = {r0.x, r0.y, r0.z, r0.w, r1.x, r1.y, r1.z, r1.w};
+      const float bi[8] = {i0.x, i0.y, i0.z, i0.w, i1.x, i1.y, i1.z, i1.w};
+#pragma unroll
+      for (int t = 0; t < 8; ++t) {
+        accR[j] = __builtin_amdgcn_mfma_f32_16x16x4f32(ac[t], br[t], accR[j], 0, 0, 0);
+        accI[j] = __builtin_amdgcn_mfma_f32_16x16x4f32(ac[t], bi[t], accI[j], 0, 0, 0);
+      }
+    }
+    if (more) {
+      stash_b(Bs[buf ^ 1], rb, k0 + 32);
+#pragma unroll
+      for (int t = 0; t < 8; ++t) ac[t] = an[t];
+      mask_a(ac, k0 + 32);
+    }
+    __syncthreads();
+    buf ^= 1;
+  }
+  if (!active) return;
+
+  // accumulator element v of lane l is (4(l/16) + v, l%16)
+  float2* Cp = g.C ? static_cast<float2*>(g.C) + b * g.sC : nullptr;
+  double2* Dp = g.D ? static_cast<double2*>(g.D) + b * g.sD : nullptr;
+  const double2* Ep = g.E ? static_cast<const double2*>(g.E) + b * g.sE : nullptr;
+#pragma unroll
+  for (int j = 0; j < TN; ++j)
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const int m = m0 + wave * 16 + 4 * q + v;
+      const int n = n0 + j * 16 + r;
+      if (m >= g.M || n >= g.N) continue;
+      const double2 x = cmul(g.alpha, double2{(double)accR[j][v], (double)accI[j][v]});
+      if (Cp) Cp[(long)m * g.ldc + n] = float2{(float)x.x, (float)x.y};
+      if (Dp) {
+        double2* Dc = Dp + (long)m * g.ldd + n;
+        double2 d = Ep ? Ep[(long)m * g.lde + n] : *Dc;
+        d.x += x.x;
+        d.y += x.y;
+        *Dc = d;
+      }
+    }
+}
+
+template <int NW, int TN>
+int launch_panel_f32(const Gemm& g, hipStream_t s) {
+  const int tm = (g.M + 16 * NW - 1) / (16 * NW), tn = (g.N + 16 * TN - 1) / (16 * TN);
+  const long gx = (long)tm * tn * ((g.batch + 7) / 8 * 8);
+  if (gx > 0x7fffffffL) return fail(AFQ_E_UNSUPPORTED, "gemm_lp: grid too large");
+  hipLaunchKernelGGL((panel_f32_kernel<NW, TN>), dim3((unsigned)gx), dim3(64 * NW), 0, s, g, tm, tn);
+  return launch_status("panel_f32_kernel");
+}
+
 template <bool BF, bool AC, bool BC, bool AT, bool BT, typename TA, typename TB, typename TCo>
 int dispatch_shape(const Gemm& g, hipStream_t s) {
   if constexpr (!BC && !BT)  // wide real-B tiles (the VHS product), as in the FP64 GEMM
@@ -633,6 +776,9 @@ int dispatch(LpKind kind, const Gemm& g, hipStream_t s) {
     case LpKind::FORCE_BIAS:  // complex double x real float (transposed) -> complex double
       return dispatch_shape<BF, true, false, false, true, double, float, double2>(g, s);
     case LpKind::ONE_BODY:  // real float x complex double -> D = E + result (complex double)
+      // 49..64 columns (one spin of unrestricted walkers): the FP32 register-panel kernel
+      if (g.batch > 1 && g.M > 64 && g.N > 48 && g.N <= 64 && !g.ws && !g.a_trans && !g.b_trans && !g.a_sym)
+        return launch_panel_f32<8, 4>(g, s);
       return dispatch_shape<false, false, true, false, false, float, double, double2>(g, s);
   }
   return fail(AFQ_E_INVALID, "gemm_lp: unknown kind");
