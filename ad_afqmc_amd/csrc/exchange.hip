@@ -53,62 +53,57 @@ struct Frag {
   R bi[NB][KE];
 };
 
-template <int MODE, int NB, int KS, typename TR>
-__device__ inline void load_frag(Frag<MODE, NB, KS>& F, const TR* __restrict__ A, const double2* __restrict__ B, int O,
-                                 int N, int c, int lane) {
-  using R = typename Frag<MODE, NB, KS>::R;
-  constexpr int KE = Frag<MODE, NB, KS>::KE;
-  const int kb = c + KE * (lane >> 4);
+// One chunk as loaded: raw values at clamped (always valid) addresses, rows
+// >= O and columns >= N included.  take() converts it to the MFMA type and
+// zeroes A (rot_chol) there; that alone makes every padded product vanish:
+// f_ij = 0 for i >= O, so f_ij f_ji and f_ii vanish whenever i or j >= O, and
+// the clamped B values stay finite.  No loaded value is consumed before the
+// MFMAs of the current chunk are issued, so the next chunk's loads overlap them
+// (masking or converting at load time made the compiler wait for every load
+// first).
+template <int NB, int KS, typename TR>
+struct Raw {
+  TR a[NB][KS];
+  double2 b[NB][KS];
+};
+
+template <int NB, int KS, typename TR>
+__device__ inline void load_raw(Raw<NB, KS, TR>& X, const TR* __restrict__ A, const double2* __restrict__ B, int O,
+                                int N, int c, int lane) {
+  const int kb = c + KS * (lane >> 4);
   const int r16 = lane & 15;
-  const bool full = (kb + KE - 1 < N);
-  // Rows >= O and columns >= N are clamped to valid addresses.  F64 keeps the
-  // raw values (mask_frag zeroes A there before use, so that no value is
-  // consumed until the MFMAs of the current chunk have been issued); F32
-  // masks and converts here.
 #pragma unroll
   for (int bi = 0; bi < NB; ++bi) {
-    const int row = bi * 16 + r16;
-    const bool rok = row < O;
-    const long roff = (long)(rok ? row : O - 1) * N;
+    const long roff = (long)min(bi * 16 + r16, O - 1) * N;
 #pragma unroll
-    for (int s = 0; s < KE; ++s) {
-      const int col = full ? kb + s : min(kb + s, N - 1);
-      const double2 v = B[roff + col];
-      const TR a = A[roff + col];
-      if constexpr (MODE == F64) {
-        F.a[bi][s] = a;
-        F.br[bi][s] = v.x;
-        F.bi[bi][s] = v.y;
-      } else {
-        const bool ok = rok && (full || kb + s < N);
-        F.a[bi][s] = ok ? (R)a : (R)0;
-        F.br[bi][s] = ok ? (R)v.x : (R)0;
-        F.bi[bi][s] = ok ? (R)v.y : (R)0;
-      }
+    for (int s = 0; s < KS; ++s) {
+      const int col = min(kb + s, N - 1);
+      X.a[bi][s] = A[roff + col];
+      X.b[bi][s] = B[roff + col];
     }
   }
 }
 
-// F64: zero the A (rot_chol) entries of rows >= O and columns >= N.  That
-// alone makes every padded product vanish: f_ij = 0 for i >= O, so f_ij f_ji
-// and f_ii vanish whenever i or j >= O; the clamped B values stay finite.
-template <int MODE, int NB, int KS>
-__device__ inline void mask_frag(Frag<MODE, NB, KS>& F, int O, int N, int c, int lane) {
-  if constexpr (MODE == F64) {
-    constexpr int KE = Frag<MODE, NB, KS>::KE;
-    const int kb = c + KE * (lane >> 4);
-#pragma unroll
-    for (int bi = 0; bi < NB; ++bi) {
-      const bool rok = bi * 16 + (lane & 15) < O;
-#pragma unroll
-      for (int s = 0; s < KE; ++s) F.a[bi][s] = (rok && kb + s < N) ? F.a[bi][s] : 0.0;
-    }
-  }
-}
-
-// NB <= 3 fits two work-groups per CU (<= 256 VGPRs per lane); NB = 4 needs the whole file.
 template <int MODE, int NB, int KS, typename TR>
-__global__ __launch_bounds__(256, (NB <= 3 ? 2 : 1)) void exchange_kernel(const TR* __restrict__ rc, const double2* __restrict__ Gm, int O,
+__device__ inline void take(Frag<MODE, NB, KS>& F, const Raw<NB, KS, TR>& X, int O, int N, int c, int lane) {
+  using R = typename Frag<MODE, NB, KS>::R;
+  const int kb = c + KS * (lane >> 4);
+#pragma unroll
+  for (int bi = 0; bi < NB; ++bi) {
+    const bool rok = bi * 16 + (lane & 15) < O;
+#pragma unroll
+    for (int s = 0; s < KS; ++s) {
+      F.a[bi][s] = (rok && kb + s < N) ? (R)X.a[bi][s] : (R)0;
+      F.br[bi][s] = (R)X.b[bi][s].x;
+      F.bi[bi][s] = (R)X.b[bi][s].y;
+    }
+  }
+}
+
+// NB <= 3, and F32 at NB = 4, fit two work-groups per CU (<= 256 VGPRs per
+// lane); F64 at NB = 4 needs the whole file (exchange_split_kernel serves it).
+template <int MODE, int NB, int KS, typename TR>
+__global__ __launch_bounds__(256, (NB <= 3 || MODE == F32 ? 2 : 1)) void exchange_kernel(const TR* __restrict__ rc, const double2* __restrict__ Gm, int O,
                                                        int N, int Gn, int W, int nchunk, int cpg, long T8,
                                                        double2* __restrict__ coul, double2* __restrict__ kpart) {
   using Acc = std::conditional_t<MODE == F64, v4d, v4f>;
@@ -143,11 +138,12 @@ __global__ __launch_bounds__(256, (NB <= 3 ? 2 : 1)) void exchange_kernel(const 
 
   using Fr = Frag<MODE, NB, KS>;
   constexpr int CW = 4 * Fr::KE;
-  Fr cur, nxt;
-  load_frag<MODE, NB, KS, TR>(cur, A, B, O, N, 0, lane);
-  mask_frag<MODE, NB, KS>(cur, O, N, 0, lane);
+  Fr cur;
+  Raw<NB, KS, TR> nxt;
+  load_raw<NB, KS, TR>(nxt, A, B, O, N, 0, lane);
+  take<MODE, NB, KS, TR>(cur, nxt, O, N, 0, lane);
   for (int c = 0; c < N; c += CW) {
-    if (c + CW < N) load_frag<MODE, NB, KS, TR>(nxt, A, B, O, N, c + CW, lane);
+    if (c + CW < N) load_raw<NB, KS, TR>(nxt, A, B, O, N, c + CW, lane);
 #pragma unroll
     for (int s = 0; s < KS; ++s)
 #pragma unroll
@@ -162,10 +158,7 @@ __global__ __launch_bounds__(256, (NB <= 3 ? 2 : 1)) void exchange_kernel(const 
             accI[bi][bj] = __builtin_amdgcn_mfma_f32_16x16x4f32(cur.a[bi][s], cur.bi[bj][s], accI[bi][bj], 0, 0, 0);
           }
         }
-    if (c + CW < N) {
-      cur = nxt;
-      mask_frag<MODE, NB, KS>(cur, O, N, c + CW, lane);
-    }
+    if (c + CW < N) take<MODE, NB, KS, TR>(cur, nxt, O, N, c + CW, lane);
   }
 
   // element v of lane l sits at (row, col) = (l/16 + 4v, l%16) for f64 and
@@ -655,13 +648,14 @@ int launch_split(const double* rc, const double2* Gm, int O, int N, int Gn, int 
 template <int MODE, typename TR>
 int launch_mode(const TR* rc, const double2* Gm, int O, int N, int Gn, int W, int nchunk, double2* coul,
                 double2* kpart, hipStream_t s) {
-  // chunk widths chosen so the FP64 fragments fit next to the accumulators:
-  // NB = 3 uses 8-wide chunks (two work-groups per CU), NB = 4 (49 <= O <= 64) 4-wide ones
+  // chunk widths chosen so the fragments fit next to the accumulators: NB = 3
+  // uses 8-wide chunks (two work-groups per CU), NB = 4 (49 <= O <= 64) 4-wide
+  // ones in FP64 and 8-wide ones in F32
   switch ((O + 15) / 16) {
     case 1: return launch_nb<MODE, 1, 4, TR>(rc, Gm, O, N, Gn, W, nchunk, coul, kpart, s);
     case 2: return launch_nb<MODE, 2, 4, TR>(rc, Gm, O, N, Gn, W, nchunk, coul, kpart, s);
     case 3: return launch_nb<MODE, 3, 2, TR>(rc, Gm, O, N, Gn, W, nchunk, coul, kpart, s);
-    default: return launch_nb<MODE, 4, (MODE == F64 ? 1 : 4), TR>(rc, Gm, O, N, Gn, W, nchunk, coul, kpart, s);
+    default: return launch_nb<MODE, 4, (MODE == F64 ? 1 : 2), TR>(rc, Gm, O, N, Gn, W, nchunk, coul, kpart, s);
   }
 }
 
