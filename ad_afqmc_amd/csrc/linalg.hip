@@ -109,6 +109,7 @@ __global__ __launch_bounds__(256) void gj_det_inv_kernel(DetInvSet s0, DetInvSet
     if (j < n && j >= jlo) {
       const double2 rk = prow[j];
       const int i0 = want_inv ? rg : k + 1 + rg;
+#pragma unroll 4
       for (int i = i0; i < n; i += 4) {
         if (i == k) continue;
         const double2 f = fcol[i];
