@@ -314,10 +314,6 @@ int apply_exp_h1(afq_job* j, const double2* src, long lds, long ss, int s, doubl
 int apply_exp_h1_lp(afq_job* j, const double2* src, long lds, long ss, int s, double2* dst, long ldd, long sd,
                     hipStream_t st) {
   const int N = j->N;
-  const int cols = j->restricted ? j->wcols : j->nocc[s];
-  // 33..48 columns: the FP64 register-panel product is the faster one (2.2 vs
-  // 3.1 ms per block at N=200, O=40, 512 walkers), and exact
-  if (cols > 32 && cols <= 48 && N > 64) return apply_exp_h1(j, src, lds, ss, s, dst, ldd, sd, st);
   Gemm g;
   g.M = N;
   g.N = (int)(j->restricted ? j->wcols : j->nocc[s]);

@@ -887,9 +887,10 @@ int dispatch(LpKind kind, const Gemm& g, hipStream_t s) {
     case LpKind::FORCE_BIAS:  // complex double x real float (transposed) -> complex double
       return dispatch_shape<BF, true, false, false, true, double, float, double2>(g, s);
     case LpKind::ONE_BODY:  // real float x complex double -> D = E + result (complex double)
-      // 49..64 columns (one spin of unrestricted walkers): the FP32 register-panel kernel
-      if (g.batch > 1 && g.M > 64 && g.N > 48 && g.N <= 64 && !g.ws && !g.a_trans && !g.b_trans && !g.a_sym)
-        return launch_panel_f32<8, 4>(g, s);
+      // 33..64 columns (restricted walkers at O = 40, one spin of unrestricted
+      // walkers at O = 60): the FP32 register-panel kernel
+      if (g.batch > 1 && g.M > 64 && g.N > 32 && g.N <= 64 && !g.ws && !g.a_trans && !g.b_trans && !g.a_sym)
+        return g.N <= 48 ? launch_panel_f32<8, 3>(g, s) : launch_panel_f32<8, 4>(g, s);
       return dispatch_shape<false, false, true, false, false, float, double, double2>(g, s);
   }
   return fail(AFQ_E_INVALID, "gemm_lp: unknown kind");
