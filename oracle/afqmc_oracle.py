@@ -262,28 +262,51 @@ def apply_trotprop(walkers: Walkers, shifted_fields, chol, exp_h1, dt, n_exp_ter
     return np.array([apply_exp_vhs(w, v, exp_h1, n_exp_terms) for w, v in zip(walkers, vhs)])
 
 
+def step_terms(fields: np.ndarray, fb: np.ndarray, mf_shifts: np.ndarray, dt: float):
+    """Field shifts, shifted fields, shift_term and fb_term of one phaseless
+    step, propagation.py:296-301.  Returns (field_shifts, shifted, shift_term,
+    fb_term)."""
+    sqdt = math.sqrt(dt)
+    field_shifts = -sqdt * (1.0j * fb - mf_shifts)  # :296
+    shifted = fields - field_shifts  # :297
+    shift_term = np.sum(shifted * mf_shifts, axis=1)  # :298
+    fb_term = np.sum(fields * field_shifts - field_shifts * field_shifts / 2.0, axis=1)  # :299-301
+    return field_shifts, shifted, shift_term, fb_term
+
+
+def importance_function(shift_term, fb_term, ov_new, ov_old, e_shift, h0_prop, dt: float):
+    """Unclipped complex importance function and the phase angle of the
+    phaseless constraint, propagation.py:306-319.  Returns (imp, theta)."""
+    sqdt = math.sqrt(dt)
+    ratio = ov_new / ov_old
+    imp = np.exp(-sqdt * shift_term + fb_term + dt * (e_shift + h0_prop)) * ratio  # :306-314
+    theta = np.angle(np.exp(-sqdt * shift_term) * ratio)  # :315-319
+    return imp, theta
+
+
+def phaseless_factor(imp, theta):
+    """|I| cos(theta) with the reference's clipping, propagation.py:320-327."""
+    ph = np.abs(imp) * np.cos(theta)  # :320
+    ph = np.where(np.isnan(ph), 0.0, ph)
+    ph = np.where(ph < 1.0e-3, 0.0, ph)
+    return np.where(ph > 100.0, 0.0, ph)  # :321-327
+
+
 def propagate_constrained(trial, state: dict, fields: np.ndarray, ham: dict, dt: float, n_exp_terms=6) -> dict:
     """propagator_afqmc.propagate_constrained, propagation.py:274-336.
 
     state: walkers, weights (W,), overlaps (W,) complex, e_estimate, pop_control_ene_shift.
     ham: chol (G,N*N), rot_chol, mf_shifts, h0_prop, exp_h1.
-    Returns a new state dict (also carrying force_bias and the phaseless factor)."""
+    Returns a new state dict (also carrying the force bias, the shifted fields,
+    the unclipped complex importance function and the phaseless factor)."""
     W = fields.shape[0]
-    sqdt = math.sqrt(dt)
     fb = trial.calc_force_bias(state["walkers"], ham["rot_chol"])  # :295
-    field_shifts = -sqdt * (1.0j * fb - ham["mf_shifts"])  # :296
-    shifted = fields - field_shifts  # :297
-    shift_term = np.sum(shifted * ham["mf_shifts"], axis=1)  # :298
-    fb_term = np.sum(fields * field_shifts - field_shifts * field_shifts / 2.0, axis=1)  # :299-301
-    walkers = apply_trotprop(state["walkers"], shifted, ham["chol"], ham["exp_h1"], dt, n_exp_terms)
+    _, shifted, shift_term, fb_term = step_terms(fields, fb, ham["mf_shifts"], dt)  # :296-301
+    walkers = apply_trotprop(state["walkers"], shifted, ham["chol"], ham["exp_h1"], dt, n_exp_terms)  # :302-304
     ov_new = trial.calc_overlap(walkers)  # :305
-    ratio = ov_new / state["overlaps"]
-    imp = np.exp(-sqdt * shift_term + fb_term + dt * (state["pop_control_ene_shift"] + ham["h0_prop"])) * ratio
-    theta = np.angle(np.exp(-sqdt * shift_term) * ratio)  # :315-319
-    ph = np.abs(imp) * np.cos(theta)  # :320
-    ph = np.where(np.isnan(ph), 0.0, ph)
-    ph = np.where(ph < 1.0e-3, 0.0, ph)
-    ph = np.where(ph > 100.0, 0.0, ph)  # :321-327
+    imp, theta = importance_function(shift_term, fb_term, ov_new, state["overlaps"], state["pop_control_ene_shift"],
+                                     ham["h0_prop"], dt)
+    ph = phaseless_factor(imp, theta)
     weights = ph * state["weights"]
     weights = np.where(weights > 100.0, 0.0, weights)  # :328-331
     e_shift = state["e_estimate"] - 0.1 * np.log(np.sum(weights) / W) / dt  # :332-334
@@ -294,6 +317,7 @@ def propagate_constrained(trial, state: dict, fields: np.ndarray, ham: dict, dt:
         overlaps=ov_new,
         pop_control_ene_shift=e_shift,
         force_bias=fb,
+        importance=imp,
         phaseless_factor=ph,
         shifted_fields=shifted,
     )
@@ -432,17 +456,29 @@ def reject_outliers(data, obs, m=10.0, min_threshold=1e-5):
 # ---------------------------------------------------------------------------
 PINS = {
     "green/det_overlap/RHFTrial/UHFTrial": "reference golden scalars tests/test_wavefunctions.py:128-184 "
-    "(replayed inputs, tests/golden/golden_wavefunctions.npz); brute-force second quantisation on N<=4; "
-    "W->W.R invariance",
-    "rhf/uhf_measurement_intermediates": "W=T closed-form RHF energy; shapes tests/test_hamiltonian.py:112-135",
-    "propagation_intermediates": "fb(W=T) = -i mf_shifts invariant; expm vs eigen-decomposition of symmetric h1_mod; "
-    "v0 vs explicit loop",
-    "apply_exp_vhs/apply_trotprop": "Taylor vs scipy expm(sqrt(dt) VHS) within the 6th-order remainder bound",
-    "propagate_constrained": "dt->0 leaves weights at 1; x=0, W=T gives ratio exp(dt(E_shift+h0_prop)); "
-    "phaseless clipping known answers",
-    "qr_restricted/qr_unrestricted": "Q^H Q = I, span preserved, overlap phase preserved (UHF sign fix)",
-    "sr_indices": "equal weights -> identity; brute-force comb on small W",
-    "blocking_analysis/reject_outliers": "i.i.d. samples: error ~ sigma/sqrt(n); planted outlier removed",
-    "propagate_phaseless/block_scan": "H6 fixture mixed energy within error bar of -3.2382 "
-    "(tests/test_driver.py:74-86), statistical only (threefry stream not reproducible: parity unpinned bit-wise)",
+    "(replayed inputs, tests/golden/golden_wavefunctions.npz; tests/test_oracle.py::test_golden_*); brute-force "
+    "second quantisation on 3 orbitals (test_energy_bruteforce_*, test_force_bias_bruteforce); W->W.R invariance",
+    "rhf/uhf_measurement_intermediates": "W=T closed-form RHF energy (test_energy_at_trial_closed_form)",
+    "propagation_intermediates": "mf_shifts: fb(W=T) = -i mf_shifts (test_fb_equals_minus_i_mf_at_trial); "
+    "h0_prop, v0, v1: the Fock-space operator identity H = -h0_prop + sum h1_mod E + 1/2 sum (V_g - m_g)^2 "
+    "(tests/test_oracle_importance.py::test_mean_field_decomposition_is_the_hamiltonian); exp_h1 vs the "
+    "eigendecomposition of the symmetric h1_mod (test_propagation_intermediates_closed_forms)",
+    "apply_exp_vhs/apply_trotprop": "error equals the first omitted Taylor term VHS^6 W/6! "
+    "(test_taylor_within_remainder_bound)",
+    "step_terms/importance_function/propagate_constrained": "Gaussian mean of the unclipped importance function "
+    "equals 1 - dt (E_L - E_shift) + O(dt^2) (Gauss-Hermite quadrature, Richardson in dt; "
+    "test_girsanov_mean_of_importance_function); W=T, x=0 closed form exp(dt(E_shift+h0_prop)) "
+    "det(T^T exp(-dt h1_mod) T) (test_trial_walker_zero_field_closed_form); a sign error planted in fb_term, "
+    "shift_term, h0_prop or E_shift fails the Girsanov pin (test_mutations_fail_the_pins); dt->0 leaves weights "
+    "at 1 (test_propagate_dt_to_zero_keeps_weights); clipping known answers (GPU test)",
+    "phaseless_factor": "clipping known answers (tests/test_gpu_parity.py::test_phaseless_clipping_known_answers) "
+    "and |I| at W=T (test_trial_walker_zero_field_closed_form)",
+    "qr_restricted/qr_unrestricted": "Q^H Q = I, span preserved, overlap phase preserved (UHF sign fix) "
+    "(test_qr_invariants)",
+    "sr_indices": "equal weights -> identity; brute-force comb on small W (test_sr_known_answers)",
+    "blocking_analysis/reject_outliers": "i.i.d. samples: error ~ sigma/sqrt(n); planted outlier removed "
+    "(test_blocking_and_outliers)",
+    "propagate_phaseless/block_scan": "composition of the pinned pieces above in the order of sampling.py:52-94, "
+    "146-168; H6 fixture mixed energy within the error bar of -3.2382 (tests/test_driver.py:74-86; statistical "
+    "only: the threefry field stream is not reproducible, so bit-level driver energies are parity unpinned)",
 }
