@@ -84,11 +84,14 @@ _SIGNATURES = {
     "afq_stochastic_reconfiguration": ([P, P, P, P, P, P, P], C.c_int),
     "afq_block": ([P, P, P, P, P, P, C.c_int, C.c_uint64, C.POINTER(C.c_uint64), P, P], C.c_int),
     "afq_normal_fill": ([P, C.c_int64, C.c_uint64, C.c_uint64, P], C.c_int),
+    "afq_sr_zeta": ([P, C.c_uint64, C.c_uint64, P], C.c_int),
     "afq_job_profile": ([P, C.c_int, C.c_int], C.c_int),
     "afq_job_profile_read": ([P, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_int64)], C.c_int),
     "afq_walkers_orthonormalize": ([P, C.c_int32, C.c_int32, C.c_int32, P], C.c_int),
     "afq_walkers_sr_workspace": ([C.c_int32, C.c_int64], C.c_int64),
     "afq_walkers_sr": ([P, C.c_int64, P, C.c_int64, C.c_int32, P, P, P, P, P], C.c_int),
+    "afq_walkers_sr_noalloc": ([P, C.c_int64, P, C.c_int64, C.c_int32, P, P, P, P], C.c_int),
+    "afq_walkers_sr_noalloc_max_walkers": ([], C.c_int64),
     "afq_sr_block": ([P, P, P, P, P, P, C.c_int, C.c_uint64, P, P, P], C.c_int),
     "afq_job_reserve_fields": ([P, C.c_int32, P], C.c_int),
     "afq_segment_energy_vjp": ([P, P, P, C.c_double, P, C.c_int32, C.c_int32, P, P, P, P], C.c_int),

@@ -255,13 +255,20 @@ int energy_pipeline(afq_job* j, const void* up, const void* dn, double2* out, hi
   }
   int nch[2] = {1, 1};
   for (int s = 0; s < j->nspin_meas; ++s) {
-    ProfScope ps(j, AFQ_K_EXCHANGE, st);
     double2* coul = j->coul + (long)s * j->W * j->G;
     double2* kpart = j->kpart + (long)s * j->W * j->kchunk_max;
     if (j->bf16x3) {
-      AFQ_TRY(k_split_bf16(reinterpret_cast<const double*>(j->green[s]), true, j->nocc[s], j->N, j->W, j->gsplit, st));
+      {
+        ProfScope pm(j, AFQ_K_ENERGY_MISC, st);
+        AFQ_TRY(k_split_bf16(reinterpret_cast<const double*>(j->green[s]), true, j->nocc[s], j->N, j->W, j->gsplit,
+                             st));
+      }
+      ProfScope ps(j, AFQ_K_EXCHANGE, st);
       AFQ_TRY(k_exchange_bf16(j->rcs[s], j->gsplit, j->nocc[s], j->N, j->G, j->W, coul, kpart, &nch[s], st));
-    } else if (j->lp)
+      continue;
+    }
+    ProfScope ps(j, AFQ_K_EXCHANGE, st);
+    if (j->lp)
       AFQ_TRY(k_exchange_lp(j->rot_chol32[s], j->green[s], j->nocc[s], j->N, j->G, j->W, coul, kpart, &nch[s], st));
     else
       AFQ_TRY(k_exchange(j->rot_chol[s], j->green[s], j->nocc[s], j->N, j->G, j->W, coul, kpart, &nch[s], st));
@@ -1201,6 +1208,11 @@ int afq_stochastic_reconfiguration(afq_job_t j, void* up, void* dn, double* weig
 int afq_normal_fill(double* out, int64_t n, uint64_t seed, uint64_t counter, void* stream) {
   if (!out || n < 0) return fail(AFQ_E_INVALID, "bad argument");
   return k_normal_fill(out, n, seed, counter, S(stream));
+}
+
+int afq_sr_zeta(double* out, uint64_t seed, uint64_t counter, void* stream) {
+  if (!out) return fail(AFQ_E_INVALID, "out is NULL");
+  return k_sr_zeta_host(out, seed, counter, S(stream));
 }
 
 }  // extern "C"

@@ -58,5 +58,6 @@ int k_normal_fill(double* out, long n, uint64_t seed, uint64_t counter, hipStrea
 int k_normal_fill_dev(double* out, long n, uint64_t seed, const uint64_t* counter_dev, hipStream_t s);
 int k_sr_zeta(double* zeta, uint64_t seed, const uint64_t* counter_dev, hipStream_t s);
 int k_counter_incr(uint64_t* counter_dev, hipStream_t s);
+int k_sr_zeta_host(double* zeta, uint64_t seed, uint64_t counter, hipStream_t s);
 
 }  // namespace afq

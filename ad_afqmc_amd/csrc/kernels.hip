@@ -364,9 +364,10 @@ __global__ void normal_fill_kernel(double* out, long n, uint64_t seed, uint64_t 
 
 // SR offset zeta ~ U[0, 1) of block `*counter_dev`: Philox on a key distinct from
 // the field stream's, then the counter advances to the next block.
-__global__ void sr_zeta_kernel(double* zeta, uint64_t seed, const uint64_t* counter_dev) {
+// counter_dev (when not null) supplies the counter from device memory.
+__global__ void sr_zeta_kernel(double* zeta, uint64_t seed, uint64_t counter, const uint64_t* counter_dev) {
   if (threadIdx.x != 0 || blockIdx.x != 0) return;
-  const uint64_t c = *counter_dev;
+  const uint64_t c = counter_dev ? *counter_dev : counter;
   const uint64_t key = seed ^ 0x9E3779B97F4A7C15ull;
   const uint4 r = philox4x32_10(uint4{0xFFFFFFFFu, 0xFFFFFFFFu, (uint32_t)c, (uint32_t)(c >> 32)},
                                 uint2{(uint32_t)key, (uint32_t)(key >> 32)});
@@ -546,7 +547,11 @@ int k_normal_fill_dev(double* out, long n, uint64_t seed, const uint64_t* counte
   return launch_status("normal_fill_kernel");
 }
 int k_sr_zeta(double* zeta, uint64_t seed, const uint64_t* counter_dev, hipStream_t s) {
-  hipLaunchKernelGGL(sr_zeta_kernel, dim3(1), dim3(64), 0, s, zeta, seed, counter_dev);
+  hipLaunchKernelGGL(sr_zeta_kernel, dim3(1), dim3(64), 0, s, zeta, seed, (uint64_t)0, counter_dev);
+  return launch_status("sr_zeta_kernel");
+}
+int k_sr_zeta_host(double* zeta, uint64_t seed, uint64_t counter, hipStream_t s) {
+  hipLaunchKernelGGL(sr_zeta_kernel, dim3(1), dim3(64), 0, s, zeta, seed, counter, (const uint64_t*)nullptr);
   return launch_status("sr_zeta_kernel");
 }
 int k_counter_incr(uint64_t* counter_dev, hipStream_t s) {

@@ -29,6 +29,30 @@ def ptr(t: Optional[torch.Tensor]) -> C.c_void_p:
     return C.c_void_p(0 if t is None else t.data_ptr())
 
 
+def check_tensor(name: str, t, shape, dtype) -> None:
+    """Raises ValueError unless t is a contiguous device tensor of exactly this
+    shape and dtype: the kernels index raw pointers by the job's sizes, so a
+    mismatched buffer must be refused before it reaches them."""
+    if not isinstance(t, torch.Tensor):
+        raise ValueError(f"{name} must be a torch.Tensor, got {type(t).__name__}")
+    if t.dtype != dtype:
+        raise ValueError(f"{name} has dtype {t.dtype}, expected {dtype}")
+    if tuple(t.shape) != tuple(int(x) for x in shape):
+        raise ValueError(f"{name} has shape {tuple(t.shape)}, expected {tuple(int(x) for x in shape)}")
+    if not t.is_contiguous():
+        raise ValueError(f"{name} must be contiguous")
+    if not t.is_cuda:
+        raise ValueError(f"{name} must be a device tensor (it is on {t.device})")
+
+
+def walker_shapes(norb: int, nelec: Tuple[int, int], walker_type: str, n: int):
+    """Buffer shapes of n walkers: restricted (n, N, nelec_up) and no second
+    buffer; unrestricted (n, N, nelec_up) and (n, N, nelec_dn)."""
+    if walker_type == "restricted":
+        return (n, norb, nelec[0]), None
+    return (n, norb, nelec[0]), (n, norb, nelec[1])
+
+
 def as_f64(x) -> torch.Tensor:
     if isinstance(x, torch.Tensor):
         return x.to(device=device(), dtype=torch.float64).contiguous()
@@ -116,9 +140,16 @@ class EngineJob:
         nat.call("afq_job_set_hamiltonian", self._h, float(h0), ptr(h1), ptr(chol), stream_ptr())
 
     def set_trial(self, mo_up, mo_dn, rdm1):
+        N = self.norb
         up = as_f64(mo_up)
+        check_tensor("mo_up", up, (N, self.nelec[0]), torch.float64)
         dn = None if mo_dn is None else as_f64(mo_dn)
+        if self.trial_type == "uhf":
+            if dn is None:
+                raise ValueError("a UHF trial needs mo_dn")
+            check_tensor("mo_dn", dn, (N, self.nelec[1]), torch.float64)
         rdm = as_f64(rdm1)
+        check_tensor("rdm1", rdm, (2, N, N), torch.float64)
         nat.call("afq_job_set_trial", self._h, ptr(up), ptr(dn), ptr(rdm), stream_ptr())
 
     def build_measurement(self):
@@ -153,11 +184,30 @@ class EngineJob:
 
     # ------------------------------------------------------------ measurements
     def _w(self, up: torch.Tensor, dn: Optional[torch.Tensor]):
-        n = up.shape[0]
-        if n > self.capacity:
-            raise ValueError(f"{n} walkers exceed the job capacity {self.capacity}")
+        if not isinstance(up, torch.Tensor) or up.dim() != 3:
+            raise ValueError("walkers must be a (n_walkers, norb, nocc) tensor")
+        n = int(up.shape[0])
+        if n < 1 or n > self.capacity:
+            raise ValueError(f"{n} walkers outside [1, job capacity {self.capacity}]")
+        su, sd = walker_shapes(self.norb, self.nelec, self.walker_type, n)
+        check_tensor("walkers_up" if sd is not None else "walkers", up, su, torch.complex128)
+        if sd is None:
+            if dn is not None:
+                raise ValueError("restricted walkers take one buffer (walkers_dn must be None)")
+        else:
+            if dn is None:
+                raise ValueError("unrestricted walkers need walkers_dn")
+            check_tensor("walkers_dn", dn, sd, torch.complex128)
         self.set_batch(n)
         return ptr(up), ptr(dn)
+
+    def _state(self, n, weights=None, overlaps=None, scalars=None):
+        if weights is not None:
+            check_tensor("weights", weights, (n,), torch.float64)
+        if overlaps is not None:
+            check_tensor("overlaps", overlaps, (n,), torch.complex128)
+        if scalars is not None:
+            check_tensor("scalars", scalars, (2,), torch.float64)
 
     def overlap(self, up, dn=None) -> torch.Tensor:
         pu, pd = self._w(up, dn)
@@ -188,10 +238,16 @@ class EngineJob:
     # ------------------------------------------------------------ propagation (in place)
     def trotprop(self, up, dn, shifted_fields: torch.Tensor):
         pu, pd = self._w(up, dn)
+        check_tensor("shifted_fields", shifted_fields, (up.shape[0], self.nchol), torch.complex128)
         nat.call("afq_apply_trotprop", self._h, pu, pd, ptr(shifted_fields), stream_ptr())
 
     def propagate(self, up, dn, fields, weights, overlaps, scalars, fb_out=None):
         pu, pd = self._w(up, dn)
+        n = up.shape[0]
+        check_tensor("fields", fields, (n, self.nchol), torch.float64)
+        self._state(n, weights, overlaps, scalars)
+        if fb_out is not None:
+            check_tensor("force_bias_out", fb_out, (n, self.nchol), torch.complex128)
         nat.call(
             "afq_propagate_constrained",
             self._h,
@@ -211,6 +267,10 @@ class EngineJob:
 
     def stochastic_reconfiguration(self, up, dn, weights, zeta: torch.Tensor, idx_out=None):
         pu, pd = self._w(up, dn)
+        self._state(up.shape[0], weights)
+        check_tensor("zeta", zeta, (1,), torch.float64)
+        if idx_out is not None:
+            check_tensor("indices_out", idx_out, (up.shape[0],), torch.int32)
         nat.call("afq_stochastic_reconfiguration", self._h, pu, pd, ptr(weights), ptr(zeta), ptr(idx_out), stream_ptr())
 
     # ------------------------------------------------------------ live kernel timing
@@ -226,6 +286,9 @@ class EngineJob:
     def block(self, up, dn, weights, overlaps, scalars, n_prop_steps: int, seed: int, counter: int):
         """One sampler block on the device; returns ({block_energy, block_weight} tensor, new counter)."""
         pu, pd = self._w(up, dn)
+        self._state(up.shape[0], weights, overlaps, scalars)
+        if int(n_prop_steps) < 1:
+            raise ValueError("n_prop_steps must be >= 1")
         out = torch.empty(2, dtype=torch.float64, device=up.device)
         ctr = C.c_uint64(counter)
         nat.call(
@@ -253,9 +316,10 @@ class EngineJob:
         """One block + comb resampling with the stream counter in device memory
         (counter_dev: one int64 element, advanced on the device); capturable in a
         HIP graph.  out (2,) receives {block_energy, block_weight}."""
-        if counter_dev.dtype != torch.int64 or counter_dev.numel() != 1 or not counter_dev.is_cuda:
-            raise ValueError("counter_dev must be a one-element int64 device tensor")
+        check_tensor("counter_dev", counter_dev, (1,), torch.int64)
+        check_tensor("block_out", out, (2,), torch.float64)
         pu, pd = self._w(up, dn)
+        self._state(up.shape[0], weights, overlaps, scalars)
         nat.call(
             "afq_sr_block",
             self._h,
@@ -311,13 +375,31 @@ def normal_fill(n: int, seed: int, counter: int) -> torch.Tensor:
     return out
 
 
+def _walker_buffer(name: str, w) -> Tuple[int, int, int]:
+    if not isinstance(w, torch.Tensor) or w.dim() != 3:
+        raise ValueError(f"{name} must be a (n_walkers, norb, ncols) tensor")
+    check_tensor(name, w, tuple(w.shape), torch.complex128)
+    return tuple(int(x) for x in w.shape)
+
+
+def sr_zeta(seed: int, counter: int) -> torch.Tensor:
+    """The comb offset afq_sr_block draws for the block keyed by (seed, counter)."""
+    out = torch.empty(1, dtype=torch.float64, device=device())
+    nat.call("afq_sr_zeta", ptr(out), C.c_uint64(seed), C.c_uint64(counter), stream_ptr())
+    return out
+
+
 def walkers_orthonormalize(w: torch.Tensor):
-    W, N, c = w.shape
+    W, N, c = _walker_buffer("walkers", w)
     nat.call("afq_walkers_orthonormalize", ptr(w), int(W), int(N), int(c), stream_ptr())
 
 
 def walkers_sr(up: torch.Tensor, dn: Optional[torch.Tensor], weights: torch.Tensor, zeta: torch.Tensor):
-    W = up.shape[0]
+    W = _walker_buffer("walkers_up", up)[0]
+    if dn is not None and _walker_buffer("walkers_dn", dn)[0] != W:
+        raise ValueError("walkers_up and walkers_dn hold different walker counts")
+    check_tensor("weights", weights, (W,), torch.float64)
+    check_tensor("zeta", zeta, (1,), torch.float64)
     eu = up[0].numel()
     ed = 0 if dn is None else dn[0].numel()
     ws_bytes = nat.lib().afq_walkers_sr_workspace(int(W), int(max(eu, ed)))

@@ -60,8 +60,10 @@ class JobResult:
 
 
 def footprint_bytes(spec: JobSpec, n_prop_steps: int) -> int:
-    """Device bytes of a job: libafq.so's own buffers plus the caller-owned
-    walkers, weights and overlaps."""
+    """Device bytes a running job holds: libafq.so's own buffers (Hamiltonian,
+    trial, intermediates, scratch and the field buffer) plus the caller-owned
+    walkers, weights, overlaps and scalars.  The executor keeps nothing else
+    per job once it is set up."""
     from . import _native as nat
 
     d = nat.JobDesc(
@@ -166,16 +168,27 @@ class GpuExecutor:
         self.torch = torch
         self.n_prop_steps = int(n_prop_steps)
         self.use_graphs = bool(use_graphs)
+        self._setup = None
 
     def start(self, spec: JobSpec):
+        """Builds the job on a setup stream shared by all jobs (so the
+        transient generator and intermediate tensors return to one pool of
+        torch's caching allocator and are reused by the next job's setup) and
+        keeps only what the job's blocks use: the engine job (with its own
+        resident copies of the Hamiltonian, trial and intermediates), the
+        walkers and the per-job scalars.  Its blocks run on a stream of its
+        own that waits for the setup."""
         torch = self.torch
         from . import hamiltonian as hm
         from . import propagation as pr
         from . import wavefunctions as wf
         from .binding import engine_for
 
+        if self._setup is None:
+            self._setup = torch.cuda.Stream()
+        setup = self._setup
         stream = torch.cuda.Stream()
-        with torch.cuda.stream(stream):
+        with torch.cuda.stream(setup):
             h = _device_system(spec)
             trial = wf.rhf(spec.norb, (spec.nocc, spec.nocc))
             prop = pr.propagator_afqmc(dt=spec.dt, n_walkers=spec.n_walkers, walker_type="restricted",
@@ -185,7 +198,6 @@ class GpuExecutor:
             H = hm.hamiltonian(spec.norb)
             ham = H.build_measurement_intermediates(ham, trial, wave)
             ham = H.build_propagation_intermediates(ham, prop, trial, wave)
-            ham.pop("chol")  # the job keeps its own resident copy
             pd = prop.init_prop_data(trial, wave, ham, seed=spec.seed)
             job = engine_for(trial, wave, ham, "restricted", spec.n_walkers, prop)
             job.reserve_fields(self.n_prop_steps)
@@ -194,14 +206,21 @@ class GpuExecutor:
             overlaps = job.overlap(up, None)
             e0 = pd["e_estimate"].reshape(())
             scalars = torch.stack([e0, e0.clone()]).contiguous()
-            nblocks = max(1, -(-spec.n_steps // self.n_prop_steps))
             counter = torch.zeros(1, dtype=torch.int64, device="cuda")
             out = torch.empty(2, dtype=torch.float64, device="cuda")
-            ev = torch.cuda.Event()
-            ev.record(stream)
-        return dict(spec=spec, stream=stream, job=job, wave=wave, ham=ham, up=up, weights=weights,
-                    overlaps=overlaps, scalars=scalars, counter=counter, out=out, graph=None, issued=0,
-                    nblocks=nblocks, outs=[], event=ev)
+            ready = torch.cuda.Event()
+            ready.record(setup)
+        # the binding in `wave` refers to every uploaded array (chol included):
+        # dropping wave, ham, pd and h here returns them to the setup pool
+        del h, wave, ham, pd
+        stream.wait_event(ready)
+        for t in (up, weights, overlaps, scalars, counter, out):
+            t.record_stream(stream)
+        nblocks = max(1, -(-spec.n_steps // self.n_prop_steps))
+        ev = torch.cuda.Event()
+        ev.record(stream)
+        return dict(spec=spec, stream=stream, job=job, up=up, weights=weights, overlaps=overlaps, scalars=scalars,
+                    counter=counter, out=out, graph=None, issued=0, nblocks=nblocks, outs=[], event=ev)
 
     def ready(self, h) -> bool:
         return h["issued"] < h["nblocks"] and h["event"].query()

@@ -1,11 +1,21 @@
 #!/usr/bin/env python
 """Walker-step throughput of the phaseless-AFQMC engine on one MI355X.
 
-One bench step = one device sampler block on the synthetic RHF workload
-(N=200, O=40, G=1200, 512 walkers, dt=0.005): 10 phaseless steps with
+One bench step = one device sampler block (afq_sr_block) on the synthetic UHF
+workload where the mixed-precision target is stated (N=300, O=60/58, G=2000,
+256 walkers, dt=0.005), in split-BF16 x3 by default: 10 phaseless steps with
 on-device fields, QR re-orthonormalisation, overlap and local-energy
-measurement, population-control update, then comb resampling.
+measurement, the population-control update, then comb resampling and the
+overlaps of the new population.
 value = walkers * 10 * steps / wall time of the timed region (walker-steps/s).
+The same workload is timed in FP64 over the same number of blocks
+(speedup_vs_fp64) and in FP32; the RHF N=200 workload, the H10 chain and the
+64-job serving stream are reported beside it.
+
+The generator is the stated one with the Cholesky-vector scale s_g multiplied
+by 0.3/sqrt(N), so that the walker population lives and block energies are
+defined (at the stated scale every walker is clipped within a few steps; the
+dense work per block is the same, which `stated_generator` shows).
 
 Prints one JSON line.  --gpus N > 1 is refused: this engine runs on one GPU.
 """
@@ -117,27 +127,49 @@ def launch_flops(cls, N, Oa, Ob, G, W, unrestricted=False, vhs_cols=None):
     return None
 
 
-def _oracle_block(cfg, W, seed):
-    """Set up the FP64 oracle on the workload and return a closure that runs one
-    block (n_prop_steps steps + QR + energy) of W walkers, and the step count."""
+def _oracle_setup(name, W, seed, l_coef):
+    """FP64 oracle on workload `name` with W walkers near the trial: returns
+    (step(), measure(), n_prop_steps).  step() runs one phaseless step of all
+    W walkers; measure() the block's QR, overlaps, energies and bookkeeping."""
     import numpy as np
 
     from ad_afqmc_amd import synthetic as syn
     from oracle import afqmc_oracle as O
 
-    s = syn.make_system(cfg["norb"], cfg["nocc_up"], cfg["nchol"], seed=0)
-    dt, nsteps = cfg["dt"], cfg["n_prop_steps"]
-    h1s, rh, rc = O.rhf_measurement_intermediates(s.h1, s.chol, s.mo_up)
-    pi = O.propagation_intermediates(s.h0, h1s, s.chol, s.rdm1(), dt)
-    trial = O.RHFTrial(s.norb, s.nelec, s.mo_up)
+    cfg = syn.CONFIGS[name]
+    N, G = cfg["norb"], cfg["nchol"]
+    Oa = cfg["nocc_up"]
+    Ob = cfg.get("nocc_dn", Oa)
+    uhf = cfg["uhf"]
+    s = syn.make_system(N, Oa, G, nocc_dn=Ob, uhf=uhf, seed=0, l_scale=l_coef / math.sqrt(N))
+    dt = cfg["dt"]
+    wtype = "unrestricted" if uhf else "restricted"
+    if uhf:
+        h1s, rh, rc = O.uhf_measurement_intermediates(s.h1, s.chol, s.mo_up, s.mo_dn)
+        trial = O.UHFTrial(N, (Oa, Ob), s.mo_up, s.mo_dn)
+    else:
+        h1s, rh, rc = O.rhf_measurement_intermediates(s.h1, s.chol, s.mo_up)
+        trial = O.RHFTrial(N, (Oa, Oa), s.mo_up)
+    pi = O.propagation_intermediates(s.h0, h1s, s.chol, s.rdm1(), dt, wtype)
     ham = dict(h0=s.h0, chol=s.chol, rot_h1=rh, rot_chol=rc, mf_shifts=pi["mf_shifts"], h0_prop=pi["h0_prop"],
                exp_h1=pi["exp_h1"])
-    w0 = O.init_walkers(s.rdm1(), s.nelec, W, "restricted")
+    w0 = O.init_walkers(s.rdm1(), s.nelec, W, wtype)
     e0 = float(np.mean(trial.calc_energy(w0, s.h0, rh, rc).real))
-    state = dict(walkers=w0, weights=np.ones(W), overlaps=trial.calc_overlap(w0), e_estimate=e0,
-                 pop_control_ene_shift=e0)
-    fields = np.random.default_rng(seed).standard_normal((nsteps, W, s.nchol))
-    return (lambda: O.block_scan(trial, state, fields, ham, dt)), nsteps
+    st = {"s": dict(walkers=w0, weights=np.ones(W), overlaps=trial.calc_overlap(w0), e_estimate=e0,
+                    pop_control_ene_shift=e0)}
+    rng = np.random.default_rng(seed)
+
+    def step():
+        st["s"] = O.propagate_constrained(trial, st["s"], rng.standard_normal((W, G)), ham, dt)
+
+    def measure():
+        x = st["s"]
+        x["walkers"] = O.orthonormalize(x["walkers"])
+        x["overlaps"] = trial.calc_overlap(x["walkers"])
+        e = np.real(trial.calc_energy(x["walkers"], ham["h0"], ham["rot_h1"], ham["rot_chol"]))
+        return float(np.sum(e * x["weights"]) / np.sum(x["weights"]))
+
+    return step, measure, cfg["n_prop_steps"]
 
 
 def _one_thread():
@@ -152,48 +184,67 @@ def _one_thread():
         return None
 
 
-def cpu_baseline(cfg, sample_walkers=8, seed=0):
-    """The FP64 oracle timed on one host core on a bounded sample: a few walkers
-    of the same workload through one block (10 steps + QR + energy)."""
+def _time_sample(name, W, sample_steps, seed, l_coef):
+    """Seconds for `sample_steps` oracle steps and for one measurement."""
+    step, measure, m = _oracle_setup(name, W, seed, l_coef)
+    t0 = time.perf_counter()
+    for _ in range(sample_steps):
+        step()
+    t_steps = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    e = measure()
+    t_meas = time.perf_counter() - t0
+    return t_steps, t_meas, m, e
+
+
+def cpu_baseline(name, sample_walkers, sample_steps, l_coef=0.3, seed=0):
+    """The FP64 oracle on one host core, on a bounded sample of the workload:
+    `sample_walkers` walkers through `sample_steps` phaseless steps and one
+    measurement (QR + overlaps + energies), scaled to walker-steps/s of a block
+    of m steps with one measurement: W m / (m t_step + t_meas)."""
     limits = _one_thread()
     try:
-        run, nsteps = _oracle_block(cfg, sample_walkers, seed)
-        t0 = time.perf_counter()
-        run()
-        t = time.perf_counter() - t0
+        t_steps, t_meas, m, e = _time_sample(name, sample_walkers, sample_steps, seed, l_coef)
     finally:
         if limits is not None:
             limits.restore_original_limits()
+    t_step = t_steps / sample_steps
     W = sample_walkers
     return {
-        "value": W * nsteps / t,
+        "value": W * m / (m * t_step + t_meas),
         "unit": "walker-steps/s",
         "cores": 1,
         "kind": "port",
-        "sample": f"{W} walkers x 1 block ({nsteps} steps + QR + energy) of the same workload, FP64 NumPy oracle, "
-        f"1 BLAS thread, {t:.1f} s",
+        "sample": f"{W} walkers x {sample_steps} steps ({t_steps:.1f} s) + one QR/overlap/energy measurement "
+        f"({t_meas:.1f} s) of {name} (Cholesky scale {l_coef}/sqrt(N), live population, mean energy {e:.4f}), "
+        f"scaled to blocks of {m} steps + 1 measurement; FP64 NumPy oracle, 1 BLAS thread",
     }
 
 
-def _cpu_worker(cfg, W, seed, barrier, queue):
+def _cpu_worker(name, W, sample_steps, seed, l_coef, barrier, queue):
     _limits = _one_thread()
-    run, nsteps = _oracle_block(cfg, W, seed)
+    step, measure, m = _oracle_setup(name, W, seed, l_coef)
     barrier.wait(timeout=900)
     t0 = time.perf_counter()
-    run()
-    queue.put((W * nsteps, t0, time.perf_counter()))
+    for _ in range(sample_steps):
+        step()
+    t1 = time.perf_counter()
+    measure()
+    queue.put((W, sample_steps, m, t0, t1, time.perf_counter()))
 
 
-def cpu_baseline_parallel(cfg, sample_walkers=8, procs=16):
+def cpu_baseline_parallel(name, sample_walkers, sample_steps, procs=16, l_coef=0.3):
     """The same oracle sample in `procs` walker-sharded processes of one BLAS
     thread each (the reference's MPI walker parallelism with seed + rank):
-    aggregate walker-steps over the span from the first start to the last end.
-    Worker processes are spawned fresh, before the caller touches the GPU."""
+    aggregate walker-steps/s over the span from the first start to the last end,
+    scaled like cpu_baseline.  Workers are spawned fresh, before the caller
+    touches the GPU."""
     import multiprocessing as mp
 
     ctx = mp.get_context("spawn")
     barrier, queue = ctx.Barrier(procs), ctx.Queue()
-    ps = [ctx.Process(target=_cpu_worker, args=(cfg, sample_walkers, 1 + r, barrier, queue)) for r in range(procs)]
+    ps = [ctx.Process(target=_cpu_worker, args=(name, sample_walkers, sample_steps, 1 + r, l_coef, barrier, queue))
+          for r in range(procs)]
     for p_ in ps:
         p_.start()
     res = []
@@ -210,15 +261,21 @@ def cpu_baseline_parallel(cfg, sample_walkers=8, procs=16):
                 p_.terminate()
     if len(res) < procs:
         return None
+    m = res[0][2]
+    start = min(r[3] for r in res)
+    span_steps = max(r[4] for r in res) - start
+    span_all = max(r[5] for r in res) - start
+    t_meas = span_all - span_steps
+    t_step = span_steps / sample_steps
     work = sum(r[0] for r in res)
-    span = max(r[2] for r in res) - min(r[1] for r in res)
     return {
-        "value": work / span,
+        "value": work * m / (m * t_step + t_meas),
         "unit": "walker-steps/s",
         "cores": procs,
         "kind": "port",
-        "sample": f"{procs} processes x {sample_walkers} walkers x 1 block of the same workload, FP64 NumPy oracle, "
-        f"1 BLAS thread each, {span:.1f} s span",
+        "sample": f"{procs} processes x {sample_walkers} walkers x {sample_steps} steps + one measurement of {name} "
+        f"(live population), {span_all:.1f} s span, scaled to blocks of {m} steps; FP64 NumPy oracle, "
+        f"1 BLAS thread each",
     }
 
 
@@ -235,67 +292,88 @@ def host_description():
     return {"cpu_count": os.cpu_count(), "cpu_model": model}
 
 
-def precision_study(cfg_name: str, n_blocks: int, n_eql: int, l_coef: float = 0.3):
-    """Energy of each precision mode against FP64 on identical Philox fields and
-    SR offsets: the trajectories stay correlated, so the paired block-energy
-    differences measure the systematic error with a small error bar."""
-    import numpy as np
+def _new_job(system, name, precision, nsteps):
+    """Façade objects and an engine job of workload `name` in `precision`, with
+    the device field buffer reserved for blocks of nsteps steps."""
     import torch
 
     from ad_afqmc_amd import hamiltonian as hm
     from ad_afqmc_amd import propagation as pr
-    from ad_afqmc_amd import stat_utils
-    from ad_afqmc_amd import synthetic as syn
     from ad_afqmc_amd import wavefunctions as wf
     from ad_afqmc_amd.binding import engine_for
 
-    cfg = syn.CONFIGS[cfg_name]
-    N, G, W = cfg["norb"], cfg["nchol"], cfg["n_walkers"]
+    cfg = syn_config(name)
+    N, W = cfg["norb"], cfg["n_walkers"]
+    uhf = cfg["uhf"]
     Oa = cfg["nocc_up"]
     Ob = cfg.get("nocc_dn", Oa)
-    uhf = cfg["uhf"]
-    # Cholesky scale l_coef/sqrt(N): a correlation energy small enough for a stable
-    # walker population at dt = 0.005 (the stated generator, l_coef = sqrt(N), kills it)
-    lsc = l_coef / math.sqrt(N)
-    s = syn.make_system(N, Oa, G, nocc_dn=Ob, uhf=uhf, seed=0, l_scale=lsc)
-    nsteps, dt = cfg["n_prop_steps"], cfg["dt"]
+    wave, ham = system.wave_data(), system.ham_data()
     trial = wf.uhf(N, (Oa, Ob)) if uhf else wf.rhf(N, (Oa, Oa))
     wtype = "unrestricted" if uhf else "restricted"
-    zetas = torch.rand(n_eql + n_blocks, dtype=torch.float64, device="cuda",
-                       generator=torch.Generator(device="cuda").manual_seed(7))
+    prop = pr.propagator_afqmc(dt=cfg["dt"], n_walkers=W, walker_type=wtype, precision=precision)
+    H = hm.hamiltonian(N)
+    ham = H.build_measurement_intermediates(ham, trial, wave)
+    ham = H.build_propagation_intermediates(ham, prop, trial, wave)
+    pd = prop.init_prop_data(trial, wave, ham, seed=0)
+    job = engine_for(trial, wave, ham, wtype, W, prop)
+    job.reserve_fields(nsteps)
+    up, dn = (t.clone() if t is not None else None for t in pd["walkers"].tensors())
+    state = dict(up=up, dn=dn, weights=pd["weights"].clone(), overlaps=job.overlap(up, dn),
+                 scalars=torch.stack([pd["e_estimate"], pd["e_estimate"].clone()]).contiguous(),
+                 counter=torch.zeros(1, dtype=torch.int64, device=up.device),
+                 out=torch.empty(2, dtype=torch.float64, device=up.device))
+    return job, state, (wave, ham, pd, trial, prop)
+
+
+def _sr_block(job, st, nsteps, seed):
+    """One device block + comb resampling + overlaps of the new population
+    (afq_sr_block); returns a copy of {block_energy, block_weight}."""
+    job.sr_block(st["up"], st["dn"], st["weights"], st["overlaps"], st["scalars"], nsteps, seed, st["counter"],
+                 st["out"])
+    return st["out"].clone()
+
+
+def _clone_state(st):
+    return {k: (v.clone() if v is not None else None) for k, v in st.items()}
+
+
+def precision_study(cfg_name: str, n_blocks: int, n_eql: int, l_coef: float = 0.3):
+    """Energy of each precision mode against FP64 on identical Philox fields and
+    comb offsets (afq_sr_block draws both from the block counter): the
+    trajectories stay correlated, so the paired block-energy differences
+    measure the systematic error with a small error bar."""
+    import numpy as np
+    import torch
+
+    from ad_afqmc_amd import stat_utils
+
+    cfg = syn_config(cfg_name)
+    W, nsteps = cfg["n_walkers"], cfg["n_prop_steps"]
+    s = make_system(cfg_name, l_coef)
     out = {}
     for prec in ("fp64", "fp32", "bf16x3"):
-        wave, ham = s.wave_data(), s.ham_data()
-        prop = pr.propagator_afqmc(dt=dt, n_walkers=W, walker_type=wtype, precision=prec)
-        H = hm.hamiltonian(N)
-        ham = H.build_measurement_intermediates(ham, trial, wave)
-        ham = H.build_propagation_intermediates(ham, prop, trial, wave)
-        pd = prop.init_prop_data(trial, wave, ham, seed=0)
-        job = engine_for(trial, wave, ham, wtype, W, prop)
-        up, dn = (t.clone() if t is not None else None for t in pd["walkers"].tensors())
-        weights = pd["weights"].clone()
-        overlaps = job.overlap(up, dn)
-        scalars = torch.stack([pd["e_estimate"], pd["e_estimate"].clone()]).contiguous()
-        counter, blocks = 0, []
+        job, st, keep = _new_job(s, cfg_name, prec, nsteps)
+        blocks = []
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         for i in range(n_eql + n_blocks):
-            o, counter = job.block(up, dn, weights, overlaps, scalars, nsteps, 99, counter)
-            job.stochastic_reconfiguration(up, dn, weights, zetas[i : i + 1])
+            o = _sr_block(job, st, nsteps, 99)
             blocks.append(o)
             # e_estimate follows the block energy as in the reference driver (driver.py:168)
-            scalars[0] = 0.9 * scalars[0] + 0.1 * o[0]
+            st["scalars"][0] = 0.9 * st["scalars"][0] + 0.1 * o[0]
         torch.cuda.synchronize()
         t = time.perf_counter() - t0
         b = torch.stack(blocks).cpu().numpy()[n_eql:]
         mean, err = stat_utils.blocking_analysis(b[:, 1], b[:, 0])
         out[prec] = dict(blocks=b, energy=float(mean), error=float(err) if err is not None else None,
                          walker_steps_per_s=W * nsteps * (n_eql + n_blocks) / t, finite=bool(np.all(np.isfinite(b))))
-        del job, wave, ham, pd, up, dn, weights, overlaps, scalars
+        job.close()
+        del job, st, keep
         torch.cuda.empty_cache()
     ref = out["fp64"]["blocks"]
     summary = {"workload": cfg_name, "l_scale": f"{l_coef}/sqrt(N)",
-               "blocks": n_blocks, "equilibration_blocks": n_eql, "steps_per_block": nsteps}
+               "blocks": n_blocks, "equilibration_blocks": n_eql, "steps_per_block": nsteps,
+               "block": "afq_sr_block (steps, QR, energy, comb resampling, overlaps of the new population)"}
     for prec, r in out.items():
         d = r["blocks"][:, 0] - ref[:, 0]
         summary[prec] = {
@@ -307,61 +385,40 @@ def precision_study(cfg_name: str, n_blocks: int, n_eql: int, l_coef: float = 0.
             "walker_steps_per_s": r["walker_steps_per_s"],
             "finite": r["finite"],
         }
-    summary.update(_restart_paired(s, cfg, trial, wtype, n_blocks, n_eql, zetas))
+    summary.update(_restart_paired(s, cfg_name, n_blocks, n_eql))
     return summary
 
 
-def _restart_paired(s, cfg, trial, wtype, n_blocks, n_eql, zetas):
+def _restart_paired(s, cfg_name, n_blocks, n_eql):
     """Per-block error without trajectory divergence: along the FP64 trajectory,
     each block is re-run from the same FP64 state (walkers, weights, overlaps,
-    shift, field counter) in each reduced-precision mode, and the local energies
+    shift, block counter) in each reduced-precision mode, and the local energies
     of the same FP64 walkers are measured in each mode.  The means of these
     differences are the systematic errors per block; their spread is small
     because both sides start from identical states."""
     import numpy as np
-    import torch
 
-    from ad_afqmc_amd import hamiltonian as hm
-    from ad_afqmc_amd import propagation as pr
-    from ad_afqmc_amd.binding import engine_for
-
-    N, W = cfg["norb"], cfg["n_walkers"]
-    nsteps, dt = cfg["n_prop_steps"], cfg["dt"]
-    jobs, waves = {}, {}
+    cfg = syn_config(cfg_name)
+    nsteps = cfg["n_prop_steps"]
+    jobs, states, keep = {}, {}, {}
     for prec in ("fp64", "fp32", "bf16x3"):
-        wave, ham = s.wave_data(), s.ham_data()
-        prop = pr.propagator_afqmc(dt=dt, n_walkers=W, walker_type=wtype, precision=prec)
-        H = hm.hamiltonian(N)
-        ham = H.build_measurement_intermediates(ham, trial, wave)
-        ham = H.build_propagation_intermediates(ham, prop, trial, wave)
-        jobs[prec] = engine_for(trial, wave, ham, wtype, W, prop)
-        waves[prec] = (wave, ham, prop)
-    wave, ham, prop = waves["fp64"]
-    pd = prop.init_prop_data(trial, wave, ham, seed=0)
-    up, dn = (t.clone() if t is not None else None for t in pd["walkers"].tensors())
-    weights = pd["weights"].clone()
-    J = jobs["fp64"]
-    overlaps = J.overlap(up, dn)
-    scalars = torch.stack([pd["e_estimate"], pd["e_estimate"].clone()]).contiguous()
-    counter = 0
+        jobs[prec], states[prec], keep[prec] = _new_job(s, cfg_name, prec, nsteps)
+    J, st = jobs["fp64"], states["fp64"]
     blk = {p: [] for p in jobs}
     ene = {p: [] for p in jobs}
-    cl = lambda t: None if t is None else t.clone()
     for i in range(n_eql + n_blocks):
-        state = (cl(up), cl(dn), weights.clone(), overlaps.clone(), scalars.clone(), counter)
+        snap = _clone_state(st)
         if i >= n_eql:
             for p, job in jobs.items():
-                e = job.energy(up, dn).real
-                ene[p].append(float((weights * e).sum() / weights.sum()))
-        o, counter = J.block(up, dn, weights, overlaps, scalars, nsteps, 99, counter)
+                e = job.energy(st["up"], st["dn"]).real
+                ene[p].append(float((st["weights"] * e).sum() / st["weights"].sum()))
+        o = _sr_block(J, st, nsteps, 99)
         if i >= n_eql:
             blk["fp64"].append(float(o[0]))
             for p in ("fp32", "bf16x3"):
-                u2, d2, w2, ov2, sc2, c0 = (cl(x) if isinstance(x, torch.Tensor) else x for x in state)
-                o2, _ = jobs[p].block(u2, d2, w2, ov2, sc2, nsteps, 99, c0)
+                o2 = _sr_block(jobs[p], _clone_state(snap), nsteps, 99)
                 blk[p].append(float(o2[0]))
-        J.stochastic_reconfiguration(up, dn, weights, zetas[i : i + 1])
-        scalars[0] = 0.9 * scalars[0] + 0.1 * o[0]
+        st["scalars"][0] = 0.9 * st["scalars"][0] + 0.1 * o[0]
     out = {}
     for p in ("fp32", "bf16x3"):
         res = {}
@@ -381,13 +438,16 @@ def syn_config(name):
     return syn.CONFIGS[name]
 
 
-def make_system(name):
-    """The stated synthetic generator for workload `name` (seed 0)."""
+def make_system(name, l_coef=0.3):
+    """The synthetic generator for workload `name` (seed 0) with the Cholesky
+    scale s_g multiplied by l_coef/sqrt(N); l_coef=None gives the stated scale."""
     from ad_afqmc_amd import synthetic as syn
 
     cfg = syn.CONFIGS[name]
     Oa = cfg["nocc_up"]
-    return syn.make_system(cfg["norb"], Oa, cfg["nchol"], nocc_dn=cfg.get("nocc_dn", Oa), uhf=cfg["uhf"], seed=0)
+    ls = 1.0 if l_coef is None else l_coef / math.sqrt(cfg["norb"])
+    return syn.make_system(cfg["norb"], Oa, cfg["nchol"], nocc_dn=cfg.get("nocc_dn", Oa), uhf=cfg["uhf"], seed=0,
+                           l_scale=ls)
 
 
 def _precision_errors(name):
@@ -410,52 +470,29 @@ def _precision_errors(name):
 
 
 def time_config(system, name, precision, steps, warmup, dominant_roofline=False, breakdown=False, nsteps=None):
-    """Times `steps` device blocks (10 phaseless steps + QR + energy + comb
-    resampling each) of workload `name` in `precision`, after `warmup` blocks.
-    With dominant_roofline the warm-up times every kernel class and the timed
-    region brackets the dominant class with HIP events on the launch stream."""
+    """Times `steps` device blocks (afq_sr_block: n steps + QR + energy + comb
+    resampling + overlaps) of workload `name` in `precision`, after `warmup`
+    blocks.  With dominant_roofline the warm-up times every kernel class and
+    the timed region brackets the dominant class with HIP events on the launch
+    stream."""
     import numpy as np
     import torch
 
     from ad_afqmc_amd import engine as E
-    from ad_afqmc_amd import hamiltonian as hm
-    from ad_afqmc_amd import propagation as pr
-    from ad_afqmc_amd import wavefunctions as wf
-    from ad_afqmc_amd.binding import engine_for
 
     cfg = syn_config(name)
     N, G, W = cfg["norb"], cfg["nchol"], cfg["n_walkers"]
     uhf = cfg["uhf"]
     Oa = cfg["nocc_up"]
     Ob = cfg.get("nocc_dn", Oa)
-    nsteps, dt = nsteps or cfg["n_prop_steps"], cfg["dt"]
-    wave, ham = system.wave_data(), system.ham_data()
-    trial = wf.uhf(N, (Oa, Ob)) if uhf else wf.rhf(N, (Oa, Oa))
-    wtype = "unrestricted" if uhf else "restricted"
-    prop = pr.propagator_afqmc(dt=dt, n_walkers=W, walker_type=wtype, precision=precision)
-    H = hm.hamiltonian(N)
-    ham = H.build_measurement_intermediates(ham, trial, wave)
-    ham = H.build_propagation_intermediates(ham, prop, trial, wave)
-    pd = prop.init_prop_data(trial, wave, ham, seed=0)
-    job = engine_for(trial, wave, ham, wtype, W, prop)
-    up, dn = (t.clone() if t is not None else None for t in pd["walkers"].tensors())
-    weights = pd["weights"].clone()
-    overlaps = job.overlap(up, dn)
-    scalars = torch.stack([pd["e_estimate"], pd["e_estimate"].clone()]).contiguous()
-    zetas = torch.rand(warmup + steps + 8, dtype=torch.float64, device=up.device)
-    counter = 0
-
-    def one_block(i):
-        nonlocal counter
-        out, counter = job.block(up, dn, weights, overlaps, scalars, nsteps, 1234, counter)
-        job.stochastic_reconfiguration(up, dn, weights, zetas[i : i + 1])
-        return out
+    nsteps = nsteps or cfg["n_prop_steps"]
+    job, st, keep = _new_job(system, name, precision, nsteps)
 
     classes = E.nat.KERNEL_CLASSES if dominant_roofline or breakdown else []
     for c in classes:
         job.profile(c, True)
     for i in range(warmup):
-        one_block(i)
+        _sr_block(job, st, nsteps, 1234)
     torch.cuda.synchronize()
     bd = {c: job.profile_read(c) for c in classes}
     for c in classes:
@@ -464,14 +501,14 @@ def time_config(system, name, precision, steps, warmup, dominant_roofline=False,
     if breakdown:
         tot = sum(v[0] for v in bd.values())
         for c, (ms, n) in sorted(bd.items(), key=lambda kv: -kv[1][0]):
-            print(f"[bench] {c:14s} {ms / max(warmup, 1):9.3f} ms/block  {n // max(warmup, 1):5d} launches/block"
-                  f"  {100 * ms / max(tot, 1e-9):5.1f}%", file=sys.stderr)
+            print(f"[bench] {name} {precision} {c:14s} {ms / max(warmup, 1):9.3f} ms/block  "
+                  f"{n // max(warmup, 1):5d} launches/block  {100 * ms / max(tot, 1e-9):5.1f}%", file=sys.stderr)
     dominant = max(bd, key=lambda c: bd[c][0]) if dominant_roofline else None
     if dominant:
         job.profile(dominant, True)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    outs = [one_block(warmup + i) for i in range(steps)]
+    outs = [_sr_block(job, st, nsteps, 1234) for _ in range(steps)]
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
     if dominant:
@@ -487,7 +524,7 @@ def time_config(system, name, precision, steps, warmup, dominant_roofline=False,
             ach = per_launch / avg_s
             roof = {"kernel": dominant, "bound": "mfma", "achieved": ach / 1e12, "peak": peak / 1e12,
                     "unit": "TFLOP/s", "frac": ach / peak, "traffic": None, "avg_launch_ms": avg_s * 1e3,
-                    "flops_per_launch": per_launch}
+                    "launches": dom_n, "flops_per_launch": per_launch}
             # HBM bytes per launch and the MFMA pipe's busy fraction, both from
             # rocprofv3 counter runs of this kernel on this workload
             for fname, field in (("traffic.json", "traffic"), ("mfma_busy.json", "mfma_busy")):
@@ -501,10 +538,109 @@ def time_config(system, name, precision, steps, warmup, dominant_roofline=False,
                 except Exception:
                     pass
             res["roofline"] = roof
-    res.update(value=W * nsteps * steps / elapsed, elapsed=elapsed, blocks=torch.stack(outs).cpu().numpy())
+    blocks = torch.stack(outs).cpu().numpy()
+    res.update(value=W * nsteps * steps / elapsed, elapsed=elapsed, blocks=blocks,
+               ms_per_block=elapsed / steps * 1e3)
     job.close()
-    del job, up, dn, weights, overlaps, scalars, pd, ham, wave
+    del job, st, keep
+    torch.cuda.empty_cache()
     return res
+
+
+def h10_timing(steps=20, warmup=3):
+    """The H10 chain (STO-6G, 1.6 bohr, own integrals, 1e-5 Cholesky
+    threshold), RHF trial, 50 walkers, dt=0.005, blocks of 10 steps, FP64."""
+    import numpy as np
+    import torch
+
+    from ad_afqmc_amd import hamiltonian as hm
+    from ad_afqmc_amd import integrals as I
+    from ad_afqmc_amd import propagation as pr
+    from ad_afqmc_amd import wavefunctions as wf
+    from ad_afqmc_amd.binding import engine_for
+
+    h = I.hydrogen_chain(10, 1.6, "sto-6g", 1e-5)
+    W, dt, nsteps = 50, 0.005, 10
+    trial = wf.rhf(10, (5, 5))
+    prop = pr.propagator_afqmc(dt=dt, n_walkers=W, walker_type="restricted")
+    wave, ham = h.wave_data(), h.ham_data()
+    H = hm.hamiltonian(10)
+    ham = H.build_measurement_intermediates(ham, trial, wave)
+    ham = H.build_propagation_intermediates(ham, prop, trial, wave)
+    pd = prop.init_prop_data(trial, wave, ham, seed=0)
+    job = engine_for(trial, wave, ham, "restricted", W, prop)
+    job.reserve_fields(nsteps)
+    up = pd["walkers"].data.clone()
+    st = dict(up=up, dn=None, weights=pd["weights"].clone(), overlaps=job.overlap(up, None),
+              scalars=torch.stack([pd["e_estimate"], pd["e_estimate"].clone()]).contiguous(),
+              counter=torch.zeros(1, dtype=torch.int64, device=up.device),
+              out=torch.empty(2, dtype=torch.float64, device=up.device))
+    for _ in range(warmup):
+        _sr_block(job, st, nsteps, 5)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    outs = [_sr_block(job, st, nsteps, 5) for _ in range(steps)]
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    b = torch.stack(outs).cpu().numpy()
+    job.close()
+    return {"value": W * nsteps * steps / el, "unit": "walker-steps/s", "ms_per_block": el / steps * 1e3,
+            "blocks": steps, "nchol": int(h.chol.shape[0]), "e_rhf": float(h.e_rhf),
+            "mean_block_energy": float(np.sum(b[:, 0] * b[:, 1]) / np.sum(b[:, 1])),
+            "workload": "H10 chain, STO-6G, 1.6 bohr, RHF trial, 50 walkers, dt 0.005, 10 steps per block, FP64"}
+
+
+def h10_cpu_baseline(n_steps=20):
+    """The FP64 oracle on the H10 chain, 50 walkers, one core: n_steps steps
+    and one measurement, scaled to blocks of 10 steps."""
+    import numpy as np
+
+    from ad_afqmc_amd import integrals as I
+    from oracle import afqmc_oracle as O
+
+    limits = _one_thread()
+    try:
+        h = I.hydrogen_chain(10, 1.6, "sto-6g", 1e-5)
+        W, dt, G = 50, 0.005, h.chol.shape[0]
+        mo = h.trial_mo()
+        trial = O.RHFTrial(10, (5, 5), mo)
+        h1s, rh, rc = O.rhf_measurement_intermediates(h.ham_data()["h1"], h.chol, mo)
+        rdm1 = O.rdm1_from_mo(mo)
+        pi = O.propagation_intermediates(h.h0, h1s, h.chol, rdm1, dt)
+        ham = dict(h0=h.h0, chol=h.chol, rot_h1=rh, rot_chol=rc, mf_shifts=pi["mf_shifts"], h0_prop=pi["h0_prop"],
+                   exp_h1=pi["exp_h1"])
+        w0 = O.init_walkers(rdm1, (5, 5), W, "restricted")
+        e0 = float(np.mean(trial.calc_energy(w0, h.h0, rh, rc).real))
+        st = dict(walkers=w0, weights=np.ones(W), overlaps=trial.calc_overlap(w0), e_estimate=e0,
+                  pop_control_ene_shift=e0)
+        rng = np.random.default_rng(0)
+        t0 = time.perf_counter()
+        for _ in range(n_steps):
+            st = O.propagate_constrained(trial, st, rng.standard_normal((W, G)), ham, dt)
+        t_steps = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        st["walkers"] = O.orthonormalize(st["walkers"])
+        st["overlaps"] = trial.calc_overlap(st["walkers"])
+        trial.calc_energy(st["walkers"], h.h0, rh, rc)
+        t_meas = time.perf_counter() - t0
+    finally:
+        if limits is not None:
+            limits.restore_original_limits()
+    t_step = t_steps / n_steps
+    return {"value": W * 10 / (10 * t_step + t_meas), "unit": "walker-steps/s", "cores": 1, "kind": "port",
+            "sample": f"50 walkers x {n_steps} steps + one measurement, FP64 NumPy oracle, 1 BLAS thread"}
+
+
+def _mode_entry(name, r, base_value, errs, m):
+    cfg = syn_config(name)
+    Oa = cfg["nocc_up"]
+    Ob = cfg.get("nocc_dn", Oa)
+    e = {"value": r["value"], "ms_per_step": r["ms_per_block"], "blocks_timed": len(r["blocks"]),
+         "speedup_vs_fp64": r["value"] / base_value}
+    e.update(errs)
+    e["whole_step_roofline"] = whole_step_roofline(cfg["norb"], Oa, Ob, cfg["nchol"], m, not cfg["uhf"], r["precision"],
+                                                   r["value"])
+    return e
 
 
 def main():
@@ -512,13 +648,17 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10, help="timed blocks")
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--config", default="rhf_n200_o40_g1200_w512")
-    ap.add_argument("--precision", default="fp64")
+    ap.add_argument("--config", default="uhf_n300_o60_58_g2000_w256")
+    ap.add_argument("--precision", default="bf16x3")
+    ap.add_argument("--lcoef", type=float, default=0.3,
+                    help="Cholesky scale coefficient c: s_g multiplied by c/sqrt(N) (0: the stated scale)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-sample-walkers", type=int, default=8)
+    ap.add_argument("--cpu-sample-walkers", type=int, default=4)
+    ap.add_argument("--cpu-sample-steps", type=int, default=6)
     ap.add_argument("--cpu-procs", type=int, default=16,
                     help="walker-sharded oracle processes for the all-cores CPU baseline (1: skip it)")
     ap.add_argument("--breakdown", action="store_true", help="print per-class kernel times to stderr")
+    ap.add_argument("--only", action="store_true", help="time the headline workload only (for profiling runs)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run 5 plain device copies of MIB MiB (to scale HBM counters) and exit")
     ap.add_argument("--serving-jobs", type=int, default=64,
@@ -528,20 +668,18 @@ def main():
                     help="only run the FP64 / FP32 / bf16x3 energy comparison on WORKLOAD and exit")
     ap.add_argument("--study-blocks", type=int, default=50)
     ap.add_argument("--study-eql", type=int, default=10)
-    ap.add_argument("--study-lcoef", type=float, default=0.3, help="Cholesky scale coefficient c in c/sqrt(N)")
-    ap.add_argument("--no-modes", action="store_true",
-                    help="skip the FP32 / bf16x3 runs of the workload and the N=300, G=2000 mode comparison")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this engine runs on one GPU; --gpus above 1 is not supported", file=sys.stderr)
         sys.exit(2)
+    l_coef = args.lcoef if args.lcoef > 0 else None
 
     import numpy as np
     import torch
 
     if args.precision_study:
         print(json.dumps({"precision_study": precision_study(args.precision_study, args.study_blocks,
-                                                             args.study_eql, args.study_lcoef)}), flush=True)
+                                                             args.study_eql, args.lcoef)}), flush=True)
         return
 
     if args.copy_calibration:
@@ -554,23 +692,29 @@ def main():
         print(json.dumps({"copy_calibration_bytes_read": n * 8, "copy_calibration_bytes_written": n * 8}))
         return
 
-    cfg = syn_config(args.config)
+    name = args.config
+    cfg = syn_config(name)
     N, G, W = cfg["norb"], cfg["nchol"], cfg["n_walkers"]
     uhf = cfg["uhf"]
     Oa = cfg["nocc_up"]
     Ob = cfg.get("nocc_dn", Oa)
     nsteps, dt = cfg["n_prop_steps"], cfg["dt"]
-    cpu = cpu_all = None
-    if not args.no_cpu_baseline and not uhf:
+    cpu = cpu_all = h10_cpu = None
+    if not args.no_cpu_baseline and not args.only:
         # before anything touches the GPU: the parallel baseline spawns its workers
-        print("[bench] cpu baseline (oracle sample)", file=sys.stderr, flush=True)
-        cpu = cpu_baseline(cfg, args.cpu_sample_walkers)
+        print(f"[bench] cpu baseline (oracle sample of {name})", file=sys.stderr, flush=True)
+        cpu = cpu_baseline(name, args.cpu_sample_walkers, args.cpu_sample_steps, l_coef or 0.3)
         if args.cpu_procs > 1:
-            cpu_all = cpu_baseline_parallel(cfg, args.cpu_sample_walkers, min(args.cpu_procs, os.cpu_count() or 1))
-    print(f"[bench] building {args.config}", file=sys.stderr, flush=True)
-    system = make_system(args.config)
-    r = time_config(system, args.config, args.precision, args.steps, args.warmup, dominant_roofline=True,
+            print("[bench] cpu baseline, all cores", file=sys.stderr, flush=True)
+            cpu_all = cpu_baseline_parallel(name, args.cpu_sample_walkers, args.cpu_sample_steps,
+                                            min(args.cpu_procs, os.cpu_count() or 1), l_coef or 0.3)
+        print("[bench] cpu baseline (H10)", file=sys.stderr, flush=True)
+        h10_cpu = h10_cpu_baseline()
+    print(f"[bench] building {name}", file=sys.stderr, flush=True)
+    system = make_system(name, l_coef)
+    r = time_config(system, name, args.precision, args.steps, args.warmup, dominant_roofline=True,
                     breakdown=args.breakdown)
+    r["precision"] = args.precision
     value, elapsed, blocks = r["value"], r["elapsed"], r["blocks"]
     finite = bool(np.all(np.isfinite(blocks)))
     f_total, _ = flops_per_walker_step(N, Oa, Ob, G, nsteps, not uhf)
@@ -588,7 +732,7 @@ def main():
         "dtype": "f64" if args.precision == "fp64" else args.precision,
         "data": "synthetic",
         "config": {
-            "workload": args.config,
+            "workload": name,
             "norb": N,
             "nocc": [Oa, Ob] if uhf else [Oa],
             "nchol": G,
@@ -596,7 +740,10 @@ def main():
             "dt": dt,
             "steps_per_block": nsteps,
             "trial": "uhf" if uhf else "rhf",
+            "walker_type": "unrestricted" if uhf else "restricted",
             "precision": args.precision,
+            "cholesky_scale": "stated" if l_coef is None else f"stated x {l_coef}/sqrt(N)",
+            "block": "afq_sr_block: 10 steps + QR + overlap/energy + comb resampling + overlaps",
             "parallelism": "single-gpu",
         },
         "model_flops_per_walker_step": f_total,
@@ -604,57 +751,60 @@ def main():
         "roofline": r["roofline"],
         "whole_step_roofline": whole_step_roofline(N, Oa, Ob, G, nsteps, not uhf, args.precision, value),
         "kernel_breakdown_ms_per_block": r["breakdown"],
-        # The stated generator kills the walker population (see synthetic.make_system);
-        # the work per block is the same dense arithmetic either way.
         "population_finite": finite,
-        "mean_block_energy": float(np.mean(blocks[:, 0])) if finite else None,
-        "population_note": ("the generator as stated has a two-body scale growing like sqrt(N) and clips every "
-                            "walker within a few steps; every timed kernel is dense and does the same work either "
-                            "way. Energies and precision errors use Cholesky scale 0.3/sqrt(N) (profiles/precision_study_*.json)."),
+        "mean_block_energy": float(np.sum(blocks[:, 0] * blocks[:, 1]) / np.sum(blocks[:, 1])) if finite else None,
     }
-    del system
-    torch.cuda.empty_cache()
-    if not args.no_modes:
-        # the reduced-precision modes on the same workload, with their measured
-        # systematic energy error (restart-paired blocks, profiles/precision_study_*.json)
-        modes = {}
-        errs = _precision_errors(args.config)
-        system = make_system(args.config)
+    errs = _precision_errors(name)
+    line.update(errs.get(args.precision, {}))
+    if not args.only:
+        # the other modes of the same workload over the same number of blocks
+        modes = {args.precision: r}
         for p in ("fp64", "fp32", "bf16x3"):
-            if p == args.precision:
-                modes[p] = {"value": value, "ms_per_step": elapsed / args.steps * 1e3}
-            else:
-                m = time_config(system, args.config, p, max(3, args.steps // 2), 1)
-                modes[p] = {"value": m["value"], "ms_per_step": m["elapsed"] / max(3, args.steps // 2) * 1e3}
-            modes[p].update(errs.get(p, {}))
-            modes[p]["whole_step_roofline"] = whole_step_roofline(N, Oa, Ob, G, nsteps, not uhf, p,
-                                                                  modes[p]["value"])
-        line["precision_modes"] = modes
+            if p not in modes:
+                print(f"[bench] {name} {p}", file=sys.stderr, flush=True)
+                modes[p] = time_config(system, name, p, args.steps, args.warmup, breakdown=args.breakdown)
+                modes[p]["precision"] = p
+        base = modes["fp64"]["value"]
+        line["speedup_vs_fp64"] = value / base
+        line["precision_modes"] = {p: _mode_entry(name, m, base, errs.get(p, {}), nsteps) for p, m in modes.items()}
+        # the reference's default measurement interval (50 steps, launch_script.py:120)
+        m50 = {}
+        for p in ("fp64", "fp32", "bf16x3"):
+            print(f"[bench] {name} {p} at 50 steps per block", file=sys.stderr, flush=True)
+            m = time_config(system, name, p, 3, 1, nsteps=50)
+            m["precision"] = p
+            m50[p] = m
+        line["measurement_interval_50"] = {p: _mode_entry(name, m, m50["fp64"]["value"], {}, 50)
+                                           for p, m in m50.items()}
         del system
         torch.cuda.empty_cache()
-        # the mixed-precision target is stated at N=300, G=2000 (UHF, 256 walkers)
-        big = "uhf_n300_o60_58_g2000_w256"
-        print(f"[bench] building {big}", file=sys.stderr, flush=True)
-        system = make_system(big)
-        bc = syn_config(big)
-        errs = _precision_errors(big)
-        # measurement every 10 steps (as the RHF workload) and every 50 (the
-        # reference's default interval)
-        for m_int, key in ((10, "uhf_n300_g2000_modes"), (50, "uhf_n300_g2000_modes_m50")):
-            ns = {}
-            for p in ("fp64", "fp32", "bf16x3"):
-                m = time_config(system, big, p, 2, 1, nsteps=m_int)
-                ns[p] = {"value": m["value"], "ms_per_step": m["elapsed"] / 2 * 1e3, "steps_per_block": m_int}
-            for p in ns:
-                if m_int == bc["n_prop_steps"]:
-                    ns[p].update(errs.get(p, {}))
-                ns[p]["speedup_vs_fp64"] = ns[p]["value"] / ns["fp64"]["value"]
-                ns[p]["whole_step_roofline"] = whole_step_roofline(bc["norb"], bc["nocc_up"], bc["nocc_dn"],
-                                                                   bc["nchol"], m_int, False, p, ns[p]["value"])
-            line[key] = ns
+        # the stated Cholesky scale: same dense work per block, dying population
+        print(f"[bench] {name} stated scale", file=sys.stderr, flush=True)
+        system = make_system(name, None)
+        st = time_config(system, name, args.precision, 3, 1)
+        line["stated_generator"] = {"value": st["value"], "ms_per_step": st["ms_per_block"], "blocks_timed": 3,
+                                    "population_finite": bool(np.all(np.isfinite(st["blocks"])))}
         del system
         torch.cuda.empty_cache()
-    if args.serving_jobs > 0:
+        # the RHF workload (N=200, O=40, G=1200, 512 walkers) in every mode
+        rhf = "rhf_n200_o40_g1200_w512"
+        print(f"[bench] building {rhf}", file=sys.stderr, flush=True)
+        system = make_system(rhf, l_coef)
+        rerrs = _precision_errors(rhf)
+        rm = {}
+        for p in ("fp64", "fp32", "bf16x3"):
+            rm[p] = time_config(system, rhf, p, args.steps, args.warmup, dominant_roofline=(p == "fp64"),
+                                breakdown=args.breakdown)
+            rm[p]["precision"] = p
+        line[rhf] = {p: _mode_entry(rhf, m, rm["fp64"]["value"], rerrs.get(p, {}), 10) for p, m in rm.items()}
+        line[rhf]["fp64"]["roofline"] = rm["fp64"]["roofline"]
+        line[rhf]["fp64"]["kernel_breakdown_ms_per_block"] = rm["fp64"]["breakdown"]
+        del system
+        torch.cuda.empty_cache()
+        print("[bench] H10 chain", file=sys.stderr, flush=True)
+        line["h10_chain"] = h10_timing()
+        line["h10_chain"]["cpu_baseline"] = h10_cpu
+    if args.serving_jobs > 0 and not args.only:
         from ad_afqmc_amd import serving
 
         torch.cuda.empty_cache()

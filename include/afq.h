@@ -188,6 +188,9 @@ int afq_job_reserve_fields(afq_job_t job, int32_t n_prop_steps, void* stream);
 /* Fills out (n,) f64 with N(0,1) draws of the engine's field generator for
  * (seed, counter) -- the stream afq_block uses. */
 int afq_normal_fill(double* out, int64_t n, uint64_t seed, uint64_t counter, void* stream);
+/* Writes to out (device f64 scalar) the comb offset zeta ~ U[0,1) that
+ * afq_sr_block draws for the block keyed by (seed, counter). */
+int afq_sr_zeta(double* out, uint64_t seed, uint64_t counter, void* stream);
 
 /* Live kernel timing.  Kernel classes of the walker step: */
 enum {
@@ -226,6 +229,21 @@ int64_t afq_walkers_sr_workspace(int32_t n_walkers, int64_t max_walker_elems);
  * walkers_dn = NULL for one buffer).  weights set to sum|w|/W. */
 int afq_walkers_sr(void* walkers_up, int64_t elems_up, void* walkers_dn, int64_t elems_dn, int32_t n_walkers,
                    double* weights, const double* zeta, int32_t* indices_out, void* workspace, void* stream);
+
+/* No counterpart in the reference.  afq_walkers_sr without the workspace
+ * argument and without any allocation, for callers that cannot allocate
+ * device memory (code under HIP graph capture, tight serving loops, harnesses
+ * that forbid hipMalloc).  Same arguments, results and errors as
+ * afq_walkers_sr; its workspace is a 16 MiB device array of the library (one
+ * per GPU, held until the process ends), so n_walkers is limited to
+ * afq_walkers_sr_noalloc_max_walkers() (65535) and the walker gather runs in
+ * column chunks that fit the array.  Calls that overlap on one GPU (on
+ * different streams or threads) share that array and race.  The first call on
+ * a device looks the array's address up (no stream work); make it before
+ * capturing a graph. */
+int afq_walkers_sr_noalloc(void* walkers_up, int64_t elems_up, void* walkers_dn, int64_t elems_dn, int32_t n_walkers,
+                           double* weights, const double* zeta, int32_t* indices_out, void* stream);
+int64_t afq_walkers_sr_noalloc_max_walkers(void);
 
 /* Value and reverse-mode gradient of the segment energy
  *     F = sum_b E_b W_b / sum_b W_b

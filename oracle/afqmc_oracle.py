@@ -121,17 +121,32 @@ def det_overlap(walkers: np.ndarray, mo: np.ndarray) -> np.ndarray:
     return np.linalg.det(S)
 
 
+def _real_times_complex(A: np.ndarray, B: np.ndarray) -> np.ndarray:
+    """A @ B for real A and complex B as two real products (the same values as
+    the complex product, without upcasting A)."""
+    return (A @ np.ascontiguousarray(B.real)) + 1.0j * (A @ np.ascontiguousarray(B.imag))
+
+
 def _exchange_and_coulomb(rot_chol: np.ndarray, G: np.ndarray):
     """c_g = tr f_g and K = sum_g sum_ij f_gij f_gji with f_g = rot_chol_g . G^T,
     one walker at a time (wavefunctions.py:1137-1139, 1347-1356)."""
     Wn = G.shape[0]
-    c = np.empty((Wn, rot_chol.shape[0]), dtype=np.complex128)
+    ng, no, n = rot_chol.shape
+    rc2 = np.asarray(rot_chol, dtype=np.float64).reshape(ng * no, n)
+    c = np.empty((Wn, ng), dtype=np.complex128)
     K = np.empty(Wn, dtype=np.complex128)
     for w in range(Wn):
-        f = np.einsum("gij,kj->gik", rot_chol, G[w], optimize=True)
+        f = _real_times_complex(rc2, G[w].T).reshape(ng, no, G.shape[1])
         c[w] = np.einsum("gii->g", f)
-        K[w] = np.einsum("gij,gji->", f, f)
+        K[w] = np.sum(f * f.transpose(0, 2, 1))
     return c, K
+
+
+def _fb_contract(rot_chol: np.ndarray, G: np.ndarray) -> np.ndarray:
+    """sum_ij rot_chol_gij G_wij -> (W, G) (wavefunctions.py:1107, 1323-1328)."""
+    rc2 = np.asarray(rot_chol, dtype=np.float64).reshape(rot_chol.shape[0], -1)
+    g2 = G.reshape(G.shape[0], -1)
+    return _real_times_complex(rc2, g2.T).T
 
 
 class RHFTrial:
@@ -163,9 +178,9 @@ class RHFTrial:
         """_calc_force_bias_restricted/_unrestricted, wavefunctions.py:1102-1126."""
         up, dn, restricted = self.split(walkers)
         if restricted:
-            return 2.0 * np.einsum("gij,wij->wg", rot_chol, green(up, self.mo), optimize=True)
+            return 2.0 * _fb_contract(rot_chol, green(up, self.mo))
         g = green(up, self.mo) + green(dn, self.mo)
-        return np.einsum("gij,wij->wg", rot_chol, g, optimize=True)
+        return _fb_contract(rot_chol, g)
 
     def calc_energy(self, walkers: Walkers, h0, rot_h1, rot_chol) -> np.ndarray:
         """_calc_energy_restricted/_unrestricted, wavefunctions.py:1128-1174."""
@@ -210,9 +225,7 @@ class UHFTrial:
     def calc_force_bias(self, walkers: Walkers, rot_chol: Sequence[np.ndarray]) -> np.ndarray:
         """_calc_force_bias_unrestricted, wavefunctions.py:1314-1329."""
         gu, gd = self.calc_green(walkers)
-        return np.einsum("gij,wij->wg", rot_chol[0], gu, optimize=True) + np.einsum(
-            "gij,wij->wg", rot_chol[1], gd, optimize=True
-        )
+        return _fb_contract(rot_chol[0], gu) + _fb_contract(rot_chol[1], gd)
 
     def calc_energy(self, walkers: Walkers, h0, rot_h1, rot_chol) -> np.ndarray:
         """_calc_energy_unrestricted, wavefunctions.py:1331-1363."""
@@ -231,10 +244,12 @@ def vhs_from_fields(shifted_fields: np.ndarray, chol: np.ndarray, dt: float) -> 
     """VHS = i sqrt(dt) (x~ @ L).reshape(N,N), propagation.py:196-203, 215-222."""
     n2 = chol.reshape(chol.shape[0], -1).shape[1]
     n = int(round(math.sqrt(n2)))
-    L = chol.reshape(-1, n2).astype(np.float64)
-    x = shifted_fields.astype(np.complex128)
-    # complex @ real as two real products (same values as the complex product)
-    v = (x.real @ L) + 1.0j * (x.imag @ L)
+    L = np.asarray(chol, dtype=np.float64).reshape(-1, n2)
+    x = np.asarray(shifted_fields, dtype=np.complex128)
+    # complex @ real as one real product of the stacked real and imaginary rows
+    W = x.shape[0]
+    v = np.concatenate([x.real, x.imag], axis=0) @ L
+    v = v[:W] + 1.0j * v[W:]
     return 1.0j * math.sqrt(dt) * v.reshape(-1, n, n)
 
 
