@@ -210,6 +210,10 @@ struct afq_job {
   // (Wr, 4, Gpad); rows padded to 16, k to 32
   uint16_t* lt_split = nullptr;
   uint16_t* xs_split = nullptr;
+  // split-BF16 x3 fused propagator: exp(-dt/2 h1) - I per propagated spin as
+  // three bf16 planes (k_split_e1_bf16)
+  uint16_t* e1s[2] = {nullptr, nullptr};
+  uint16_t* vsplit = nullptr;             // (W, 4, 16 ceil(N/16), 32 ceil(N/32)) split VHS planes
   // symmetric Cholesky vectors: the VHS product over block-packed columns
   // (sym_packed_cols), used when set_hamiltonian finds L_g = L_g^T; npk = 0 when
   // packing would not save work at this N

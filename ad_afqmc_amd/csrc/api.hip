@@ -7,6 +7,7 @@
 
 #include "gemm_bf3.h"
 #include "kernels.h"
+#include "trot_fused.h"
 #include "pipeline.h"
 
 namespace afq {
@@ -80,6 +81,9 @@ void release(afq_job* j) {
   fr(j->rot_chol32[0]);
   if (!j->rhf) fr(j->rot_chol32[1]);
   fr(j->vhs32);
+  fr(j->e1s[0]);
+  fr(j->e1s[1]);
+  fr(j->vsplit);
   fr(j->t32[0]);
   fr(j->t32[1]);
   fr(j->rcs[0]);
@@ -394,10 +398,30 @@ int trotprop_lp(afq_job* j, void* up, void* dn, const double2* xs, hipStream_t s
       AFQ_TRY(gemm_lp(LpKind::VHS, false, g, st));
     }
   }
+  PanelCols pc(j);
+#ifndef AFQ_NO_FUSED_TROT
+  if (j->e1s[0]) {
+    // split-BF16 x3: one-body products, Taylor terms and running sum in one kernel
+    TrotFused f;
+    f.N = N;
+    f.W = W;
+    f.nspin = j->nspin_prop;
+    f.nterms = j->d.n_exp_terms;
+    for (int s = 0; s < j->nspin_prop; ++s) {
+      f.wb[s] = static_cast<double2*>(s == 0 ? up : dn);
+      f.c[s] = pc.c[s];
+      f.e1[s] = j->e1s[s];
+    }
+    f.vs = j->vsplit;
+    ProfScope ps(j, AFQ_K_TAYLOR, st);
+    AFQ_TRY(k_split_vhs_bf16(j->sym_chol ? j->vhsp32 : j->vhs32, j->sym_chol ? j->npk : NN, j->sym_chol, N, W,
+                             j->vsplit, st));
+    return trot_fused_bf3(f, st);
+  }
+#endif
   // The walker buffers of both spins are propagated as the column blocks of
   // one (W, N, c_up + c_dn) panel, so each Taylor term is one product that
   // reads the shared VHS once.
-  PanelCols pc(j);
   const long per = (long)N * pc.ctot;
   double2* t0 = j->tbuf[0];
   double2* acc = j->tbuf[2];
@@ -800,6 +824,11 @@ int alloc_job(afq_job* j, bool dry, int64_t* total) {
       A_(j->gsplit, W * 4 * bf16_opad((int)std::max(j->nocc[0], j->nocc[1])) * np);
       A_(j->lt_split, 16 * ((N * N + 15) / 16) * 2 * gp);
       A_(j->xs_split, bf16_opad((int)W) * 4 * gp);
+      if (trot_fused_supported((int)N)) {
+        A_(j->e1s[0], trot_fused_e1_elems((int)N));
+        if (!j->restricted) A_(j->e1s[1], trot_fused_e1_elems((int)N));
+        A_(j->vsplit, trot_fused_vhs_elems((int)N, (int)W));
+      }
     } else {
       A_(j->chol32, G * N * N);
     }
@@ -1049,6 +1078,7 @@ int afq_build_propagation_intermediates(afq_job_t j, void* stream) {
     AFQ_TRY(k_scale(As, hmod, -0.5 * j->d.dt, N, st));
     AFQ_TRY(expm_device(As, N, j->exp_h1[s], j->tmp_nn + 5 * NN, j->dscratch, st));
     if (j->lp) AFQ_TRY(k_minus_identity_f32(j->exp_h1[s], j->exp_h1m32[s], N, st));
+    if (j->e1s[s]) AFQ_TRY(k_split_e1_bf16(j->exp_h1[s], N, j->e1s[s], st));
   }
   AFQ_HIP(hipMemcpyAsync(&j->h0_prop_host, j->h0_prop, sizeof(double2), hipMemcpyDeviceToHost, st));
   AFQ_HIP(hipStreamSynchronize(st));
@@ -1112,6 +1142,7 @@ int afq_set_intermediate(afq_job_t j, int which, int spin, const void* src, void
     case AFQ_INT_EXP_H1:
       j->have_exp[spin] = true;
       if (j->lp) AFQ_TRY(k_minus_identity_f32(j->exp_h1[spin], j->exp_h1m32[spin], (int)N, S(stream)));
+      if (j->e1s[spin]) AFQ_TRY(k_split_e1_bf16(j->exp_h1[spin], (int)N, j->e1s[spin], S(stream)));
       break;
     default: break;
   }

@@ -36,6 +36,25 @@ namespace {
 
 enum Mode { F64 = 0, F32 = 1 };
 
+// Bytes of rot_chol a gamma group spans (its chunks run back to back on one
+// XCD, so the slice stays in that XCD's L2 while all walkers pass over it;
+// 1.25 and 0.625 MB measured no faster at N=300, G=2000, 256 walkers).
+constexpr long kGroupSliceBytes = 2560L * 1024;
+
+// The epilogue's transpose tile is private to each wave, and a wave's LDS
+// accesses are performed in issue order: ordering them needs no work-group
+// barrier, only that the compiler keep them in program order.
+__device__ inline void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Permutation of the four 16-byte k-groups of LDS row r: rows r and r' < 16
+// of a fragment then occupy distinct 16-byte bank groups both within every 8
+// consecutive rows (128 bytes of banks) and within all 16 (256 bytes).
+__device__ inline int kswz(int r) { return ((r >> 1) ^ (r >> 3)) & 3; }
+
 __device__ inline double wave_sum(double v) {
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
@@ -175,7 +194,7 @@ __global__ __launch_bounds__(256, (NB <= 3 || MODE == F32 ? 2 : 1)) void exchang
         const int r = MODE == F64 ? (lane >> 4) + 4 * v : 4 * (lane >> 4) + v;
         S[r * 17 + cc] = double2{(double)accR[bj][bi][v], (double)accI[bj][bi][v]};
       }
-      __syncthreads();
+      wave_lds_sync();
       const double wgt = (bi == bj) ? 1.0 : 2.0;
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
@@ -192,7 +211,7 @@ __global__ __launch_bounds__(256, (NB <= 3 || MODE == F32 ? 2 : 1)) void exchang
           }
         }
       }
-      __syncthreads();
+      wave_lds_sync();
     }
   kr = wave_sum(kr);
   ki = wave_sum(ki);
@@ -441,11 +460,13 @@ __global__ __launch_bounds__(256, 2) void exchange_bf16_kernel(const v8bf* __res
       if (i < PER) stage[t] = Bw[(long)(i / (4 * ROWS)) * plane + (long)((i / 4) % ROWS) * rowv + kv + (i & 3)];
     }
   };
+  // the four 16-byte k-groups of a row are permuted by kswz(row) so that the
+  // 16 rows of a fragment read fall in distinct banks
   auto store = [&](int buf) {
 #pragma unroll
     for (int t = 0; t < LD; ++t) {
       const int i = tid + t * 256;
-      if (i < PER) bsh[buf][i] = stage[t];
+      if (i < PER) bsh[buf][(i & ~3) | ((i & 3) ^ kswz((i >> 2) & 15))] = stage[t];
     }
   };
 
@@ -487,7 +508,7 @@ __global__ __launch_bounds__(256, 2) void exchange_bf16_kernel(const v8bf* __res
 #pragma unroll
       for (int b = 0; b < NB; ++b) al[b] = Al[(long)b * 16 * rowv + kv];
     }
-    const v8bf* B = bsh[buf] + r16 * 4 + q4;  // + plane * 4 ROWS + 16 b * 4
+    const v8bf* B = bsh[buf] + r16 * 4 + (q4 ^ kswz(r16));  // + plane * 4 ROWS + 16 b * 4
     v8bf br[NB], bi[NB];
 #pragma unroll
     for (int b = 0; b < NB; ++b) {
@@ -545,7 +566,7 @@ __global__ __launch_bounds__(256, 2) void exchange_bf16_kernel(const v8bf* __res
     for (int bj = bi; bj < NB; ++bj) {
 #pragma unroll
       for (int v = 0; v < 4; ++v) S[(4 * q4 + v) * 17 + cc] = double2{(double)accR[bj][bi][v], (double)accI[bj][bi][v]};
-      __syncthreads();
+      wave_lds_sync();
       const double wgt = (bi == bj) ? 1.0 : 2.0;
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
@@ -562,7 +583,7 @@ __global__ __launch_bounds__(256, 2) void exchange_bf16_kernel(const v8bf* __res
           }
         }
       }
-      __syncthreads();
+      wave_lds_sync();
     }
   kr = wave_sum(kr);
   ki = wave_sum(ki);
@@ -611,7 +632,7 @@ int launch_nb(const TR* rc, const double2* Gm, int O, int N, int Gn, int W, int 
               hipStream_t s) {
   // gamma chunks per group: a group's rot_chol slice ~2.5 MB of L2
   const long chunk_bytes = 4L * O * N * (long)sizeof(TR);
-  const int cpg = (int)std::max(1L, std::min((long)nchunk, (2560L * 1024) / std::max(1L, chunk_bytes)));
+  const int cpg = (int)std::max(1L, std::min((long)nchunk, (kGroupSliceBytes) / std::max(1L, chunk_bytes)));
   const long ngroups = (nchunk + cpg - 1) / cpg;
   const long T = ngroups * cpg * (long)W;
   const long T8 = (T + 7) / 8;
@@ -626,7 +647,7 @@ template <int KS>
 int launch_split(const double* rc, const double2* Gm, int O, int N, int Gn, int W, int nchunk, double2* coul,
                  double2* kpart, hipStream_t s) {
   const long chunk_bytes = 4L * O * N * (long)sizeof(double);
-  const int cpg = (int)std::max(1L, std::min((long)nchunk, (2560L * 1024) / std::max(1L, chunk_bytes)));
+  const int cpg = (int)std::max(1L, std::min((long)nchunk, (kGroupSliceBytes) / std::max(1L, chunk_bytes)));
   const long ngroups = (nchunk + cpg - 1) / cpg;
   const long T = ngroups * cpg * (long)W;
   const long T8 = (T + 7) / 8;
@@ -700,7 +721,7 @@ int k_exchange_bf16(const uint16_t* rcs, const uint16_t* gs, int O, int N, int G
   *nchunk_out = nchunk;
   const int Opad = bf16_opad(O), Npad = bf16_npad(N);
   const long chunk_bytes = 4L * 2 * Opad * Npad * 2;
-  const int cpg = (int)std::max(1L, std::min((long)nchunk, (2560L * 1024) / std::max(1L, chunk_bytes)));
+  const int cpg = (int)std::max(1L, std::min((long)nchunk, (kGroupSliceBytes) / std::max(1L, chunk_bytes)));
   const long ngroups = (nchunk + cpg - 1) / cpg;
   const long T = ngroups * cpg * (long)W;
   const long T8 = (T + 7) / 8;

@@ -21,6 +21,7 @@
 // MFMA f64 16x16x4 lane layout (gfx950): lane l feeds A[l%16][l/16] and
 // B[l/16][l%16]; accumulator element v of lane l is D[l/16 + 4v][l%16].
 #include "afq_internal.h"
+#include "group.h"
 
 namespace afq {
 namespace {
@@ -163,11 +164,11 @@ __device__ inline void store_b(double* Bs, const double (&r)[C::RB][4][2], int t
   }
 }
 
-// grid: x = tiles (XCD-remapped when xcd_remap == 1), y = batch, z = K split;
-// xcd_remap == 2: x = tiles x batch rounded up to 8 entries, y = z = 1.
+// One work-group's tile `tile` of batch entry b, K slice kz (the whole
+// product when kchunk >= K and ws is null).
 template <bool AC, bool BC, bool AT, bool BT, int WM, int WN, int TM, int TN>
-__global__ __launch_bounds__(64 * WM * WN) void gemm_f64_kernel(Gemm g, int tiles_m, int tiles_n, int xcd_remap,
-                                                                 long T8, int kchunk, double* ws) {
+__device__ __forceinline__ void gemm_f64_body(const Gemm& g, int tiles_m, int tiles_n, long tile, long b, int kz,
+                                              int kchunk, double* ws) {
   using C = Cfg<AC, BC, AT, BT, WM, WN, TM, TN>;
   constexpr bool CC = AC || BC;
   __shared__ double As[2][C::PA * BK * C::LDA];
@@ -175,24 +176,10 @@ __global__ __launch_bounds__(64 * WM * WN) void gemm_f64_kernel(Gemm g, int tile
 
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wm = wave / WN, wn = wave % WN;
-  long tile = blockIdx.x;
-  long b = blockIdx.y;
-  if (xcd_remap == 1) {
-    tile = (tile & 7) * T8 + (tile >> 3);
-  } else if (xcd_remap == 2) {
-    // batched, 1-D grid: work-group L runs on XCD L % 8; that XCD takes the
-    // batch entries b = L % 8 (mod 8), all T8 tiles of one entry in a row, so
-    // an operand shared by the tiles of an entry is fetched into one L2
-    const long p = tile >> 3;
-    b = (tile & 7) + 8 * (p / T8);
-    tile = p % T8;
-    if (b >= g.batch) return;  // uniform per work-group
-  }
   const int mt = (int)(tile % tiles_m);
   const long ntl = tile / tiles_m;
   if (ntl >= tiles_n) return;  // uniform per work-group
   const int m0 = mt * C::BM, n0 = (int)ntl * C::BN;
-  const int kz = blockIdx.z;
   const int kbeg = kz * kchunk;
   const int kend = min(g.K, kbeg + kchunk);
   const double* A = static_cast<const double*>(g.A) + b * g.sA * (AC ? 2 : 1);
@@ -322,6 +309,42 @@ __global__ __launch_bounds__(64 * WM * WN) void gemm_f64_kernel(Gemm g, int tile
       }
 }
 
+// grid: x = tiles (XCD-remapped when xcd_remap == 1), y = batch, z = K split;
+// xcd_remap == 2: x = tiles x batch rounded up to 8 entries, y = z = 1.
+template <bool AC, bool BC, bool AT, bool BT, int WM, int WN, int TM, int TN>
+__global__ __launch_bounds__(64 * WM * WN) void gemm_f64_kernel(Gemm g, int tiles_m, int tiles_n, int xcd_remap,
+                                                                 long T8, int kchunk, double* ws) {
+  long tile = blockIdx.x;
+  long b = blockIdx.y;
+  if (xcd_remap == 1) {
+    tile = (tile & 7) * T8 + (tile >> 3);
+  } else if (xcd_remap == 2) {
+    // batched, 1-D grid: work-group L runs on XCD L % 8; that XCD takes the
+    // batch entries b = L % 8 (mod 8), all T8 tiles of one entry in a row, so
+    // an operand shared by the tiles of an entry is fetched into one L2
+    const long p = tile >> 3;
+    b = (tile & 7) + 8 * (p / T8);
+    tile = p % T8;
+    if (b >= g.batch) return;  // uniform per work-group
+  }
+  gemm_f64_body<AC, BC, AT, BT, WM, WN, TM, TN>(g, tiles_m, tiles_n, tile, b, (int)blockIdx.z, kchunk, ws);
+}
+
+// Grouped batched products: work-group i belongs to the job j with
+// prefix[j] <= i < prefix[j + 1]; within it, tile = local % tiles and batch
+// entry = local / tiles.  No K split.
+template <bool AC, bool BC, bool AT, bool BT>
+__global__ __launch_bounds__(256) void gemm_f64_grouped_kernel(const GemmJob* __restrict__ jobs,
+                                                               const int* __restrict__ prefix, int njobs) {
+  const int item = blockIdx.x;
+  const int j = group_find(prefix, njobs, item);
+  const Gemm g = jobs[j].g;
+  const int tm = jobs[j].tiles_m, tn = jobs[j].tiles_n;
+  const int local = item - prefix[j];
+  const int tiles = tm * tn;
+  gemm_f64_body<AC, BC, AT, BT, 2, 2, 2, 2>(g, tm, tn, local % tiles, local / tiles, 0, g.K, nullptr);
+}
+
 // C = alpha * sum_kz ws[kz] + beta * C  (fixed summation order)
 template <bool CC>
 __global__ void splitk_reduce_kernel(Gemm g, const double* ws, int ksplit) {
@@ -360,6 +383,34 @@ __global__ void splitk_reduce_kernel(Gemm g, const double* ws, int ksplit) {
     *Cr = r;
     if (g.D) static_cast<double*>(g.D)[b * g.sD + (long)m * g.ldd + n] += r;
   }
+}
+
+// Grouped complex reduction (batch 1 per job).
+__global__ void splitk_reduce_grouped_kernel(const ReduceJob* __restrict__ jobs, const int* __restrict__ prefix,
+                                             int njobs) {
+  const int item = blockIdx.x;
+  const int j = group_find(prefix, njobs, item);
+  const Gemm& g = jobs[j].g;
+  const double* ws = jobs[j].ws;
+  const int ksplit = jobs[j].ksplit;
+  const long plane = (long)g.M * g.N;
+  const long idx = (long)(item - prefix[j]) * 256 + threadIdx.x;
+  if (idx >= plane) return;
+  const int m = (int)(idx / g.N), n = (int)(idx % g.N);
+  double2 sum = {0.0, 0.0};
+  for (int z = 0; z < ksplit; ++z) {
+    const double2 v = reinterpret_cast<const double2*>(ws)[(long)z * plane + idx];
+    sum.x += v.x;
+    sum.y += v.y;
+  }
+  double2 r = cmul(g.alpha, sum);
+  double2* Cc = static_cast<double2*>(g.C) + (long)m * g.ldc + n;
+  if (g.beta.x != 0.0 || g.beta.y != 0.0) {
+    const double2 o = cmul(g.beta, *Cc);
+    r.x += o.x;
+    r.y += o.y;
+  }
+  *Cc = r;
 }
 
 template <bool AC, bool BC, bool AT, bool BT, int WM, int WN, int TM, int TN>
@@ -572,6 +623,41 @@ int gemm_splitk_reduce(const Gemm& g, const double* ws, int ksplit, bool cplx, h
   else
     hipLaunchKernelGGL((splitk_reduce_kernel<false>), rg, dim3(256), 0, s, g, ws, ksplit);
   return launch_status("splitk_reduce_kernel");
+}
+
+GemmJob gemm_group_job(const Gemm& g) {
+  GemmJob j;
+  j.g = g;
+  j.tiles_m = (g.M + 63) / 64;
+  j.tiles_n = (g.N + 63) / 64;
+  return j;
+}
+
+int gemm_f64_grouped(const GroupTab<GemmJob>& t, bool a_cplx, bool b_cplx, bool a_trans, bool b_trans,
+                     hipStream_t s) {
+  if (t.njobs < 1 || t.total < 1) return AFQ_OK;
+  const dim3 grid((unsigned)t.total), blk(256);
+#define GG(AC, BC, AT, BT) \
+  hipLaunchKernelGGL((gemm_f64_grouped_kernel<AC, BC, AT, BT>), grid, blk, 0, s, t.params, t.prefix, t.njobs)
+  if (!a_cplx && b_cplx && a_trans && !b_trans)
+    GG(false, true, true, false);
+  else if (a_cplx && b_cplx && a_trans && b_trans)
+    GG(true, true, true, true);
+  else if (a_cplx && b_cplx && a_trans && !b_trans)
+    GG(true, true, true, false);
+  else if (a_cplx && b_cplx && !a_trans && !b_trans)
+    GG(true, true, false, false);
+  else
+    return fail(AFQ_E_UNSUPPORTED, "gemm_f64_grouped: operand form");
+#undef GG
+  return launch_status("gemm_f64_grouped_kernel");
+}
+
+int splitk_reduce_grouped(const GroupTab<ReduceJob>& t, hipStream_t s) {
+  if (t.njobs < 1 || t.total < 1) return AFQ_OK;
+  hipLaunchKernelGGL(splitk_reduce_grouped_kernel, dim3((unsigned)t.total), dim3(256), 0, s, t.params, t.prefix,
+                     t.njobs);
+  return launch_status("splitk_reduce_grouped_kernel");
 }
 
 int gemm_f64(const Gemm& g, hipStream_t s) {

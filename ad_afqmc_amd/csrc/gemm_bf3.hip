@@ -16,6 +16,7 @@
 // force bias: few output tiles, a long K).
 #include "afq_internal.h"
 #include "gemm_bf3.h"
+#include "group.h"
 
 typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
 
@@ -36,25 +37,21 @@ __device__ inline v4f mma(v8bf a, v8bf b, v4f c) { return __builtin_amdgcn_mfma_
 constexpr int LROW = 72;  // bf16 per LDS row: hi (32), lo (32), pad (8)
 
 template <int BMR, int BN, int WM, int WN>
-__global__ __launch_bounds__(64 * WM * WN) void gemm_bf3_lds_kernel(Bf3 p, int tiles_m, int tiles_n, int xcd_remap,
-                                                                     long T8, int kchunk, double* ws) {
+__device__ __forceinline__ void gemm_bf3_body(const Bf3& p, int tiles_m, int tiles_n, long tile, long b, int kz,
+                                              int kchunk, double* ws, __bf16* __restrict__ lds) {
   constexpr int NT = 64 * WM * WN;
   constexpr int TM = BMR / (16 * WM), TN = BN / (16 * WN);
   constexpr int CA = BMR * 8, CB = BN * 8;  // 16-byte chunks per stage
   constexpr int RA = (CA + NT - 1) / NT, RB = (CB + NT - 1) / NT;
   static_assert(TM * 16 * WM == BMR && TN * 16 * WN == BN, "tile");
-  extern __shared__ __align__(16) __bf16 lds[];
   __bf16* As = lds;                      // [2][BMR][LROW]
   __bf16* Bs = lds + 2 * BMR * LROW;     // [2][BN][LROW]
 
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wm = wave / WN, wn = wave % WN;
-  long tile = blockIdx.x;
-  if (xcd_remap) tile = (tile & 7) * T8 + (tile >> 3);
   const int mt = (int)(tile % tiles_m);
   const long ntl = tile / tiles_m;
   if (ntl >= tiles_n) return;  // uniform per work-group
-  const long b = blockIdx.y;
   const int m0r = mt * BMR;  // first real row
   const int n0 = (int)ntl * BN;
   const __bf16* Ab = reinterpret_cast<const __bf16*>(p.A) + b * p.sA;
@@ -90,7 +87,6 @@ __global__ __launch_bounds__(64 * WM * WN) void gemm_bf3_lds_kernel(Bf3 p, int t
 #pragma unroll
     for (int j = 0; j < TN; ++j) acc[i][j] = v4f{0.f, 0.f, 0.f, 0.f};
 
-  const int kz = blockIdx.z;
   const int ks0 = kz * kchunk / 32, nk = min(p.K, (kz + 1) * kchunk) / 32;
   v8bf ra[RA], rb[RB];
   auto fetch = [&](int ks) {
@@ -171,6 +167,28 @@ __global__ __launch_bounds__(64 * WM * WN) void gemm_bf3_lds_kernel(Bf3 p, int t
         const double orr = p.alpha.x * xr - p.alpha.y * xi, oi = p.alpha.x * xi + p.alpha.y * xr;
         C[(long)m * p.ldc + n] = float2{(float)orr, (float)oi};
       }
+}
+
+template <int BMR, int BN, int WM, int WN>
+__global__ __launch_bounds__(64 * WM * WN) void gemm_bf3_lds_kernel(Bf3 p, int tiles_m, int tiles_n, int xcd_remap,
+                                                                     long T8, int kchunk, double* ws) {
+  extern __shared__ __align__(16) __bf16 lds_bf3[];
+  long tile = blockIdx.x;
+  if (xcd_remap) tile = (tile & 7) * T8 + (tile >> 3);
+  gemm_bf3_body<BMR, BN, WM, WN>(p, tiles_m, tiles_n, tile, blockIdx.y, blockIdx.z, kchunk, ws, lds_bf3);
+}
+
+// Grouped (batch-1 products): work-group i belongs to job j with
+// prefix[j] <= i < prefix[j + 1]; local index = tile + tiles * kz.
+__global__ __launch_bounds__(512) void gemm_bf3_grouped_kernel(const Bf3Job* __restrict__ jobs,
+                                                               const int* __restrict__ prefix, int njobs) {
+  extern __shared__ __align__(16) __bf16 lds_bf3[];
+  const int item = blockIdx.x;
+  const int j = group_find(prefix, njobs, item);
+  const Bf3Job J = jobs[j];
+  const int local = item - prefix[j];
+  const int tiles = J.tiles_m * J.tiles_n;
+  gemm_bf3_body<128, 128, 2, 4>(J.p, J.tiles_m, J.tiles_n, local % tiles, 0, local / tiles, J.kchunk, J.ws, lds_bf3);
 }
 
 // K slices for a complex128 output: split until about 1024 work-groups are in
@@ -300,6 +318,41 @@ int gemm_bf3(const Bf3& p, hipStream_t s) {
   // bias (0.17 and 0.14 ms), against 1.12 / 0.56 / 0.29 / 0.25 ms for fragments
   // read straight from global memory
   return launch_lds<128, 128, 2, 4>(p, s);
+}
+
+int bf3_group_job(const Bf3& p, Bf3Job* out) {
+  if (p.batch != 1) return fail(AFQ_E_INVALID, "gemm_bf3 grouped: batch-1 products only");
+  if (p.K <= 0 || p.K % 32 != 0) return fail(AFQ_E_INVALID, "gemm_bf3: K must be a positive multiple of 32");
+  Bf3Job j;
+  j.p = p;
+  j.tiles_m = (2 * p.M + 127) / 128;
+  j.tiles_n = (p.N + 127) / 128;
+  j.ksplit = 1;
+  j.kchunk = p.K;
+  j.ws = nullptr;
+  if (p.c_f64) {
+    const int ks = bf3_ksplit(p, (long)j.tiles_m * j.tiles_n, &j.kchunk);
+    if (ks < 1) return ks;
+    j.ksplit = ks;
+    j.ws = static_cast<double*>(p.ws);
+  }
+  *out = j;
+  return AFQ_OK;
+}
+
+int gemm_bf3_grouped(const GroupTab<Bf3Job>& t, hipStream_t s) {
+  if (t.njobs < 1 || t.total < 1) return AFQ_OK;
+  const size_t lds = sizeof(__bf16) * 2 * (128 + 128) * LROW;
+  static bool raised = false;
+  if (!raised && lds > 65536) {
+    hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(gemm_bf3_grouped_kernel),
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (e != hipSuccess) return fail(AFQ_E_HIP, std::string("gemm_bf3 LDS attribute: ") + hipGetErrorString(e));
+    raised = true;
+  }
+  hipLaunchKernelGGL(gemm_bf3_grouped_kernel, dim3((unsigned)t.total), dim3(512), lds, s, t.params, t.prefix,
+                     t.njobs);
+  return launch_status("gemm_bf3_grouped_kernel");
 }
 
 int k_split_rows_bf16(const double* src, bool cplx, long rows, int K, long lds, int kpad, long ldd, long pd,

@@ -9,6 +9,7 @@
 //   * CGS2 re-orthonormalisation of an N x O walker panel, kept for the
 //     job-free walker entry point, where no Gram or triangle workspace exists.
 #include "afq_internal.h"
+#include "group.h"
 
 namespace afq {
 namespace {
@@ -35,19 +36,16 @@ __device__ inline double2 crecip(double2 a) {
 // pivots, hence the determinant, are the same).
 // blockIdx.y selects one of up to two matrix sets (the two spins' overlap
 // matrices factor in one launch, so a CU holds two work-groups at W = 256).
-__global__ __launch_bounds__(256) void gj_det_inv_kernel(DetInvSet s0, DetInvSet s1) {
-  extern __shared__ double2 a[];
+__device__ __forceinline__ void gj_det_inv_body(const DetInvSet& set, long b, double2* __restrict__ a) {
   __shared__ double2 prow[64];
   __shared__ double2 fcol[64];
   __shared__ int piv[64];
   __shared__ int pk;
-  const DetInvSet& set = blockIdx.y == 0 ? s0 : s1;
   const int n = set.n;
   double2* __restrict__ det = set.det;
   double2* __restrict__ inv = set.inv;
   const int tid = threadIdx.x;
   const int j = tid & 63, rg = tid >> 6;
-  const long b = blockIdx.x;
   const double2* src = set.S + b * set.strideS;
   const int nn = n * n;
   const bool want_inv = inv != nullptr;
@@ -146,6 +144,21 @@ __global__ __launch_bounds__(256) void gj_det_inv_kernel(DetInvSet s0, DetInvSet
   }
 }
 
+__global__ __launch_bounds__(256) void gj_det_inv_kernel(DetInvSet s0, DetInvSet s1) {
+  extern __shared__ double2 lds_gj[];
+  gj_det_inv_body(blockIdx.y == 0 ? s0 : s1, blockIdx.x, lds_gj);
+}
+
+// Grouped: work-group i factors matrix i - prefix[j] of job j's set.
+__global__ __launch_bounds__(256) void gj_det_inv_grouped_kernel(const DetInvSet* __restrict__ sets,
+                                                                 const int* __restrict__ prefix, int njobs) {
+  extern __shared__ double2 lds_gj[];
+  const int item = blockIdx.x;
+  const int j = group_find(prefix, njobs, item);
+  const DetInvSet set = sets[j];
+  gj_det_inv_body(set, item - prefix[j], lds_gj);
+}
+
 __device__ inline double wave_sum(double v) {
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
@@ -208,12 +221,11 @@ __global__ __launch_bounds__(256) void cgs2_kernel(double2* __restrict__ Wp, lon
 // Hermitian positive-definite n x n matrices (n <= 64), one 256-thread
 // work-group per matrix in dynamic LDS; thread t owns column t % 64 for the
 // rows t/64, t/64 + 4, ...  The strict lower triangle of R is written as zero.
-__global__ __launch_bounds__(256) void cholesky_upper_kernel(const double2* __restrict__ S, long strideS, int n,
-                                                             double2* __restrict__ R, long strideR) {
-  extern __shared__ double2 a[];
+__device__ __forceinline__ void cholesky_upper_body(const double2* __restrict__ S, long strideS, int n,
+                                                    double2* __restrict__ R, long strideR, long b,
+                                                    double2* __restrict__ a) {
   const int tid = threadIdx.x;
   const int j = tid & 63, rg = tid >> 6;
-  const long b = blockIdx.x;
   const double2* src = S + b * strideS;
   const int nn = n * n;
   for (int i = tid; i < nn; i += 256) a[i] = src[i];
@@ -245,9 +257,42 @@ __global__ __launch_bounds__(256) void cholesky_upper_kernel(const double2* __re
   }
 }
 
+__global__ __launch_bounds__(256) void cholesky_upper_kernel(const double2* __restrict__ S, long strideS, int n,
+                                                             double2* __restrict__ R, long strideR) {
+  extern __shared__ double2 lds_ch[];
+  cholesky_upper_body(S, strideS, n, R, strideR, blockIdx.x, lds_ch);
+}
+
+__global__ __launch_bounds__(256) void cholesky_grouped_kernel(const CholSet* __restrict__ sets,
+                                                              const int* __restrict__ prefix, int njobs) {
+  extern __shared__ double2 lds_ch[];
+  const int item = blockIdx.x;
+  const int j = group_find(prefix, njobs, item);
+  const CholSet c = sets[j];
+  cholesky_upper_body(c.S, c.strideS, c.n, c.R, c.strideR, item - prefix[j], lds_ch);
+}
+
 __global__ void conj_kernel(const double2* __restrict__ x, double2* __restrict__ y, long n) {
   const long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
   if (i < n) y[i] = double2{x[i].x, -x[i].y};
+}
+
+// Grouped copies / conjugate copies: work-group i covers elements
+// [(i - prefix[j]) * 1024, +1024) of job j's vector.
+__global__ __launch_bounds__(256) void vec_grouped_kernel(const VecJob* __restrict__ jobs,
+                                                          const int* __restrict__ prefix, int njobs) {
+  const int item = blockIdx.x;
+  const int j = group_find(prefix, njobs, item);
+  const VecJob v = jobs[j];
+  const long base = (long)(item - prefix[j]) * 1024;
+#pragma unroll
+  for (int t = 0; t < 4; ++t) {
+    const long i = base + t * 256 + threadIdx.x;
+    if (i < v.n) {
+      const double2 x = v.src[i];
+      v.dst[i] = v.conj ? double2{x.x, -x.y} : x;
+    }
+  }
 }
 
 }  // namespace
@@ -275,6 +320,22 @@ int batched_det_inv_sets(const DetInvSet* sets, int nsets, int count, hipStream_
   return launch_status("gj_det_inv_kernel");
 }
 
+int det_inv_grouped(const GroupTab<DetInvSet>& t, int nmax, hipStream_t stream) {
+  if (t.njobs < 1 || t.total < 1) return AFQ_OK;
+  if (nmax < 1 || nmax > 64) return fail(AFQ_E_UNSUPPORTED, "det/inverse kernel supports 1 <= nocc <= 64");
+  const size_t lds = sizeof(double2) * nmax * nmax;
+  static bool raised = false;
+  if (!raised && lds + 2048 > 65536) {
+    hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(gj_det_inv_grouped_kernel),
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, 96 * 1024);
+    if (e != hipSuccess) return fail(AFQ_E_HIP, std::string("det/inverse LDS attribute: ") + hipGetErrorString(e));
+    raised = true;
+  }
+  hipLaunchKernelGGL(gj_det_inv_grouped_kernel, dim3((unsigned)t.total), dim3(256), lds, stream, t.params, t.prefix,
+                     t.njobs);
+  return launch_status("gj_det_inv_grouped_kernel");
+}
+
 int batched_det_inv(const double2* S, long strideS, int n, int count, double2* det, double2* inv, long strideInv,
                     hipStream_t stream) {
   const DetInvSet set{S, strideS, n, det, inv, strideInv};
@@ -297,6 +358,28 @@ int batched_cholesky_upper(const double2* S, long strideS, int n, int count, dou
   }
   hipLaunchKernelGGL(cholesky_upper_kernel, dim3(count), dim3(256), lds, stream, S, strideS, n, R, strideR);
   return launch_status("cholesky_upper_kernel");
+}
+
+int cholesky_grouped(const GroupTab<CholSet>& t, int nmax, hipStream_t stream) {
+  if (t.njobs < 1 || t.total < 1) return AFQ_OK;
+  if (nmax < 1 || nmax > 64) return fail(AFQ_E_UNSUPPORTED, "Cholesky kernel supports 1 <= n <= 64");
+  const size_t lds = sizeof(double2) * nmax * nmax;
+  static bool raised = false;
+  if (!raised && lds > 32768) {
+    hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(cholesky_grouped_kernel),
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, 64 * 1024);
+    if (e != hipSuccess) return fail(AFQ_E_HIP, std::string("Cholesky LDS attribute: ") + hipGetErrorString(e));
+    raised = true;
+  }
+  hipLaunchKernelGGL(cholesky_grouped_kernel, dim3((unsigned)t.total), dim3(256), lds, stream, t.params, t.prefix,
+                     t.njobs);
+  return launch_status("cholesky_grouped_kernel");
+}
+
+int vec_grouped(const GroupTab<VecJob>& t, hipStream_t stream) {
+  if (t.njobs < 1 || t.total < 1) return AFQ_OK;
+  hipLaunchKernelGGL(vec_grouped_kernel, dim3((unsigned)t.total), dim3(256), 0, stream, t.params, t.prefix, t.njobs);
+  return launch_status("vec_grouped_kernel");
 }
 
 int k_conj(const double2* x, double2* y, long n, hipStream_t stream) {

@@ -40,7 +40,7 @@ class JobSpec:
     seed: int = 0
     precision: str = "fp64"
     arrival: float = 0.0  # seconds after the start of the stream
-    l_scale: Optional[float] = None  # Cholesky scale (default 1/sqrt(norb))
+    l_scale: Optional[float] = None  # Cholesky scale (default 0.3/sqrt(norb): a live walker population)
 
 
 @dataclass
@@ -228,6 +228,9 @@ class GpuExecutor:
     def _block(self, h):
         h["job"].sr_block(h["up"], None, h["weights"], h["overlaps"], h["scalars"], self.n_prop_steps,
                           h["spec"].seed, h["counter"], h["out"])
+        # the driver's energy estimate follows the block energy (driver.py:168):
+        # e_estimate <- 0.9 e_estimate + 0.1 E_block, on the device
+        h["scalars"][:1].mul_(0.9).add_(h["out"][:1], alpha=0.1)
 
     def issue(self, h):
         torch = self.torch
@@ -265,7 +268,10 @@ class GpuExecutor:
 
 def _device_system(spec: JobSpec):
     """The synthetic generator of ad_afqmc_amd.synthetic, drawn on the device
-    from torch's generator seeded with the job seed (fast for large jobs)."""
+    from torch's generator seeded with the job seed (fast for large jobs), with
+    the Cholesky-vector scale s_g multiplied by spec.l_scale (0.3/sqrt(N) by
+    default; at the unscaled generator every walker is clipped within a few
+    steps)."""
     import torch
 
     N, O, G = spec.norb, spec.nocc, spec.nchol
@@ -273,7 +279,7 @@ def _device_system(spec: JobSpec):
     dev = torch.device("cuda")
     x = torch.randn((N, N), generator=gen, dtype=torch.float64, device=dev)
     h1 = torch.diag(torch.linspace(-2.0, 1.0, N, dtype=torch.float64, device=dev)) + 0.05 * (x + x.T) / 2
-    ls = spec.l_scale if spec.l_scale is not None else 1.0 / math.sqrt(N)
+    ls = spec.l_scale if spec.l_scale is not None else 0.3 / math.sqrt(N)
     sig = ls * 0.5 * torch.exp(-torch.arange(G, dtype=torch.float64, device=dev) / (0.2 * G))
     a = torch.randn((G, N, N), generator=gen, dtype=torch.float64, device=dev) * sig.view(G, 1, 1)
     chol = ((a + a.transpose(1, 2)) / 2).reshape(G, N * N).contiguous()

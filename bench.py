@@ -327,9 +327,11 @@ def _new_job(system, name, precision, nsteps):
 
 def _sr_block(job, st, nsteps, seed):
     """One device block + comb resampling + overlaps of the new population
-    (afq_sr_block); returns a copy of {block_energy, block_weight}."""
+    (afq_sr_block), then the driver's e_estimate <- 0.9 e_estimate + 0.1 E_block
+    (driver.py:168) on the device; returns a copy of {block_energy, block_weight}."""
     job.sr_block(st["up"], st["dn"], st["weights"], st["overlaps"], st["scalars"], nsteps, seed, st["counter"],
                  st["out"])
+    st["scalars"][:1].mul_(0.9).add_(st["out"][:1], alpha=0.1)
     return st["out"].clone()
 
 
@@ -357,10 +359,7 @@ def precision_study(cfg_name: str, n_blocks: int, n_eql: int, l_coef: float = 0.
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         for i in range(n_eql + n_blocks):
-            o = _sr_block(job, st, nsteps, 99)
-            blocks.append(o)
-            # e_estimate follows the block energy as in the reference driver (driver.py:168)
-            st["scalars"][0] = 0.9 * st["scalars"][0] + 0.1 * o[0]
+            blocks.append(_sr_block(job, st, nsteps, 99))
         torch.cuda.synchronize()
         t = time.perf_counter() - t0
         b = torch.stack(blocks).cpu().numpy()[n_eql:]
@@ -418,7 +417,6 @@ def _restart_paired(s, cfg_name, n_blocks, n_eql):
             for p in ("fp32", "bf16x3"):
                 o2 = _sr_block(jobs[p], _clone_state(snap), nsteps, 99)
                 blk[p].append(float(o2[0]))
-        st["scalars"][0] = 0.9 * st["scalars"][0] + 0.1 * o[0]
     out = {}
     for p in ("fp32", "bf16x3"):
         res = {}
