@@ -93,6 +93,9 @@ _SIGNATURES = {
     "afq_walkers_sr_noalloc": ([P, C.c_int64, P, C.c_int64, C.c_int32, P, P, P, P], C.c_int),
     "afq_walkers_sr_noalloc_max_walkers": ([], C.c_int64),
     "afq_sr_block": ([P, P, P, P, P, P, C.c_int, C.c_uint64, P, P, P], C.c_int),
+    "afq_group_create": ([P, P, C.c_int32, P, P, P, P, P, P, P, C.c_int32], C.c_int),
+    "afq_group_sr_block": ([P, P], C.c_int),
+    "afq_group_destroy": ([P], C.c_int),
     "afq_job_reserve_fields": ([P, C.c_int32, P], C.c_int),
     "afq_segment_energy_vjp": ([P, P, P, C.c_double, P, C.c_int32, C.c_int32, P, P, P, P], C.c_int),
 }

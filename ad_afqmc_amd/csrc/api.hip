@@ -414,8 +414,7 @@ int trotprop_lp(afq_job* j, void* up, void* dn, const double2* xs, hipStream_t s
     }
     f.vs = j->vsplit;
     ProfScope ps(j, AFQ_K_TAYLOR, st);
-    AFQ_TRY(k_split_vhs_bf16(j->sym_chol ? j->vhsp32 : j->vhs32, j->sym_chol ? j->npk : NN, j->sym_chol, N, W,
-                             j->vsplit, st));
+    AFQ_TRY(k_split_vhs_bf16(j->sym_chol ? j->vhsp32 : j->vhs32, j->sym_chol, N, W, j->vsplit, st));
     return trot_fused_bf3(f, st);
   }
 #endif
@@ -887,6 +886,15 @@ int afq_job_create(afq_job_t* out, const afq_job_desc* d) {
     release(j);
     delete j;
     return rc;
+  }
+  if (j->vsplit) {
+    // plane entries past N are never written and must read as zeros
+    const hipError_t e = hipMemset(j->vsplit, 0, sizeof(uint16_t) * trot_fused_vhs_elems(j->N, j->Wcap));
+    if (e != hipSuccess) {
+      release(j);
+      delete j;
+      return fail(AFQ_E_HIP, std::string("hipMemset: ") + hipGetErrorString(e));
+    }
   }
   if (j->rhf) {
     j->mo[1] = j->mo[0];

@@ -18,7 +18,7 @@ PKG = os.path.dirname(HERE)
 REPO = os.path.dirname(PKG)
 INCLUDE = os.path.join(REPO, "include")
 BUILD = os.path.join(REPO, "_build")
-SOURCES = ["gemm.hip", "gemm_lp.hip", "linalg.hip", "kernels.hip", "exchange.hip", "api.hip", "adjoint.hip", "gemm_bf3.hip", "noalloc.hip", "trot_fused.hip"]
+SOURCES = ["gemm.hip", "gemm_lp.hip", "linalg.hip", "kernels.hip", "exchange.hip", "api.hip", "adjoint.hip", "gemm_bf3.hip", "noalloc.hip", "trot_fused.hip", "group.hip"]
 LIB = os.path.join(PKG, "libafq.so")
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

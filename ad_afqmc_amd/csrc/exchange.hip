@@ -28,6 +28,7 @@
 #include <type_traits>
 
 #include "kernels.h"
+#include "group.h"
 
 typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
 
@@ -49,6 +50,9 @@ __device__ inline void wave_lds_sync() {
   __builtin_amdgcn_wave_barrier();
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
+
+__host__ __device__ inline int bf16_opad_d(int O) { return 16 * ((O + 15) / 16); }
+__host__ __device__ inline int bf16_npad_d(int N) { return 32 * ((N + 31) / 32); }
 
 // Permutation of the four 16-byte k-groups of LDS row r: rows r and r' < 16
 // of a fragment then occupy distinct 16-byte bank groups both within every 8
@@ -421,10 +425,9 @@ __global__ __launch_bounds__(512, 1) void exchange_split_kernel(const double* __
 // lo*hi, hi*lo are issued as three passes over all NB x NB blocks, so
 // consecutive MFMAs never depend on each other.
 template <int NB>
-__global__ __launch_bounds__(256, 2) void exchange_bf16_kernel(const v8bf* __restrict__ rcs, const v8bf* __restrict__ gs,
-                                                               int O, int Opad, int Npad, int Gn, int W, int nchunk,
-                                                               int cpg, long T8, double2* __restrict__ coul,
-                                                               double2* __restrict__ kpart) {
+__device__ __forceinline__ void exchange_bf16_body(const v8bf* __restrict__ rcs, const v8bf* __restrict__ gs, int O,
+                                                   int Opad, int Npad, int Gn, int nchunk, double2* __restrict__ coul,
+                                                   double2* __restrict__ kpart, int w, int chunk) {
   constexpr int ROWS = NB * 16;
   constexpr int PER = 4 * ROWS * 4;       // v8bf of one step's B stage: 4 planes x ROWS x 4 k-groups
   constexpr int LD = (PER + 255) / 256;   // v8bf loaded per thread per step
@@ -432,14 +435,6 @@ __global__ __launch_bounds__(256, 2) void exchange_bf16_kernel(const v8bf* __res
   __shared__ double2 scr[4][16 * 17];
   __shared__ double2 kred[4];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const long bid = blockIdx.x;
-  const long L = (bid & 7) * T8 + (bid >> 3);
-  const long per_group = (long)W * cpg;
-  const long grp = L / per_group;
-  const long rem = L - grp * per_group;
-  const int w = (int)(rem / cpg);
-  const int chunk = (int)(grp * cpg + rem % cpg);
-  if (chunk >= nchunk) return;  // uniform for the whole work-group
   const int g = chunk * 4 + wave;
   const bool valid = g < Gn;
   const int gg = valid ? g : Gn - 1;
@@ -604,12 +599,40 @@ __global__ __launch_bounds__(256, 2) void exchange_bf16_kernel(const v8bf* __res
   }
 }
 
+template <int NB>
+__global__ __launch_bounds__(256, 2) void exchange_bf16_kernel(const v8bf* __restrict__ rcs, const v8bf* __restrict__ gs,
+                                                               int O, int Opad, int Npad, int Gn, int W, int nchunk,
+                                                               int cpg, long T8, double2* __restrict__ coul,
+                                                               double2* __restrict__ kpart) {
+  const long bid = blockIdx.x;
+  const long L = (bid & 7) * T8 + (bid >> 3);
+  const long per_group = (long)W * cpg;
+  const long grp = L / per_group;
+  const long rem = L - grp * per_group;
+  const int w = (int)(rem / cpg);
+  const int chunk = (int)(grp * cpg + rem % cpg);
+  if (chunk >= nchunk) return;  // uniform for the whole work-group
+  exchange_bf16_body<NB>(rcs, gs, O, Opad, Npad, Gn, nchunk, coul, kpart, w, chunk);
+}
+
+// Grouped: job j owns work-groups [prefix[j], prefix[j + 1]), chunk-major
+// (consecutive work-groups share a rot_chol chunk across walkers).
+template <int NB>
+__global__ __launch_bounds__(256, 2) void exchange_bf16_grouped_kernel(const ExchJob* __restrict__ jobs,
+                                                                       const int* __restrict__ prefix, int njobs) {
+  const int item = blockIdx.x;
+  const int j = group_find(prefix, njobs, item);
+  const ExchJob e = jobs[j];
+  const int local = item - prefix[j];
+  exchange_bf16_body<NB>(reinterpret_cast<const v8bf*>(e.rcs), reinterpret_cast<const v8bf*>(e.gs), e.O, e.Opad,
+                         e.Npad, e.Gn, e.nchunk, e.coul, e.kpart, local % e.W, local / e.W);
+}
+
 // out[r][plane][o][k] (o < Opad, k < Npad, zero padded) from src[r][o][k]
 // (o < O, k < N; real or interleaved complex double): planes hi, lo of
 // bf16(float(x)) and bf16(float(x) - hi), for the real then the imaginary part.
-__global__ void split_bf16_kernel(const double* __restrict__ src, int cplx, int O, int N, int Opad, int Npad,
-                                  long rows, __bf16* __restrict__ out) {
-  const long idx = blockIdx.x * (long)blockDim.x + threadIdx.x;
+__device__ __forceinline__ void split_bf16_body(const double* __restrict__ src, int cplx, int O, int N, int Opad,
+                                                int Npad, long rows, __bf16* __restrict__ out, long idx) {
   const long per = (long)Opad * Npad;
   if (idx >= rows * per) return;
   const long r = idx / per;
@@ -625,6 +648,20 @@ __global__ void split_bf16_kernel(const double* __restrict__ src, int cplx, int 
     base[(2 * c) * per] = h;
     base[(2 * c + 1) * per] = (__bf16)(xf - (float)h);
   }
+}
+
+__global__ void split_bf16_kernel(const double* __restrict__ src, int cplx, int O, int N, int Opad, int Npad,
+                                  long rows, __bf16* __restrict__ out) {
+  split_bf16_body(src, cplx, O, N, Opad, Npad, rows, out, blockIdx.x * (long)blockDim.x + threadIdx.x);
+}
+
+__global__ void split_bf16_grouped_kernel(const SplitJob* __restrict__ jobs, const int* __restrict__ prefix,
+                                          int njobs) {
+  const int item = blockIdx.x;
+  const int j = group_find(prefix, njobs, item);
+  const SplitJob J = jobs[j];
+  split_bf16_body(J.src, J.cplx, J.O, J.N, bf16_opad_d(J.O), bf16_npad_d(J.N), J.rows,
+                  reinterpret_cast<__bf16*>(J.out), (long)(item - prefix[j]) * 256 + threadIdx.x);
 }
 
 template <int MODE, int NB, int KS, typename TR>
@@ -702,8 +739,19 @@ int k_exchange_lp(const float* rc, const double2* Gm, int O, int N, int Gn, int 
   return launch_mode<F32, float>(rc, Gm, O, N, Gn, W, nchunk, coul, kpart, s);
 }
 
-int bf16_opad(int O) { return 16 * ((O + 15) / 16); }
-int bf16_npad(int N) { return 32 * ((N + 31) / 32); }
+int bf16_opad(int O) { return bf16_opad_d(O); }
+int bf16_npad(int N) { return bf16_npad_d(N); }
+
+long split_items(const SplitJob& j) {
+  return (j.rows * bf16_opad_d(j.O) * (long)bf16_npad_d(j.N) + 255) / 256;
+}
+
+int split_bf16_grouped(const GroupTab<SplitJob>& t, hipStream_t s) {
+  if (t.njobs < 1 || t.total < 1) return AFQ_OK;
+  hipLaunchKernelGGL(split_bf16_grouped_kernel, dim3((unsigned)t.total), dim3(256), 0, s, t.params, t.prefix,
+                     t.njobs);
+  return launch_status("split_bf16_grouped_kernel");
+}
 
 int k_split_bf16(const double* src, bool cplx, int O, int N, long rows, uint16_t* out, hipStream_t s) {
   const int Opad = bf16_opad(O), Npad = bf16_npad(N);
@@ -712,6 +760,19 @@ int k_split_bf16(const double* src, bool cplx, int O, int N, long rows, uint16_t
   hipLaunchKernelGGL(split_bf16_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, src, cplx ? 1 : 0, O, N,
                      Opad, Npad, rows, reinterpret_cast<__bf16*>(out));
   return launch_status("split_bf16_kernel");
+}
+
+int exchange_bf16_grouped(const GroupTab<ExchJob>& t, int nb, hipStream_t s) {
+  if (t.njobs < 1 || t.total < 1) return AFQ_OK;
+  const dim3 grid((unsigned)t.total), blk(256);
+  switch (nb) {
+    case 1: hipLaunchKernelGGL(exchange_bf16_grouped_kernel<1>, grid, blk, 0, s, t.params, t.prefix, t.njobs); break;
+    case 2: hipLaunchKernelGGL(exchange_bf16_grouped_kernel<2>, grid, blk, 0, s, t.params, t.prefix, t.njobs); break;
+    case 3: hipLaunchKernelGGL(exchange_bf16_grouped_kernel<3>, grid, blk, 0, s, t.params, t.prefix, t.njobs); break;
+    case 4: hipLaunchKernelGGL(exchange_bf16_grouped_kernel<4>, grid, blk, 0, s, t.params, t.prefix, t.njobs); break;
+    default: return fail(AFQ_E_UNSUPPORTED, "exchange kernel supports 1 <= nocc <= 64");
+  }
+  return launch_status("exchange_bf16_grouped_kernel");
 }
 
 int k_exchange_bf16(const uint16_t* rcs, const uint16_t* gs, int O, int N, int Gn, int W, double2* coul,

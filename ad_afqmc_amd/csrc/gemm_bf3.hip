@@ -246,9 +246,8 @@ int launch_lds(const Bf3& p, hipStream_t s) {
 // ------------------------------------------------------------------ operand preparation
 // dst[r][q][k] (k < kpad, rows r < rows) from a row-major complex (or real) double
 // matrix src[r][k] (k < K): bf16 planes, zero for k >= K.
-__global__ void split_rows_kernel(const double* __restrict__ src, int cplx, long rows, int K, long lds, int kpad,
-                                  long ldd, long pd, __bf16* __restrict__ dst) {
-  const long idx = blockIdx.x * (long)blockDim.x + threadIdx.x;
+__device__ __forceinline__ void split_rows_body(const double* __restrict__ src, int cplx, long rows, int K, long lds,
+                                                int kpad, long ldd, long pd, __bf16* __restrict__ dst, long idx) {
   if (idx >= rows * kpad) return;
   const long r = idx / kpad;
   const int k = (int)(idx % kpad);
@@ -272,6 +271,20 @@ __global__ void split_rows_kernel(const double* __restrict__ src, int cplx, long
     d[2 * pd] = hi;
     d[3 * pd] = (__bf16)(fi - (float)hi);
   }
+}
+
+__global__ void split_rows_kernel(const double* __restrict__ src, int cplx, long rows, int K, long lds, int kpad,
+                                  long ldd, long pd, __bf16* __restrict__ dst) {
+  split_rows_body(src, cplx, rows, K, lds, kpad, ldd, pd, dst, blockIdx.x * (long)blockDim.x + threadIdx.x);
+}
+
+__global__ void split_rows_grouped_kernel(const RowSplitJob* __restrict__ jobs, const int* __restrict__ prefix,
+                                          int njobs) {
+  const int item = blockIdx.x;
+  const int j = group_find(prefix, njobs, item);
+  const RowSplitJob J = jobs[j];
+  split_rows_body(J.src, J.cplx, J.rows, J.K, J.lds, J.kpad, J.ldd, J.pd, reinterpret_cast<__bf16*>(J.dst),
+                  (long)(item - prefix[j]) * 256 + threadIdx.x);
 }
 
 // L (G x NN, row-major doubles) -> Lt[n][q][g] (bf16 hi/lo planes, g < gpad,
@@ -362,6 +375,15 @@ int k_split_rows_bf16(const double* src, bool cplx, long rows, int K, long lds, 
   hipLaunchKernelGGL(split_rows_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, src, cplx ? 1 : 0, rows, K,
                      lds, kpad, ldd, pd, reinterpret_cast<__bf16*>(dst));
   return launch_status("split_rows_kernel");
+}
+
+long row_split_items(const RowSplitJob& j) { return (j.rows * (long)j.kpad + 255) / 256; }
+
+int split_rows_grouped(const GroupTab<RowSplitJob>& t, hipStream_t s) {
+  if (t.njobs < 1 || t.total < 1) return AFQ_OK;
+  hipLaunchKernelGGL(split_rows_grouped_kernel, dim3((unsigned)t.total), dim3(256), 0, s, t.params, t.prefix,
+                     t.njobs);
+  return launch_status("split_rows_grouped_kernel");
 }
 
 int k_split_chol_t_bf16(const double* L, int G, long NN, long ncols, int sym_N, int gpad, long ldd, long pd,

@@ -90,4 +90,93 @@ struct ReduceJob {
 };
 int splitk_reduce_grouped(const GroupTab<ReduceJob>& t, hipStream_t s);
 
+// Split-BF16 x3 exchange of each job (k_exchange_bf16's operands: rot_chol
+// planes rcs, Green's-function planes gs); work items: W * nchunk, all jobs of
+// one launch with ceil(O / 16) = nb.
+struct ExchJob {
+  const uint16_t* rcs;
+  const uint16_t* gs;
+  int O, Opad, Npad, Gn, W, nchunk;
+  double2* coul;
+  double2* kpart;
+};
+int exchange_bf16_grouped(const GroupTab<ExchJob>& t, int nb, hipStream_t s);
+
+// k_split_bf16 of each job's rows x (O, N) operand; work items: split_items.
+struct SplitJob {
+  const double* src;
+  int cplx, O, N;
+  long rows;
+  uint16_t* out;
+};
+long split_items(const SplitJob& j);
+int split_bf16_grouped(const GroupTab<SplitJob>& t, hipStream_t s);
+
+// k_split_rows_bf16 of each job's rows; work items: row_split_items.
+struct RowSplitJob {
+  const double* src;
+  int cplx;
+  long rows;
+  int K;
+  long lds;
+  int kpad;
+  long ldd, pd;
+  uint16_t* dst;
+};
+long row_split_items(const RowSplitJob& j);
+int split_rows_grouped(const GroupTab<RowSplitJob>& t, hipStream_t s);
+
+// Everything the small per-walker and per-job kernels of a phaseless block
+// read or write for one job (restricted walkers of an RHF trial).
+struct BlockJob {
+  int N, O, G, W;  // orbitals, walker columns, Cholesky vectors, walkers
+  int nsteps;      // phaseless steps per block
+  double dt, sqdt, h0, clip;
+  uint64_t seed;
+  // caller state
+  double2* walkers;
+  double* weights;
+  double2* overlaps;
+  double* scalars;  // e_estimate, pop_control_ene_shift
+  uint64_t* counter;
+  double* out;  // block energy, block weight
+  // job buffers
+  double* fields;  // (nsteps, W, G)
+  const double2* fb;
+  const double2* mf;
+  double2* xs;
+  double2* terms;
+  const double2* det;
+  double2* ov_new;
+  const double2* h0_prop;
+  const double2* green;
+  const double* rot_h1;
+  double2* ene1;
+  const double2* coul;
+  const double2* kpart;
+  int nch;
+  double2* energies;
+  double* cum;
+  int32_t* idx;
+  double* zeta;
+  double2* wgather;
+};
+// Work items of the grouped block kernels: WALKER = one per walker,
+// WALKER256 = one per 256 walkers, JOB = one per job, RNG = one per 256 field
+// pairs, GATHER = one per 1024 elements of each walker.
+enum BlockItems { BI_WALKER, BI_WALKER256, BI_JOB, BI_RNG, BI_GATHER, BI_NKINDS };
+long block_items(const BlockJob& j, int kind);
+// The block's small kernels over all jobs; `t` carries the prefix of the
+// kernel's item kind.
+int g_rng(const GroupTab<BlockJob>& t, hipStream_t s);                 // RNG: fields of the block
+int g_field_shift(const GroupTab<BlockJob>& t, int step, hipStream_t s);  // WALKER
+int g_overlap(const GroupTab<BlockJob>& t, bool to_overlaps, hipStream_t s);  // WALKER256: det^2 -> ov_new | overlaps
+int g_weight_update(const GroupTab<BlockJob>& t, hipStream_t s);       // JOB
+int g_ene1(const GroupTab<BlockJob>& t, hipStream_t s);                // WALKER
+int g_energy_finalize(const GroupTab<BlockJob>& t, hipStream_t s);     // WALKER
+int g_block_finalize(const GroupTab<BlockJob>& t, hipStream_t s);      // JOB
+int g_sr_index(const GroupTab<BlockJob>& t, hipStream_t s);            // JOB: zeta of the block counter, comb indices
+int g_gather(const GroupTab<BlockJob>& t, hipStream_t s);              // GATHER: walkers[idx] -> wgather
+int g_block_tail(const GroupTab<BlockJob>& t, hipStream_t s);          // JOB: counter + 1, e_estimate update
+
 }  // namespace afq

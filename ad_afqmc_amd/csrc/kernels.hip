@@ -1,6 +1,7 @@
 // Elementwise / reduction kernels of the walker step (FP64), and the fused
 // exchange-energy kernel.  Launch helpers are declared in kernels.h.
 #include "kernels.h"
+#include "group.h"
 
 namespace afq {
 namespace {
@@ -120,10 +121,9 @@ __global__ void overlap_combine_kernel(const double2* d0, const double2* d1, dou
 }
 
 // ene1[w] = sum_s f_s sum_{ij} G_s[w,i,j] rot_h1_s[i,j]
-__global__ void ene1_kernel(const double2* G0, const double* rh0, int n0, const double2* G1, const double* rh1, int n1,
-                            int N, double f0, double2* out) {
+__device__ __forceinline__ void ene1_body(const double2* G0, const double* rh0, int n0, const double2* G1,
+                                          const double* rh1, int n1, int N, double f0, double2* out, long w) {
   __shared__ double2 red[4];
-  const long w = blockIdx.x;
   double2 acc = {0.0, 0.0};
   const long l0 = (long)n0 * N;
   const double2* g0 = G0 + w * l0;
@@ -143,11 +143,16 @@ __global__ void ene1_kernel(const double2* G0, const double* rh0, int n0, const 
   if (threadIdx.x == 0) out[w] = acc;
 }
 
+__global__ void ene1_kernel(const double2* G0, const double* rh0, int n0, const double2* G1, const double* rh1, int n1,
+                            int N, double f0, double2* out) {
+  ene1_body(G0, rh0, n0, G1, rh1, n1, N, f0, out, blockIdx.x);
+}
+
 // E = h0 + ene1 + 1/2 sum_g (c_up + c_dn)^2 - 1/2 (K_up + K_dn)
-__global__ void energy_finalize_kernel(const double2* ene1, const double2* c0, const double2* c1, const double2* k0,
-                                       const double2* k1, int Gn, int nch0, int nch1, double h0, double2* out) {
+__device__ __forceinline__ void energy_finalize_body(const double2* ene1, const double2* c0, const double2* c1,
+                                                     const double2* k0, const double2* k1, int Gn, int nch0, int nch1,
+                                                     double h0, double2* out, long w) {
   __shared__ double2 red[4];
-  const long w = blockIdx.x;
   double2 J = {0.0, 0.0};
   for (int g = threadIdx.x; g < Gn; g += 256) {
     const double2 a = c0[w * Gn + g];
@@ -172,12 +177,16 @@ __global__ void energy_finalize_kernel(const double2* ene1, const double2* c0, c
   }
 }
 
+__global__ void energy_finalize_kernel(const double2* ene1, const double2* c0, const double2* c1, const double2* k0,
+                                       const double2* k1, int Gn, int nch0, int nch1, double h0, double2* out) {
+  energy_finalize_body(ene1, c0, c1, k0, k1, Gn, nch0, nch1, h0, out, blockIdx.x);
+}
+
 // ------------------------------------------------------------------ propagation
 // s = -sqrt(dt) (i fb - mf); xs = x - s; shift = sum xs*mf; fbt = sum x*s - s*s/2
-__global__ void field_shift_kernel(const double2* fb, const double* x, const double2* mf, int Gn, double sqdt,
-                                   double2* xs, double2* terms) {
+__device__ __forceinline__ void field_shift_body(const double2* fb, const double* x, const double2* mf, int Gn,
+                                                 double sqdt, double2* xs, double2* terms, long w) {
   __shared__ double2 red[4];
-  const long w = blockIdx.x;
   double2 sh = {0.0, 0.0}, ft = {0.0, 0.0};
   for (int g = threadIdx.x; g < Gn; g += 256) {
     const double2 f = fb[w * Gn + g];
@@ -201,11 +210,16 @@ __global__ void field_shift_kernel(const double2* fb, const double* x, const dou
   }
 }
 
+__global__ void field_shift_kernel(const double2* fb, const double* x, const double2* mf, int Gn, double sqdt,
+                                   double2* xs, double2* terms) {
+  field_shift_body(fb, x, mf, Gn, sqdt, xs, terms, blockIdx.x);
+}
+
 // Phaseless importance function, clipping, weight update and the
 // population-control shift (single work-group of 1024 threads).
-__global__ __launch_bounds__(1024) void weight_update_kernel(const double2* ov_new, double2* overlaps,
-                                                             const double2* terms, double* weights, double* scalars,
-                                                             const double2* h0_prop, double dt, int W) {
+__device__ __forceinline__ void weight_update_body(const double2* ov_new, double2* overlaps, const double2* terms,
+                                                   double* weights, double* scalars, const double2* h0_prop, double dt,
+                                                   int W) {
   __shared__ double red[16];
   const double sqdt = sqrt(dt);
   const double pcs = scalars[1];
@@ -239,10 +253,16 @@ __global__ __launch_bounds__(1024) void weight_update_kernel(const double2* ov_n
   }
 }
 
+__global__ __launch_bounds__(1024) void weight_update_kernel(const double2* ov_new, double2* overlaps,
+                                                             const double2* terms, double* weights, double* scalars,
+                                                             const double2* h0_prop, double dt, int W) {
+  weight_update_body(ov_new, overlaps, terms, weights, scalars, h0_prop, dt, W);
+}
+
 // Block bookkeeping after the step loop: energy clipping around e_estimate,
 // weighted block energy, and pop_control_ene_shift <- 0.9 shift + 0.1 E_block.
-__global__ __launch_bounds__(1024) void block_finalize_kernel(const double2* energies, const double* weights,
-                                                              double* scalars, double clip, int W, double* out) {
+__device__ __forceinline__ void block_finalize_body(const double2* energies, const double* weights, double* scalars,
+                                                    double clip, int W, double* out) {
   __shared__ double r1[16], r2[16];
   const double e_est = scalars[0];
   double sw = 0.0, swe = 0.0;
@@ -273,11 +293,15 @@ __global__ __launch_bounds__(1024) void block_finalize_kernel(const double2* ene
   }
 }
 
+__global__ __launch_bounds__(1024) void block_finalize_kernel(const double2* energies, const double* weights,
+                                                              double* scalars, double clip, int W, double* out) {
+  block_finalize_body(energies, weights, scalars, clip, W, out);
+}
+
 // ------------------------------------------------------------------ population control
 // Comb resampling (single work-group): cum = cumsum|w| (sequential per thread
 // chunk + block scan), z_i = total (i + zeta)/W, idx_i = first j with cum_j >= z_i.
-__global__ __launch_bounds__(1024) void sr_index_kernel(double* weights, const double* zeta, int W, double* cum,
-                                                        int32_t* idx) {
+__device__ __forceinline__ void sr_index_body(double* weights, const double* zeta, int W, double* cum, int32_t* idx) {
   __shared__ double part[1024];
   const int t = threadIdx.x, T = blockDim.x;
   const int chunk = (W + T - 1) / T;
@@ -320,6 +344,11 @@ __global__ __launch_bounds__(1024) void sr_index_kernel(double* weights, const d
   for (int i = t; i < W; i += T) weights[i] = avg;
 }
 
+__global__ __launch_bounds__(1024) void sr_index_kernel(double* weights, const double* zeta, int W, double* cum,
+                                                        int32_t* idx) {
+  sr_index_body(weights, zeta, W, cum, idx);
+}
+
 __global__ void gather_walkers_kernel(const double2* src, double2* dst, const int32_t* idx, long per) {
   const long w = blockIdx.y;
   const long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
@@ -343,8 +372,8 @@ __device__ inline uint4 philox4x32_10(uint4 c, uint2 k) {
 }
 
 // counter_dev (when not null) supplies the stream counter from device memory.
-__global__ void normal_fill_kernel(double* out, long n, uint64_t seed, uint64_t counter, const uint64_t* counter_dev) {
-  const long pair = blockIdx.x * (long)blockDim.x + threadIdx.x;
+__device__ __forceinline__ void normal_fill_body(double* out, long n, uint64_t seed, uint64_t counter,
+                                                 const uint64_t* counter_dev, long pair) {
   if (2 * pair >= n) return;
   if (counter_dev) counter = *counter_dev;
   const uint4 r = philox4x32_10(uint4{(uint32_t)pair, (uint32_t)((uint64_t)pair >> 32), (uint32_t)counter,
@@ -362,17 +391,26 @@ __global__ void normal_fill_kernel(double* out, long n, uint64_t seed, uint64_t 
   if (2 * pair + 1 < n) out[2 * pair + 1] = rad * s;
 }
 
+__global__ void normal_fill_kernel(double* out, long n, uint64_t seed, uint64_t counter, const uint64_t* counter_dev) {
+  normal_fill_body(out, n, seed, counter, counter_dev, blockIdx.x * (long)blockDim.x + threadIdx.x);
+}
+
 // SR offset zeta ~ U[0, 1) of block `*counter_dev`: Philox on a key distinct from
 // the field stream's, then the counter advances to the next block.
 // counter_dev (when not null) supplies the counter from device memory.
-__global__ void sr_zeta_kernel(double* zeta, uint64_t seed, uint64_t counter, const uint64_t* counter_dev) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+__device__ __forceinline__ void sr_zeta_body(double* zeta, uint64_t seed, uint64_t counter,
+                                             const uint64_t* counter_dev) {
   const uint64_t c = counter_dev ? *counter_dev : counter;
   const uint64_t key = seed ^ 0x9E3779B97F4A7C15ull;
   const uint4 r = philox4x32_10(uint4{0xFFFFFFFFu, 0xFFFFFFFFu, (uint32_t)c, (uint32_t)(c >> 32)},
                                 uint2{(uint32_t)key, (uint32_t)(key >> 32)});
   const uint64_t a = ((uint64_t)r.x << 21) ^ ((uint64_t)r.y >> 11);
   *zeta = (double)(a & ((1ull << 53) - 1)) * (1.0 / 9007199254740992.0);
+}
+
+__global__ void sr_zeta_kernel(double* zeta, uint64_t seed, uint64_t counter, const uint64_t* counter_dev) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  sr_zeta_body(zeta, seed, counter, counter_dev);
 }
 
 __global__ void counter_incr_kernel(uint64_t* counter_dev) {
@@ -426,9 +464,127 @@ __global__ void sym_pack_kernel(const double* __restrict__ L, int N, long npk, l
 
 inline unsigned blocks_for(long n, int t) { return (unsigned)((n + t - 1) / t); }
 
+// ------------------------------------------------------------------ grouped block kernels
+#define GJ_FIND                                   \
+  const int item = blockIdx.x;                    \
+  const int jn = group_find(prefix, njobs, item); \
+  const BlockJob& J = jobs[jn];                   \
+  const long local = item - prefix[jn];
+
+__global__ void g_rng_kernel(const BlockJob* __restrict__ jobs, const int* __restrict__ prefix, int njobs) {
+  GJ_FIND
+  normal_fill_body(J.fields, (long)J.nsteps * J.W * J.G, J.seed, 0, J.counter, local * 256 + threadIdx.x);
+}
+
+__global__ void g_field_shift_kernel(const BlockJob* __restrict__ jobs, const int* __restrict__ prefix, int njobs,
+                                     int step) {
+  GJ_FIND
+  field_shift_body(J.fb, J.fields + (long)step * J.W * J.G, J.mf, J.G, J.sqdt, J.xs, J.terms, local);
+}
+
+__global__ void g_overlap_kernel(const BlockJob* __restrict__ jobs, const int* __restrict__ prefix, int njobs,
+                                 int to_overlaps) {
+  GJ_FIND
+  const long w = local * 256 + threadIdx.x;
+  if (w >= J.W) return;
+  (to_overlaps ? J.overlaps : J.ov_new)[w] = cmul(J.det[w], J.det[w]);
+}
+
+__global__ __launch_bounds__(1024) void g_weight_update_kernel(const BlockJob* __restrict__ jobs,
+                                                               const int* __restrict__ prefix, int njobs) {
+  GJ_FIND
+  (void)local;
+  weight_update_body(J.ov_new, J.overlaps, J.terms, J.weights, J.scalars, J.h0_prop, J.dt, J.W);
+}
+
+__global__ void g_ene1_kernel(const BlockJob* __restrict__ jobs, const int* __restrict__ prefix, int njobs) {
+  GJ_FIND
+  ene1_body(J.green, J.rot_h1, J.O, nullptr, nullptr, 0, J.N, 2.0, J.ene1, local);
+}
+
+__global__ void g_energy_finalize_kernel(const BlockJob* __restrict__ jobs, const int* __restrict__ prefix,
+                                         int njobs) {
+  GJ_FIND
+  energy_finalize_body(J.ene1, J.coul, nullptr, J.kpart, nullptr, J.G, J.nch, 0, J.h0, J.energies, local);
+}
+
+__global__ __launch_bounds__(1024) void g_block_finalize_kernel(const BlockJob* __restrict__ jobs,
+                                                                const int* __restrict__ prefix, int njobs) {
+  GJ_FIND
+  (void)local;
+  block_finalize_body(J.energies, J.weights, J.scalars, J.clip, J.W, J.out);
+}
+
+__global__ __launch_bounds__(1024) void g_sr_index_kernel(const BlockJob* __restrict__ jobs,
+                                                          const int* __restrict__ prefix, int njobs) {
+  GJ_FIND
+  (void)local;
+  if (threadIdx.x == 0) sr_zeta_body(J.zeta, J.seed, 0, J.counter);
+  __syncthreads();
+  sr_index_body(J.weights, J.zeta, J.W, J.cum, J.idx);
+}
+
+__global__ void g_gather_kernel(const BlockJob* __restrict__ jobs, const int* __restrict__ prefix, int njobs) {
+  GJ_FIND
+  const long per = (long)J.N * J.O;
+  const long chunks = (per + 1023) / 1024;
+  const long w = local / chunks;
+  const long base = (local % chunks) * 1024;
+  const long src = (long)J.idx[w] * per;
+#pragma unroll
+  for (int t = 0; t < 4; ++t) {
+    const long i = base + t * 256 + threadIdx.x;
+    if (i < per) J.wgather[w * per + i] = J.walkers[src + i];
+  }
+}
+
+__global__ void g_block_tail_kernel(const BlockJob* __restrict__ jobs, const int* __restrict__ prefix, int njobs) {
+  GJ_FIND
+  (void)local;
+  if (threadIdx.x != 0) return;
+  *J.counter += 1;
+  // the driver's energy estimate follows the block energy (driver.py:168)
+  J.scalars[0] = 0.9 * J.scalars[0] + 0.1 * J.out[0];
+}
+#undef GJ_FIND
+
 }  // namespace
 
 // ------------------------------------------------------------------ launchers
+long block_items(const BlockJob& j, int kind) {
+  switch (kind) {
+    case BI_WALKER: return j.W;
+    case BI_WALKER256: return (j.W + 255) / 256;
+    case BI_JOB: return 1;
+    case BI_RNG: return (((long)j.nsteps * j.W * j.G + 1) / 2 + 255) / 256;
+    case BI_GATHER: return (long)j.W * (((long)j.N * j.O + 1023) / 1024);
+    default: return 0;
+  }
+}
+
+#define GJ_LAUNCH(kern, threads, ...)                                                                       \
+  do {                                                                                                   \
+    if (t.njobs < 1 || t.total < 1) return AFQ_OK;                                                       \
+    hipLaunchKernelGGL(kern, dim3((unsigned)t.total), dim3(threads), 0, s, t.params, t.prefix, t.njobs, \
+                       ##__VA_ARGS__);                                                                   \
+    return launch_status(#kern);                                                                         \
+  } while (0)
+int g_rng(const GroupTab<BlockJob>& t, hipStream_t s) { GJ_LAUNCH(g_rng_kernel, 256); }
+int g_field_shift(const GroupTab<BlockJob>& t, int step, hipStream_t s) {
+  GJ_LAUNCH(g_field_shift_kernel, 256, step);
+}
+int g_overlap(const GroupTab<BlockJob>& t, bool to_overlaps, hipStream_t s) {
+  GJ_LAUNCH(g_overlap_kernel, 256, to_overlaps ? 1 : 0);
+}
+int g_weight_update(const GroupTab<BlockJob>& t, hipStream_t s) { GJ_LAUNCH(g_weight_update_kernel, 1024); }
+int g_ene1(const GroupTab<BlockJob>& t, hipStream_t s) { GJ_LAUNCH(g_ene1_kernel, 256); }
+int g_energy_finalize(const GroupTab<BlockJob>& t, hipStream_t s) { GJ_LAUNCH(g_energy_finalize_kernel, 256); }
+int g_block_finalize(const GroupTab<BlockJob>& t, hipStream_t s) { GJ_LAUNCH(g_block_finalize_kernel, 1024); }
+int g_sr_index(const GroupTab<BlockJob>& t, hipStream_t s) { GJ_LAUNCH(g_sr_index_kernel, 1024); }
+int g_gather(const GroupTab<BlockJob>& t, hipStream_t s) { GJ_LAUNCH(g_gather_kernel, 256); }
+int g_block_tail(const GroupTab<BlockJob>& t, hipStream_t s) { GJ_LAUNCH(g_block_tail_kernel, 64); }
+#undef GJ_LAUNCH
+
 int k_sym_h1(const double* in, double* out, int n, hipStream_t s) {
   const long tot = 2L * n * n;
   hipLaunchKernelGGL(sym_h1_kernel, dim3(blocks_for(tot, 256)), dim3(256), 0, s, in, out, n);

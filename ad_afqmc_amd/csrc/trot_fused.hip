@@ -16,11 +16,11 @@
 //     products (hi.hi + hi.lo + mid.hi + lo.hi), and the identity part is
 //     added in FP64 from the walker buffer: t0 = X + (E - I) X;
 //   * a Taylor term is VHS . term / n on the split-BF16 x3 scheme (hi.hi +
-//     lo.hi + hi.lo per real product).  The VHS is split once per step into
-//     four bf16 planes of the full N x N matrix (k_split_vhs_bf16, from the
-//     packed lower block triangle when L is symmetric), so an A fragment is
-//     one 16-byte load per plane straight from global memory (L2) and the
-//     inner loop does no conversion;
+//     lo.hi + hi.lo per real product).  The VHS product writes the VHS as four
+//     bf16 planes of the full N x N matrix (from the packed lower block
+//     triangle when L is symmetric, mirrored), so an A fragment is one
+//     16-byte load per plane straight from global memory (L2) and the inner
+//     loop does no conversion;
 //   * the running sum of the terms stays in FP32 registers (the terms are
 //     complex64 values) and is added to t0 in FP64.
 // t0 and t0 + sum are kept in the walker buffer itself (FP64): every lane
@@ -30,6 +30,7 @@
 // other slabs and terms come from L2 (a walker's slabs run on one XCD).
 #include "kernels.h"
 #include "trot_fused.h"
+#include "group.h"
 
 typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
 typedef __bf16 v4bf __attribute__((ext_vector_type(4)));
@@ -51,21 +52,16 @@ __device__ __forceinline__ int pofs(int n, int k, int kpl) {
   return n * kpl + ((((k >> 3) ^ (n & 15)) << 3) | (k & 7));
 }
 
+__device__ __forceinline__ long sym_packed_cols_d(int N) {
+  const long nb = (N + 15) / 16;
+  return nb * (nb + 1) / 2 * 256;
+}
+
 __device__ __forceinline__ void split1(float y, __bf16& hi, __bf16& lo) {
   hi = (__bf16)y;
   lo = (__bf16)(y - (float)hi);
 }
 
-struct FusedArgs {
-  double2* wb[2];        // walker buffers (W, N, c[s]), updated in place
-  int c[2];              // columns per spin buffer
-  int nsl[2];            // column slabs per spin
-  int N, R, K32, KPL, W;  // rows, 16-row blocks, k extent (32-multiple), LDS column length (128-multiple)
-  const __bf16* vs;      // VHS planes re hi, re lo, im hi, im lo of (16 R) x K32 per walker
-  long ep;               // plane stride of vs and e1 (elements)
-  const __bf16* e1[2];   // E - I of each spin: planes hi, mid, lo of (16 R) x K32, zero padded
-  int nterms;            // n_exp_terms: Taylor terms 1 .. nterms - 1
-};
 
 // -x of 8 bf16 (sign bits flipped)
 __device__ __forceinline__ v8bf neg8(const v8bf& x) {
@@ -75,21 +71,24 @@ __device__ __forceinline__ v8bf neg8(const v8bf& x) {
   return __builtin_bit_cast(v8bf, u);
 }
 
+// XCD-aware order of a job's work-groups: item i (i mod 8 = its XCD) is slab
+// (i / 8) mod nsl of walker i mod 8 + 8 ((i / 8) / nsl), so all slabs of a
+// walker run on one XCD and its VHS is read into one L2.
+__device__ __forceinline__ void fused_item(const FusedArgs& a, long i, int* w, int* sl) {
+  const int nsl = a.nsl[0] + a.nsl[1];
+  const long it = i >> 3;
+  *w = (int)(i & 7) + 8 * (int)(it / nsl);
+  *sl = (int)(it % nsl);
+}
+
 template <int TN, int RBW>
-__global__ __launch_bounds__(64 * FW, 1) void trot_fused_bf3_kernel(FusedArgs a) {
+__device__ __forceinline__ void trot_fused_body(const FusedArgs& a, int w, int sl) {
   constexpr int CS = 16 * TN;
   constexpr int KPLMAX = (RBW * FW * 16 + 63) / 64 * 64;
   __shared__ __align__(16) __bf16 P[4 * CS * KPLMAX];
   const int tid = threadIdx.x, lane = tid & 63;
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int r = lane & 15, q = lane >> 4;
-  // XCD-aware order: XCD x (work-groups bid = x mod 8) takes walkers x, x + 8, ...,
-  // with all slabs of a walker in a row, so the walker's VHS is read into one L2
-  const int nsl = a.nsl[0] + a.nsl[1];
-  const long it = (long)(blockIdx.x >> 3);
-  const int w = (int)(blockIdx.x & 7) + 8 * (int)(it / nsl);
-  if (w >= a.W) return;  // uniform for the work-group
-  int sl = (int)(it % nsl);
   const int s = sl < a.nsl[0] ? 0 : 1;
   if (s) sl -= a.nsl[0];
   const int cs = a.c[s], c0 = sl * CS, N = a.N, kpl = a.KPL;
@@ -212,25 +211,25 @@ __global__ __launch_bounds__(64 * FW, 1) void trot_fused_bf3_kernel(FusedArgs a)
 #pragma unroll
       for (int ii = 0; ii < RBW; ++ii) {
         if (ii >= nrb) continue;
+        // B fragments of every column block, so that consecutive MFMAs go to
+        // the 2 TN independent accumulators of this row block in turn
+        v8bf b[TN][6];
 #pragma unroll
         for (int j = 0; j < TN; ++j) {
-          v8bf brh, brl, bih, bil;
-          bfrag(j, k0, brh, brl, bih, bil);
-          const v8bf mbih = neg8(bih), mbil = neg8(bil);
-          // re += ar br - ai bi ; im += ar bi + ai br, each as hi*hi + lo*hi + hi*lo
-          aR[ii][j] = mfma_bf16(c[ii][0], brh, aR[ii][j]);
-          aR[ii][j] = mfma_bf16(c[ii][1], brh, aR[ii][j]);
-          aR[ii][j] = mfma_bf16(c[ii][0], brl, aR[ii][j]);
-          aR[ii][j] = mfma_bf16(c[ii][2], mbih, aR[ii][j]);
-          aR[ii][j] = mfma_bf16(c[ii][3], mbih, aR[ii][j]);
-          aR[ii][j] = mfma_bf16(c[ii][2], mbil, aR[ii][j]);
-          aI[ii][j] = mfma_bf16(c[ii][0], bih, aI[ii][j]);
-          aI[ii][j] = mfma_bf16(c[ii][1], bih, aI[ii][j]);
-          aI[ii][j] = mfma_bf16(c[ii][0], bil, aI[ii][j]);
-          aI[ii][j] = mfma_bf16(c[ii][2], brh, aI[ii][j]);
-          aI[ii][j] = mfma_bf16(c[ii][3], brh, aI[ii][j]);
-          aI[ii][j] = mfma_bf16(c[ii][2], brl, aI[ii][j]);
+          bfrag(j, k0, b[j][0], b[j][1], b[j][2], b[j][3]);
+          b[j][4] = neg8(b[j][2]);
+          b[j][5] = neg8(b[j][3]);
         }
+        // re += ar br - ai bi ; im += ar bi + ai br, each as hi*hi + lo*hi + hi*lo
+        constexpr int PR[6][2] = {{0, 0}, {1, 0}, {0, 1}, {2, 4}, {3, 4}, {2, 5}};  // (A part, B part) of re
+        constexpr int PI[6][2] = {{0, 2}, {1, 2}, {0, 3}, {2, 0}, {3, 0}, {2, 1}};  // of im
+#pragma unroll
+        for (int t = 0; t < 6; ++t)
+#pragma unroll
+          for (int j = 0; j < TN; ++j) {
+            aR[ii][j] = mfma_bf16(c[ii][PR[t][0]], b[j][PR[t][1]], aR[ii][j]);
+            aI[ii][j] = mfma_bf16(c[ii][PI[t][0]], b[j][PI[t][1]], aI[ii][j]);
+          }
         load(ii, kn);
       }
     }
@@ -339,59 +338,141 @@ __global__ __launch_bounds__(64 * FW, 1) void trot_fused_bf3_kernel(FusedArgs a)
 }
 
 template <int TN, int RBW>
-int launch_tn(const FusedArgs& a, long gx, hipStream_t st) {
+__global__ __launch_bounds__(64 * FW, 1) void trot_fused_bf3_kernel(FusedArgs a) {
+  int w, sl;
+  fused_item(a, blockIdx.x, &w, &sl);
+  if (w >= a.W) return;  // uniform for the work-group
+  trot_fused_body<TN, RBW>(a, w, sl);
+}
+
+// Grouped: job j owns work-groups [prefix[j], prefix[j + 1]) (a multiple of 8
+// from a multiple of 8, so the XCD order within a job is kept).
+template <int TN, int RBW>
+__global__ __launch_bounds__(64 * FW, 1) void trot_fused_grouped_kernel(const FusedArgs* __restrict__ jobs,
+                                                                        const int* __restrict__ prefix, int njobs) {
+  const int item = blockIdx.x;
+  const int j = group_find(prefix, njobs, item);
+  const FusedArgs a = jobs[j];
+  int w, sl;
+  fused_item(a, item - prefix[j], &w, &sl);
+  if (w >= a.W) return;
+  trot_fused_body<TN, RBW>(a, w, sl);
+}
+
+// One launch: the single job `a` (tab == nullptr) or the grouped table.
+template <int TN, int RBW>
+int launch_tn(const FusedArgs& a, const GroupTab<FusedArgs>* tab, long gx, hipStream_t st) {
+  if (tab) {
+    hipLaunchKernelGGL((trot_fused_grouped_kernel<TN, RBW>), dim3((unsigned)tab->total), dim3(64 * FW), 0, st,
+                       tab->params, tab->prefix, tab->njobs);
+    return launch_status("trot_fused_grouped_kernel");
+  }
   hipLaunchKernelGGL((trot_fused_bf3_kernel<TN, RBW>), dim3((unsigned)gx), dim3(64 * FW), 0, st, a);
   return launch_status("trot_fused_bf3_kernel");
 }
 
 template <int TN>
-int launch_rbw(const FusedArgs& a, int rbw, long gx, hipStream_t st) {
+int launch_rbw(const FusedArgs& a, const GroupTab<FusedArgs>* tab, int rbw, long gx, hipStream_t st) {
   switch (rbw) {
-    case 1: return launch_tn<TN, 1>(a, gx, st);
-    case 2: return launch_tn<TN, 2>(a, gx, st);
-    case 3: return launch_tn<TN, 3>(a, gx, st);
+    case 1: return launch_tn<TN, 1>(a, tab, gx, st);
+    case 2: return launch_tn<TN, 2>(a, tab, gx, st);
+    case 3: return launch_tn<TN, 3>(a, tab, gx, st);
     case 4:
       // three column blocks at four row blocks per wave would exceed the LDS
-      if constexpr (TN < 3) return launch_tn<TN, 4>(a, gx, st);
+      if constexpr (TN < 3) return launch_tn<TN, 4>(a, tab, gx, st);
       break;
     default: break;
   }
   return fail(AFQ_E_UNSUPPORTED, "trot_fused_bf3: no kernel for this shape");
 }
 
-// VHS (complex64, packed lower block triangle when sym, else N x N) of every
-// walker -> its four bf16 planes (re hi, re lo, im hi, im lo) of the full
-// (16 R) x K32 matrix, zero past N.  Thread (w, row i, 8-group g) writes one
-// 16-byte run per plane.
-__global__ void split_vhs_kernel(const float2* __restrict__ vhs, long sV, int sym, int N, int rows, int K32, long ep,
-                                 int W, __bf16* __restrict__ out) {
-  const int kg = K32 >> 3;
-  const long idx = blockIdx.x * (long)blockDim.x + threadIdx.x;
-  if (idx >= (long)W * rows * kg) return;
-  const int g = (int)(idx % kg);
-  const long wi = idx / kg;
-  const int i = (int)(wi % rows);
-  const long w = wi / rows;
-  const float2* V = vhs + w * sV;
-  v8bf rh, rl, ih, il;
+// VHS -> planes, one 64-thread group per 16 x 16 block of a walker's VHS (four
+// blocks per work-group): thread (r, q) reads entries (r, 4q .. 4q + 3) of the
+// block (the block's 2 KB are read contiguously) and writes their bf16 parts
+// at row 16 ib + r, columns 16 kb + 4q ..; for an off-diagonal packed block
+// the transposed block (kb, ib) is written too, through a per-block LDS tile
+// so that each thread again writes 4 consecutive entries of one row.
+__device__ __forceinline__ void put4(__bf16* base, long ep, long off, const float2 (&x)[4], int nvalid) {
+  __bf16 rh[4], rl[4], ih[4], il[4];
 #pragma unroll
-  for (int t = 0; t < 8; ++t) {
-    const int k = 8 * g + t;
-    float2 x = {0.f, 0.f};
-    if (i < N && k < N) x = V[sym ? sym_off(i, k) : (long)i * N + k];
-    __bf16 h, l;
-    split1(x.x, h, l);
-    rh[t] = h;
-    rl[t] = l;
-    split1(x.y, h, l);
-    ih[t] = h;
-    il[t] = l;
+  for (int t = 0; t < 4; ++t) {
+    split1(x[t].x, rh[t], rl[t]);
+    split1(x[t].y, ih[t], il[t]);
   }
-  __bf16* o = out + 4 * ep * w + (long)i * K32 + 8 * g;
-  *reinterpret_cast<v8bf*>(o) = rh;
-  *reinterpret_cast<v8bf*>(o + ep) = rl;
-  *reinterpret_cast<v8bf*>(o + 2 * ep) = ih;
-  *reinterpret_cast<v8bf*>(o + 3 * ep) = il;
+  if (nvalid == 4 && (off & 3) == 0) {
+    *reinterpret_cast<v4bf*>(base + off) = v4bf{rh[0], rh[1], rh[2], rh[3]};
+    *reinterpret_cast<v4bf*>(base + ep + off) = v4bf{rl[0], rl[1], rl[2], rl[3]};
+    *reinterpret_cast<v4bf*>(base + 2 * ep + off) = v4bf{ih[0], ih[1], ih[2], ih[3]};
+    *reinterpret_cast<v4bf*>(base + 3 * ep + off) = v4bf{il[0], il[1], il[2], il[3]};
+  } else {
+    for (int t = 0; t < nvalid; ++t) {
+      base[off + t] = rh[t];
+      base[ep + off + t] = rl[t];
+      base[2 * ep + off + t] = ih[t];
+      base[3 * ep + off + t] = il[t];
+    }
+  }
+}
+
+__device__ __forceinline__ void split_vhs_body(const VhsSplit& v, long blk) {
+  __shared__ float2 tile[4][16][17];
+  const int N = v.N, nb = (N + 15) / 16, K32 = 32 * ((N + 31) / 32);
+  const long ep = 16L * nb * K32;
+  const int sub = threadIdx.x >> 6, t = threadIdx.x & 63;
+  const long nblk = v.sym ? (long)nb * (nb + 1) / 2 : (long)nb * nb;
+  const long b = blk * 4 + sub;  // (walker, block) index
+  const bool active = b < (long)v.W * nblk;
+  const long w = active ? b / nblk : 0;
+  const int bi = active ? (int)(b % nblk) : 0;
+  int ib, kb;
+  if (v.sym) {
+    sym_block(bi, &ib, &kb);
+  } else {
+    ib = bi / nb;
+    kb = bi % nb;
+  }
+  const int r = t >> 2, q = t & 3;
+  float2 x[4] = {};
+  if (active) {
+    if (v.sym) {
+      const float2* src = v.vhs + w * (sym_packed_cols_d(N)) + ((long)bi << 8) + r * 16 + 4 * q;
+#pragma unroll
+      for (int u = 0; u < 4; ++u) x[u] = src[u];
+    } else {
+      const int i = 16 * ib + r;
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const int k = 16 * kb + 4 * q + u;
+        if (i < N && k < N) x[u] = v.vhs[w * (long)N * N + (long)i * N + k];
+      }
+    }
+  }
+  __bf16* base = reinterpret_cast<__bf16*>(v.out) + 4 * ep * w;
+  {
+    const int i = 16 * ib + r, k0 = 16 * kb + 4 * q;
+    if (active && i < N && k0 < N) put4(base, ep, (long)i * K32 + k0, x, min(4, N - k0));
+  }
+  if (!v.sym) return;
+#pragma unroll
+  for (int u = 0; u < 4; ++u) tile[sub][r][4 * q + u] = x[u];
+  __syncthreads();
+  if (!active || ib == kb) return;
+  // transposed block: row 16 kb + r, columns 16 ib + 4q ..
+  float2 y[4];
+#pragma unroll
+  for (int u = 0; u < 4; ++u) y[u] = tile[sub][4 * q + u][r];
+  const int i = 16 * kb + r, k0 = 16 * ib + 4 * q;
+  if (i < N && k0 < N) put4(base, ep, (long)i * K32 + k0, y, min(4, N - k0));
+}
+
+__global__ __launch_bounds__(256) void split_vhs_kernel(VhsSplit v) { split_vhs_body(v, blockIdx.x); }
+
+__global__ __launch_bounds__(256) void split_vhs_grouped_kernel(const VhsSplit* __restrict__ jobs,
+                                                                const int* __restrict__ prefix, int njobs) {
+  const int item = blockIdx.x;
+  const int j = group_find(prefix, njobs, item);
+  const VhsSplit v = jobs[j];
+  split_vhs_body(v, item - prefix[j]);
 }
 
 // (E - I) split into three bf16 planes of rows x ld (zero padded)
@@ -435,19 +516,34 @@ int fused_tn(const int* c, int nspin, int N) {
 
 bool trot_fused_supported(int N) { return N >= 1 && N <= 512; }
 
+long vhs_split_items(const VhsSplit& v) {
+  const long nb = (v.N + 15) / 16;
+  const long nblk = v.sym ? nb * (nb + 1) / 2 : nb * nb;
+  return ((long)v.W * nblk + 3) / 4;
+}
+
+int k_split_vhs_bf16(const float2* vhs, bool sym, int N, int W, uint16_t* out, hipStream_t st) {
+  const VhsSplit v{vhs, sym, N, W, out};
+  const long n = vhs_split_items(v);
+  if (n > 0x7fffffffL) return fail(AFQ_E_UNSUPPORTED, "split_vhs: grid too large");
+  hipLaunchKernelGGL(split_vhs_kernel, dim3((unsigned)n), dim3(256), 0, st, v);
+  return launch_status("split_vhs_kernel");
+}
+
+int split_vhs_grouped(const GroupTab<VhsSplit>& t, hipStream_t st) {
+  if (t.njobs < 1 || t.total < 1) return AFQ_OK;
+  hipLaunchKernelGGL(split_vhs_grouped_kernel, dim3((unsigned)t.total), dim3(256), 0, st, t.params, t.prefix,
+                     t.njobs);
+  return launch_status("split_vhs_grouped_kernel");
+}
+
 long trot_fused_plane(int N) { return 16L * ((N + 15) / 16) * (32L * ((N + 31) / 32)); }
 
 long trot_fused_e1_elems(int N) { return 3 * trot_fused_plane(N); }
 
 long trot_fused_vhs_elems(int N, int W) { return 4 * trot_fused_plane(N) * W; }
 
-int k_split_vhs_bf16(const float2* vhs, long sV, bool sym, int N, int W, uint16_t* out, hipStream_t st) {
-  const int rows = 16 * ((N + 15) / 16), K32 = 32 * ((N + 31) / 32);
-  const long n = (long)W * rows * (K32 / 8);
-  hipLaunchKernelGGL(split_vhs_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, vhs, sV, sym ? 1 : 0, N,
-                     rows, K32, trot_fused_plane(N), W, reinterpret_cast<__bf16*>(out));
-  return launch_status("split_vhs_kernel");
-}
+
 
 int k_split_e1_bf16(const double* E, int N, uint16_t* out, hipStream_t st) {
   const int rows = 16 * ((N + 15) / 16), ld = 32 * ((N + 31) / 32);
@@ -457,9 +553,10 @@ int k_split_e1_bf16(const double* E, int N, uint16_t* out, hipStream_t st) {
   return launch_status("split_e1_kernel");
 }
 
-int trot_fused_bf3(const TrotFused& p, hipStream_t st) {
+int trot_fused_args(const TrotFused& p, int tn, FusedArgs* out) {
   if (!trot_fused_supported(p.N)) return fail(AFQ_E_UNSUPPORTED, "trot_fused_bf3: N above 512");
   if (p.nspin < 1 || p.nspin > 2 || p.W < 1 || p.nterms < 1) return fail(AFQ_E_INVALID, "trot_fused_bf3: bad shape");
+  if (!p.vs) return fail(AFQ_E_INVALID, "trot_fused_bf3: missing VHS planes");
   FusedArgs a{};
   const int R = (p.N + 15) / 16;
   a.N = p.N;
@@ -470,7 +567,6 @@ int trot_fused_bf3(const TrotFused& p, hipStream_t st) {
   a.vs = reinterpret_cast<const __bf16*>(p.vs);
   a.ep = 16L * R * a.K32;
   a.nterms = p.nterms;
-  const int tn = fused_tn(p.c, p.nspin, p.N);
   for (int s = 0; s < 2; ++s) {
     const bool on = s < p.nspin;
     a.wb[s] = on ? p.wb[s] : nullptr;
@@ -479,15 +575,32 @@ int trot_fused_bf3(const TrotFused& p, hipStream_t st) {
     a.e1[s] = reinterpret_cast<const __bf16*>(on ? p.e1[s] : nullptr);
     if (on && (!p.wb[s] || !p.e1[s] || p.c[s] < 1)) return fail(AFQ_E_INVALID, "trot_fused_bf3: missing operand");
   }
-  const int rbw = (R + FW - 1) / FW;
-  // the static LDS planes hold KPLMAX(rbw) >= KPL entries per column
-  const long gx = 8L * ((p.W + 7) / 8) * (a.nsl[0] + a.nsl[1]);
-  if (gx > 0x7fffffffL) return fail(AFQ_E_UNSUPPORTED, "trot_fused_bf3: grid too large");
-  if (!p.vs) return fail(AFQ_E_INVALID, "trot_fused_bf3: missing VHS planes");
+  *out = a;
+  return AFQ_OK;
+}
+
+long trot_fused_items(const FusedArgs& a) { return 8L * ((a.W + 7) / 8) * (a.nsl[0] + a.nsl[1]); }
+
+int trot_fused_grouped(const GroupTab<FusedArgs>& t, int tn, int rbw, hipStream_t st) {
+  if (t.njobs < 1 || t.total < 1) return AFQ_OK;
+  const FusedArgs none{};
   switch (tn) {
-    case 1: return launch_rbw<1>(a, rbw, gx, st);
-    case 2: return launch_rbw<2>(a, rbw, gx, st);
-    default: return launch_rbw<3>(a, rbw, gx, st);
+    case 1: return launch_rbw<1>(none, &t, rbw, 0, st);
+    case 2: return launch_rbw<2>(none, &t, rbw, 0, st);
+    default: return launch_rbw<3>(none, &t, rbw, 0, st);
+  }
+}
+
+int trot_fused_bf3(const TrotFused& p, hipStream_t st) {
+  FusedArgs a;
+  AFQ_TRY(trot_fused_args(p, fused_tn(p.c, p.nspin, p.N), &a));
+  const int rbw = (a.R + FW - 1) / FW;  // the static LDS planes hold KPLMAX(rbw) >= KPL entries per column
+  const long gx = trot_fused_items(a);
+  if (gx > 0x7fffffffL) return fail(AFQ_E_UNSUPPORTED, "trot_fused_bf3: grid too large");
+  switch (fused_tn(p.c, p.nspin, p.N)) {
+    case 1: return launch_rbw<1>(a, nullptr, rbw, gx, st);
+    case 2: return launch_rbw<2>(a, nullptr, rbw, gx, st);
+    default: return launch_rbw<3>(a, nullptr, rbw, gx, st);
   }
 }
 

@@ -368,6 +368,60 @@ class EngineJob:
         return out[0], gh1, gL
 
 
+class JobGroup:
+    """Several engine jobs stepped together: each `sr_block()` runs one
+    afq_sr_block for every member, every kernel of the block launched once
+    over all members (afq_group_sr_block).  Members: restricted walkers of an
+    RHF trial in bf16x3 (N <= 512) with fields reserved for n_prop_steps.  The
+    group keeps each member's walkers, weights, overlaps, scalars, counter and
+    block output tensors (it holds references to them); a block also advances
+    each e_estimate to 0.9 e_estimate + 0.1 E_block.  Capturable in a HIP
+    graph."""
+
+    def __init__(self, members, n_prop_steps: int):
+        """members: sequence of (job, walkers, weights, overlaps, scalars,
+        counter_dev, block_out, seed)."""
+        if not members:
+            raise ValueError("a group needs at least one job")
+        self._h = C.c_void_p()
+        self._keep = []
+        n = len(members)
+        P = C.c_void_p
+        jobs, walkers, weights, overlaps, scalars, counters, outs = ((P * n)() for _ in range(7))
+        seeds = (C.c_uint64 * n)()
+        for i, (job, up, w, ov, sc, cnt, out, seed) in enumerate(members):
+            pu, _ = job._w(up, None)
+            job._state(up.shape[0], w, ov, sc)
+            check_tensor("counter_dev", cnt, (1,), torch.int64)
+            check_tensor("block_out", out, (2,), torch.float64)
+            jobs[i] = job._h.value
+            walkers[i] = pu.value
+            weights[i] = w.data_ptr()
+            overlaps[i] = ov.data_ptr()
+            scalars[i] = sc.data_ptr()
+            counters[i] = cnt.data_ptr()
+            outs[i] = out.data_ptr()
+            seeds[i] = int(seed)
+            self._keep.append((job, up, w, ov, sc, cnt, out))
+        nat.call("afq_group_create", C.byref(self._h), jobs, n, walkers, weights, overlaps, scalars, counters, outs,
+                 seeds, int(n_prop_steps))
+
+    def sr_block(self):
+        nat.call("afq_group_sr_block", self._h, stream_ptr())
+
+    def close(self):
+        if self._h:
+            nat.lib().afq_group_destroy(self._h)
+            self._h = C.c_void_p()
+        self._keep = []
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 def normal_fill(n: int, seed: int, counter: int) -> torch.Tensor:
     """N(0,1) draws of the engine's Philox4x32-10 field generator."""
     out = torch.empty(int(n), dtype=torch.float64, device=device())

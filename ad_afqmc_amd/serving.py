@@ -266,6 +266,107 @@ class GpuExecutor:
         h.clear()
 
 
+class GroupedExecutor(GpuExecutor):
+    """Runs all admitted jobs together: every round issues one grouped block
+    (afq_group_sr_block: each kernel of the block launched once over all
+    member jobs) on one stream, for every admitted job with blocks left.  A job
+    admitted while a round is in flight joins the next round; a finished job
+    leaves at the end of its last round.  The group (the device table of the
+    members' launch parameters) is rebuilt when the member set changes, and
+    with use_graphs a member set that lasts more than one round is captured
+    into a HIP graph on its second round and replayed afterwards.  Jobs must
+    be restricted-walker RHF jobs in bf16x3 (the grouped block's scope)."""
+
+    def __init__(self, n_prop_steps: int = 10, use_graphs: bool = True):
+        super().__init__(n_prop_steps, use_graphs)
+        self._stream = None
+        self._members = ()
+        self._group = None
+        self._graph = None
+        self._rounds_same = 0
+        self._event = None
+        self.rounds = 0
+
+    def start(self, spec: JobSpec):
+        if spec.precision != "bf16x3":
+            raise ValueError("grouped serving runs bf16x3 jobs")
+        h = super().start(spec)
+        self._live.append(h)
+        return h
+
+    def _round_done(self) -> bool:
+        return self._event is None or self._event.query()
+
+    def ready(self, h) -> bool:
+        return h["issued"] < h["nblocks"] and h["event"].query() and self._round_done()
+
+    def _drop_group(self):
+        if self._group is not None:
+            if self._event is not None:
+                self._event.synchronize()
+            self._graph = None
+            self._group.close()
+            self._group = None
+            self._members = ()
+            self._rounds_same = 0
+
+    def issue(self, h):
+        """Issues one grouped block for every admitted job with blocks left
+        (h among them); other members' issue() calls in this round find them
+        already issued."""
+        torch = self.torch
+        from .engine import JobGroup
+
+        if not self._round_done():
+            return
+        if self._stream is None:
+            self._stream = torch.cuda.Stream()
+        live = [x for x in self._live if x["issued"] < x["nblocks"] and x["event"].query()]
+        members = tuple(id(x) for x in live)
+        stream = self._stream
+        with torch.cuda.stream(stream):
+            if members != self._members:
+                self._drop_group()
+                for x in live:
+                    stream.wait_stream(x["stream"])
+                self._group = JobGroup([(x["job"], x["up"], x["weights"], x["overlaps"], x["scalars"], x["counter"],
+                                         x["out"], x["spec"].seed) for x in live], self.n_prop_steps)
+                self._members = members
+            if self._graph is not None:
+                self._graph.replay()
+            elif self.use_graphs and self._rounds_same == 1:
+                g = torch.cuda.CUDAGraph()
+                with torch.cuda.graph(g, stream=stream):
+                    self._group.sr_block()  # recorded, not run
+                self._graph = g
+                g.replay()
+            else:
+                self._group.sr_block()
+            self._rounds_same += 1
+            for x in live:
+                x["outs"].append(x["out"].clone())
+                x["issued"] += 1
+            ev = torch.cuda.Event()
+            ev.record(stream)
+            self._event = ev
+            for x in live:
+                x["event"] = ev
+        self.rounds += 1
+
+    @property
+    def _live(self):
+        return self.__dict__.setdefault("_handles", [])
+
+    def done(self, h) -> bool:
+        return h["issued"] >= h["nblocks"] and h["event"].query()
+
+    def release(self, h):
+        self._drop_group()
+        if h in self._live:
+            self._live.remove(h)
+        super().release(h)
+
+
 def _device_system(spec: JobSpec):
     """The synthetic generator of ad_afqmc_amd.synthetic, drawn on the device
     from torch's generator seeded with the job seed (fast for large jobs), with

@@ -805,17 +805,32 @@ def main():
     if args.serving_jobs > 0 and not args.only:
         from ad_afqmc_amd import serving
 
+        def serve(executor_cls, realtime):
+            specs = serving.poisson_stream(args.serving_jobs, rate=4.0, seed=0, precision=args.serving_precision)
+            ex = executor_cls(n_prop_steps=10)
+            sch = serving.Scheduler(ex, hbm_cap_bytes=200e9, max_active=64)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            res = sch.run(specs, realtime=realtime)
+            torch.cuda.synchronize()
+            summ = serving.summarize(res, time.perf_counter() - t0)
+            summ.update(max_concurrent_jobs=sch.max_concurrent, peak_admitted_gb=sch.peak_used / 1e9)
+            if hasattr(ex, "rounds"):
+                summ["grouped_rounds"] = ex.rounds
+            return summ
+
         torch.cuda.empty_cache()
-        print(f"[bench] serving stream: {args.serving_jobs} jobs", file=sys.stderr, flush=True)
-        specs = serving.poisson_stream(args.serving_jobs, rate=4.0, seed=0, precision=args.serving_precision)
-        sch = serving.Scheduler(serving.GpuExecutor(n_prop_steps=10), hbm_cap_bytes=200e9, max_active=64)
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        res = sch.run(specs, realtime=True)
-        torch.cuda.synchronize()
-        summ = serving.summarize(res, time.perf_counter() - t0)
-        summ.update(arrival_rate_per_s=4.0, precision=args.serving_precision, hbm_cap_gb=200,
-                    max_concurrent_jobs=sch.max_concurrent, peak_admitted_gb=sch.peak_used / 1e9, hip_graphs=True,
+        out = {}
+        # one grouped block per round over all admitted jobs, and one stream per job
+        for mode, cls in (("grouped", serving.GroupedExecutor), ("streams", serving.GpuExecutor)):
+            print(f"[bench] serving stream ({mode}): {args.serving_jobs} jobs", file=sys.stderr, flush=True)
+            out[mode] = serve(cls, True)
+            print(f"[bench] serving capacity ({mode}): all jobs at t=0", file=sys.stderr, flush=True)
+            out[mode]["capacity_all_at_t0"] = serve(cls, False)
+        summ = dict(out["grouped"])
+        summ.update(arrival_rate_per_s=4.0, precision=args.serving_precision, hbm_cap_gb=200, hip_graphs=True,
+                    executor="grouped: one launch per kernel over all admitted jobs per block (afq_group_sr_block)",
+                    per_job_streams=out["streams"],
                     workload="N~U{30..250}, O=N/5, G=5N, W~U{64..512}, 300 steps, Poisson 4 jobs/s")
         line["serving"] = summ
     line["cpu_baseline"] = cpu

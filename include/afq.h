@@ -245,6 +245,30 @@ int afq_walkers_sr_noalloc(void* walkers_up, int64_t elems_up, void* walkers_dn,
                            double* weights, const double* zeta, int32_t* indices_out, void* stream);
 int64_t afq_walkers_sr_noalloc_max_walkers(void);
 
+/* ---- grouped blocks (serving) ----
+ * No single counterpart in the reference, whose many-job workloads (LNO
+ * fragments) run one job after another (lnoutils.py:175-177): one afq_sr_block
+ * (sampling.py:52-94, 146-168 with the comb of sr.py:24-48) for every job of a
+ * group, each kernel of the block launched once over all the jobs.  Jobs:
+ * restricted walkers of an RHF trial, split-BF16 x3 precision, N <= 512, with
+ * their fields reserved for n_prop_steps.  Per job i the group keeps the
+ * caller's walkers[i] (W, N, O) c128, weights[i] (W,), overlaps[i] (W,) c128,
+ * scalars[i] (e_estimate, pop_control_ene_shift), rng_counters[i] (device
+ * uint64) and block_out[i] (block energy, block weight), all device buffers
+ * that must outlive the group.  A grouped block updates them exactly as
+ * afq_sr_block(job_i, ..., seeds[i], rng_counters[i], block_out[i]) does, then
+ * advances e_estimate to 0.9 e_estimate + 0.1 E_block (driver.py:168).
+ * afq_group_create copies the per-job launch parameters to the device
+ * (synchronous); afq_group_sr_block is stream-ordered, allocates nothing and
+ * may be captured in a HIP graph.  A job may belong to one group at a time. */
+typedef struct afq_group* afq_group_t;
+int afq_group_create(afq_group_t* out, const afq_job_t* jobs, int32_t n_jobs, void* const* walkers,
+                     double* const* weights, void* const* overlaps, double* const* scalars,
+                     uint64_t* const* rng_counters, double* const* block_out, const uint64_t* seeds,
+                     int32_t n_prop_steps);
+int afq_group_sr_block(afq_group_t group, void* stream);
+int afq_group_destroy(afq_group_t group);
+
 /* Value and reverse-mode gradient of the segment energy
  *     F = sum_b E_b W_b / sum_b W_b
  * over n_blocks blocks of n_steps phaseless steps (afq_block's step, QR,
