@@ -114,7 +114,8 @@ int gemm_lp(LpKind kind, bool bf16x3, const Gemm& g, hipStream_t stream);
 // ---------------------------------------------------------------- small dense linear algebra
 // In-place Gauss-Jordan with partial pivoting (|re|+|im| pivot metric, first
 // maximum wins, as LAPACK's izamax) on `count` n x n complex matrices.  Writes
-// det (count,) and, if inv != nullptr, the inverse.  n <= 64.
+// det (count,) and, if inv != nullptr, the inverse.  n <= 256: up to n = 64
+// the matrix is factored in LDS, above in global memory (in inv itself).
 int batched_det_inv(const double2* S, long strideS, int n, int count, double2* det, double2* inv, long strideInv,
                     hipStream_t stream);
 // One launch over one or two sets of `count` matrices each (e.g. both spins).
@@ -125,11 +126,16 @@ struct DetInvSet {
   double2* det;
   double2* inv;  // nullptr: determinant only
   long strideInv;
+  // n > 64 without an inverse: global scratch of n x n per matrix, matrix
+  // stride strideS (S itself may be given: it is then overwritten)
+  double2* work = nullptr;
 };
+constexpr int kDetInvMaxN = 256;
 int batched_det_inv_sets(const DetInvSet* sets, int nsets, int count, hipStream_t stream);
 
 // Upper Cholesky factor R (S = R^H R, positive real diagonal, zero strict lower
-// triangle) of `count` Hermitian positive-definite n x n matrices, n <= 64.
+// triangle) of `count` Hermitian positive-definite n x n matrices, n <= 256
+// (above n = 64 factored in place in R; R may be S).
 int batched_cholesky_upper(const double2* S, long strideS, int n, int count, double2* R, long strideR,
                            hipStream_t stream);
 int k_conj(const double2* x, double2* y, long n, hipStream_t stream);
@@ -214,6 +220,9 @@ struct afq_job {
   // three bf16 planes (k_split_e1_bf16)
   uint16_t* e1s[2] = {nullptr, nullptr};
   uint16_t* vsplit = nullptr;             // (W, 4, 16 ceil(N/16), 32 ceil(N/32)) split VHS planes
+  // nocc > 64: stored f blocks of the wide exchange path (k_exchange_wide)
+  double2* xwide = nullptr;
+  long xwide_elems = 0;
   // symmetric Cholesky vectors: the VHS product over block-packed columns
   // (sym_packed_cols), used when set_hamiltonian finds L_g = L_g^T; npk = 0 when
   // packing would not save work at this N

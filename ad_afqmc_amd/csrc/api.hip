@@ -84,6 +84,7 @@ void release(afq_job* j) {
   fr(j->e1s[0]);
   fr(j->e1s[1]);
   fr(j->vsplit);
+  fr(j->xwide);
   fr(j->t32[0]);
   fr(j->t32[1]);
   fr(j->rcs[0]);
@@ -153,7 +154,8 @@ int green_pipeline(afq_job* j, const void* up, const void* dn, bool want_inv, bo
         ProfScope ps(j, AFQ_K_OVLP_GEMM, st);
         AFQ_TRY(gemm_f64(g, st));
       }
-      sets[s] = DetInvSet{j->S[s], (long)O * O, O, j->det[s], want_inv ? j->Sinv[s] : nullptr, (long)O * O};
+      // above n = 64 a determinant alone is factored in place in S
+      sets[s] = DetInvSet{j->S[s], (long)O * O, O, j->det[s], want_inv ? j->Sinv[s] : nullptr, (long)O * O, j->S[s]};
     }
     ProfScope ps(j, AFQ_K_DETINV, st);
     AFQ_TRY(batched_det_inv_sets(sets, j->nspin_meas, W, st));
@@ -261,7 +263,7 @@ int energy_pipeline(afq_job* j, const void* up, const void* dn, double2* out, hi
   for (int s = 0; s < j->nspin_meas; ++s) {
     double2* coul = j->coul + (long)s * j->W * j->G;
     double2* kpart = j->kpart + (long)s * j->W * j->kchunk_max;
-    if (j->bf16x3) {
+    if (j->bf16x3 && j->nocc[s] <= kExchangeRegMaxO) {
       {
         ProfScope pm(j, AFQ_K_ENERGY_MISC, st);
         AFQ_TRY(k_split_bf16(reinterpret_cast<const double*>(j->green[s]), true, j->nocc[s], j->N, j->W, j->gsplit,
@@ -272,7 +274,10 @@ int energy_pipeline(afq_job* j, const void* up, const void* dn, double2* out, hi
       continue;
     }
     ProfScope ps(j, AFQ_K_EXCHANGE, st);
-    if (j->lp)
+    if (j->nocc[s] > kExchangeRegMaxO)
+      AFQ_TRY(k_exchange_wide(j->rot_chol[s], j->green[s], j->nocc[s], j->N, j->G, j->W, j->xwide, j->xwide_elems,
+                              coul, kpart, &nch[s], st));
+    else if (j->lp)
       AFQ_TRY(k_exchange_lp(j->rot_chol32[s], j->green[s], j->nocc[s], j->N, j->G, j->W, coul, kpart, &nch[s], st));
     else
       AFQ_TRY(k_exchange(j->rot_chol[s], j->green[s], j->nocc[s], j->N, j->G, j->W, coul, kpart, &nch[s], st));
@@ -734,7 +739,8 @@ int validate_desc(const afq_job_desc* d) {
   if (!(d->dt > 0.0)) return fail(AFQ_E_INVALID, "dt must be positive");
   if (d->precision != AFQ_PREC_FP64 && d->precision != AFQ_PREC_FP32 && d->precision != AFQ_PREC_BF16X3)
     return fail(AFQ_E_INVALID, "unknown precision");
-  if (d->nelec_up > 64 || d->nelec_dn > 64) return fail(AFQ_E_UNSUPPORTED, "nocc > 64 not supported");
+  if (d->nelec_up > kDetInvMaxN || d->nelec_dn > kDetInvMaxN)
+    return fail(AFQ_E_UNSUPPORTED, "nocc > " + std::to_string(kDetInvMaxN) + " not supported");
   if (d->nchol > 65535 || d->n_walkers > 65535) return fail(AFQ_E_UNSUPPORTED, "nchol and n_walkers must be <= 65535");
   return AFQ_OK;
 }
@@ -792,6 +798,11 @@ int alloc_job(afq_job* j, bool dry, int64_t* total) {
     A_(j->Sinv[s], W * O * O);
     A_(j->det[s], W);
     A_(j->green[s], W * O * N);
+  }
+  const int omax = std::max(j->nocc[0], j->nocc[1]);
+  if (omax > kExchangeRegMaxO) {
+    j->xwide_elems = exchange_wide_ws_elems(omax, (int)G, (int)W);
+    A_(j->xwide, j->xwide_elems);
   }
   A_(j->fb, W * G);
   A_(j->xs, W * G);

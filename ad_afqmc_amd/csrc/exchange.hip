@@ -419,18 +419,21 @@ __global__ __launch_bounds__(512, 1) void exchange_split_kernel(const double* __
 // zeros, so every lane loads its 8 consecutive k as one 16-byte vector and no
 // bounds masks or conversions remain in the loop.  The four waves of a
 // work-group share one walker: its Green's-function planes for the next 32-deep
-// step are staged once per work-group in LDS (double-buffered, one barrier per
-// step) instead of being read by every wave from L2; each wave reads its own
-// rot_chol fragments straight from global memory.  The three products hi*hi,
-// lo*hi, hi*lo are issued as three passes over all NB x NB blocks, so
-// consecutive MFMAs never depend on each other.
+// step are copied once per work-group from global memory straight into LDS
+// (global_load_lds, 16 bytes a lane, double-buffered, one barrier per step);
+// each wave reads its own rot_chol fragments into registers one step ahead.
+// The three products lo*hi, hi*hi, hi*lo are issued as three passes over all
+// NB x NB blocks, so consecutive MFMAs never depend on each other; the lo
+// fragments of the next step are requested as soon as the first pass has
+// consumed this step's, the hi ones at the start of the step.
 template <int NB>
 __device__ __forceinline__ void exchange_bf16_body(const v8bf* __restrict__ rcs, const v8bf* __restrict__ gs, int O,
                                                    int Opad, int Npad, int Gn, int nchunk, double2* __restrict__ coul,
                                                    double2* __restrict__ kpart, int w, int chunk) {
   constexpr int ROWS = NB * 16;
-  constexpr int PER = 4 * ROWS * 4;       // v8bf of one step's B stage: 4 planes x ROWS x 4 k-groups
-  constexpr int LD = (PER + 255) / 256;   // v8bf loaded per thread per step
+  constexpr int PER = 4 * ROWS * 4;  // v8bf of one step's B stage: 4 planes x ROWS x 4 k-groups
+  static_assert(PER % 256 == 0, "stage must split into whole 64-lane copies per wave");
+  constexpr int CPW = PER / 256;  // 64-lane copies per wave per step
   __shared__ v8bf bsh[2][PER];
   __shared__ double2 scr[4][16 * 17];
   __shared__ double2 kred[4];
@@ -446,24 +449,30 @@ __device__ __forceinline__ void exchange_bf16_body(const v8bf* __restrict__ rcs,
   const v8bf* Ah = rcs + (long)gg * 2 * plane + r16 * rowv + q4;
   const v8bf* Al = Ah + plane;
   const v8bf* Bw = gs + (long)w * 4 * plane;
-  // stage slot i of a step: plane i / (4 ROWS), row (i / 4) % ROWS, k-group i % 4
-  v8bf stage[LD];
-  auto fetch = [&](int kv) {
+  // LDS slot s of a stage holds plane s / (4 ROWS), row (s / 4) % ROWS and
+  // k-group (s & 3) ^ kswz(row % 16): the four 16-byte k-groups of a row are
+  // permuted so that the 16 rows of a fragment read fall in distinct banks.
+  // Copy t of this wave fills slots [64 (4 t + wave), +64), one per lane, so
+  // each lane fetches the global element its slot holds.
+  long src[CPW];
 #pragma unroll
-    for (int t = 0; t < LD; ++t) {
-      const int i = tid + t * 256;
-      if (i < PER) stage[t] = Bw[(long)(i / (4 * ROWS)) * plane + (long)((i / 4) % ROWS) * rowv + kv + (i & 3)];
-    }
-  };
-  // the four 16-byte k-groups of a row are permuted by kswz(row) so that the
-  // 16 rows of a fragment read fall in distinct banks
-  auto store = [&](int buf) {
+  for (int t = 0; t < CPW; ++t) {
+    const int s = 64 * (4 * t + wave) + lane;
+    const int row = (s >> 2) % ROWS;
+    src[t] = (long)(s / (4 * ROWS)) * plane + (long)row * rowv + ((s & 3) ^ kswz(row & 15));
+  }
+  auto copy = [&](int kv, int buf) {
 #pragma unroll
-    for (int t = 0; t < LD; ++t) {
-      const int i = tid + t * 256;
-      if (i < PER) bsh[buf][(i & ~3) | ((i & 3) ^ kswz((i >> 2) & 15))] = stage[t];
-    }
+    for (int t = 0; t < CPW; ++t)
+      __builtin_amdgcn_global_load_lds(static_cast<const void*>(Bw + src[t] + kv),
+                                       (__attribute__((address_space(3))) void*)&bsh[buf][64 * (4 * t + wave)], 16,
+                                       0, 0);
   };
+  // s_waitcnt vmcnt(0): the copies of this wave have landed in LDS (before
+  // the barrier that publishes them to the other waves), and so have the
+  // rot_chol fragments of the next step.  The builtin lets the compiler count
+  // this wait, so it adds none of its own for those fragments.
+  auto copies_done = [] { __builtin_amdgcn_s_waitcnt(0x0F70); };
 
   v4f accR[NB][NB], accI[NB][NB];
 #pragma unroll
@@ -473,84 +482,75 @@ __device__ __forceinline__ void exchange_bf16_body(const v8bf* __restrict__ rcs,
       accR[i][j] = v4f{0.f, 0.f, 0.f, 0.f};
       accI[i][j] = v4f{0.f, 0.f, 0.f, 0.f};
     }
-  // This wave's rot_chol fragments go straight to registers.  With NB = 4
-  // (two waves per SIMD whatever the register count) the hi ones are loaded
-  // one step ahead and the lo ones at the start of their step, so that their
-  // loads overlap the hi x hi pass; smaller NB keep to the registers that
-  // leave three or more waves per SIMD: both are loaded in their step.
-  constexpr bool PF = NB == 4;
   v8bf ah[NB], al[NB], nh[NB];
-  if (PF) {
 #pragma unroll
-    for (int b = 0; b < NB; ++b) ah[b] = Ah[(long)b * 16 * rowv];
+  for (int b = 0; b < NB; ++b) {
+    ah[b] = Ah[(long)b * 16 * rowv];
+    al[b] = Al[(long)b * 16 * rowv];
   }
-  fetch(0);
-  store(0);
-  __syncthreads();
+  copy(0, 0);
   int buf = 0;
+  // The issue order of a step is pinned with scheduling barriers: B reads,
+  // the next step's copies and hi fragments, the lo x hi pass, the next lo
+  // fragments, then the hi x hi and hi x lo passes.  All LDS reads of a step
+  // precede its copies into the other buffer, so no read waits on a copy.
   for (int kv = 0; kv < rowv; kv += 4) {
-    const bool more = kv + 4 < rowv;
-    if (more) fetch(kv + 4);
-    if (PF && more) {
-#pragma unroll
-      for (int b = 0; b < NB; ++b) nh[b] = Ah[(long)b * 16 * rowv + kv + 4];
-    }
-    if (!PF) {
-#pragma unroll
-      for (int b = 0; b < NB; ++b) ah[b] = Ah[(long)b * 16 * rowv + kv];
-    }
-    if (PF) {
-#pragma unroll
-      for (int b = 0; b < NB; ++b) al[b] = Al[(long)b * 16 * rowv + kv];
-    }
-    const v8bf* B = bsh[buf] + r16 * 4 + (q4 ^ kswz(r16));  // + plane * 4 ROWS + 16 b * 4
-    v8bf br[NB], bi[NB];
-#pragma unroll
-    for (int b = 0; b < NB; ++b) {
-      br[b] = B[b * 64];
-      bi[b] = B[2 * 4 * ROWS + b * 64];
-    }
-#pragma unroll
-    for (int i = 0; i < NB; ++i)
-#pragma unroll
-      for (int j = 0; j < NB; ++j) {
-        accR[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], br[j], accR[i][j], 0, 0, 0);
-        accI[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bi[j], accI[i][j], 0, 0, 0);
-      }
-    if (!PF) {
-#pragma unroll
-      for (int b = 0; b < NB; ++b) al[b] = Al[(long)b * 16 * rowv + kv];
-    }
-#pragma unroll
-    for (int i = 0; i < NB; ++i)
-#pragma unroll
-      for (int j = 0; j < NB; ++j) {
-        accR[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[i], br[j], accR[i][j], 0, 0, 0);
-        accI[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[i], bi[j], accI[i][j], 0, 0, 0);
-      }
-#pragma unroll
-    for (int b = 0; b < NB; ++b) {
-      br[b] = B[4 * ROWS + b * 64];
-      bi[b] = B[3 * 4 * ROWS + b * 64];
-    }
-#pragma unroll
-    for (int i = 0; i < NB; ++i)
-#pragma unroll
-      for (int j = 0; j < NB; ++j) {
-        accR[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], br[j], accR[i][j], 0, 0, 0);
-        accI[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bi[j], accI[i][j], 0, 0, 0);
-      }
-    if (more) {
-      store(buf ^ 1);
-      if (PF) {
-#pragma unroll
-        for (int b = 0; b < NB; ++b) ah[b] = nh[b];
-      }
-    }
+    // this step's stage and fragments have arrived (the wait opens the step,
+    // so that nothing requested before the loop is left to count)
+    copies_done();
     __syncthreads();
+    // the next step's k offset; the last step fetches its own again (into
+    // the idle buffer and registers), which keeps the loop free of branches
+    // and the compiler's load counting exact
+    const int kn = kv + 4 < rowv ? kv + 4 : kv;
+    const v8bf* B = bsh[buf] + r16 * 4 + (q4 ^ kswz(r16));  // + plane * 4 ROWS + 16 b * 4
+    v8bf brh[NB], bih[NB], brl[NB], bil[NB];
+#pragma unroll
+    for (int b = 0; b < NB; ++b) {
+      brh[b] = B[b * 64];
+      bih[b] = B[2 * 4 * ROWS + b * 64];
+    }
+#pragma unroll
+    for (int b = 0; b < NB; ++b) {
+      brl[b] = B[4 * ROWS + b * 64];
+      bil[b] = B[3 * 4 * ROWS + b * 64];
+    }
+    copy(kn, buf ^ 1);
+#pragma unroll
+    for (int b = 0; b < NB; ++b) nh[b] = Ah[(long)b * 16 * rowv + kn];
+    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+    for (int i = 0; i < NB; ++i)
+#pragma unroll
+      for (int j = 0; j < NB; ++j) {
+        accR[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[i], brh[j], accR[i][j], 0, 0, 0);
+        accI[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[i], bih[j], accI[i][j], 0, 0, 0);
+      }
+    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+    for (int b = 0; b < NB; ++b) al[b] = Al[(long)b * 16 * rowv + kn];
+    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+    for (int i = 0; i < NB; ++i)
+#pragma unroll
+      for (int j = 0; j < NB; ++j) {
+        accR[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], brh[j], accR[i][j], 0, 0, 0);
+        accI[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bih[j], accI[i][j], 0, 0, 0);
+      }
+#pragma unroll
+    for (int i = 0; i < NB; ++i)
+#pragma unroll
+      for (int j = 0; j < NB; ++j) {
+        accR[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], brl[j], accR[i][j], 0, 0, 0);
+        accI[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bil[j], accI[i][j], 0, 0, 0);
+      }
+#pragma unroll
+    for (int b = 0; b < NB; ++b) ah[b] = nh[b];
+    __builtin_amdgcn_sched_barrier(0);
     buf ^= 1;
   }
-
+  // no copy into LDS may be in flight when the work-group ends
+  copies_done();
   // accumulator element v of lane l sits at (4(l/16) + v, l%16)
   const int cc = lane & 15;
   double kr = 0.0, ki = 0.0, cr = 0.0, ci = 0.0;
@@ -664,6 +664,58 @@ __global__ void split_bf16_grouped_kernel(const SplitJob* __restrict__ jobs, con
                   reinterpret_cast<__bf16*>(J.out), (long)(item - prefix[j]) * 256 + threadIdx.x);
 }
 
+// c_g = tr f_g and K_g = sum_ij f_ij f_ji from stored f blocks: f holds, for
+// each walker w, gcur consecutive O x O matrices f_{g0 + g} (row-major, one
+// after another).  Work-group (w, c) covers the 4-g chunk c of the pass: it
+// writes coul[w, g0 + 4c + t] and kpart[w, g0 / 4 + c] = sum_t K.
+__global__ __launch_bounds__(256) void exchange_wide_reduce_kernel(const double2* __restrict__ f, int O, int gcur,
+                                                                   int g0, int Gn, int nchunk,
+                                                                   double2* __restrict__ coul,
+                                                                   double2* __restrict__ kpart) {
+  __shared__ double red[4][4];
+  const int w = blockIdx.x, c = blockIdx.y;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const long OO = (long)O * O;
+  double ksr = 0.0, ksi = 0.0;
+  for (int t = 0; t < 4; ++t) {
+    const int g = 4 * c + t;
+    if (g >= gcur) break;  // uniform for the work-group
+    const double2* fg = f + ((long)w * gcur + g) * OO;
+    double kr = 0.0, ki = 0.0, cr = 0.0, ci = 0.0;
+    for (long e = tid; e < OO; e += 256) {
+      const int i = (int)(e / O), jj = (int)(e - (long)i * O);
+      const double2 x = fg[e], y = fg[(long)jj * O + i];
+      kr += x.x * y.x - x.y * y.y;
+      ki += x.x * y.y + x.y * y.x;
+      if (i == jj) {
+        cr += x.x;
+        ci += x.y;
+      }
+    }
+    kr = wave_sum(kr);
+    ki = wave_sum(ki);
+    cr = wave_sum(cr);
+    ci = wave_sum(ci);
+    if (lane == 0) {
+      red[wave][0] = kr;
+      red[wave][1] = ki;
+      red[wave][2] = cr;
+      red[wave][3] = ci;
+    }
+    __syncthreads();
+    if (tid == 0) {
+      double s[4] = {0.0, 0.0, 0.0, 0.0};
+      for (int q = 0; q < 4; ++q)
+        for (int v = 0; v < 4; ++v) s[v] += red[q][v];
+      coul[(long)w * Gn + g0 + g] = double2{s[2], s[3]};
+      ksr += s[0];
+      ksi += s[1];
+    }
+    __syncthreads();
+  }
+  if (tid == 0) kpart[(long)w * nchunk + g0 / 4 + c] = double2{ksr, ksi};
+}
+
 template <int MODE, int NB, int KS, typename TR>
 int launch_nb(const TR* rc, const double2* Gm, int O, int N, int Gn, int W, int nchunk, double2* coul, double2* kpart,
               hipStream_t s) {
@@ -729,6 +781,50 @@ int k_exchange(const double* rc, const double2* Gm, int O, int N, int Gn, int W,
   // 48 < O <= 64: two waves per f-block (the 64 x 64 block alone would take 256 registers)
   if (O > 48) return launch_split<2>(rc, Gm, O, N, Gn, W, nchunk, coul, kpart, s);
   return launch_mode<F64, double>(rc, Gm, O, N, Gn, W, nchunk, coul, kpart, s);
+}
+
+long exchange_wide_ws_elems(int O, int Gn, int W) {
+  // a pass of gc g's (a multiple of 4, at least one chunk) within 256 MB
+  const long per_g = (long)W * O * O;
+  const long budget = (256L << 20) / (long)sizeof(double2);
+  long gc = std::max(4L, budget / std::max(1L, per_g) / 4 * 4);
+  gc = std::min(gc, 4L * exchange_nchunk(Gn));
+  return gc * per_g;
+}
+
+int k_exchange_wide(const double* rc, const double2* Gm, int O, int N, int Gn, int W, double2* ws, long ws_elems,
+                    double2* coul, double2* kpart, int* nchunk_out, hipStream_t s) {
+  if (O < 1 || N < 1 || Gn < 1 || W < 1) return fail(AFQ_E_INVALID, "exchange: sizes must be positive");
+  const int nchunk = exchange_nchunk(Gn);
+  *nchunk_out = nchunk;
+  const long per_g = (long)W * O * O;
+  const long gc = ws_elems / per_g / 4 * 4;
+  if (!ws || gc < 4) return fail(AFQ_E_INVALID, "exchange: workspace below one 4-g chunk");
+  for (long g0 = 0; g0 < Gn; g0 += gc) {
+    const int gcur = (int)std::min(gc, (long)Gn - g0);
+    // f_{g0..g0+gcur}[w] = rot_chol rows (gcur O x N) . G_w^T (N x O)
+    Gemm g;
+    g.M = gcur * O;
+    g.N = O;
+    g.K = N;
+    g.batch = W;
+    g.A = rc + g0 * O * (long)N;
+    g.lda = N;
+    g.sA = 0;
+    g.b_cplx = true;
+    g.b_trans = true;
+    g.B = Gm;
+    g.ldb = N;
+    g.sB = (long)O * N;
+    g.C = ws;
+    g.ldc = O;
+    g.sC = (long)gcur * O * O;
+    AFQ_TRY(gemm_f64(g, s));
+    hipLaunchKernelGGL(exchange_wide_reduce_kernel, dim3(W, (gcur + 3) / 4), dim3(256), 0, s, ws, O, gcur, (int)g0,
+                       Gn, nchunk, coul, kpart);
+    AFQ_TRY(launch_status("exchange_wide_reduce_kernel"));
+  }
+  return AFQ_OK;
 }
 
 int k_exchange_lp(const float* rc, const double2* Gm, int O, int N, int Gn, int W, double2* coul, double2* kpart,

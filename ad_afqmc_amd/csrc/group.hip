@@ -89,9 +89,8 @@ struct afq_group {
   afq::DevTab<afq::Bf3Job> fb, vhs;
   afq::DevTab<afq::ReduceJob> fb_red;
   afq::DevTab<afq::RowSplitJob> xsplit;
-  afq::DevTab<afq::FusedArgs> trot;
+  std::map<int, afq::DevTab<afq::FusedArgs>> trot;  // by 8 x column blocks + row blocks per wave
   afq::DevTab<afq::VhsSplit> vsplit;
-  int trot_tn = 1, trot_rbw = 1;
   std::map<int, afq::DevTab<afq::ExchJob>> exch;  // by ceil(O / 16)
 };
 
@@ -104,6 +103,8 @@ int check_job(const afq_job* j, int nsteps) {
     return fail(AFQ_E_UNSUPPORTED, "group: jobs with restricted walkers and an RHF trial only");
   if (!j->bf16x3 || !j->e1s[0] || !j->vsplit)
     return fail(AFQ_E_UNSUPPORTED, "group: split-BF16 x3 jobs with the fused propagator only (N <= 512)");
+  if (j->nocc[0] > kExchangeRegMaxO)
+    return fail(AFQ_E_UNSUPPORTED, "group: nocc <= 64 only (the register-blocked exchange)");
   if ((long)j->W * j->G * nsteps > j->fields_cap)
     return fail(AFQ_E_STATE, "group: call afq_job_reserve_fields for this n_prop_steps first");
   return AFQ_OK;
@@ -121,20 +122,15 @@ int build(afq_group* g, afq_job* const* jobs, void* const* walkers, double* cons
   std::vector<Bf3Job> fbj(n), vhj(n);
   std::vector<ReduceJob> red(n);
   std::vector<RowSplitJob> rsp(n);
-  std::vector<FusedArgs> fa(n);
+  std::map<int, std::vector<FusedArgs>> fa;
   std::vector<VhsSplit> vsj(n);
   std::map<int, std::vector<ExchJob>> ex;
   std::vector<long> it_gemm_ov(n), it_gemm_gr(n), it_gemm_gm(n), it_gemm_tr(n), it_w(n), it_vec(n), it_sp(n),
-      it_fb(n), it_vh(n), it_red(n), it_rsp(n), it_trot(n), it_vs(n);
-  std::map<int, std::vector<long>> it_ex;
+      it_fb(n), it_vh(n), it_red(n), it_rsp(n), it_vs(n);
+  std::map<int, std::vector<long>> it_ex, it_trot;
   std::vector<long> it_bj[BI_NKINDS];
   for (auto& v : it_bj) v.resize(n);
 
-  // one column-block width for the whole group's propagator launch
-  g->trot_tn = 1;
-  int rmax = 1;
-  for (int i = 0; i < n; ++i) rmax = std::max(rmax, (jobs[i]->N + 15) / 16);
-  g->trot_rbw = (rmax + 7) / 8;
   for (int i = 0; i < n; ++i) {
     afq_job* j = jobs[i];
     const int N = j->N, O = j->nocc[0], G = j->G, W = j->W, c = j->wcols;
@@ -222,7 +218,7 @@ int build(afq_group* g, afq_job* const* jobs, void* const* walkers, double* cons
     gr[i] = gemm_group_job(h);
     it_gemm_gr[i] = (long)gr[i].tiles_m * gr[i].tiles_n * W;
     dinv[i] = DetInvSet{j->S[0], (long)O * O, O, j->det[0], j->Sinv[0], (long)O * O};
-    ddet[i] = DetInvSet{j->S[0], (long)O * O, O, j->det[0], nullptr, (long)O * O};
+    ddet[i] = DetInvSet{j->S[0], (long)O * O, O, j->det[0], nullptr, (long)O * O, j->S[0]};
     it_w[i] = W;
 
     // Cholesky QR (cholqr2_panel): tmp = conj(W); S = tmp^T W; S = R^H R; R^-1; tmp = W R^-1; W = tmp
@@ -333,7 +329,6 @@ int build(afq_group* g, afq_job* const* jobs, void* const* walkers, double* cons
     vsj[i] = VhsSplit{static_cast<const float2*>(v.C), j->sym_chol, N, W, j->vsplit};
     it_vs[i] = vhs_split_items(vsj[i]);
 
-    // fused propagator
     TrotFused tf;
     tf.N = N;
     tf.W = W;
@@ -343,8 +338,13 @@ int build(afq_group* g, afq_job* const* jobs, void* const* walkers, double* cons
     tf.c[0] = c;
     tf.e1[0] = j->e1s[0];
     tf.vs = j->vsplit;
-    AFQ_TRY(trot_fused_args(tf, g->trot_tn, &fa[i]));
-    it_trot[i] = trot_fused_items(fa[i]);
+    // fused propagator, one launch per (column blocks, row blocks per wave)
+    // class: each job runs the configuration its single-job launch picks
+    const int tn = trot_fused_tn(tf.c, 1, N), tkey = 8 * tn + trot_fused_rbw(N);
+    FusedArgs a;
+    AFQ_TRY(trot_fused_args(tf, tn, &a));
+    fa[tkey].push_back(a);
+    it_trot[tkey].push_back(trot_fused_items(a));
 
     // exchange, grouped by the 16-row block count of the f-blocks
     const int nb = (O + 15) / 16;
@@ -371,7 +371,7 @@ int build(afq_group* g, afq_job* const* jobs, void* const* walkers, double* cons
   AFQ_TRY(m.table(rsp, it_rsp, &g->xsplit));
   AFQ_TRY(m.table(vhj, it_vh, &g->vhs));
   AFQ_TRY(m.table(vsj, it_vs, &g->vsplit));
-  AFQ_TRY(m.table(fa, it_trot, &g->trot));
+  for (auto& kv : fa) AFQ_TRY(m.table(kv.second, it_trot[kv.first], &g->trot[kv.first]));
   for (auto& kv : ex) AFQ_TRY(m.table(kv.second, it_ex[kv.first], &g->exch[kv.first]));
   return AFQ_OK;
 }
@@ -399,7 +399,7 @@ int g_block(afq_group* g, hipStream_t st) {
     AFQ_TRY(split_rows_grouped(g->xsplit.view(), st));
     AFQ_TRY(gemm_bf3_grouped(g->vhs.view(), st));
     AFQ_TRY(split_vhs_grouped(g->vsplit.view(), st));
-    AFQ_TRY(trot_fused_grouped(g->trot.view(), g->trot_tn, g->trot_rbw, st));
+    for (auto& kv : g->trot) AFQ_TRY(trot_fused_grouped(kv.second.view(), kv.first / 8, kv.first % 8, st));
     AFQ_TRY(g_green(g, i + 1 < g->nsteps, false, false, st));
     AFQ_TRY(g_overlap(g->bj[BI_WALKER256].view(), false, st));
     AFQ_TRY(g_weight_update(g->bj[BI_JOB].view(), st));

@@ -21,6 +21,13 @@ int k_exchange(const double* rc, const double2* Gm, int O, int N, int Gn, int W,
                int* nchunk_out, hipStream_t s);
 int k_exchange_lp(const float* rc, const double2* Gm, int O, int N, int Gn, int W, double2* coul, double2* kpart,
                   int* nchunk_out, hipStream_t s);
+// nocc > 64 (any precision, FP64 arithmetic): f_g of a pass of g's for all
+// walkers by the batched FP64 product into ws, then c_g and the K_g of each
+// 4-g chunk; ws holds exchange_wide_ws_elems(O, Gn, W) complex elements.
+long exchange_wide_ws_elems(int O, int Gn, int W);
+int k_exchange_wide(const double* rc, const double2* Gm, int O, int N, int Gn, int W, double2* ws, long ws_elems,
+                    double2* coul, double2* kpart, int* nchunk_out, hipStream_t s);
+constexpr int kExchangeRegMaxO = 64;  // largest nocc of the register-blocked kernels
 // Split-BF16 operands of the exchange: rows x (O, N) real (2 planes) or complex
 // (4 planes) doubles -> bf16 (hi, lo) planes of bf16_opad(O) x bf16_npad(N), zero padded.
 int bf16_opad(int O);

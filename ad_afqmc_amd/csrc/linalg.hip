@@ -25,21 +25,29 @@ __device__ inline double2 crecip(double2 a) {
   return {r / den, -1.0 / den};
 }
 
-// One 256-thread work-group per matrix; the matrix lives in dynamic LDS (n*n
-// complex, n <= 64).  Thread t owns column t % 64 for the rows t/64, t/64 + 4,
-// ..., so the elimination needs no index division and reads the pivot row and
-// factor column as LDS broadcasts.  Partial pivoting selects the first row of
-// maximal |re|+|im| in the pivot column (LAPACK izamax convention).  With an
-// inverse requested the sweep is Gauss-Jordan in place (the inverse ends with
-// its columns permuted by the row interchanges, undone at the end); for the
-// determinant alone only the trailing rows and columns are eliminated (the
-// pivots, hence the determinant, are the same).
+// One 256-thread work-group per matrix.  Up to n = 64 the matrix lives in
+// dynamic LDS; above (n <= 256) it is factored in global memory, in the
+// inverse's buffer or the set's scratch (it stays in L2).  Thread t owns the
+// columns t % 64 + 64 c for the rows t/64, t/64 + 4, ..., so the elimination
+// needs no index division and reads the pivot row and factor column as LDS
+// broadcasts.  Partial pivoting selects the first row of maximal |re|+|im| in
+// the pivot column (LAPACK izamax convention).  With an inverse requested the
+// sweep is Gauss-Jordan in place (the inverse ends with its columns permuted
+// by the row interchanges, undone at the end); for the determinant alone only
+// the trailing rows and columns are eliminated (the pivots, hence the
+// determinant, are the same).
 // blockIdx.y selects one of up to two matrix sets (the two spins' overlap
 // matrices factor in one launch, so a CU holds two work-groups at W = 256).
-__device__ __forceinline__ void gj_det_inv_body(const DetInvSet& set, long b, double2* __restrict__ a) {
-  __shared__ double2 prow[64];
-  __shared__ double2 fcol[64];
-  __shared__ int piv[64];
+// NMAX: the largest n of the launch (64 or kDetInvMaxN); each thread covers
+// NMAX / 64 columns (and pivot-search rows).
+constexpr int kLdsN = 64;  // largest n factored in LDS
+
+template <int NMAX>
+__device__ __forceinline__ void gj_det_inv_body(const DetInvSet& set, long b, double2* __restrict__ lds) {
+  constexpr int kColsPer = NMAX / 64;
+  __shared__ double2 prow[NMAX];
+  __shared__ double2 fcol[NMAX];
+  __shared__ int piv[NMAX];
   __shared__ int pk;
   const int n = set.n;
   double2* __restrict__ det = set.det;
@@ -49,7 +57,10 @@ __device__ __forceinline__ void gj_det_inv_body(const DetInvSet& set, long b, do
   const double2* src = set.S + b * set.strideS;
   const int nn = n * n;
   const bool want_inv = inv != nullptr;
-  for (int i = tid; i < nn; i += 256) a[i] = src[i];
+  double2* a = lds;
+  if (n > kLdsN) a = want_inv ? inv + b * set.strideInv : set.work + b * set.strideS;
+  if (a != src)
+    for (int i = tid; i < nn; i += 256) a[i] = src[i];
   __syncthreads();
 
   double2 dprod = {1.0, 0.0};
@@ -58,9 +69,17 @@ __device__ __forceinline__ void gj_det_inv_body(const DetInvSet& set, long b, do
     if (tid < 64) {
       double v = -1.0;
       int idx = tid;
-      if (tid >= k && tid < n) {
-        const double2 x = a[tid * n + k];
-        v = fabs(x.x) + fabs(x.y);
+#pragma unroll
+      for (int m = 0; m < kColsPer; ++m) {
+        const int r = tid + 64 * m;
+        if (r >= k && r < n) {
+          const double2 x = a[r * n + k];
+          const double vr = fabs(x.x) + fabs(x.y);
+          if (vr > v) {  // rows ascend: a tie keeps the lower row
+            v = vr;
+            idx = r;
+          }
+        }
       }
 #pragma unroll
       for (int off = 32; off > 0; off >>= 1) {
@@ -87,38 +106,84 @@ __device__ __forceinline__ void gj_det_inv_body(const DetInvSet& set, long b, do
     // read: the pivot row (scaled) and the rows to swap (group 0), the factor
     // column after the swap (group 1)
     const int jlo = want_inv ? 0 : k;
-    double2 xp = {0.0, 0.0}, xk = {0.0, 0.0}, fc = {0.0, 0.0};
-    if (rg == 0 && j >= jlo && j < n) {
-      xp = a[p * n + j];
-      xk = a[k * n + j];
-    } else if (rg == 1 && j < n) {
-      fc = (j == k) ? d : a[(j == p ? k : j) * n + k];
+    double2 xp[kColsPer], xk[kColsPer], fc[kColsPer];
+#pragma unroll
+    for (int c = 0; c < kColsPer; ++c) {
+      const int jc = j + 64 * c;
+      xp[c] = xk[c] = fc[c] = double2{0.0, 0.0};
+      if (rg == 0 && jc >= jlo && jc < n) {
+        xp[c] = a[p * n + jc];
+        xk[c] = a[k * n + jc];
+      } else if (rg == 1 && jc < n) {
+        fc[c] = (jc == k) ? d : a[(jc == p ? k : jc) * n + k];
+      }
     }
     __syncthreads();
-    if (rg == 0 && j >= jlo && j < n) {
-      const double2 r = (j == k) ? dinv : cmul(xp, dinv);
-      prow[j] = r;
-      a[k * n + j] = r;
-      if (p != k) a[p * n + j] = xk;
-    } else if (rg == 1 && j < n) {
-      fcol[j] = fc;
+#pragma unroll
+    for (int c = 0; c < kColsPer; ++c) {
+      const int jc = j + 64 * c;
+      if (rg == 0 && jc >= jlo && jc < n) {
+        const double2 r = (jc == k) ? dinv : cmul(xp[c], dinv);
+        prow[jc] = r;
+        a[k * n + jc] = r;
+        if (p != k) a[p * n + jc] = xk[c];
+      } else if (rg == 1 && jc < n) {
+        fcol[jc] = fc[c];
+      }
     }
     __syncthreads();
-    if (j < n && j >= jlo) {
-      const double2 rk = prow[j];
-      const int i0 = want_inv ? rg : k + 1 + rg;
+    const int i0 = want_inv ? rg : k + 1 + rg;
+    if constexpr (NMAX == kLdsN) {
+      // all of this thread's reads first (one LDS latency per step: the
+      // compiler cannot tell the matrix from fcol, so a read-modify-write
+      // loop would wait for every store before the next read)
+      if (j < n && j >= jlo) {
+        const double2 rk = prow[j];
+        double2 xv[16], fv[16];
+#pragma unroll
+        for (int t = 0; t < 16; ++t) {
+          const int i = i0 + 4 * t;
+          xv[t] = fv[t] = double2{0.0, 0.0};
+          if (i < n) {
+            fv[t] = fcol[i];
+            xv[t] = a[i * n + j];
+          }
+        }
+#pragma unroll
+        for (int t = 0; t < 16; ++t) {
+          const int i = i0 + 4 * t;
+          if (i < n && i != k) {
+            const double2 f = fv[t];
+            if (j == k) {
+              xv[t] = double2{-(f.x * rk.x - f.y * rk.y), -(f.x * rk.y + f.y * rk.x)};
+            } else {
+              xv[t].x -= f.x * rk.x - f.y * rk.y;
+              xv[t].y -= f.x * rk.y + f.y * rk.x;
+            }
+            a[i * n + j] = xv[t];
+          }
+        }
+      }
+    } else {
+#pragma unroll
+      for (int c = 0; c < kColsPer; ++c) {
+        const int jc = j + 64 * c;
+        if (jc < n && jc >= jlo) {
+          const double2 rk = prow[jc];
 #pragma unroll 4
-      for (int i = i0; i < n; i += 4) {
-        if (i == k) continue;
-        const double2 f = fcol[i];
-        double2* x = &a[i * n + j];
-        if (j == k) {
-          *x = double2{-(f.x * rk.x - f.y * rk.y), -(f.x * rk.y + f.y * rk.x)};
-        } else {
-          double2 y = *x;
-          y.x -= f.x * rk.x - f.y * rk.y;
-          y.y -= f.x * rk.y + f.y * rk.x;
-          *x = y;
+          for (int i = i0; i < n; i += 4) {
+            if (i == k) continue;
+            const double2 f = fcol[i];
+            double2* x = &a[i * n + jc];
+            if (jc == k) {
+              *x = double2{-(f.x * rk.x - f.y * rk.y), -(f.x * rk.y + f.y * rk.x)};
+            } else {
+              double2 y = *x;
+              y.x -= f.x * rk.x - f.y * rk.y;
+              y.y -= f.x * rk.y + f.y * rk.x;
+              *x = y;
+            }
+          }
         }
       }
     }
@@ -140,23 +205,26 @@ __device__ __forceinline__ void gj_det_inv_body(const DetInvSet& set, long b, do
     }
     __syncthreads();
     double2* dst = inv + b * set.strideInv;
-    for (int i = tid; i < nn; i += 256) dst[i] = a[i];
+    if (dst != a)
+      for (int i = tid; i < nn; i += 256) dst[i] = a[i];
   }
 }
 
+template <int NMAX>
 __global__ __launch_bounds__(256) void gj_det_inv_kernel(DetInvSet s0, DetInvSet s1) {
   extern __shared__ double2 lds_gj[];
-  gj_det_inv_body(blockIdx.y == 0 ? s0 : s1, blockIdx.x, lds_gj);
+  gj_det_inv_body<NMAX>(blockIdx.y == 0 ? s0 : s1, blockIdx.x, lds_gj);
 }
 
 // Grouped: work-group i factors matrix i - prefix[j] of job j's set.
+template <int NMAX>
 __global__ __launch_bounds__(256) void gj_det_inv_grouped_kernel(const DetInvSet* __restrict__ sets,
                                                                  const int* __restrict__ prefix, int njobs) {
   extern __shared__ double2 lds_gj[];
   const int item = blockIdx.x;
   const int j = group_find(prefix, njobs, item);
   const DetInvSet set = sets[j];
-  gj_det_inv_body(set, item - prefix[j], lds_gj);
+  gj_det_inv_body<NMAX>(set, item - prefix[j], lds_gj);
 }
 
 __device__ inline double wave_sum(double v) {
@@ -218,58 +286,111 @@ __global__ __launch_bounds__(256) void cgs2_kernel(double2* __restrict__ Wp, lon
 }
 
 // Upper Cholesky factor R (S = R^H R, positive real diagonal) of `count`
-// Hermitian positive-definite n x n matrices (n <= 64), one 256-thread
-// work-group per matrix in dynamic LDS; thread t owns column t % 64 for the
-// rows t/64, t/64 + 4, ...  The strict lower triangle of R is written as zero.
+// Hermitian positive-definite n x n matrices (n <= 256), one 256-thread
+// work-group per matrix, in dynamic LDS up to n = 64 and in place in R above;
+// thread t owns the columns t % 64 + 64 c for the rows t/64, t/64 + 4, ...
+// The strict lower triangle of R is written as zero.
+template <int NMAX>
 __device__ __forceinline__ void cholesky_upper_body(const double2* __restrict__ S, long strideS, int n,
                                                     double2* __restrict__ R, long strideR, long b,
-                                                    double2* __restrict__ a) {
+                                                    double2* __restrict__ lds) {
+  constexpr int kColsPer = NMAX / 64;
   const int tid = threadIdx.x;
   const int j = tid & 63, rg = tid >> 6;
   const double2* src = S + b * strideS;
+  double2* dst = R + b * strideR;
   const int nn = n * n;
-  for (int i = tid; i < nn; i += 256) a[i] = src[i];
+  double2* a = n > kLdsN ? dst : lds;
+  if (a != src)
+    for (int i = tid; i < nn; i += 256) a[i] = src[i];
   __syncthreads();
   for (int k = 0; k < n; ++k) {
     const double dinv = 1.0 / sqrt(a[k * n + k].x);
     __syncthreads();
-    if (rg == 0 && j >= k && j < n) {
-      const double2 x = a[k * n + j];
-      a[k * n + j] = (j == k) ? double2{1.0 / dinv, 0.0} : double2{x.x * dinv, x.y * dinv};
+#pragma unroll
+    for (int c = 0; c < kColsPer; ++c) {
+      const int jc = j + 64 * c;
+      if (rg == 0 && jc >= k && jc < n) {
+        const double2 x = a[k * n + jc];
+        a[k * n + jc] = (jc == k) ? double2{1.0 / dinv, 0.0} : double2{x.x * dinv, x.y * dinv};
+      }
     }
     __syncthreads();
-    if (j > k && j < n) {
-      const double2 rkj = a[k * n + j];
-      for (int i = k + 1 + rg; i <= j; i += 4) {
-        const double2 rki = a[k * n + i];  // S_ij -= conj(R_ki) R_kj
-        double2 x = a[i * n + j];
-        x.x -= rki.x * rkj.x + rki.y * rkj.y;
-        x.y -= rki.x * rkj.y - rki.y * rkj.x;
-        a[i * n + j] = x;
+    if constexpr (NMAX == kLdsN) {
+      // reads first, then the updates (as in the Gauss-Jordan sweep)
+      if (j > k && j < n) {
+        const double2 rkj = a[k * n + j];
+        double2 xv[16], rv[16];
+#pragma unroll
+        for (int t = 0; t < 16; ++t) {
+          const int i = k + 1 + rg + 4 * t;
+          xv[t] = rv[t] = double2{0.0, 0.0};
+          if (i <= j) {
+            rv[t] = a[k * n + i];
+            xv[t] = a[i * n + j];
+          }
+        }
+#pragma unroll
+        for (int t = 0; t < 16; ++t) {
+          const int i = k + 1 + rg + 4 * t;
+          if (i <= j) {  // S_ij -= conj(R_ki) R_kj
+            xv[t].x -= rv[t].x * rkj.x + rv[t].y * rkj.y;
+            xv[t].y -= rv[t].x * rkj.y - rv[t].y * rkj.x;
+            a[i * n + j] = xv[t];
+          }
+        }
+      }
+    } else {
+#pragma unroll
+      for (int c = 0; c < kColsPer; ++c) {
+        const int jc = j + 64 * c;
+        if (jc > k && jc < n) {
+          const double2 rkj = a[k * n + jc];
+          for (int i = k + 1 + rg; i <= jc; i += 4) {
+            const double2 rki = a[k * n + i];  // S_ij -= conj(R_ki) R_kj
+            double2 x = a[i * n + jc];
+            x.x -= rki.x * rkj.x + rki.y * rkj.y;
+            x.y -= rki.x * rkj.y - rki.y * rkj.x;
+            a[i * n + jc] = x;
+          }
+        }
       }
     }
     __syncthreads();
   }
-  double2* dst = R + b * strideR;
   for (int i = tid; i < nn; i += 256) {
     const int r = i / n, c = i - r * n;
-    dst[i] = (c >= r) ? a[i] : double2{0.0, 0.0};
+    if (c < r)
+      dst[i] = double2{0.0, 0.0};
+    else if (dst != a)
+      dst[i] = a[i];
   }
 }
 
+template <int NMAX>
 __global__ __launch_bounds__(256) void cholesky_upper_kernel(const double2* __restrict__ S, long strideS, int n,
                                                              double2* __restrict__ R, long strideR) {
   extern __shared__ double2 lds_ch[];
-  cholesky_upper_body(S, strideS, n, R, strideR, blockIdx.x, lds_ch);
+  cholesky_upper_body<NMAX>(S, strideS, n, R, strideR, blockIdx.x, lds_ch);
 }
 
+template <int NMAX>
 __global__ __launch_bounds__(256) void cholesky_grouped_kernel(const CholSet* __restrict__ sets,
                                                               const int* __restrict__ prefix, int njobs) {
   extern __shared__ double2 lds_ch[];
   const int item = blockIdx.x;
   const int j = group_find(prefix, njobs, item);
   const CholSet c = sets[j];
-  cholesky_upper_body(c.S, c.strideS, c.n, c.R, c.strideR, item - prefix[j], lds_ch);
+  cholesky_upper_body<NMAX>(c.S, c.strideS, c.n, c.R, c.strideR, item - prefix[j], lds_ch);
+}
+
+// Raises a kernel's dynamic LDS limit to 96 KB (once per kernel).
+int raise_lds(const void* kernel, bool* done, const char* what) {
+  if (*done) return AFQ_OK;
+  hipError_t e = hipFuncSetAttribute(kernel, hipFuncAttributeMaxDynamicSharedMemorySize, 96 * 1024);
+  if (e != hipSuccess) return fail(AFQ_E_HIP, std::string(what) + " LDS attribute: " + hipGetErrorString(e));
+  *done = true;
+  return AFQ_OK;
 }
 
 __global__ void conj_kernel(const double2* __restrict__ x, double2* __restrict__ y, long n) {
@@ -297,43 +418,59 @@ __global__ __launch_bounds__(256) void vec_grouped_kernel(const VecJob* __restri
 
 }  // namespace
 
-int batched_det_inv_sets(const DetInvSet* sets, int nsets, int count, hipStream_t stream) {
-  if (nsets < 1 || nsets > 2) return fail(AFQ_E_INVALID, "det/inverse: one or two matrix sets");
-  int nmax = 0;
-  for (int i = 0; i < nsets; ++i) {
-    if (sets[i].n < 1 || sets[i].n > 64) return fail(AFQ_E_UNSUPPORTED, "det/inverse kernel supports 1 <= nocc <= 64");
-    nmax = std::max(nmax, sets[i].n);
-  }
-  if (count <= 0) return AFQ_OK;
-  const size_t lds = sizeof(double2) * nmax * nmax;
-  if (lds + 2048 > 65536) {
-    static bool raised = false;
-    if (!raised) {
-      hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(gj_det_inv_kernel),
-                                         hipFuncAttributeMaxDynamicSharedMemorySize, 96 * 1024);
-      if (e != hipSuccess) return fail(AFQ_E_HIP, std::string("det/inverse LDS attribute: ") + hipGetErrorString(e));
-      raised = true;
-    }
-  }
-  hipLaunchKernelGGL(gj_det_inv_kernel, dim3(count, nsets), dim3(256), lds, stream, sets[0],
+// Dynamic LDS of the det/inverse and Cholesky kernels: the largest matrix
+// factored in LDS (n <= 64; larger ones are factored in global memory).
+static size_t matrix_lds_bytes(int nmax) {
+  const int m = std::min(nmax, kLdsN);
+  return sizeof(double2) * m * m;
+}
+
+static int check_det_inv_set(const DetInvSet& s) {
+  if (s.n < 1 || s.n > kDetInvMaxN)
+    return fail(AFQ_E_UNSUPPORTED, "det/inverse kernel supports 1 <= n <= " + std::to_string(kDetInvMaxN));
+  if (s.n > kLdsN && !s.inv && !s.work)
+    return fail(AFQ_E_INVALID, "det/inverse: n > 64 without an inverse needs a scratch matrix per entry");
+  return AFQ_OK;
+}
+
+template <int NMAX>
+static int launch_det_inv(const DetInvSet* sets, int nsets, int count, size_t lds, hipStream_t stream) {
+  static bool raised = false;
+  AFQ_TRY(raise_lds(reinterpret_cast<const void*>(gj_det_inv_kernel<NMAX>), &raised, "det/inverse"));
+  hipLaunchKernelGGL(gj_det_inv_kernel<NMAX>, dim3(count, nsets), dim3(256), lds, stream, sets[0],
                      sets[nsets > 1 ? 1 : 0]);
   return launch_status("gj_det_inv_kernel");
 }
 
+int batched_det_inv_sets(const DetInvSet* sets, int nsets, int count, hipStream_t stream) {
+  if (nsets < 1 || nsets > 2) return fail(AFQ_E_INVALID, "det/inverse: one or two matrix sets");
+  int nmax = 0;
+  for (int i = 0; i < nsets; ++i) {
+    AFQ_TRY(check_det_inv_set(sets[i]));
+    nmax = std::max(nmax, sets[i].n);
+  }
+  if (count <= 0) return AFQ_OK;
+  const size_t lds = matrix_lds_bytes(nmax);
+  return nmax <= kLdsN ? launch_det_inv<kLdsN>(sets, nsets, count, lds, stream)
+                       : launch_det_inv<kDetInvMaxN>(sets, nsets, count, lds, stream);
+}
+
+template <int NMAX>
+static int launch_det_inv_grouped(const GroupTab<DetInvSet>& t, size_t lds, hipStream_t stream) {
+  static bool raised = false;
+  AFQ_TRY(raise_lds(reinterpret_cast<const void*>(gj_det_inv_grouped_kernel<NMAX>), &raised, "det/inverse"));
+  hipLaunchKernelGGL(gj_det_inv_grouped_kernel<NMAX>, dim3((unsigned)t.total), dim3(256), lds, stream, t.params,
+                     t.prefix, t.njobs);
+  return launch_status("gj_det_inv_grouped_kernel");
+}
+
 int det_inv_grouped(const GroupTab<DetInvSet>& t, int nmax, hipStream_t stream) {
   if (t.njobs < 1 || t.total < 1) return AFQ_OK;
-  if (nmax < 1 || nmax > 64) return fail(AFQ_E_UNSUPPORTED, "det/inverse kernel supports 1 <= nocc <= 64");
-  const size_t lds = sizeof(double2) * nmax * nmax;
-  static bool raised = false;
-  if (!raised && lds + 2048 > 65536) {
-    hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(gj_det_inv_grouped_kernel),
-                                       hipFuncAttributeMaxDynamicSharedMemorySize, 96 * 1024);
-    if (e != hipSuccess) return fail(AFQ_E_HIP, std::string("det/inverse LDS attribute: ") + hipGetErrorString(e));
-    raised = true;
-  }
-  hipLaunchKernelGGL(gj_det_inv_grouped_kernel, dim3((unsigned)t.total), dim3(256), lds, stream, t.params, t.prefix,
-                     t.njobs);
-  return launch_status("gj_det_inv_grouped_kernel");
+  if (nmax < 1 || nmax > kDetInvMaxN)
+    return fail(AFQ_E_UNSUPPORTED, "det/inverse kernel supports 1 <= n <= " + std::to_string(kDetInvMaxN));
+  const size_t lds = matrix_lds_bytes(nmax);
+  return nmax <= kLdsN ? launch_det_inv_grouped<kLdsN>(t, lds, stream)
+                       : launch_det_inv_grouped<kDetInvMaxN>(t, lds, stream);
 }
 
 int batched_det_inv(const double2* S, long strideS, int n, int count, double2* det, double2* inv, long strideInv,
@@ -342,38 +479,41 @@ int batched_det_inv(const double2* S, long strideS, int n, int count, double2* d
   return batched_det_inv_sets(&set, 1, count, stream);
 }
 
+template <int NMAX>
+static int launch_cholesky(const double2* S, long strideS, int n, int count, double2* R, long strideR, size_t lds,
+                           hipStream_t stream) {
+  static bool raised = false;
+  AFQ_TRY(raise_lds(reinterpret_cast<const void*>(cholesky_upper_kernel<NMAX>), &raised, "Cholesky"));
+  hipLaunchKernelGGL(cholesky_upper_kernel<NMAX>, dim3(count), dim3(256), lds, stream, S, strideS, n, R, strideR);
+  return launch_status("cholesky_upper_kernel");
+}
+
 int batched_cholesky_upper(const double2* S, long strideS, int n, int count, double2* R, long strideR,
                            hipStream_t stream) {
-  if (n < 1 || n > 64) return fail(AFQ_E_UNSUPPORTED, "Cholesky kernel supports 1 <= n <= 64");
+  if (n < 1 || n > kDetInvMaxN)
+    return fail(AFQ_E_UNSUPPORTED, "Cholesky kernel supports 1 <= n <= " + std::to_string(kDetInvMaxN));
   if (count <= 0) return AFQ_OK;
-  const size_t lds = sizeof(double2) * n * n;
-  if (lds > 65536) {
-    static bool raised = false;
-    if (!raised) {
-      hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(cholesky_upper_kernel),
-                                         hipFuncAttributeMaxDynamicSharedMemorySize, 96 * 1024);
-      if (e != hipSuccess) return fail(AFQ_E_HIP, std::string("Cholesky LDS attribute: ") + hipGetErrorString(e));
-      raised = true;
-    }
-  }
-  hipLaunchKernelGGL(cholesky_upper_kernel, dim3(count), dim3(256), lds, stream, S, strideS, n, R, strideR);
-  return launch_status("cholesky_upper_kernel");
+  const size_t lds = matrix_lds_bytes(n);
+  return n <= kLdsN ? launch_cholesky<kLdsN>(S, strideS, n, count, R, strideR, lds, stream)
+                    : launch_cholesky<kDetInvMaxN>(S, strideS, n, count, R, strideR, lds, stream);
+}
+
+template <int NMAX>
+static int launch_cholesky_grouped(const GroupTab<CholSet>& t, size_t lds, hipStream_t stream) {
+  static bool raised = false;
+  AFQ_TRY(raise_lds(reinterpret_cast<const void*>(cholesky_grouped_kernel<NMAX>), &raised, "Cholesky"));
+  hipLaunchKernelGGL(cholesky_grouped_kernel<NMAX>, dim3((unsigned)t.total), dim3(256), lds, stream, t.params,
+                     t.prefix, t.njobs);
+  return launch_status("cholesky_grouped_kernel");
 }
 
 int cholesky_grouped(const GroupTab<CholSet>& t, int nmax, hipStream_t stream) {
   if (t.njobs < 1 || t.total < 1) return AFQ_OK;
-  if (nmax < 1 || nmax > 64) return fail(AFQ_E_UNSUPPORTED, "Cholesky kernel supports 1 <= n <= 64");
-  const size_t lds = sizeof(double2) * nmax * nmax;
-  static bool raised = false;
-  if (!raised && lds > 32768) {
-    hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(cholesky_grouped_kernel),
-                                       hipFuncAttributeMaxDynamicSharedMemorySize, 64 * 1024);
-    if (e != hipSuccess) return fail(AFQ_E_HIP, std::string("Cholesky LDS attribute: ") + hipGetErrorString(e));
-    raised = true;
-  }
-  hipLaunchKernelGGL(cholesky_grouped_kernel, dim3((unsigned)t.total), dim3(256), lds, stream, t.params, t.prefix,
-                     t.njobs);
-  return launch_status("cholesky_grouped_kernel");
+  if (nmax < 1 || nmax > kDetInvMaxN)
+    return fail(AFQ_E_UNSUPPORTED, "Cholesky kernel supports 1 <= n <= " + std::to_string(kDetInvMaxN));
+  const size_t lds = matrix_lds_bytes(nmax);
+  return nmax <= kLdsN ? launch_cholesky_grouped<kLdsN>(t, lds, stream)
+                       : launch_cholesky_grouped<kDetInvMaxN>(t, lds, stream);
 }
 
 int vec_grouped(const GroupTab<VecJob>& t, hipStream_t stream) {

@@ -53,5 +53,9 @@ int trot_fused_bf3(const TrotFused& p, hipStream_t st);
 int trot_fused_args(const TrotFused& p, int tn, FusedArgs* out);
 long trot_fused_items(const FusedArgs& a);
 int trot_fused_grouped(const GroupTab<FusedArgs>& t, int tn, int rbw, hipStream_t st);
+// Column blocks per slab (1..3) the single-job launch picks for these
+// columns, and the row blocks per wave of its LDS planes for N rows.
+int trot_fused_tn(const int* c, int nspin, int N);
+int trot_fused_rbw(int N);
 
 }  // namespace afq

@@ -516,6 +516,10 @@ int fused_tn(const int* c, int nspin, int N) {
 
 bool trot_fused_supported(int N) { return N >= 1 && N <= 512; }
 
+int trot_fused_tn(const int* c, int nspin, int N) { return fused_tn(c, nspin, N); }
+
+int trot_fused_rbw(int N) { return ((N + 15) / 16 + FW - 1) / FW; }
+
 long vhs_split_items(const VhsSplit& v) {
   const long nb = (v.N + 15) / 16;
   const long nblk = v.sym ? nb * (nb + 1) / 2 : nb * nb;

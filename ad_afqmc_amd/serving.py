@@ -266,104 +266,150 @@ class GpuExecutor:
         h.clear()
 
 
-class GroupedExecutor(GpuExecutor):
-    """Runs all admitted jobs together: every round issues one grouped block
-    (afq_group_sr_block: each kernel of the block launched once over all
-    member jobs) on one stream, for every admitted job with blocks left.  A job
-    admitted while a round is in flight joins the next round; a finished job
-    leaves at the end of its last round.  The group (the device table of the
-    members' launch parameters) is rebuilt when the member set changes, and
-    with use_graphs a member set that lasts more than one round is captured
-    into a HIP graph on its second round and replayed afterwards.  Jobs must
-    be restricted-walker RHF jobs in bf16x3 (the grouped block's scope)."""
+class _Lane:
+    """One size class of GroupedExecutor: its stream, live jobs, current group
+    (and graph), the events of its rounds in flight and retired groups waiting
+    for their last round."""
 
-    def __init__(self, n_prop_steps: int = 10, use_graphs: bool = True):
+    def __init__(self, stream):
+        self.stream = stream
+        self.live = []
+        self.members = ()
+        self.group = None
+        self.graph = None
+        self.rounds_same = 0
+        self.inflight = deque()  # events of issued rounds, oldest first
+        self.retired = []  # (group, graph, event of its last round)
+
+    def busy(self) -> int:
+        while self.inflight and self.inflight[0].query():
+            self.inflight.popleft()
+        return len(self.inflight)
+
+    def reap(self):
+        """Closes the retired groups whose last round has finished."""
+        keep = []
+        for grp, graph, ev in self.retired:
+            if ev.query():
+                grp.close()
+            else:
+                keep.append((grp, graph, ev))
+        self.retired = keep
+
+
+class GroupedExecutor(GpuExecutor):
+    """Runs admitted jobs in groups: every round issues one grouped block
+    (afq_group_sr_block: each kernel of the block launched once over all
+    member jobs) for every member with blocks left.
+
+    Jobs are sorted into size classes by N (`lanes`: the class bounds), one
+    group and one stream per class, so that a class's rounds last about as
+    long as its members' blocks and the classes' rounds overlap on the device.
+    A class keeps up to `depth` rounds in flight: the next round is queued
+    behind the current one instead of waiting for it.  A job admitted while
+    rounds are in flight joins the next round; a job whose blocks are all
+    issued leaves.  When the member set changes the class's group (the device
+    tables of the members' launch parameters) is rebuilt; the old one is
+    closed once its last round has finished.  With use_graphs a member set
+    that lasts more than one round is captured into a HIP graph on its second
+    round and replayed afterwards.  Jobs must be restricted-walker RHF jobs in
+    bf16x3 (the grouped block's scope)."""
+
+    def __init__(self, n_prop_steps: int = 10, use_graphs: bool = True, lanes=(100, 175), depth: int = 2):
         super().__init__(n_prop_steps, use_graphs)
-        self._stream = None
-        self._members = ()
-        self._group = None
-        self._graph = None
-        self._rounds_same = 0
-        self._event = None
+        self.bounds = tuple(int(b) for b in lanes)
+        self.depth = max(1, int(depth))
+        self._lanes = None
         self.rounds = 0
+
+    def _lane_list(self):
+        if self._lanes is None:
+            self._lanes = [_Lane(self.torch.cuda.Stream()) for _ in range(len(self.bounds) + 1)]
+        return self._lanes
+
+    def _lane(self, spec: JobSpec) -> _Lane:
+        k = sum(1 for b in self.bounds if spec.norb > b)
+        return self._lane_list()[k]
 
     def start(self, spec: JobSpec):
         if spec.precision != "bf16x3":
             raise ValueError("grouped serving runs bf16x3 jobs")
         h = super().start(spec)
-        self._live.append(h)
+        h["lane"] = self._lane(spec)
+        h["lane"].live.append(h)
         return h
 
-    def _round_done(self) -> bool:
-        return self._event is None or self._event.query()
-
     def ready(self, h) -> bool:
-        return h["issued"] < h["nblocks"] and h["event"].query() and self._round_done()
+        if h["issued"] >= h["nblocks"]:
+            return False
+        if h["issued"] == 0 and not h["event"].query():
+            return False  # setup still running
+        return h["lane"].busy() < self.depth
 
-    def _drop_group(self):
-        if self._group is not None:
-            if self._event is not None:
-                self._event.synchronize()
-            self._graph = None
-            self._group.close()
-            self._group = None
-            self._members = ()
-            self._rounds_same = 0
+    def _retire(self, lane: _Lane):
+        if lane.group is not None:
+            ev = self.torch.cuda.Event()
+            ev.record(lane.stream)
+            lane.retired.append((lane.group, lane.graph, ev))
+        lane.group, lane.graph, lane.members, lane.rounds_same = None, None, (), 0
 
     def issue(self, h):
-        """Issues one grouped block for every admitted job with blocks left
-        (h among them); other members' issue() calls in this round find them
-        already issued."""
+        """Queues one grouped block for every job of h's class with blocks left
+        whose setup has finished (h among them); the other members' issue()
+        calls find them already issued."""
         torch = self.torch
         from .engine import JobGroup
 
-        if not self._round_done():
+        lane = h["lane"]
+        lane.reap()
+        if lane.busy() >= self.depth:
             return
-        if self._stream is None:
-            self._stream = torch.cuda.Stream()
-        live = [x for x in self._live if x["issued"] < x["nblocks"] and x["event"].query()]
+        live = [x for x in lane.live if x["issued"] < x["nblocks"] and (x["issued"] > 0 or x["event"].query())]
+        if not live:
+            return
         members = tuple(id(x) for x in live)
-        stream = self._stream
+        stream = lane.stream
         with torch.cuda.stream(stream):
-            if members != self._members:
-                self._drop_group()
+            if members != lane.members:
+                self._retire(lane)
                 for x in live:
-                    stream.wait_stream(x["stream"])
-                self._group = JobGroup([(x["job"], x["up"], x["weights"], x["overlaps"], x["scalars"], x["counter"],
-                                         x["out"], x["spec"].seed) for x in live], self.n_prop_steps)
-                self._members = members
-            if self._graph is not None:
-                self._graph.replay()
-            elif self.use_graphs and self._rounds_same == 1:
+                    if x["issued"] == 0:
+                        stream.wait_stream(x["stream"])
+                lane.group = JobGroup([(x["job"], x["up"], x["weights"], x["overlaps"], x["scalars"], x["counter"],
+                                        x["out"], x["spec"].seed) for x in live], self.n_prop_steps)
+                lane.members = members
+            if lane.graph is not None:
+                lane.graph.replay()
+            elif self.use_graphs and lane.rounds_same == 1:
                 g = torch.cuda.CUDAGraph()
                 with torch.cuda.graph(g, stream=stream):
-                    self._group.sr_block()  # recorded, not run
-                self._graph = g
+                    lane.group.sr_block()  # recorded, not run
+                lane.graph = g
                 g.replay()
             else:
-                self._group.sr_block()
-            self._rounds_same += 1
+                lane.group.sr_block()
+            lane.rounds_same += 1
             for x in live:
                 x["outs"].append(x["out"].clone())
                 x["issued"] += 1
             ev = torch.cuda.Event()
             ev.record(stream)
-            self._event = ev
+            lane.inflight.append(ev)
             for x in live:
                 x["event"] = ev
         self.rounds += 1
-
-    @property
-    def _live(self):
-        return self.__dict__.setdefault("_handles", [])
 
     def done(self, h) -> bool:
         return h["issued"] >= h["nblocks"] and h["event"].query()
 
     def release(self, h):
-        self._drop_group()
-        if h in self._live:
-            self._live.remove(h)
+        lane = h["lane"]
+        if h in lane.live:
+            lane.live.remove(h)
+        if id(h) in lane.members:
+            # the group refers to this job's buffers: it must not be replayed again
+            self._retire(lane)
+        lane.reap()
         super().release(h)
 
 

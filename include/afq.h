@@ -250,8 +250,8 @@ int64_t afq_walkers_sr_noalloc_max_walkers(void);
  * fragments) run one job after another (lnoutils.py:175-177): one afq_sr_block
  * (sampling.py:52-94, 146-168 with the comb of sr.py:24-48) for every job of a
  * group, each kernel of the block launched once over all the jobs.  Jobs:
- * restricted walkers of an RHF trial, split-BF16 x3 precision, N <= 512, with
- * their fields reserved for n_prop_steps.  Per job i the group keeps the
+ * restricted walkers of an RHF trial, split-BF16 x3 precision, N <= 512,
+ * nocc <= 64, with their fields reserved for n_prop_steps.  Per job i the group keeps the
  * caller's walkers[i] (W, N, O) c128, weights[i] (W,), overlaps[i] (W,) c128,
  * scalars[i] (e_estimate, pop_control_ene_shift), rng_counters[i] (device
  * uint64) and block_out[i] (block energy, block weight), all device buffers

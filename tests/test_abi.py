@@ -40,9 +40,9 @@ def test_version_and_validation():
     bad.dt = -1.0
     assert L.afq_job_create(C.byref(h), C.byref(bad)) == -1
     bad.dt = 0.01
-    bad.nelec_up = 70
-    bad.norb = 80
-    assert L.afq_job_create(C.byref(h), C.byref(bad)) == -5  # nocc > 64 unsupported
+    bad.nelec_up = 300
+    bad.norb = 320
+    assert L.afq_job_create(C.byref(h), C.byref(bad)) == -5  # nocc > 256 unsupported
     assert L.afq_walkers_sr_workspace(0, 10) == 0
     assert L.afq_walkers_sr_workspace(100, 10) >= 100 * 10 * 16
 

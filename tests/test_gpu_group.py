@@ -16,6 +16,8 @@ SPECS = [
     JobSpec(job_id=0, norb=30, nocc=6, nchol=150, n_walkers=64, n_steps=30, seed=11, precision="bf16x3"),
     JobSpec(job_id=1, norb=52, nocc=10, nchol=260, n_walkers=96, n_steps=30, seed=12, precision="bf16x3"),
     JobSpec(job_id=2, norb=75, nocc=17, nchol=375, n_walkers=130, n_steps=30, seed=13, precision="bf16x3"),
+    # three column blocks per propagator slab and two row blocks per wave
+    JobSpec(job_id=3, norb=140, nocc=40, nchol=700, n_walkers=64, n_steps=30, seed=14, precision="bf16x3"),
 ]
 
 
@@ -69,7 +71,8 @@ def test_group_rejects_other_precisions():
 def test_grouped_serving_alone_and_in_mix():
     specs = [JobSpec(job_id=i, norb=n, nocc=max(1, n // 5), nchol=5 * n, n_walkers=w, n_steps=st, seed=100 + i,
                      precision="bf16x3")
-             for i, (n, w, st) in enumerate([(30, 64, 60), (64, 100, 40), (96, 160, 50), (48, 80, 30)])]
+             for i, (n, w, st) in enumerate([(30, 64, 60), (64, 100, 40), (96, 160, 50), (48, 80, 30), (130, 96, 30)])]
+    # two size classes (N <= 100 and N > 100), up to two rounds in flight per class
     ex = GroupedExecutor(n_prop_steps=10)
     sch = Scheduler(ex, hbm_cap_bytes=50e9, max_active=8)
     res = sch.run(specs, realtime=False)

@@ -88,6 +88,9 @@ CASES = [
     ("rhf", "restricted", 96, 40, 40, 150, 8),
     # 48 < O <= 64: the two-wave FP64 exchange
     ("uhf", "unrestricted", 72, 60, 52, 40, 4),
+    # nocc > 64: Gauss-Jordan and Cholesky in global memory, the stored-f exchange
+    ("uhf", "unrestricted", 100, 80, 70, 60, 4),
+    ("rhf", "restricted", 210, 200, 200, 16, 2),
 ]
 
 
@@ -247,7 +250,8 @@ def test_orthonormalize_matches_oracle_up_to_phases():
     assert rel(e, otrial.calc_energy(qo, oham["h0"], oham["rot_h1"], oham["rot_chol"])) < TOL
 
 
-@pytest.mark.parametrize("case", [("uhf", "unrestricted", 30, 7, 5, 12, 5), ("rhf", "restricted", 70, 17, 17, 130, 9)])
+@pytest.mark.parametrize("case", [("uhf", "unrestricted", 30, 7, 5, 12, 5), ("rhf", "restricted", 70, 17, 17, 130, 9),
+                                  ("rhf", "restricted", 100, 80, 80, 60, 4)])
 def test_block_qr_equals_gram_schmidt_and_oracle(case):
     """The block loop's QR (Cholesky QR, twice) against Gram-Schmidt with
     re-orthogonalisation (both give R with a positive real diagonal, so the
@@ -331,6 +335,8 @@ LP_CASES = [
     ("rhf", "restricted", 96, 40, 40, 150, 8),
     # 112 walker columns (both spins) on the bf16x3 register-panel Taylor kernel; O > 48 exchanges
     ("uhf", "unrestricted", 72, 60, 52, 40, 4),
+    # nocc > 64 (the stored-f exchange in FP64 arithmetic in every mode)
+    ("rhf", "restricted", 100, 80, 80, 60, 4),
 ]
 # relative tolerances vs the FP64 oracle for one measurement / one step
 LP_TOL = {"fp32": 2e-5, "bf16x3": 5e-4}
