@@ -15,6 +15,7 @@
 // copy or memset call, so a caller may capture it in a HIP graph.
 #include <cmath>
 #include <cstring>
+#include <functional>
 #include <map>
 #include <memory>
 #include <string>
@@ -43,19 +44,22 @@ struct DevTab {
   }
 };
 
+// All tables of a group in one device allocation: table() stages them in a
+// host buffer (256-byte aligned), commit() allocates once, copies once and
+// points the tables at their device copies.
 struct DevMem {
-  std::vector<void*> blocks;
+  void* base = nullptr;
+  std::vector<unsigned char> host;
+  std::vector<std::function<void(unsigned char*)>> fixups;
   ~DevMem() {
-    for (void* p : blocks) (void)hipFree(p);
+    if (base) (void)hipFree(base);
   }
   template <class T>
-  int upload(const std::vector<T>& v, T** out) {
-    void* p = nullptr;
-    AFQ_TRY(dalloc(&p, sizeof(T) * std::max<size_t>(1, v.size())));
-    blocks.push_back(p);
-    if (!v.empty()) AFQ_HIP(hipMemcpy(p, v.data(), sizeof(T) * v.size(), hipMemcpyHostToDevice));
-    *out = static_cast<T*>(p);
-    return AFQ_OK;
+  void stage(const std::vector<T>& v, T** out) {
+    const size_t off = (host.size() + 255) & ~size_t(255);
+    host.resize(off + sizeof(T) * std::max<size_t>(1, v.size()));
+    if (!v.empty()) std::memcpy(host.data() + off, v.data(), sizeof(T) * v.size());
+    fixups.push_back([out, off](unsigned char* b) { *out = reinterpret_cast<T*>(b + off); });
   }
   // params and the prefix of per-job item counts
   template <class P>
@@ -66,10 +70,18 @@ struct DevMem {
       if (next > 0x7fffffffL) return fail(AFQ_E_UNSUPPORTED, "group: too many work-groups in one launch");
       prefix[i + 1] = (int)next;
     }
-    AFQ_TRY(upload(params, &out->params));
-    AFQ_TRY(upload(prefix, &out->prefix));
+    stage(params, &out->params);
+    stage(prefix, &out->prefix);
     out->njobs = (int)params.size();
     out->total = prefix.back();
+    return AFQ_OK;
+  }
+  int commit() {
+    AFQ_TRY(dalloc(&base, std::max<size_t>(host.size(), 256)));
+    AFQ_HIP(hipMemcpy(base, host.data(), host.size(), hipMemcpyHostToDevice));
+    for (auto& f : fixups) f(static_cast<unsigned char*>(base));
+    host = {};
+    fixups = {};
     return AFQ_OK;
   }
 };
@@ -373,7 +385,7 @@ int build(afq_group* g, afq_job* const* jobs, void* const* walkers, double* cons
   AFQ_TRY(m.table(vsj, it_vs, &g->vsplit));
   for (auto& kv : fa) AFQ_TRY(m.table(kv.second, it_trot[kv.first], &g->trot[kv.first]));
   for (auto& kv : ex) AFQ_TRY(m.table(kv.second, it_ex[kv.first], &g->exch[kv.first]));
-  return AFQ_OK;
+  return m.commit();
 }
 
 // green_pipeline over the group: S and its determinant (inverse when want_inv),

@@ -57,8 +57,10 @@ __device__ __forceinline__ void gj_det_inv_body(const DetInvSet& set, long b, do
   const double2* src = set.S + b * set.strideS;
   const int nn = n * n;
   const bool want_inv = inv != nullptr;
+  // (the n <= 64 kernel addresses LDS only, so its accesses stay LDS instructions)
   double2* a = lds;
-  if (n > kLdsN) a = want_inv ? inv + b * set.strideInv : set.work + b * set.strideS;
+  if constexpr (NMAX > kLdsN)
+    if (n > kLdsN) a = want_inv ? inv + b * set.strideInv : set.work + b * set.strideS;
   if (a != src)
     for (int i = tid; i < nn; i += 256) a[i] = src[i];
   __syncthreads();
@@ -300,7 +302,9 @@ __device__ __forceinline__ void cholesky_upper_body(const double2* __restrict__ 
   const double2* src = S + b * strideS;
   double2* dst = R + b * strideR;
   const int nn = n * n;
-  double2* a = n > kLdsN ? dst : lds;
+  double2* a = lds;
+  if constexpr (NMAX > kLdsN)
+    if (n > kLdsN) a = dst;
   if (a != src)
     for (int i = tid; i < nn; i += 256) a[i] = src[i];
   __syncthreads();

@@ -346,6 +346,14 @@ class GroupedExecutor(GpuExecutor):
             return False  # setup still running
         return h["lane"].busy() < self.depth
 
+    def _reap(self):
+        """Closes retired groups once no class has a round in flight (freeing
+        device memory waits for the whole device), or when many have piled up."""
+        lanes = self._lane_list()
+        if all(lane.busy() == 0 for lane in lanes) or sum(len(lane.retired) for lane in lanes) > 32:
+            for lane in lanes:
+                lane.reap()
+
     def _retire(self, lane: _Lane):
         if lane.group is not None:
             ev = self.torch.cuda.Event()
@@ -361,7 +369,7 @@ class GroupedExecutor(GpuExecutor):
         from .engine import JobGroup
 
         lane = h["lane"]
-        lane.reap()
+        self._reap()
         if lane.busy() >= self.depth:
             return
         live = [x for x in lane.live if x["issued"] < x["nblocks"] and (x["issued"] > 0 or x["event"].query())]
@@ -409,7 +417,7 @@ class GroupedExecutor(GpuExecutor):
         if id(h) in lane.members:
             # the group refers to this job's buffers: it must not be replayed again
             self._retire(lane)
-        lane.reap()
+        self._reap()
         super().release(h)
 
 

@@ -203,6 +203,35 @@ def test_vhs_symmetric_and_general_cholesky(case, asym):
         assert rel(npy(pd["weights"]), st["weights"]) < TOL, step
 
 
+@pytest.mark.parametrize("asym", [0.0, 1e-9], ids=["symmetric-L", "asymmetric-L"])
+@pytest.mark.parametrize("case", [("rhf", "restricted", 70, 17, 17, 130, 9), ("uhf", "unrestricted", 67, 21, 18, 97, 7)],
+                         ids=["rhf-N70", "uhf-N67"])
+def test_bf16x3_propagator_symmetric_and_general_cholesky(case, asym):
+    """The fused split-BF16 propagator on the planes split from a packed
+    symmetric VHS (mirrored) and from a general N x N one: one step of each
+    against the FP64 oracle."""
+    kind, wtype, N, ou, od, G, W = case
+    sysm, trial, prop, ham, wave, walkers, otrial, oham, owalk, _ = _setup(*case, chol_asym=asym)
+    wave_lp, ham_lp = sysm.wave_data(), sysm.ham_data()
+    prop_lp = pr.propagator_afqmc(dt=prop.dt, n_walkers=W, walker_type=wtype, precision="bf16x3")
+    H = hm.hamiltonian(N)
+    ham_lp = H.build_measurement_intermediates(ham_lp, trial, wave_lp)
+    ham_lp = H.build_propagation_intermediates(ham_lp, prop_lp, trial, wave_lp)
+    pd = prop_lp.init_prop_data(trial, wave_lp, ham_lp, seed=3, init_walkers=walkers)
+    e_est = float(pd["e_estimate"].item())
+    st = dict(walkers=owalk, weights=np.ones(W), overlaps=otrial.calc_overlap(owalk), e_estimate=e_est,
+              pop_control_ene_shift=e_est)
+    fields = np.random.default_rng(11).standard_normal((W, G))
+    pd = prop_lp.propagate_constrained(trial, ham_lp, pd, fields, wave_lp)
+    st = O.propagate_constrained(otrial, st, fields, oham, prop.dt, prop.n_exp_terms)
+    wa = npy(pd["walkers"].data)
+    for s in range(1 if wtype == "restricted" else 2):
+        got = wa if wtype == "restricted" else wa[s]
+        ref = st["walkers"] if wtype == "restricted" else st["walkers"][s]
+        assert rel(got, ref) < LP_TOL["bf16x3"], s
+    assert rel(npy(pd["weights"]), st["weights"]) < 10 * LP_TOL["bf16x3"]
+
+
 def test_trotprop_matches_oracle():
     sysm, trial, prop, ham, wave, walkers, otrial, oham, owalk, _ = _setup("uhf", "unrestricted", 20, 5, 4, 15, 4)
     rng = np.random.default_rng(5)
