@@ -615,17 +615,25 @@ __global__ __launch_bounds__(256, 2) void exchange_bf16_kernel(const v8bf* __res
   exchange_bf16_body<NB>(rcs, gs, O, Opad, Npad, Gn, nchunk, coul, kpart, w, chunk);
 }
 
-// Grouped: job j owns work-groups [prefix[j], prefix[j + 1]), chunk-major
-// (consecutive work-groups share a rot_chol chunk across walkers).
+// Grouped: job j owns work-groups [prefix[j], prefix[j + 1]) (8 T8 of them,
+// from a multiple of 8), in exchange_bf16_kernel's XCD-aware (gamma group,
+// walker, chunk) order.
 template <int NB>
 __global__ __launch_bounds__(256, 2) void exchange_bf16_grouped_kernel(const ExchJob* __restrict__ jobs,
                                                                        const int* __restrict__ prefix, int njobs) {
   const int item = blockIdx.x;
   const int j = group_find(prefix, njobs, item);
   const ExchJob e = jobs[j];
-  const int local = item - prefix[j];
+  const long local = item - prefix[j];
+  const long L = (local & 7) * e.T8 + (local >> 3);
+  const long per_group = (long)e.W * e.cpg;
+  const long grp = L / per_group;
+  const long rem = L - grp * per_group;
+  const int w = (int)(rem / e.cpg);
+  const int chunk = (int)(grp * e.cpg + rem % e.cpg);
+  if (chunk >= e.nchunk) return;  // uniform for the whole work-group
   exchange_bf16_body<NB>(reinterpret_cast<const v8bf*>(e.rcs), reinterpret_cast<const v8bf*>(e.gs), e.O, e.Opad,
-                         e.Npad, e.Gn, e.nchunk, e.coul, e.kpart, local % e.W, local / e.W);
+                         e.Npad, e.Gn, e.nchunk, e.coul, e.kpart, w, chunk);
 }
 
 // out[r][plane][o][k] (o < Opad, k < Npad, zero padded) from src[r][o][k]
@@ -856,6 +864,14 @@ int k_split_bf16(const double* src, bool cplx, int O, int N, long rows, uint16_t
   hipLaunchKernelGGL(split_bf16_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, src, cplx ? 1 : 0, O, N,
                      Opad, Npad, rows, reinterpret_cast<__bf16*>(out));
   return launch_status("split_bf16_kernel");
+}
+
+long exchange_group_items(ExchJob* e) {
+  const long chunk_bytes = 4L * 2 * e->Opad * e->Npad * 2;
+  e->cpg = (int)std::max(1L, std::min((long)e->nchunk, kGroupSliceBytes / std::max(1L, chunk_bytes)));
+  const long ngroups = (e->nchunk + e->cpg - 1) / e->cpg;
+  e->T8 = (ngroups * e->cpg * (long)e->W + 7) / 8;
+  return 8 * e->T8;
 }
 
 int exchange_bf16_grouped(const GroupTab<ExchJob>& t, int nb, hipStream_t s) {

@@ -331,8 +331,9 @@ __global__ __launch_bounds__(64 * WM * WN) void gemm_f64_kernel(Gemm g, int tile
 }
 
 // Grouped batched products: work-group i belongs to the job j with
-// prefix[j] <= i < prefix[j + 1]; within it, tile = local % tiles and batch
-// entry = local / tiles.  No K split.
+// prefix[j] <= i < prefix[j + 1] (a multiple of 8); within it the local index
+// is mapped as gemm_f64_kernel's xcd_remap == 2: XCD local % 8 takes the batch
+// entries local % 8 (mod 8), all tiles of one entry in a row.  No K split.
 template <bool AC, bool BC, bool AT, bool BT>
 __global__ __launch_bounds__(256) void gemm_f64_grouped_kernel(const GemmJob* __restrict__ jobs,
                                                                const int* __restrict__ prefix, int njobs) {
@@ -342,7 +343,10 @@ __global__ __launch_bounds__(256) void gemm_f64_grouped_kernel(const GemmJob* __
   const int tm = jobs[j].tiles_m, tn = jobs[j].tiles_n;
   const int local = item - prefix[j];
   const int tiles = tm * tn;
-  gemm_f64_body<AC, BC, AT, BT, 2, 2, 2, 2>(g, tm, tn, local % tiles, local / tiles, 0, g.K, nullptr);
+  const int p = local >> 3;
+  const int b = (local & 7) + 8 * (p / tiles);
+  if (b >= g.batch) return;  // uniform per work-group
+  gemm_f64_body<AC, BC, AT, BT, 2, 2, 2, 2>(g, tm, tn, p % tiles, b, 0, g.K, nullptr);
 }
 
 // C = alpha * sum_kz ws[kz] + beta * C  (fixed summation order)
@@ -632,6 +636,8 @@ GemmJob gemm_group_job(const Gemm& g) {
   j.tiles_n = (g.N + 63) / 64;
   return j;
 }
+
+long gemm_group_items(const GemmJob& j) { return (long)j.tiles_m * j.tiles_n * ((j.g.batch + 7) / 8 * 8); }
 
 int gemm_f64_grouped(const GroupTab<GemmJob>& t, bool a_cplx, bool b_cplx, bool a_trans, bool b_trans,
                      hipStream_t s) {

@@ -179,7 +179,8 @@ __global__ __launch_bounds__(64 * WM * WN) void gemm_bf3_lds_kernel(Bf3 p, int t
 }
 
 // Grouped (batch-1 products): work-group i belongs to job j with
-// prefix[j] <= i < prefix[j + 1]; local index = tile + tiles * kz.
+// prefix[j] <= i < prefix[j + 1] (a multiple of 8); local index = t + tx * kz,
+// t remapped to the XCD-aware tile order as gemm_bf3_lds_kernel's.
 __global__ __launch_bounds__(512) void gemm_bf3_grouped_kernel(const Bf3Job* __restrict__ jobs,
                                                                const int* __restrict__ prefix, int njobs) {
   extern __shared__ __align__(16) __bf16 lds_bf3[];
@@ -187,8 +188,12 @@ __global__ __launch_bounds__(512) void gemm_bf3_grouped_kernel(const Bf3Job* __r
   const int j = group_find(prefix, njobs, item);
   const Bf3Job J = jobs[j];
   const int local = item - prefix[j];
-  const int tiles = J.tiles_m * J.tiles_n;
-  gemm_bf3_body<128, 128, 2, 4>(J.p, J.tiles_m, J.tiles_n, local % tiles, 0, local / tiles, J.kchunk, J.ws, lds_bf3);
+  const int kz = local / J.tx;
+  if (kz >= J.ksplit) return;  // padding past the last K slice (uniform)
+  long t = local - (long)kz * J.tx;
+  if (J.T8 > 0) t = (t & 7) * J.T8 + (t >> 3);
+  if (t >= (long)J.tiles_m * J.tiles_n) return;
+  gemm_bf3_body<128, 128, 2, 4>(J.p, J.tiles_m, J.tiles_n, t, 0, kz, J.kchunk, J.ws, lds_bf3);
 }
 
 // K slices for a complex128 output: split until about 1024 work-groups are in
@@ -340,6 +345,9 @@ int bf3_group_job(const Bf3& p, Bf3Job* out) {
   j.p = p;
   j.tiles_m = (2 * p.M + 127) / 128;
   j.tiles_n = (p.N + 127) / 128;
+  const long tiles = (long)j.tiles_m * j.tiles_n;
+  j.T8 = tiles >= 64 ? (tiles + 7) / 8 : 0;
+  j.tx = (int)(j.T8 > 0 ? j.T8 * 8 : tiles);
   j.ksplit = 1;
   j.kchunk = p.K;
   j.ws = nullptr;
@@ -352,6 +360,8 @@ int bf3_group_job(const Bf3& p, Bf3Job* out) {
   *out = j;
   return AFQ_OK;
 }
+
+long bf3_group_items(const Bf3Job& j) { return ((long)j.tx * j.ksplit + 7) / 8 * 8; }
 
 int gemm_bf3_grouped(const GroupTab<Bf3Job>& t, hipStream_t s) {
   if (t.njobs < 1 || t.total < 1) return AFQ_OK;

@@ -40,7 +40,10 @@ struct GemmJob {
   Gemm g;
   int tiles_m = 0, tiles_n = 0;
 };
-GemmJob gemm_group_job(const Gemm& g);  // work items: tiles_m * tiles_n * g.batch
+GemmJob gemm_group_job(const Gemm& g);
+// work items: tiles_m * tiles_n * (g.batch rounded up to 8); a multiple of 8,
+// so every job's items start on XCD 0 and the XCD order below holds per job
+long gemm_group_items(const GemmJob& j);
 int gemm_f64_grouped(const GroupTab<GemmJob>& t, bool a_cplx, bool b_cplx, bool a_trans, bool b_trans,
                      hipStream_t s);
 
@@ -76,9 +79,13 @@ int vec_grouped(const GroupTab<VecJob>& t, hipStream_t s);
 struct Bf3Job {
   Bf3 p;
   int tiles_m = 0, tiles_n = 0, ksplit = 1, kchunk = 0;
+  int tx = 0;    // work items per K slice (tiles, rounded up to 8 when remapped)
+  long T8 = 0;   // > 0: gemm_bf3's XCD-aware tile order (the M-tiles of an L panel on one XCD)
   double* ws = nullptr;
 };
 int bf3_group_job(const Bf3& p, Bf3Job* out);
+// work items: tx * ksplit rounded up to 8 (every job starts on XCD 0)
+long bf3_group_items(const Bf3Job& j);
 int gemm_bf3_grouped(const GroupTab<Bf3Job>& t, hipStream_t s);
 
 // C = alpha sum_kz ws[kz] + beta C for batch-1 complex outputs; work items:
@@ -99,7 +106,11 @@ struct ExchJob {
   int O, Opad, Npad, Gn, W, nchunk;
   double2* coul;
   double2* kpart;
+  int cpg = 1;  // chunks per gamma group (k_exchange_bf16's XCD-aware order)
+  long T8 = 0;  // work items / 8
 };
+// fills cpg and T8 of e (its other fields set); returns its work items (8 T8)
+long exchange_group_items(ExchJob* e);
 int exchange_bf16_grouped(const GroupTab<ExchJob>& t, int nb, hipStream_t s);
 
 // k_split_bf16 of each job's rows x (O, N) operand; work items: split_items.

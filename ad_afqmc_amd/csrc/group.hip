@@ -207,7 +207,7 @@ int build(afq_group* g, afq_job* const* jobs, void* const* walkers, double* cons
     o.ldc = O;
     o.sC = (long)O * O;
     ov[i] = gemm_group_job(o);
-    it_gemm_ov[i] = (long)ov[i].tiles_m * ov[i].tiles_n * W;
+    it_gemm_ov[i] = gemm_group_items(ov[i]);
     // G = (W S^-1)^T
     Gemm h;
     h.M = O;
@@ -228,7 +228,7 @@ int build(afq_group* g, afq_job* const* jobs, void* const* walkers, double* cons
     h.ldc = N;
     h.sC = (long)O * N;
     gr[i] = gemm_group_job(h);
-    it_gemm_gr[i] = (long)gr[i].tiles_m * gr[i].tiles_n * W;
+    it_gemm_gr[i] = gemm_group_items(gr[i]);
     dinv[i] = DetInvSet{j->S[0], (long)O * O, O, j->det[0], j->Sinv[0], (long)O * O};
     ddet[i] = DetInvSet{j->S[0], (long)O * O, O, j->det[0], nullptr, (long)O * O, j->S[0]};
     it_w[i] = W;
@@ -256,7 +256,7 @@ int build(afq_group* g, afq_job* const* jobs, void* const* walkers, double* cons
     q.ldc = c;
     q.sC = cc;
     gm[i] = gemm_group_job(q);
-    it_gemm_gm[i] = (long)gm[i].tiles_m * gm[i].tiles_n * W;
+    it_gemm_gm[i] = gemm_group_items(gm[i]);
     ch[i] = CholSet{j->S[0], cc, c, j->S[0], cc};
     dqr[i] = DetInvSet{j->S[0], cc, c, j->det[0], j->Sinv[0], cc};
     Gemm t;
@@ -276,7 +276,7 @@ int build(afq_group* g, afq_job* const* jobs, void* const* walkers, double* cons
     t.ldc = c;
     t.sC = per;
     tr[i] = gemm_group_job(t);
-    it_gemm_tr[i] = (long)tr[i].tiles_m * tr[i].tiles_n * W;
+    it_gemm_tr[i] = gemm_group_items(tr[i]);
 
     // force bias (force_bias_gemm, split-BF16 x3): split G, product with the split rot_chol
     const long P = (long)bf16_opad(O) * bf16_npad(N);
@@ -303,7 +303,7 @@ int build(afq_group* g, afq_job* const* jobs, void* const* walkers, double* cons
     f.ws = j->gemm_ws;
     f.ws_elems = j->gemm_ws_elems;
     AFQ_TRY(bf3_group_job(f, &fbj[i]));
-    it_fb[i] = (long)fbj[i].tiles_m * fbj[i].tiles_n * fbj[i].ksplit;
+    it_fb[i] = bf3_group_items(fbj[i]);
     red[i].g.M = W;
     red[i].g.N = G;
     red[i].g.C = j->fb;
@@ -337,7 +337,7 @@ int build(afq_group* g, afq_job* const* jobs, void* const* walkers, double* cons
     v.C = j->sym_chol ? static_cast<void*>(j->vhsp32) : static_cast<void*>(j->vhs32);
     v.ldc = ncols;
     AFQ_TRY(bf3_group_job(v, &vhj[i]));
-    it_vh[i] = (long)vhj[i].tiles_m * vhj[i].tiles_n * vhj[i].ksplit;
+    it_vh[i] = bf3_group_items(vhj[i]);
     vsj[i] = VhsSplit{static_cast<const float2*>(v.C), j->sym_chol, N, W, j->vsplit};
     it_vs[i] = vhs_split_items(vsj[i]);
 
@@ -361,8 +361,8 @@ int build(afq_group* g, afq_job* const* jobs, void* const* walkers, double* cons
     // exchange, grouped by the 16-row block count of the f-blocks
     const int nb = (O + 15) / 16;
     ExchJob e{j->rcs[0], j->gsplit, O, bf16_opad(O), bf16_npad(N), G, W, exchange_nchunk(G), j->coul, j->kpart};
+    it_ex[nb].push_back(exchange_group_items(&e));
     ex[nb].push_back(e);
-    it_ex[nb].push_back((long)W * e.nchunk);
   }
   DevMem& m = g->mem;
   for (int k = 0; k < BI_NKINDS; ++k) AFQ_TRY(m.table(bjs, it_bj[k], &g->bj[k]));

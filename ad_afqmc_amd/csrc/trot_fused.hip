@@ -85,6 +85,9 @@ template <int TN, int RBW>
 __device__ __forceinline__ void trot_fused_body(const FusedArgs& a, int w, int sl) {
   constexpr int CS = 16 * TN;
   constexpr int KPLMAX = (RBW * FW * 16 + 63) / 64 * 64;
+  // B fragments read once per k-step for all row blocks, except in the
+  // shapes whose registers would then spill
+  constexpr bool BONCE = !(TN == 3 && RBW >= 3) && !(TN == 2 && RBW >= 4);
   __shared__ __align__(16) __bf16 P[4 * CS * KPLMAX];
   const int tid = threadIdx.x, lane = tid & 63;
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
@@ -143,8 +146,9 @@ __device__ __forceinline__ void trot_fused_body(const FusedArgs& a, int w, int s
   // wave multiplies each of its row blocks in turn, then immediately loads
   // that row block's A fragments for the next step, so every load has a whole
   // step of MFMAs (all row blocks) to arrive without a second register set;
-  // the B fragments are re-read from LDS per row block.  Row blocks past this
-  // wave's count are neither loaded nor multiplied (a wave-uniform test).
+  // the B fragments of a k-step are read from LDS once for all row blocks
+  // (BONCE) or per row block.  Row blocks past this wave's count are neither
+  // loaded nor multiplied (a wave-uniform test).
 
   // aR/aI = (E - I) . panel
   auto onebody = [&]() {
@@ -166,21 +170,30 @@ __device__ __forceinline__ void trot_fused_body(const FusedArgs& a, int w, int s
       if (ii < nrb) load(ii, 0);
     for (int k0 = 0; k0 < a.K32; k0 += 32) {
       const int kn = min(k0 + 32, a.K32 - 32);
+      // B fragments of the k-step, read once for all of this wave's row blocks
+      // (per row block where the registers are short)
+      v8bf b[TN][4];
+      if constexpr (BONCE) {
+#pragma unroll
+        for (int j = 0; j < TN; ++j) bfrag(j, k0, b[j][0], b[j][1], b[j][2], b[j][3]);
+      }
 #pragma unroll
       for (int ii = 0; ii < RBW; ++ii) {
         if (ii >= nrb) continue;
+        if constexpr (!BONCE) {
+#pragma unroll
+          for (int j = 0; j < TN; ++j) bfrag(j, k0, b[j][0], b[j][1], b[j][2], b[j][3]);
+        }
 #pragma unroll
         for (int j = 0; j < TN; ++j) {
-          v8bf brh, brl, bih, bil;
-          bfrag(j, k0, brh, brl, bih, bil);
-          aR[ii][j] = mfma_bf16(c[ii][0], brh, aR[ii][j]);
-          aR[ii][j] = mfma_bf16(c[ii][0], brl, aR[ii][j]);
-          aR[ii][j] = mfma_bf16(c[ii][1], brh, aR[ii][j]);
-          aR[ii][j] = mfma_bf16(c[ii][2], brh, aR[ii][j]);
-          aI[ii][j] = mfma_bf16(c[ii][0], bih, aI[ii][j]);
-          aI[ii][j] = mfma_bf16(c[ii][0], bil, aI[ii][j]);
-          aI[ii][j] = mfma_bf16(c[ii][1], bih, aI[ii][j]);
-          aI[ii][j] = mfma_bf16(c[ii][2], bih, aI[ii][j]);
+          aR[ii][j] = mfma_bf16(c[ii][0], b[j][0], aR[ii][j]);
+          aR[ii][j] = mfma_bf16(c[ii][0], b[j][1], aR[ii][j]);
+          aR[ii][j] = mfma_bf16(c[ii][1], b[j][0], aR[ii][j]);
+          aR[ii][j] = mfma_bf16(c[ii][2], b[j][0], aR[ii][j]);
+          aI[ii][j] = mfma_bf16(c[ii][0], b[j][2], aI[ii][j]);
+          aI[ii][j] = mfma_bf16(c[ii][0], b[j][3], aI[ii][j]);
+          aI[ii][j] = mfma_bf16(c[ii][1], b[j][2], aI[ii][j]);
+          aI[ii][j] = mfma_bf16(c[ii][2], b[j][2], aI[ii][j]);
         }
         load(ii, kn);
       }
@@ -208,18 +221,23 @@ __device__ __forceinline__ void trot_fused_body(const FusedArgs& a, int w, int s
       if (ii < nrb) load(ii, 0);
     for (int k0 = 0; k0 < a.K32; k0 += 32) {
       const int kn = min(k0 + 32, a.K32 - 32);
-#pragma unroll
-      for (int ii = 0; ii < RBW; ++ii) {
-        if (ii >= nrb) continue;
-        // B fragments of every column block, so that consecutive MFMAs go to
-        // the 2 TN independent accumulators of this row block in turn
-        v8bf b[TN][6];
+      // B fragments of every column block, read once per k-step for all of
+      // this wave's row blocks; consecutive MFMAs go to the 2 TN independent
+      // accumulators of a row block in turn
+      v8bf b[TN][6];
+      auto bload = [&]() {
 #pragma unroll
         for (int j = 0; j < TN; ++j) {
           bfrag(j, k0, b[j][0], b[j][1], b[j][2], b[j][3]);
           b[j][4] = neg8(b[j][2]);
           b[j][5] = neg8(b[j][3]);
         }
+      };
+      if constexpr (BONCE) bload();
+#pragma unroll
+      for (int ii = 0; ii < RBW; ++ii) {
+        if (ii >= nrb) continue;
+        if constexpr (!BONCE) bload();
         // re += ar br - ai bi ; im += ar bi + ai br, each as hi*hi + lo*hi + hi*lo
         constexpr int PR[6][2] = {{0, 0}, {1, 0}, {0, 1}, {2, 4}, {3, 4}, {2, 5}};  // (A part, B part) of re
         constexpr int PI[6][2] = {{0, 2}, {1, 2}, {0, 3}, {2, 0}, {3, 0}, {2, 1}};  // of im
