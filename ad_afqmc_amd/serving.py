@@ -102,7 +102,8 @@ class Scheduler:
         pending = deque(sorted(specs, key=lambda s: s.arrival))
         waiting: deque = deque()
         active = {}  # job_id -> (spec, handle, bytes)
-        results: Dict[int, JobResult] = {s.job_id: JobResult(s.job_id, s.arrival) for s in specs}
+        # without realtime every job is there at t = 0, and its latency counts from then
+        results: Dict[int, JobResult] = {s.job_id: JobResult(s.job_id, s.arrival if realtime else 0.0) for s in specs}
         t0 = self.clock()
         now = lambda: (self.clock() - t0)
         while pending or waiting or active:
@@ -313,12 +314,19 @@ class GroupedExecutor(GpuExecutor):
     closed once its last round has finished.  With use_graphs a member set
     that lasts more than one round is captured into a HIP graph on its second
     round and replayed afterwards.  Jobs must be restricted-walker RHF jobs in
-    bf16x3 (the grouped block's scope)."""
+    bf16x3 (the grouped block's scope).
 
-    def __init__(self, n_prop_steps: int = 10, use_graphs: bool = True, lanes=(100, 175), depth: int = 2):
+    Jobs above `solo_above` orbitals (None: none) are not grouped: they run as
+    GpuExecutor runs every job, on their own stream with their own graph; such
+    a job's kernels fill the device alone, where grouping the small jobs saves
+    their launches and fills the device between them."""
+
+    def __init__(self, n_prop_steps: int = 10, use_graphs: bool = True, lanes=(100, 175), depth: int = 2,
+                 solo_above=None):
         super().__init__(n_prop_steps, use_graphs)
         self.bounds = tuple(int(b) for b in lanes)
         self.depth = max(1, int(depth))
+        self.solo_above = None if solo_above is None else int(solo_above)
         self._lanes = None
         self.rounds = 0
 
@@ -335,11 +343,16 @@ class GroupedExecutor(GpuExecutor):
         if spec.precision != "bf16x3":
             raise ValueError("grouped serving runs bf16x3 jobs")
         h = super().start(spec)
+        if self.solo_above is not None and spec.norb > self.solo_above:
+            h["lane"] = None
+            return h
         h["lane"] = self._lane(spec)
         h["lane"].live.append(h)
         return h
 
     def ready(self, h) -> bool:
+        if h["lane"] is None:
+            return super().ready(h)
         if h["issued"] >= h["nblocks"]:
             return False
         if h["issued"] == 0 and not h["event"].query():
@@ -370,6 +383,9 @@ class GroupedExecutor(GpuExecutor):
 
         lane = h["lane"]
         self._reap()
+        if lane is None:
+            super().issue(h)
+            return
         if lane.busy() >= self.depth:
             return
         live = [x for x in lane.live if x["issued"] < x["nblocks"] and (x["issued"] > 0 or x["event"].query())]
@@ -412,11 +428,12 @@ class GroupedExecutor(GpuExecutor):
 
     def release(self, h):
         lane = h["lane"]
-        if h in lane.live:
-            lane.live.remove(h)
-        if id(h) in lane.members:
-            # the group refers to this job's buffers: it must not be replayed again
-            self._retire(lane)
+        if lane is not None:
+            if h in lane.live:
+                lane.live.remove(h)
+            if id(h) in lane.members:
+                # the group refers to this job's buffers: it must not be replayed again
+                self._retire(lane)
         self._reap()
         super().release(h)
 

@@ -68,12 +68,14 @@ def test_group_rejects_other_precisions():
     ex.release(h)
 
 
-def test_grouped_serving_alone_and_in_mix():
+@pytest.mark.parametrize("solo_above", [None, 100], ids=["all-grouped", "large-solo"])
+def test_grouped_serving_alone_and_in_mix(solo_above):
     specs = [JobSpec(job_id=i, norb=n, nocc=max(1, n // 5), nchol=5 * n, n_walkers=w, n_steps=st, seed=100 + i,
                      precision="bf16x3")
              for i, (n, w, st) in enumerate([(30, 64, 60), (64, 100, 40), (96, 160, 50), (48, 80, 30), (130, 96, 30)])]
-    # two size classes (N <= 100 and N > 100), up to two rounds in flight per class
-    ex = GroupedExecutor(n_prop_steps=10)
+    # two size classes (N <= 100 and N > 100), up to two rounds in flight per
+    # class; or the N = 130 job on its own stream
+    ex = GroupedExecutor(n_prop_steps=10, solo_above=solo_above)
     sch = Scheduler(ex, hbm_cap_bytes=50e9, max_active=8)
     res = sch.run(specs, realtime=False)
     assert ex.rounds >= 6 and sch.max_concurrent >= 2
@@ -81,5 +83,6 @@ def test_grouped_serving_alone_and_in_mix():
         r = res[s.job_id]
         assert r.error is None and r.block_energies.shape == (s.n_steps // 10, 2)
         assert np.all(np.isfinite(r.block_energies)) and np.all(r.block_energies[:, 1] > 0)
-        alone = Scheduler(GroupedExecutor(n_prop_steps=10), hbm_cap_bytes=50e9).run([s], realtime=False)[s.job_id]
+        alone = Scheduler(GroupedExecutor(n_prop_steps=10, solo_above=solo_above), hbm_cap_bytes=50e9).run(
+            [s], realtime=False)[s.job_id]
         assert np.array_equal(alone.block_energies, r.block_energies), s.job_id
