@@ -805,9 +805,10 @@ def main():
     if args.serving_jobs > 0 and not args.only:
         from ad_afqmc_amd import serving
 
-        def serve(executor_cls, realtime):
-            specs = serving.poisson_stream(args.serving_jobs, rate=4.0, seed=0, precision=args.serving_precision)
-            ex = executor_cls(n_prop_steps=10)
+        def serve(make_executor, realtime, nmax=250, rate=4.0):
+            specs = serving.poisson_stream(args.serving_jobs, rate=rate, seed=0, nmax=nmax,
+                                           precision=args.serving_precision)
+            ex = make_executor()
             sch = serving.Scheduler(ex, hbm_cap_bytes=200e9, max_active=64)
             torch.cuda.synchronize()
             t0 = time.perf_counter()
@@ -819,18 +820,32 @@ def main():
                 summ["grouped_rounds"] = ex.rounds
             return summ
 
+        executors = {
+            # one stream and one captured graph per job
+            "streams": lambda: serving.GpuExecutor(n_prop_steps=10),
+            # every job in grouped blocks (afq_group_sr_block), size classes N <= 100, <= 175, > 175
+            "grouped": lambda: serving.GroupedExecutor(n_prop_steps=10),
+            # jobs with N <= 100 grouped, larger ones on their own streams
+            "grouped_small_solo_large": lambda: serving.GroupedExecutor(n_prop_steps=10, lanes=(), solo_above=100),
+        }
         torch.cuda.empty_cache()
         out = {}
-        # one grouped block per round over all admitted jobs, and one stream per job
-        for mode, cls in (("grouped", serving.GroupedExecutor), ("streams", serving.GpuExecutor)):
+        for mode, make in executors.items():
             print(f"[bench] serving stream ({mode}): {args.serving_jobs} jobs", file=sys.stderr, flush=True)
-            out[mode] = serve(cls, True)
+            out[mode] = serve(make, True)
             print(f"[bench] serving capacity ({mode}): all jobs at t=0", file=sys.stderr, flush=True)
-            out[mode]["capacity_all_at_t0"] = serve(cls, False)
-        summ = dict(out["grouped"])
+            out[mode]["capacity_all_at_t0"] = serve(make, False)
+        # a stream of small jobs only (N <= 80, 16 jobs/s), where launches weigh most
+        small = {}
+        for mode in ("streams", "grouped"):
+            print(f"[bench] small-job stream ({mode})", file=sys.stderr, flush=True)
+            small[mode] = serve(executors[mode], True, nmax=80, rate=16.0)
+            small[mode]["capacity_all_at_t0"] = serve(executors[mode], False, nmax=80, rate=16.0)
+        summ = dict(out["streams"])
         summ.update(arrival_rate_per_s=4.0, precision=args.serving_precision, hbm_cap_gb=200, hip_graphs=True,
-                    executor="grouped: one launch per kernel over all admitted jobs per block (afq_group_sr_block)",
-                    per_job_streams=out["streams"],
+                    executor="streams: one HIP stream and one captured afq_sr_block graph per job",
+                    grouped=out["grouped"], grouped_small_solo_large=out["grouped_small_solo_large"],
+                    small_jobs=dict(small, workload="N~U{30..80}, O=N/5, G=5N, W~U{64..512}, 300 steps, Poisson 16 jobs/s"),
                     workload="N~U{30..250}, O=N/5, G=5N, W~U{64..512}, 300 steps, Poisson 4 jobs/s")
         line["serving"] = summ
     line["cpu_baseline"] = cpu
