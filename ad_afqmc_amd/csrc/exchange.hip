@@ -53,6 +53,12 @@ __device__ inline void wave_lds_sync() {
 
 __host__ __device__ inline int bf16_opad_d(int O) { return 16 * ((O + 15) / 16); }
 __host__ __device__ inline int bf16_npad_d(int N) { return 32 * ((N + 31) / 32); }
+// Position of (row o, column k) in an Opad x Npad bf16 plane stored by 32-deep
+// k step: [k / 32][o][k % 32].  Products that contract over the whole plane
+// (the force bias) see the same permutation on both operands.
+__host__ __device__ inline long bf16_tile(int o, int k, int Opad) {
+  return ((long)(k >> 5) * Opad + o) * 32 + (k & 31);
+}
 
 // Permutation of the four 16-byte k-groups of LDS row r: rows r and r' < 16
 // of a fragment then occupy distinct 16-byte bank groups both within every 8
@@ -446,7 +452,9 @@ __device__ __forceinline__ void exchange_bf16_body(const v8bf* __restrict__ rcs,
   const long plane = (long)Opad * Npad / 8;
   const int rowv = Npad / 8;
   const int r16 = lane & 15, q4 = lane >> 4;
-  const v8bf* Ah = rcs + (long)gg * 2 * plane + r16 * rowv + q4;
+  // a k step of a plane is Opad rows x 4 k-groups (bf16_tile); fragment b
+  // of step kv / 4 starts at kv * Opad + 64 b
+  const v8bf* Ah = rcs + (long)gg * 2 * plane + r16 * 4 + q4;
   const v8bf* Al = Ah + plane;
   const v8bf* Bw = gs + (long)w * 4 * plane;
   // LDS slot s of a stage holds plane s / (4 ROWS), row (s / 4) % ROWS and
@@ -459,12 +467,12 @@ __device__ __forceinline__ void exchange_bf16_body(const v8bf* __restrict__ rcs,
   for (int t = 0; t < CPW; ++t) {
     const int s = 64 * (4 * t + wave) + lane;
     const int row = (s >> 2) % ROWS;
-    src[t] = (long)(s / (4 * ROWS)) * plane + (long)row * rowv + ((s & 3) ^ kswz(row & 15));
+    src[t] = (long)(s / (4 * ROWS)) * plane + (long)row * 4 + ((s & 3) ^ kswz(row & 15));
   }
   auto copy = [&](int kv, int buf) {
 #pragma unroll
     for (int t = 0; t < CPW; ++t)
-      __builtin_amdgcn_global_load_lds(static_cast<const void*>(Bw + src[t] + kv),
+      __builtin_amdgcn_global_load_lds(static_cast<const void*>(Bw + src[t] + (long)kv * ROWS),
                                        (__attribute__((address_space(3))) void*)&bsh[buf][64 * (4 * t + wave)], 16,
                                        0, 0);
   };
@@ -485,8 +493,8 @@ __device__ __forceinline__ void exchange_bf16_body(const v8bf* __restrict__ rcs,
   v8bf ah[NB], al[NB], nh[NB];
 #pragma unroll
   for (int b = 0; b < NB; ++b) {
-    ah[b] = Ah[(long)b * 16 * rowv];
-    al[b] = Al[(long)b * 16 * rowv];
+    ah[b] = Ah[b * 64];
+    al[b] = Al[b * 64];
   }
   copy(0, 0);
   int buf = 0;
@@ -517,7 +525,7 @@ __device__ __forceinline__ void exchange_bf16_body(const v8bf* __restrict__ rcs,
     }
     copy(kn, buf ^ 1);
 #pragma unroll
-    for (int b = 0; b < NB; ++b) nh[b] = Ah[(long)b * 16 * rowv + kn];
+    for (int b = 0; b < NB; ++b) nh[b] = Ah[b * 64 + (long)kn * ROWS];
     __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
     for (int i = 0; i < NB; ++i)
@@ -528,7 +536,7 @@ __device__ __forceinline__ void exchange_bf16_body(const v8bf* __restrict__ rcs,
       }
     __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
-    for (int b = 0; b < NB; ++b) al[b] = Al[(long)b * 16 * rowv + kn];
+    for (int b = 0; b < NB; ++b) al[b] = Al[b * 64 + (long)kn * ROWS];
     __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
     for (int i = 0; i < NB; ++i)
@@ -636,9 +644,12 @@ __global__ __launch_bounds__(256, 2) void exchange_bf16_grouped_kernel(const Exc
                          e.Npad, e.Gn, e.nchunk, e.coul, e.kpart, w, chunk);
 }
 
-// out[r][plane][o][k] (o < Opad, k < Npad, zero padded) from src[r][o][k]
-// (o < O, k < N; real or interleaved complex double): planes hi, lo of
-// bf16(float(x)) and bf16(float(x) - hi), for the real then the imaginary part.
+// out[r][plane][tile(o, k)] (o < Opad, k < Npad, zero padded) from
+// src[r][o][k] (o < O, k < N; real or interleaved complex double): planes hi,
+// lo of bf16(float(x)) and bf16(float(x) - hi), for the real then the
+// imaginary part.  Within a plane the elements are stored by 32-deep k step,
+// then row, then k (bf16_tile), so the 16 rows x 32 k of one MFMA operand
+// fragment are 1 KB of consecutive bytes.
 __device__ __forceinline__ void split_bf16_body(const double* __restrict__ src, int cplx, int O, int N, int Opad,
                                                 int Npad, long rows, __bf16* __restrict__ out, long idx) {
   const long per = (long)Opad * Npad;
@@ -647,7 +658,7 @@ __device__ __forceinline__ void split_bf16_body(const double* __restrict__ src, 
   const int o = (int)((idx % per) / Npad), k = (int)(idx % Npad);
   const bool in = o < O && k < N;
   const int np = cplx ? 4 : 2;
-  __bf16* base = out + r * np * per + (long)o * Npad + k;
+  __bf16* base = out + r * np * per + bf16_tile(o, k, Opad);
   for (int c = 0; c < (cplx ? 2 : 1); ++c) {
     double x = 0.0;
     if (in) x = cplx ? src[2 * ((r * O + o) * (long)N + k) + c] : src[(r * O + o) * (long)N + k];
