@@ -208,9 +208,12 @@ struct afq_job {
   float* rot_chol32[2] = {nullptr, nullptr}; // (G, nocc_s, N)
   float2* vhs32 = nullptr;                // (W, N, N)
   float2* t32[2] = {nullptr, nullptr};    // (W, N, wmax) Taylor terms
-  // split-BF16 x3 exchange operands (bf16 bit patterns): rot_chol (G, 2, Opad, Npad)
-  // per spin and one walker batch of G (W, 4, Opad, Npad)
+  // split-BF16 x3 exchange and force-bias operands (bf16 bit patterns):
+  // rot_chol (G, 2, Opad, Npad) per spin in the exchange's tiled layout (rcs)
+  // and with the planes interleaved (rcs_fb), and one walker batch of G
+  // (W, 4, Opad, Npad) in either layout
   uint16_t* rcs[2] = {nullptr, nullptr};
+  uint16_t* rcs_fb[2] = {nullptr, nullptr};
   uint16_t* gsplit = nullptr;
   // split-BF16 x3 VHS operands: L^T (NNr, 2, Gpad) and the shifted fields
   // (Wr, 4, Gpad); rows padded to 16, k to 32

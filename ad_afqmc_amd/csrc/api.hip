@@ -89,6 +89,8 @@ void release(afq_job* j) {
   fr(j->t32[1]);
   fr(j->rcs[0]);
   if (!j->rhf) fr(j->rcs[1]);
+  fr(j->rcs_fb[0]);
+  if (!j->rhf) fr(j->rcs_fb[1]);
   fr(j->gsplit);
   fr(j->lt_split);
   fr(j->xs_split);
@@ -219,7 +221,7 @@ int force_bias_gemm(afq_job* j, double2* out, hipStream_t st) {
     if (j->bf16x3) {
       // over the flattened, zero-padded (Opad, Npad) orbital pairs: split G_w, reuse the split rot_chol
       const long P = (long)bf16_opad(O) * bf16_npad(j->N);
-      AFQ_TRY(k_split_bf16(reinterpret_cast<const double*>(j->green[s]), true, O, j->N, j->W, j->gsplit, st));
+      AFQ_TRY(k_split_bf16(reinterpret_cast<const double*>(j->green[s]), true, O, j->N, j->W, true, j->gsplit, st));
       Bf3 b;
       b.M = j->W;
       b.N = j->G;
@@ -227,11 +229,13 @@ int force_bias_gemm(afq_job* j, double2* out, hipStream_t st) {
       b.a_cplx = true;
       b.A = j->gsplit;
       b.lda = 4 * P;
-      b.pA = P;
+      b.pA = 32;
+      b.kA = 4 * 32;
       b.a_rows = j->Wcap;
-      b.B = j->rcs[s];
+      b.B = j->rcs_fb[s];
       b.ldb = 2 * P;
-      b.pB = P;
+      b.pB = 32;
+      b.kB = 2 * 32;
       b.b_rows = j->G;
       b.alpha = g.alpha;
       b.beta = g.beta;
@@ -266,8 +270,8 @@ int energy_pipeline(afq_job* j, const void* up, const void* dn, double2* out, hi
     if (j->bf16x3 && j->nocc[s] <= kExchangeRegMaxO) {
       {
         ProfScope pm(j, AFQ_K_ENERGY_MISC, st);
-        AFQ_TRY(k_split_bf16(reinterpret_cast<const double*>(j->green[s]), true, j->nocc[s], j->N, j->W, j->gsplit,
-                             st));
+        AFQ_TRY(k_split_bf16(reinterpret_cast<const double*>(j->green[s]), true, j->nocc[s], j->N, j->W, false,
+                             j->gsplit, st));
       }
       ProfScope ps(j, AFQ_K_EXCHANGE, st);
       AFQ_TRY(k_exchange_bf16(j->rcs[s], j->gsplit, j->nocc[s], j->N, j->G, j->W, coul, kpart, &nch[s], st));
@@ -380,8 +384,8 @@ int trotprop_lp(afq_job* j, void* up, void* dn, const double2* xs, hipStream_t s
     if (j->bf16x3) {
       // on pre-split operands: x~ split per step, L^T split once in set_hamiltonian
       const int gpad = bf16_npad(j->G);
-      AFQ_TRY(k_split_rows_bf16(reinterpret_cast<const double*>(xs), true, W, j->G, j->G, gpad, 4L * gpad, gpad,
-                                j->xs_split, st));
+      AFQ_TRY(k_split_rows_bf16(reinterpret_cast<const double*>(xs), true, W, j->G, j->G, gpad, 4L * gpad, 32,
+                                4 * 32, j->xs_split, st));
       Bf3 v;
       v.M = W;
       v.N = (int)ncols;
@@ -389,11 +393,13 @@ int trotprop_lp(afq_job* j, void* up, void* dn, const double2* xs, hipStream_t s
       v.a_cplx = true;
       v.A = j->xs_split;
       v.lda = 4L * gpad;
-      v.pA = gpad;
+      v.pA = 32;
+      v.kA = 4 * 32;
       v.a_rows = bf16_opad(j->Wcap);
       v.B = j->lt_split;
       v.ldb = 2L * gpad;
-      v.pB = gpad;
+      v.pB = 32;
+      v.kB = 2 * 32;
       v.b_rows = (int)(16 * ((ncols + 15) / 16));
       v.alpha = g.alpha;
       v.C = out;
@@ -831,6 +837,8 @@ int alloc_job(afq_job* j, bool dry, int64_t* total) {
       const long np = bf16_npad(j->N), gp = bf16_npad(j->G);
       A_(j->rcs[0], G * 2 * bf16_opad(j->nocc[0]) * np);
       if (!j->rhf) A_(j->rcs[1], G * 2 * bf16_opad(j->nocc[1]) * np);
+      A_(j->rcs_fb[0], G * 2 * bf16_opad(j->nocc[0]) * np);
+      if (!j->rhf) A_(j->rcs_fb[1], G * 2 * bf16_opad(j->nocc[1]) * np);
       A_(j->gsplit, W * 4 * bf16_opad((int)std::max(j->nocc[0], j->nocc[1])) * np);
       A_(j->lt_split, 16 * ((N * N + 15) / 16) * 2 * gp);
       A_(j->xs_split, bf16_opad((int)W) * 4 * gp);
@@ -913,6 +921,7 @@ int afq_job_create(afq_job_t* out, const afq_job_desc* d) {
     j->rot_chol[1] = j->rot_chol[0];
     j->rot_chol32[1] = j->rot_chol32[0];
     j->rcs[1] = j->rcs[0];
+    j->rcs_fb[1] = j->rcs_fb[0];
   }
   *out = j;
   return AFQ_OK;
@@ -958,7 +967,7 @@ int afq_job_set_hamiltonian(afq_job_t j, double h0, const double* h1, const doub
   const long ncols = j->sym_chol ? j->npk : N * N;
   if (j->bf16x3) {
     const long gp = bf16_npad(j->G);
-    AFQ_TRY(k_split_chol_t_bf16(j->chol, j->G, N * N, ncols, j->sym_chol ? (int)N : 0, (int)gp, 2 * gp, gp,
+    AFQ_TRY(k_split_chol_t_bf16(j->chol, j->G, N * N, ncols, j->sym_chol ? (int)N : 0, (int)gp, 2 * gp, 32, 2 * 32,
                                 j->lt_split, S(stream)));
   } else if (j->lp) {
     if (j->sym_chol)
@@ -1035,7 +1044,10 @@ int afq_build_measurement_intermediates(afq_job_t j, void* stream) {
   if (j->lp)
     for (int s = 0; s < nsp; ++s) {
       AFQ_TRY(k_to_f32(j->rot_chol[s], j->rot_chol32[s], (long)j->G * j->nocc[s] * N, st));
-      if (j->bf16x3) AFQ_TRY(k_split_bf16(j->rot_chol[s], false, j->nocc[s], N, j->G, j->rcs[s], st));
+      if (j->bf16x3) {
+        AFQ_TRY(k_split_bf16(j->rot_chol[s], false, j->nocc[s], N, j->G, false, j->rcs[s], st));
+        AFQ_TRY(k_split_bf16(j->rot_chol[s], false, j->nocc[s], N, j->G, true, j->rcs_fb[s], st));
+      }
     }
   j->have_meas = true;
   return AFQ_OK;
@@ -1154,7 +1166,10 @@ int afq_set_intermediate(afq_job_t j, int which, int spin, const void* src, void
     case AFQ_INT_ROT_CHOL:
       j->have_rot_chol[spin] = true;
       if (j->lp) AFQ_TRY(k_to_f32(j->rot_chol[spin], j->rot_chol32[spin], G * O * N, S(stream)));
-      if (j->bf16x3) AFQ_TRY(k_split_bf16(j->rot_chol[spin], false, (int)O, (int)N, G, j->rcs[spin], S(stream)));
+      if (j->bf16x3) {
+        AFQ_TRY(k_split_bf16(j->rot_chol[spin], false, (int)O, (int)N, G, false, j->rcs[spin], S(stream)));
+        AFQ_TRY(k_split_bf16(j->rot_chol[spin], false, (int)O, (int)N, G, true, j->rcs_fb[spin], S(stream)));
+      }
       break;
     case AFQ_INT_MF_SHIFTS: j->have_mf = true; break;
     case AFQ_INT_H0_PROP: j->have_h0p = true; break;

@@ -649,29 +649,34 @@ __global__ __launch_bounds__(256, 2) void exchange_bf16_grouped_kernel(const Exc
 // lo of bf16(float(x)) and bf16(float(x) - hi), for the real then the
 // imaginary part.  Within a plane the elements are stored by 32-deep k step,
 // then row, then k (bf16_tile), so the 16 rows x 32 k of one MFMA operand
-// fragment are 1 KB of consecutive bytes.
+// fragment are 1 KB of consecutive bytes (the exchange's layout).  With ilv the
+// planes of a row are instead interleaved every 32 elements of that order,
+// out[r][t / 32][plane][t % 32] with t = bf16_tile(o, k): the force-bias
+// product reads a row's 32-deep step of all planes as whole 128-byte lines.
 __device__ __forceinline__ void split_bf16_body(const double* __restrict__ src, int cplx, int O, int N, int Opad,
-                                                int Npad, long rows, __bf16* __restrict__ out, long idx) {
+                                                int Npad, long rows, int ilv, __bf16* __restrict__ out, long idx) {
   const long per = (long)Opad * Npad;
   if (idx >= rows * per) return;
   const long r = idx / per;
   const int o = (int)((idx % per) / Npad), k = (int)(idx % Npad);
   const bool in = o < O && k < N;
   const int np = cplx ? 4 : 2;
-  __bf16* base = out + r * np * per + bf16_tile(o, k, Opad);
+  const long t = bf16_tile(o, k, Opad);
+  const long pst = ilv ? 32 : per;  // plane stride
+  __bf16* base = out + r * np * per + (ilv ? (t >> 5) * (32 * np) + (t & 31) : t);
   for (int c = 0; c < (cplx ? 2 : 1); ++c) {
     double x = 0.0;
     if (in) x = cplx ? src[2 * ((r * O + o) * (long)N + k) + c] : src[(r * O + o) * (long)N + k];
     const float xf = (float)x;
     const __bf16 h = (__bf16)xf;
-    base[(2 * c) * per] = h;
-    base[(2 * c + 1) * per] = (__bf16)(xf - (float)h);
+    base[(2 * c) * pst] = h;
+    base[(2 * c + 1) * pst] = (__bf16)(xf - (float)h);
   }
 }
 
 __global__ void split_bf16_kernel(const double* __restrict__ src, int cplx, int O, int N, int Opad, int Npad,
-                                  long rows, __bf16* __restrict__ out) {
-  split_bf16_body(src, cplx, O, N, Opad, Npad, rows, out, blockIdx.x * (long)blockDim.x + threadIdx.x);
+                                  long rows, int ilv, __bf16* __restrict__ out) {
+  split_bf16_body(src, cplx, O, N, Opad, Npad, rows, ilv, out, blockIdx.x * (long)blockDim.x + threadIdx.x);
 }
 
 __global__ void split_bf16_grouped_kernel(const SplitJob* __restrict__ jobs, const int* __restrict__ prefix,
@@ -679,7 +684,7 @@ __global__ void split_bf16_grouped_kernel(const SplitJob* __restrict__ jobs, con
   const int item = blockIdx.x;
   const int j = group_find(prefix, njobs, item);
   const SplitJob J = jobs[j];
-  split_bf16_body(J.src, J.cplx, J.O, J.N, bf16_opad_d(J.O), bf16_npad_d(J.N), J.rows,
+  split_bf16_body(J.src, J.cplx, J.O, J.N, bf16_opad_d(J.O), bf16_npad_d(J.N), J.rows, J.ilv,
                   reinterpret_cast<__bf16*>(J.out), (long)(item - prefix[j]) * 256 + threadIdx.x);
 }
 
@@ -868,12 +873,12 @@ int split_bf16_grouped(const GroupTab<SplitJob>& t, hipStream_t s) {
   return launch_status("split_bf16_grouped_kernel");
 }
 
-int k_split_bf16(const double* src, bool cplx, int O, int N, long rows, uint16_t* out, hipStream_t s) {
+int k_split_bf16(const double* src, bool cplx, int O, int N, long rows, bool ilv, uint16_t* out, hipStream_t s) {
   const int Opad = bf16_opad(O), Npad = bf16_npad(N);
   const long n = rows * Opad * (long)Npad;
   if (n <= 0) return AFQ_OK;
   hipLaunchKernelGGL(split_bf16_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, src, cplx ? 1 : 0, O, N,
-                     Opad, Npad, rows, reinterpret_cast<__bf16*>(out));
+                     Opad, Npad, rows, ilv ? 1 : 0, reinterpret_cast<__bf16*>(out));
   return launch_status("split_bf16_kernel");
 }
 

@@ -2,10 +2,11 @@
 //
 //   C[b] (M x N) = alpha * A[b] (M x K, complex) . B[b]^T (N x K, real)
 //
-// Both operands are stored as bf16 planes with k contiguous: a real operand as
-// (hi, lo) = (bf16(x), bf16(x - hi)), a complex one as (re_hi, re_lo, im_hi,
-// im_lo).  Element (row r, plane q, k) of A is at A + b*sA + r*lda + q*pA + k
-// (same for B).  K is a multiple of 32 and zero-padded; rows past the valid
+// Both operands are stored as bf16 planes with k contiguous in 32-deep steps: a
+// real operand as (hi, lo) = (bf16(x), bf16(x - hi)), a complex one as (re_hi,
+// re_lo, im_hi, im_lo).  Element (row r, plane q, k) of A is at A + b*sA +
+// r*lda + (k/32)*kA + q*pA + k%32 (same for B), so the planes may be stored
+// apart or interleaved step by step.  K is a multiple of 32 and zero-padded; rows past the valid
 // ones up to the next multiple of 16 exist and hold zeros, and reads of rows
 // beyond that are clamped (their outputs are discarded).  Each real product is
 // hi*hi + lo*hi + hi*lo on v_mfma_f32_16x16x32_bf16 with FP32 accumulation.
@@ -88,15 +89,15 @@ __device__ __forceinline__ void gemm_bf3_body(const Bf3& p, int tiles_m, int til
     for (int j = 0; j < TN; ++j) acc[i][j] = v4f{0.f, 0.f, 0.f, 0.f};
 
   const int ks0 = kz * kchunk / 32, nk = min(p.K, (kz + 1) * kchunk) / 32;
+  const long kva = p.kA >> 3, kvb = p.kB >> 3;  // k step strides in units of 8 bf16
   v8bf ra[RA], rb[RB];
   auto fetch = [&](int ks) {
-    const int kv = ks * 4;  // in units of 8 bf16
 #pragma unroll
     for (int i = 0; i < RA; ++i)
-      if (la[i] >= 0) ra[i] = ga[i][kv];
+      if (la[i] >= 0) ra[i] = ga[i][ks * kva];
 #pragma unroll
     for (int i = 0; i < RB; ++i)
-      if (lb[i] >= 0) rb[i] = gb[i][kv];
+      if (lb[i] >= 0) rb[i] = gb[i][ks * kvb];
   };
   auto stash = [&](int st) {
     __bf16* a = As + st * BMR * LROW;
@@ -249,10 +250,12 @@ int launch_lds(const Bf3& p, hipStream_t s) {
 }
 
 // ------------------------------------------------------------------ operand preparation
-// dst[r][q][k] (k < kpad, rows r < rows) from a row-major complex (or real) double
-// matrix src[r][k] (k < K): bf16 planes, zero for k >= K.
+// dst[r][k / 32][q][k % 32] (strides ldd, kst, pd; k < kpad, rows r < rows)
+// from a row-major complex (or real) double matrix src[r][k] (k < K): bf16
+// planes, zero for k >= K.
 __device__ __forceinline__ void split_rows_body(const double* __restrict__ src, int cplx, long rows, int K, long lds,
-                                                int kpad, long ldd, long pd, __bf16* __restrict__ dst, long idx) {
+                                                int kpad, long ldd, long pd, long kst, __bf16* __restrict__ dst,
+                                                long idx) {
   if (idx >= rows * kpad) return;
   const long r = idx / kpad;
   const int k = (int)(idx % kpad);
@@ -265,7 +268,7 @@ __device__ __forceinline__ void split_rows_body(const double* __restrict__ src, 
       xr = src[r * lds + k];
     }
   }
-  __bf16* d = dst + r * ldd + k;
+  __bf16* d = dst + r * ldd + (k >> 5) * kst + (k & 31);
   const float fr = (float)xr;
   const __bf16 h = (__bf16)fr;
   d[0] = h;
@@ -279,8 +282,8 @@ __device__ __forceinline__ void split_rows_body(const double* __restrict__ src, 
 }
 
 __global__ void split_rows_kernel(const double* __restrict__ src, int cplx, long rows, int K, long lds, int kpad,
-                                  long ldd, long pd, __bf16* __restrict__ dst) {
-  split_rows_body(src, cplx, rows, K, lds, kpad, ldd, pd, dst, blockIdx.x * (long)blockDim.x + threadIdx.x);
+                                  long ldd, long pd, long kst, __bf16* __restrict__ dst) {
+  split_rows_body(src, cplx, rows, K, lds, kpad, ldd, pd, kst, dst, blockIdx.x * (long)blockDim.x + threadIdx.x);
 }
 
 __global__ void split_rows_grouped_kernel(const RowSplitJob* __restrict__ jobs, const int* __restrict__ prefix,
@@ -288,15 +291,16 @@ __global__ void split_rows_grouped_kernel(const RowSplitJob* __restrict__ jobs, 
   const int item = blockIdx.x;
   const int j = group_find(prefix, njobs, item);
   const RowSplitJob J = jobs[j];
-  split_rows_body(J.src, J.cplx, J.rows, J.K, J.lds, J.kpad, J.ldd, J.pd, reinterpret_cast<__bf16*>(J.dst),
+  split_rows_body(J.src, J.cplx, J.rows, J.K, J.lds, J.kpad, J.ldd, J.pd, J.kst, reinterpret_cast<__bf16*>(J.dst),
                   (long)(item - prefix[j]) * 256 + threadIdx.x);
 }
 
-// L (G x NN, row-major doubles) -> Lt[n][q][g] (bf16 hi/lo planes, g < gpad,
-// zero for g >= G) for output rows n < ncols; with sym_N > 0 output row n is
-// packed column n of a symmetric L (sym_src), else column n itself.
+// L (G x NN, row-major doubles) -> Lt[n][g / 32][q][g % 32] (strides ldd,
+// kst, pd; bf16 hi/lo planes, g < gpad, zero for g >= G) for output rows
+// n < ncols; with sym_N > 0 output row n is packed column n of a symmetric L
+// (sym_src), else column n itself.
 __global__ void split_chol_t_kernel(const double* __restrict__ L, int G, long NN, long ncols, int sym_N, int gpad,
-                                    long ldd, long pd, __bf16* __restrict__ dst) {
+                                    long ldd, long pd, long kst, __bf16* __restrict__ dst) {
   __shared__ double t[32][33];
   const long n0 = (long)blockIdx.x * 32;
   const int g0 = blockIdx.y * 32;
@@ -316,7 +320,7 @@ __global__ void split_chol_t_kernel(const double* __restrict__ L, int G, long NN
     if (n < ncols && g < gpad) {
       const float x = (float)t[tx][y];
       const __bf16 h = (__bf16)x;
-      __bf16* d = dst + n * ldd + g;
+      __bf16* d = dst + n * ldd + (g >> 5) * kst + (g & 31);
       d[0] = h;
       d[pd] = (__bf16)(x - (float)h);
     }
@@ -330,6 +334,8 @@ int gemm_bf3(const Bf3& p, hipStream_t s) {
   if (p.K <= 0 || p.K % 32 != 0) return fail(AFQ_E_INVALID, "gemm_bf3: K must be a positive multiple of 32");
   if (p.a_rows < 1 || p.b_rows < 1) return fail(AFQ_E_INVALID, "gemm_bf3: operand row counts must be positive");
   if (!p.a_cplx || p.b_cplx) return fail(AFQ_E_INVALID, "gemm_bf3: complex A and real B expected");
+  if (p.kA < 32 || p.kB < 32 || p.kA % 8 || p.kB % 8)
+    return fail(AFQ_E_INVALID, "gemm_bf3: k step strides must be multiples of 8, at least 32");
   // 128 real rows (64 complex) x 128 columns per work-group, 2 x 4 waves of
   // 4 x 2 blocks: measured fastest of the LDS tilings on the VHS product
   // (W=256, N=300, G=2000: 0.63 ms; W=512, N=200, G=1200: 0.31 ms) and the force
@@ -341,6 +347,8 @@ int gemm_bf3(const Bf3& p, hipStream_t s) {
 int bf3_group_job(const Bf3& p, Bf3Job* out) {
   if (p.batch != 1) return fail(AFQ_E_INVALID, "gemm_bf3 grouped: batch-1 products only");
   if (p.K <= 0 || p.K % 32 != 0) return fail(AFQ_E_INVALID, "gemm_bf3: K must be a positive multiple of 32");
+  if (p.kA < 32 || p.kB < 32 || p.kA % 8 || p.kB % 8)
+    return fail(AFQ_E_INVALID, "gemm_bf3: k step strides must be multiples of 8, at least 32");
   Bf3Job j;
   j.p = p;
   j.tiles_m = (2 * p.M + 127) / 128;
@@ -378,12 +386,12 @@ int gemm_bf3_grouped(const GroupTab<Bf3Job>& t, hipStream_t s) {
   return launch_status("gemm_bf3_grouped_kernel");
 }
 
-int k_split_rows_bf16(const double* src, bool cplx, long rows, int K, long lds, int kpad, long ldd, long pd,
+int k_split_rows_bf16(const double* src, bool cplx, long rows, int K, long lds, int kpad, long ldd, long pd, long kst,
                       uint16_t* dst, hipStream_t s) {
   const long n = rows * (long)kpad;
   if (n <= 0) return AFQ_OK;
   hipLaunchKernelGGL(split_rows_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, src, cplx ? 1 : 0, rows, K,
-                     lds, kpad, ldd, pd, reinterpret_cast<__bf16*>(dst));
+                     lds, kpad, ldd, pd, kst, reinterpret_cast<__bf16*>(dst));
   return launch_status("split_rows_kernel");
 }
 
@@ -396,12 +404,12 @@ int split_rows_grouped(const GroupTab<RowSplitJob>& t, hipStream_t s) {
   return launch_status("split_rows_grouped_kernel");
 }
 
-int k_split_chol_t_bf16(const double* L, int G, long NN, long ncols, int sym_N, int gpad, long ldd, long pd,
+int k_split_chol_t_bf16(const double* L, int G, long NN, long ncols, int sym_N, int gpad, long ldd, long pd, long kst,
                         uint16_t* dst, hipStream_t s) {
   const long nb = (ncols + 31) / 32;
   if (nb > 0x7fffffffL) return fail(AFQ_E_UNSUPPORTED, "split_chol_t: too large");
   hipLaunchKernelGGL(split_chol_t_kernel, dim3((unsigned)nb, (gpad + 31) / 32), dim3(256), 0, s, L, G, NN, ncols,
-                     sym_N, gpad, ldd, pd, reinterpret_cast<__bf16*>(dst));
+                     sym_N, gpad, ldd, pd, kst, reinterpret_cast<__bf16*>(dst));
   return launch_status("split_chol_t_kernel");
 }
 

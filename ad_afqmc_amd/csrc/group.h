@@ -119,6 +119,7 @@ struct SplitJob {
   int cplx, O, N;
   long rows;
   uint16_t* out;
+  int ilv;  // planes interleaved by 32-deep k step (the force-bias product's layout)
 };
 long split_items(const SplitJob& j);
 int split_bf16_grouped(const GroupTab<SplitJob>& t, hipStream_t s);
@@ -131,7 +132,7 @@ struct RowSplitJob {
   int K;
   long lds;
   int kpad;
-  long ldd, pd;
+  long ldd, pd, kst;
   uint16_t* dst;
 };
 long row_split_items(const RowSplitJob& j);

@@ -97,7 +97,7 @@ struct afq_group {
   afq::DevTab<afq::DetInvSet> di_inv, di_det, di_qr;
   afq::DevTab<afq::CholSet> chol;
   afq::DevTab<afq::VecJob> qr_conj, qr_copy, sr_copy;
-  afq::DevTab<afq::SplitJob> gsplit;
+  afq::DevTab<afq::SplitJob> gsplit, gsplit_fb;  // exchange and force-bias layouts
   afq::DevTab<afq::Bf3Job> fb, vhs;
   afq::DevTab<afq::ReduceJob> fb_red;
   afq::DevTab<afq::RowSplitJob> xsplit;
@@ -130,7 +130,7 @@ int build(afq_group* g, afq_job* const* jobs, void* const* walkers, double* cons
   std::vector<DetInvSet> dinv(n), ddet(n), dqr(n);
   std::vector<CholSet> ch(n);
   std::vector<VecJob> vconj(n), vcopy(n), vsr(n);
-  std::vector<SplitJob> sp(n);
+  std::vector<SplitJob> sp(n), sp_fb(n);
   std::vector<Bf3Job> fbj(n), vhj(n);
   std::vector<ReduceJob> red(n);
   std::vector<RowSplitJob> rsp(n);
@@ -280,7 +280,9 @@ int build(afq_group* g, afq_job* const* jobs, void* const* walkers, double* cons
 
     // force bias (force_bias_gemm, split-BF16 x3): split G, product with the split rot_chol
     const long P = (long)bf16_opad(O) * bf16_npad(N);
-    sp[i] = SplitJob{reinterpret_cast<const double*>(j->green[0]), 1, O, N, (long)W, j->gsplit};
+    sp[i] = SplitJob{reinterpret_cast<const double*>(j->green[0]), 1, O, N, (long)W, j->gsplit, 0};
+    sp_fb[i] = sp[i];
+    sp_fb[i].ilv = 1;
     it_sp[i] = split_items(sp[i]);
     Bf3 f;
     f.M = W;
@@ -289,11 +291,13 @@ int build(afq_group* g, afq_job* const* jobs, void* const* walkers, double* cons
     f.a_cplx = true;
     f.A = j->gsplit;
     f.lda = 4 * P;
-    f.pA = P;
+    f.pA = 32;
+    f.kA = 4 * 32;
     f.a_rows = j->Wcap;
-    f.B = j->rcs[0];
+    f.B = j->rcs_fb[0];
     f.ldb = 2 * P;
-    f.pB = P;
+    f.pB = 32;
+    f.kB = 2 * 32;
     f.b_rows = G;
     f.alpha = double2{2.0, 0.0};
     f.beta = double2{0.0, 0.0};
@@ -316,8 +320,8 @@ int build(afq_group* g, afq_job* const* jobs, void* const* walkers, double* cons
 
     // VHS (trotprop_lp): split the shifted fields, product with the split L^T into the planes
     const int gpad = bf16_npad(G);
-    rsp[i] = RowSplitJob{reinterpret_cast<const double*>(j->xs), 1, (long)W, G, (long)G, gpad, 4L * gpad,
-                         (long)gpad, j->xs_split};
+    rsp[i] = RowSplitJob{reinterpret_cast<const double*>(j->xs), 1, (long)W, G, (long)G, gpad, 4L * gpad, 32L,
+                         4L * 32, j->xs_split};
     it_rsp[i] = row_split_items(rsp[i]);
     const long ncols = j->sym_chol ? j->npk : NN;
     Bf3 v;
@@ -327,11 +331,13 @@ int build(afq_group* g, afq_job* const* jobs, void* const* walkers, double* cons
     v.a_cplx = true;
     v.A = j->xs_split;
     v.lda = 4L * gpad;
-    v.pA = gpad;
+    v.pA = 32;
+    v.kA = 4 * 32;
     v.a_rows = bf16_opad(j->Wcap);
     v.B = j->lt_split;
     v.ldb = 2L * gpad;
-    v.pB = gpad;
+    v.pB = 32;
+    v.kB = 2 * 32;
     v.b_rows = (int)(16 * ((ncols + 15) / 16));
     v.alpha = double2{0.0, std::sqrt(j->d.dt)};
     v.C = j->sym_chol ? static_cast<void*>(j->vhsp32) : static_cast<void*>(j->vhs32);
@@ -378,6 +384,7 @@ int build(afq_group* g, afq_job* const* jobs, void* const* walkers, double* cons
   AFQ_TRY(m.table(vcopy, it_vec, &g->qr_copy));
   AFQ_TRY(m.table(vsr, it_vec, &g->sr_copy));
   AFQ_TRY(m.table(sp, it_sp, &g->gsplit));
+  AFQ_TRY(m.table(sp_fb, it_sp, &g->gsplit_fb));
   AFQ_TRY(m.table(fbj, it_fb, &g->fb));
   AFQ_TRY(m.table(red, it_red, &g->fb_red));
   AFQ_TRY(m.table(rsp, it_rsp, &g->xsplit));
@@ -404,7 +411,7 @@ int g_block(afq_group* g, hipStream_t st) {
   for (int i = 0; i < g->nsteps; ++i) {
     // propagate(inv_ready = i > 0, keep_inv = i + 1 < nsteps)
     AFQ_TRY(g_green(g, true, true, i > 0, st));
-    AFQ_TRY(split_bf16_grouped(g->gsplit.view(), st));
+    AFQ_TRY(split_bf16_grouped(g->gsplit_fb.view(), st));
     AFQ_TRY(gemm_bf3_grouped(g->fb.view(), st));
     AFQ_TRY(splitk_reduce_grouped(g->fb_red.view(), st));
     AFQ_TRY(g_field_shift(g->bj[BI_WALKER].view(), i, st));

@@ -28,11 +28,14 @@ long exchange_wide_ws_elems(int O, int Gn, int W);
 int k_exchange_wide(const double* rc, const double2* Gm, int O, int N, int Gn, int W, double2* ws, long ws_elems,
                     double2* coul, double2* kpart, int* nchunk_out, hipStream_t s);
 constexpr int kExchangeRegMaxO = 64;  // largest nocc of the register-blocked kernels
-// Split-BF16 operands of the exchange: rows x (O, N) real (2 planes) or complex
-// (4 planes) doubles -> bf16 (hi, lo) planes of bf16_opad(O) x bf16_npad(N), zero padded.
+// Split-BF16 operands of the exchange and the force bias: rows x (O, N) real
+// (2 planes) or complex (4 planes) doubles -> bf16 (hi, lo) planes of
+// bf16_opad(O) x bf16_npad(N), zero padded, each plane tiled by 32-deep k step
+// (the exchange's layout) or, with ilv, the planes of a row interleaved every
+// 32 elements (the force-bias product's layout).
 int bf16_opad(int O);
 int bf16_npad(int N);
-int k_split_bf16(const double* src, bool cplx, int O, int N, long rows, uint16_t* out, hipStream_t s);
+int k_split_bf16(const double* src, bool cplx, int O, int N, long rows, bool ilv, uint16_t* out, hipStream_t s);
 int k_exchange_bf16(const uint16_t* rcs, const uint16_t* gs, int O, int N, int Gn, int W, double2* coul,
                     double2* kpart, int* nchunk_out, hipStream_t s);
 int k_to_f32(const double* src, float* dst, long n, hipStream_t s);
