@@ -11,7 +11,8 @@ struct TrotFused {
   double2* wb[2] = {nullptr, nullptr};        // (W, N, c[s]) complex128
   int c[2] = {0, 0};
   const uint16_t* vs = nullptr;                // VHS planes re hi, re lo, im hi, im lo per walker,
-                                               // each 16 ceil(N/16) x 32 ceil(N/32), zero past N
+                                               // each 16 ceil(N/16) x 32 ceil(N/32), zero past N,
+                                               // stored by 32-deep k step ([k/32][row][k%32])
   const uint16_t* e1[2] = {nullptr, nullptr};  // k_split_e1_bf16 output of each spin's E
 };
 // Kernel arguments of one job (bf16 pointers as __bf16).
@@ -23,6 +24,7 @@ struct FusedArgs {
   const __bf16* vs;      // VHS planes re hi, re lo, im hi, im lo of (16 R) x K32 per walker
   long ep;               // plane stride of vs and e1 (elements)
   const __bf16* e1[2];   // E - I of each spin: planes hi, mid, lo of (16 R) x K32, zero padded
+                         // (vs and e1 planes stored by 32-deep k step)
   int nterms;            // n_exp_terms: Taylor terms 1 .. nterms - 1
 };
 

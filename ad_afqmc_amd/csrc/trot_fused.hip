@@ -18,8 +18,9 @@
 //   * a Taylor term is VHS . term / n on the split-BF16 x3 scheme (hi.hi +
 //     lo.hi + hi.lo per real product).  The VHS product writes the VHS as four
 //     bf16 planes of the full N x N matrix (from the packed lower block
-//     triangle when L is symmetric, mirrored), so an A fragment is one
-//     16-byte load per plane straight from global memory (L2) and the inner
+//     triangle when L is symmetric, mirrored) stored by 32-deep k step, so
+//     an A fragment is one 16-byte load per lane and plane, 1 KB of
+//     consecutive bytes, straight from global memory (L2), and the inner
 //     loop does no conversion;
 //   * the running sum of the terms stays in FP32 registers (the terms are
 //     complex64 values) and is added to t0 in FP64.
@@ -50,6 +51,13 @@ __device__ __forceinline__ v4f mfma_bf16(const v8bf& a, const v8bf& b, const v4f
 // distinct 16-byte bank groups
 __device__ __forceinline__ int pofs(int n, int k, int kpl) {
   return n * kpl + ((((k >> 3) ^ (n & 15)) << 3) | (k & 7));
+}
+
+// bf16 element offset of entry (row i, column k) in a VHS or (E - I) plane of
+// rows x K32 entries stored by 32-deep k step, [k / 32][i][k % 32]: the 16
+// rows x 32 k of an MFMA A fragment are 1 KB of consecutive bytes
+__device__ __forceinline__ long tofs(int i, int k, int rows) {
+  return ((long)(k >> 5) * rows + i) * 32 + (k & 31);
 }
 
 __device__ __forceinline__ long sym_packed_cols_d(int N) {
@@ -161,7 +169,7 @@ __device__ __forceinline__ void trot_fused_body(const FusedArgs& a, int w, int s
       }
     v8bf c[RBW][3];
     auto load = [&](int ii, int k0) {
-      const __bf16* Er = E + (long)(16 * (wave + FW * ii) + r) * a.K32 + 8 * q + k0;
+      const __bf16* Er = E + tofs(16 * (wave + FW * ii) + r, k0 + 8 * q, 16 * a.R);
 #pragma unroll
       for (int t = 0; t < 3; ++t) c[ii][t] = *reinterpret_cast<const v8bf*>(Er + t * a.ep);
     };
@@ -212,7 +220,7 @@ __device__ __forceinline__ void trot_fused_body(const FusedArgs& a, int w, int s
       }
     v8bf c[RBW][4];
     auto load = [&](int ii, int k0) {
-      const __bf16* Vr = V + (long)(16 * (wave + FW * ii) + r) * a.K32 + 8 * q + k0;
+      const __bf16* Vr = V + tofs(16 * (wave + FW * ii) + r, k0 + 8 * q, 16 * a.R);
 #pragma unroll
       for (int t = 0; t < 4; ++t) c[ii][t] = *reinterpret_cast<const v8bf*>(Vr + t * a.ep);
     };
@@ -468,7 +476,7 @@ __device__ __forceinline__ void split_vhs_body(const VhsSplit& v, long blk) {
   __bf16* base = reinterpret_cast<__bf16*>(v.out) + 4 * ep * w;
   {
     const int i = 16 * ib + r, k0 = 16 * kb + 4 * q;
-    if (active && i < N && k0 < N) put4(base, ep, (long)i * K32 + k0, x, min(4, N - k0));
+    if (active && i < N && k0 < N) put4(base, ep, tofs(i, k0, 16 * nb), x, min(4, N - k0));
   }
   if (!v.sym) return;
 #pragma unroll
@@ -480,7 +488,7 @@ __device__ __forceinline__ void split_vhs_body(const VhsSplit& v, long blk) {
 #pragma unroll
   for (int u = 0; u < 4; ++u) y[u] = tile[sub][4 * q + u][r];
   const int i = 16 * kb + r, k0 = 16 * ib + 4 * q;
-  if (i < N && k0 < N) put4(base, ep, (long)i * K32 + k0, y, min(4, N - k0));
+  if (i < N && k0 < N) put4(base, ep, tofs(i, k0, 16 * nb), y, min(4, N - k0));
 }
 
 __global__ __launch_bounds__(256) void split_vhs_kernel(VhsSplit v) { split_vhs_body(v, blockIdx.x); }
@@ -493,7 +501,8 @@ __global__ __launch_bounds__(256) void split_vhs_grouped_kernel(const VhsSplit* 
   split_vhs_body(v, item - prefix[j]);
 }
 
-// (E - I) split into three bf16 planes of rows x ld (zero padded)
+// (E - I) split into three bf16 planes of rows x ld (zero padded), stored by
+// 32-deep k step (tofs)
 __global__ void split_e1_kernel(const double* E, int N, int rows, int ld, long plane, __bf16* out) {
   const long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
   if (i >= (long)rows * ld) return;
@@ -503,9 +512,10 @@ __global__ void split_e1_kernel(const double* E, int N, int rows, int ld, long p
   const double r1 = v - (double)(float)h;
   const __bf16 mid = (__bf16)(float)r1;
   const double r2 = r1 - (double)(float)mid;
-  out[i] = h;
-  out[plane + i] = mid;
-  out[2 * plane + i] = (__bf16)(float)r2;
+  const long o = tofs(m, k, rows);
+  out[o] = h;
+  out[plane + o] = mid;
+  out[2 * plane + o] = (__bf16)(float)r2;
 }
 
 int fused_tn(const int* c, int nspin, int N) {
