@@ -25,22 +25,193 @@ __device__ inline double2 crecip(double2 a) {
   return {r / den, -1.0 / den};
 }
 
-// One 256-thread work-group per matrix.  Up to n = 64 the matrix lives in
-// dynamic LDS; above (n <= 256) it is factored in global memory, in the
-// inverse's buffer or the set's scratch (it stays in L2).  Thread t owns the
-// columns t % 64 + 64 c for the rows t/64, t/64 + 4, ..., so the elimination
-// needs no index division and reads the pivot row and factor column as LDS
-// broadcasts.  Partial pivoting selects the first row of maximal |re|+|im| in
-// the pivot column (LAPACK izamax convention).  With an inverse requested the
-// sweep is Gauss-Jordan in place (the inverse ends with its columns permuted
-// by the row interchanges, undone at the end); for the determinant alone only
-// the trailing rows and columns are eliminated (the pivots, hence the
-// determinant, are the same).
-// blockIdx.y selects one of up to two matrix sets (the two spins' overlap
-// matrices factor in one launch, so a CU holds two work-groups at W = 256).
-// NMAX: the largest n of the launch (64 or kDetInvMaxN); each thread covers
-// NMAX / 64 columns (and pivot-search rows).
-constexpr int kLdsN = 64;  // largest n factored in LDS
+// Gauss-Jordan determinant and inverse, one 256-thread work-group per matrix,
+// partial pivoting on the largest |re|+|im| of the pivot column (LAPACK
+// izamax measure).  blockIdx.y selects one of up to two matrix sets (the two
+// spins' overlap matrices factor in one launch, so a CU holds two work-groups
+// at W = 256).
+//
+// n <= 64 (gj_reg_body): the matrix stays in registers.  Lane i of every wave
+// holds row i, wave w the columns w + 4t (t < 16).  Rows are never swapped:
+// the pivot of column k is the first (lowest) not yet used row of maximal
+// measure, found by the wave that holds column k (a DPP maximum and a ballot),
+// which publishes the pivot row index, the pivot, its reciprocal and its copy
+// of column k (the factors) through LDS, double-buffered, so one barrier per
+// column suffices, and replaces its column k by the unit vector of the pivot
+// row.  Each wave then passes its part of the pivot row through LDS, scales it
+// on 16 lanes, and updates all its columns with one branch-free formula
+// (pivot row: scaled row; other rows: a - f r).  The result R relates to the
+// inverse through the pivot sequence piv (row piv[m] was the pivot of column
+// m): A^-1[pinv[r]][piv[c]] = R[r][c], pinv the inverse permutation; det A =
+// sign(piv) prod of the pivots (the determinant alone runs the same sweep).
+//
+// n > 64 (gj_det_inv_body, n <= 256): the matrix is factored in global memory,
+// in the inverse's buffer or the set's scratch (it stays in L2), with explicit
+// row interchanges (the first row of maximal measure at or below k).  Thread t
+// owns the columns t % 64 + 64 c for the rows t/64, t/64 + 4, ..., so the
+// elimination needs no index division and reads the pivot row and factor
+// column as LDS broadcasts.  With an inverse requested the sweep is
+// Gauss-Jordan in place (the inverse ends with its columns permuted by the row
+// interchanges, undone at the end); for the determinant alone only the
+// trailing rows and columns are eliminated (the pivots, hence the determinant,
+// are the same).  NMAX = kDetInvMaxN; each thread covers NMAX / 64 columns
+// (and pivot-search rows); n <= 64 matrices of such a launch use dynamic LDS.
+constexpr int kLdsN = 64;  // largest n factored in registers (and LDS)
+
+__device__ __forceinline__ double readlane_d(double x, int l) {
+  const long long v = __double_as_longlong(x);
+  const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(v & 0xffffffffLL), l);
+  const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(v >> 32), l);
+  return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
+// max(v, v of the lane that DPP control CTRL names), rows outside ROWS kept
+template <int CTRL, int ROWS>
+__device__ __forceinline__ double dpp_fmax(double v) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_update_dpp((int)(b & 0xffffffffLL), (int)(b & 0xffffffffLL), CTRL, ROWS, 0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp((int)(b >> 32), (int)(b >> 32), CTRL, ROWS, 0xF, false);
+  return fmax(v, __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo)));
+}
+
+// maximum of v over the wave (NaN ignored): within quads, half rows and rows,
+// then rows 0-1 into row 1 and 2-3 into row 3 (row_bcast 15), rows 0-1 into
+// row 3 (row_bcast 31); lane 63 holds the result
+__device__ __forceinline__ double wave_fmax(double v) {
+  v = dpp_fmax<0xB1, 0xF>(v);   // quad_perm [1, 0, 3, 2]
+  v = dpp_fmax<0x4E, 0xF>(v);   // quad_perm [2, 3, 0, 1]
+  v = dpp_fmax<0x141, 0xF>(v);  // row_half_mirror
+  v = dpp_fmax<0x140, 0xF>(v);  // row_mirror
+  v = dpp_fmax<0x142, 0xA>(v);  // row_bcast 15
+  v = dpp_fmax<0x143, 0xC>(v);  // row_bcast 31
+  return readlane_d(v, 63);
+}
+
+// orders a wave's LDS accesses (they are performed in issue order) without a
+// work-group barrier
+__device__ inline void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+#define AFQ_GJ_CASES(X) X(0) X(1) X(2) X(3) X(4) X(5) X(6) X(7) X(8) X(9) X(10) X(11) X(12) X(13) X(14) X(15)
+
+__device__ __forceinline__ void gj_reg_body(const DetInvSet& set, long b) {
+  __shared__ double2 fcol[2][64];
+  __shared__ double2 pdd[2], pdi[2];  // pivot and its reciprocal
+  __shared__ int pks[2];
+  __shared__ int piv[64], pinv[64];
+  __shared__ double2 prow[4][16], rrow[4][16];  // per wave: pivot row, scaled
+  const int n = set.n;
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const bool want_inv = set.inv != nullptr;
+  const double2* src = set.S + b * set.strideS;
+  double2 a[16];
+#pragma unroll
+  for (int t = 0; t < 16; ++t) {
+    const int c = wave + 4 * t;
+    a[t] = (lane < n && c < n) ? src[lane * n + c] : double2{0.0, 0.0};
+  }
+  unsigned long long used = 0;  // rows already used as pivots (wave-uniform)
+  double2 dprod = {1.0, 0.0};
+  for (int k = 0; k < n; ++k) {
+    const int buf = k & 1;
+    if (wave == (k & 3)) {
+      // pivot search on column k (register k / 4 of this wave)
+      const int tk = k >> 2;
+      double2 x = a[0];
+      switch (tk) {
+#define AFQ_GET(t) \
+  case t:          \
+    x = a[t];      \
+    break;
+        AFQ_GJ_CASES(AFQ_GET)
+#undef AFQ_GET
+      }
+      const bool cand = lane < n && !((used >> lane) & 1ull);
+      const double v = cand ? fabs(x.x) + fabs(x.y) : -1.0;
+      const double m = wave_fmax(v);
+      // lowest row of maximal measure (a column of NaNs falls back to the
+      // lowest unused row, so p is always a valid row)
+      const unsigned long long best = __ballot(cand && v == m), open = __ballot(cand);
+      const int p = __builtin_ctzll(best ? best : open);
+      const double2 d = {readlane_d(x.x, p), readlane_d(x.y, p)};
+      fcol[buf][lane] = x;
+      if (lane == 0) {
+        pks[buf] = p;
+        pdd[buf] = d;
+        pdi[buf] = crecip(d);
+        piv[k] = p;
+      }
+      // column k becomes the unit vector of row p: the common update below
+      // then leaves 1 / d in row p and -f / d in the others
+      const double2 e = (lane == p) ? double2{1.0, 0.0} : double2{0.0, 0.0};
+      switch (tk) {
+#define AFQ_SET(t) \
+  case t:          \
+    a[t] = e;      \
+    break;
+        AFQ_GJ_CASES(AFQ_SET)
+#undef AFQ_SET
+      }
+    }
+    __syncthreads();
+    const int p = __builtin_amdgcn_readfirstlane(pks[buf]);
+    const double2 dinv = pdi[buf];
+    const double2 f = fcol[buf][lane];
+    used |= 1ull << p;
+    if (tid == 0) dprod = cmul(dprod, pdd[buf]);
+    // row p of this wave's columns, scaled by 1 / d, through LDS
+    if (lane == p) {
+#pragma unroll
+      for (int t = 0; t < 16; ++t) prow[wave][t] = a[t];
+    }
+    wave_lds_sync();
+    if (lane < 16) rrow[wave][lane] = cmul(prow[wave][lane], dinv);
+    wave_lds_sync();
+    // row p: r; other rows: a - f r (column k: 0 - f / d)
+    const bool isp = lane == p;
+    const double sc = isp ? 0.0 : 1.0;
+    const double2 g = isp ? double2{1.0, 0.0} : double2{-f.x, -f.y};
+    double2 r[16];
+#pragma unroll
+    for (int t = 0; t < 16; ++t) r[t] = rrow[wave][t];
+#pragma unroll
+    for (int t = 0; t < 16; ++t) {
+      const double2 y = a[t];
+      a[t].x = fma(g.x, r[t].x, fma(-g.y, r[t].y, sc * y.x));
+      a[t].y = fma(g.x, r[t].y, fma(g.y, r[t].x, sc * y.y));
+    }
+  }
+  __syncthreads();  // piv complete
+  if (tid == 0) {
+    // sign of the pivot permutation: n minus its number of cycles transpositions
+    unsigned long long seen = 0;
+    int cycles = 0;
+    for (int s0 = 0; s0 < n; ++s0) {
+      if ((seen >> s0) & 1ull) continue;
+      ++cycles;
+      for (int q = s0; !((seen >> q) & 1ull); q = piv[q]) seen |= 1ull << q;
+    }
+    const bool neg = ((n - cycles) & 1) != 0;
+    set.det[b] = neg ? double2{-dprod.x, -dprod.y} : dprod;
+  }
+  if (!want_inv) return;
+  if (tid < n) pinv[piv[tid]] = tid;
+  __syncthreads();
+  double2* dst = set.inv + b * set.strideInv;
+  if (lane < n) {
+    const long row = (long)pinv[lane] * n;
+#pragma unroll
+    for (int t = 0; t < 16; ++t) {
+      const int c = wave + 4 * t;
+      if (c < n) dst[row + piv[c]] = a[t];
+    }
+  }
+}
+#undef AFQ_GJ_CASES
 
 template <int NMAX>
 __device__ __forceinline__ void gj_det_inv_body(const DetInvSet& set, long b, double2* __restrict__ lds) {
@@ -57,10 +228,8 @@ __device__ __forceinline__ void gj_det_inv_body(const DetInvSet& set, long b, do
   const double2* src = set.S + b * set.strideS;
   const int nn = n * n;
   const bool want_inv = inv != nullptr;
-  // (the n <= 64 kernel addresses LDS only, so its accesses stay LDS instructions)
   double2* a = lds;
-  if constexpr (NMAX > kLdsN)
-    if (n > kLdsN) a = want_inv ? inv + b * set.strideInv : set.work + b * set.strideS;
+  if (n > kLdsN) a = want_inv ? inv + b * set.strideInv : set.work + b * set.strideS;
   if (a != src)
     for (int i = tid; i < nn; i += 256) a[i] = src[i];
   __syncthreads();
@@ -135,56 +304,23 @@ __device__ __forceinline__ void gj_det_inv_body(const DetInvSet& set, long b, do
     }
     __syncthreads();
     const int i0 = want_inv ? rg : k + 1 + rg;
-    if constexpr (NMAX == kLdsN) {
-      // all of this thread's reads first (one LDS latency per step: the
-      // compiler cannot tell the matrix from fcol, so a read-modify-write
-      // loop would wait for every store before the next read)
-      if (j < n && j >= jlo) {
-        const double2 rk = prow[j];
-        double2 xv[16], fv[16];
 #pragma unroll
-        for (int t = 0; t < 16; ++t) {
-          const int i = i0 + 4 * t;
-          xv[t] = fv[t] = double2{0.0, 0.0};
-          if (i < n) {
-            fv[t] = fcol[i];
-            xv[t] = a[i * n + j];
-          }
-        }
-#pragma unroll
-        for (int t = 0; t < 16; ++t) {
-          const int i = i0 + 4 * t;
-          if (i < n && i != k) {
-            const double2 f = fv[t];
-            if (j == k) {
-              xv[t] = double2{-(f.x * rk.x - f.y * rk.y), -(f.x * rk.y + f.y * rk.x)};
-            } else {
-              xv[t].x -= f.x * rk.x - f.y * rk.y;
-              xv[t].y -= f.x * rk.y + f.y * rk.x;
-            }
-            a[i * n + j] = xv[t];
-          }
-        }
-      }
-    } else {
-#pragma unroll
-      for (int c = 0; c < kColsPer; ++c) {
-        const int jc = j + 64 * c;
-        if (jc < n && jc >= jlo) {
-          const double2 rk = prow[jc];
+    for (int c = 0; c < kColsPer; ++c) {
+      const int jc = j + 64 * c;
+      if (jc < n && jc >= jlo) {
+        const double2 rk = prow[jc];
 #pragma unroll 4
-          for (int i = i0; i < n; i += 4) {
-            if (i == k) continue;
-            const double2 f = fcol[i];
-            double2* x = &a[i * n + jc];
-            if (jc == k) {
-              *x = double2{-(f.x * rk.x - f.y * rk.y), -(f.x * rk.y + f.y * rk.x)};
-            } else {
-              double2 y = *x;
-              y.x -= f.x * rk.x - f.y * rk.y;
-              y.y -= f.x * rk.y + f.y * rk.x;
-              *x = y;
-            }
+        for (int i = i0; i < n; i += 4) {
+          if (i == k) continue;
+          const double2 f = fcol[i];
+          double2* x = &a[i * n + jc];
+          if (jc == k) {
+            *x = double2{-(f.x * rk.x - f.y * rk.y), -(f.x * rk.y + f.y * rk.x)};
+          } else {
+            double2 y = *x;
+            y.x -= f.x * rk.x - f.y * rk.y;
+            y.y -= f.x * rk.y + f.y * rk.x;
+            *x = y;
           }
         }
       }
@@ -215,7 +351,10 @@ __device__ __forceinline__ void gj_det_inv_body(const DetInvSet& set, long b, do
 template <int NMAX>
 __global__ __launch_bounds__(256) void gj_det_inv_kernel(DetInvSet s0, DetInvSet s1) {
   extern __shared__ double2 lds_gj[];
-  gj_det_inv_body<NMAX>(blockIdx.y == 0 ? s0 : s1, blockIdx.x, lds_gj);
+  if constexpr (NMAX == kLdsN)
+    gj_reg_body(blockIdx.y == 0 ? s0 : s1, blockIdx.x);
+  else
+    gj_det_inv_body<NMAX>(blockIdx.y == 0 ? s0 : s1, blockIdx.x, lds_gj);
 }
 
 // Grouped: work-group i factors matrix i - prefix[j] of job j's set.
@@ -226,7 +365,10 @@ __global__ __launch_bounds__(256) void gj_det_inv_grouped_kernel(const DetInvSet
   const int item = blockIdx.x;
   const int j = group_find(prefix, njobs, item);
   const DetInvSet set = sets[j];
-  gj_det_inv_body<NMAX>(set, item - prefix[j], lds_gj);
+  if constexpr (NMAX == kLdsN)
+    gj_reg_body(set, item - prefix[j]);
+  else
+    gj_det_inv_body<NMAX>(set, item - prefix[j], lds_gj);
 }
 
 __device__ inline double wave_sum(double v) {
@@ -454,9 +596,9 @@ int batched_det_inv_sets(const DetInvSet* sets, int nsets, int count, hipStream_
     nmax = std::max(nmax, sets[i].n);
   }
   if (count <= 0) return AFQ_OK;
-  const size_t lds = matrix_lds_bytes(nmax);
-  return nmax <= kLdsN ? launch_det_inv<kLdsN>(sets, nsets, count, lds, stream)
-                       : launch_det_inv<kDetInvMaxN>(sets, nsets, count, lds, stream);
+  // n <= 64: the matrices stay in registers, no dynamic LDS
+  return nmax <= kLdsN ? launch_det_inv<kLdsN>(sets, nsets, count, 0, stream)
+                       : launch_det_inv<kDetInvMaxN>(sets, nsets, count, matrix_lds_bytes(nmax), stream);
 }
 
 template <int NMAX>
@@ -472,9 +614,8 @@ int det_inv_grouped(const GroupTab<DetInvSet>& t, int nmax, hipStream_t stream) 
   if (t.njobs < 1 || t.total < 1) return AFQ_OK;
   if (nmax < 1 || nmax > kDetInvMaxN)
     return fail(AFQ_E_UNSUPPORTED, "det/inverse kernel supports 1 <= n <= " + std::to_string(kDetInvMaxN));
-  const size_t lds = matrix_lds_bytes(nmax);
-  return nmax <= kLdsN ? launch_det_inv_grouped<kLdsN>(t, lds, stream)
-                       : launch_det_inv_grouped<kDetInvMaxN>(t, lds, stream);
+  return nmax <= kLdsN ? launch_det_inv_grouped<kLdsN>(t, 0, stream)
+                       : launch_det_inv_grouped<kDetInvMaxN>(t, matrix_lds_bytes(nmax), stream);
 }
 
 int batched_det_inv(const double2* S, long strideS, int n, int count, double2* det, double2* inv, long strideInv,
