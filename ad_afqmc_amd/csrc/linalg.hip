@@ -15,6 +15,21 @@ namespace afq {
 namespace {
 
 __device__ inline double2 cmul(double2 a, double2 b) { return {a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x}; }
+
+// 1 / (a.x + i a.y) without branches or divisions: conj(a) / |a|^2 on a
+// scaled by the power of two 2^-e (exact) that brings max(|re|, |im|) into
+// [0.5, 1), so |a|^2 can neither overflow nor underflow; 1 / |a|^2 from the
+// hardware reciprocal and two Newton steps (within about an ulp).
+__device__ inline double2 crecip_fast(double2 a) {
+  const int e = __builtin_amdgcn_frexp_exp(fmax(fabs(a.x), fabs(a.y)));
+  const double xs = __builtin_amdgcn_ldexp(a.x, -e), ys = __builtin_amdgcn_ldexp(a.y, -e);
+  const double q = fma(xs, xs, ys * ys);
+  double r = __builtin_amdgcn_rcp(q);
+  r = fma(r, fma(-q, r, 1.0), r);
+  r = fma(r, fma(-q, r, 1.0), r);
+  const double rs = __builtin_amdgcn_ldexp(r, -e);
+  return {xs * rs, -ys * rs};
+}
 __device__ inline double2 crecip(double2 a) {
   // Smith's algorithm for 1/(a.x + i a.y)
   if (fabs(a.x) >= fabs(a.y)) {
@@ -35,12 +50,12 @@ __device__ inline double2 crecip(double2 a) {
 // holds row i, wave w the columns w + 4t (t < 16).  Rows are never swapped:
 // the pivot of column k is the first (lowest) not yet used row of maximal
 // measure, found by the wave that holds column k (a DPP maximum and a ballot),
-// which publishes the pivot row index, the pivot, its reciprocal and its copy
-// of column k (the factors) through LDS, double-buffered, so one barrier per
-// column suffices, and replaces its column k by the unit vector of the pivot
-// row.  Each wave then passes its part of the pivot row through LDS, scales it
-// on 16 lanes, and updates all its columns with one branch-free formula
-// (pivot row: scaled row; other rows: a - f r).  The result R relates to the
+// which publishes the pivot row index, the pivot and its copy of column k (the
+// factors) through LDS, double-buffered, so one barrier per column suffices,
+// and replaces its column k by the unit vector of the pivot row.  Each wave
+// then passes its part of the pivot row a_p through LDS and updates all its
+// columns with one branch-free formula (pivot row: a_p / d; other rows:
+// a - (f / d) a_p).  The result R relates to the
 // inverse through the pivot sequence piv (row piv[m] was the pivot of column
 // m): A^-1[pinv[r]][piv[c]] = R[r][c], pinv the inverse permutation; det A =
 // sign(piv) prod of the pivots (the determinant alone runs the same sweep).
@@ -99,10 +114,10 @@ __device__ inline void wave_lds_sync() {
 
 __device__ __forceinline__ void gj_reg_body(const DetInvSet& set, long b) {
   __shared__ double2 fcol[2][64];
-  __shared__ double2 pdd[2], pdi[2];  // pivot and its reciprocal
+  __shared__ double2 pdd[2];  // pivot
   __shared__ int pks[2];
   __shared__ int piv[64], pinv[64];
-  __shared__ double2 prow[4][16], rrow[4][16];  // per wave: pivot row, scaled
+  __shared__ double2 prow[4][16];  // per wave: its part of the pivot row
   const int n = set.n;
   const int tid = threadIdx.x, lane = tid & 63;
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
@@ -142,7 +157,6 @@ __device__ __forceinline__ void gj_reg_body(const DetInvSet& set, long b) {
       if (lane == 0) {
         pks[buf] = p;
         pdd[buf] = d;
-        pdi[buf] = crecip(d);
         piv[k] = p;
       }
       // column k becomes the unit vector of row p: the common update below
@@ -159,25 +173,26 @@ __device__ __forceinline__ void gj_reg_body(const DetInvSet& set, long b) {
     }
     __syncthreads();
     const int p = __builtin_amdgcn_readfirstlane(pks[buf]);
-    const double2 dinv = pdi[buf];
+    const double2 d = pdd[buf];
     const double2 f = fcol[buf][lane];
     used |= 1ull << p;
-    if (tid == 0) dprod = cmul(dprod, pdd[buf]);
-    // row p of this wave's columns, scaled by 1 / d, through LDS
+    if (tid == 0) dprod = cmul(dprod, d);
+    // row p of this wave's columns through LDS (the reciprocal is formed
+    // while the copies are in flight)
     if (lane == p) {
 #pragma unroll
       for (int t = 0; t < 16; ++t) prow[wave][t] = a[t];
     }
-    wave_lds_sync();
-    if (lane < 16) rrow[wave][lane] = cmul(prow[wave][lane], dinv);
-    wave_lds_sync();
-    // row p: r; other rows: a - f r (column k: 0 - f / d)
+    const double2 dinv = crecip_fast(d);
+    // row p: a / d; other rows: a - (f / d) a_p (column k: 1 / d and -f / d)
     const bool isp = lane == p;
     const double sc = isp ? 0.0 : 1.0;
-    const double2 g = isp ? double2{1.0, 0.0} : double2{-f.x, -f.y};
+    const double2 fd = cmul(f, dinv);
+    const double2 g = isp ? dinv : double2{-fd.x, -fd.y};
+    wave_lds_sync();
     double2 r[16];
 #pragma unroll
-    for (int t = 0; t < 16; ++t) r[t] = rrow[wave][t];
+    for (int t = 0; t < 16; ++t) r[t] = prow[wave][t];
 #pragma unroll
     for (int t = 0; t < 16; ++t) {
       const double2 y = a[t];
