@@ -432,6 +432,13 @@ __global__ __launch_bounds__(512, 1) void exchange_split_kernel(const double* __
 // NB x NB blocks, so consecutive MFMAs never depend on each other; the lo
 // fragments of the next step are requested as soon as the first pass has
 // consumed this step's, the hi ones at the start of the step.
+// Block (i, j), i <= j, holds f[16i + r][16j + c] = rot_chol_i . G_j^T with the
+// accumulator element at (r, c); block (j, i), j > i, is computed with the
+// operands swapped (G_i as A, rot_chol_j as B), so its element at (r, c) is
+// f[16j + c][16i + r]: the transposed partner of the same lane's element of
+// block (i, j).  The pair products f_ab f_ba of the off-diagonal blocks then
+// need no data movement; only the NB diagonal blocks go through an LDS
+// transpose tile.
 template <int NB>
 __device__ __forceinline__ void exchange_bf16_body(const v8bf* __restrict__ rcs, const v8bf* __restrict__ gs, int O,
                                                    int Opad, int Npad, int Gn, int nchunk, double2* __restrict__ coul,
@@ -496,6 +503,22 @@ __device__ __forceinline__ void exchange_bf16_body(const v8bf* __restrict__ rcs,
     ah[b] = Ah[b * 64];
     al[b] = Al[b * 64];
   }
+  // one product pass over all blocks: rot_chol fragments a (one bf16 part),
+  // Green's-function fragments gr, gi (real and imaginary, one bf16 part)
+  auto pass = [&](const v8bf(&a)[NB], const v8bf(&gr)[NB], const v8bf(&gi)[NB]) {
+#pragma unroll
+    for (int i = 0; i < NB; ++i)
+#pragma unroll
+      for (int j = 0; j < NB; ++j) {
+        if (i <= j) {
+          accR[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i], gr[j], accR[i][j], 0, 0, 0);
+          accI[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i], gi[j], accI[i][j], 0, 0, 0);
+        } else {
+          accR[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(gr[j], a[i], accR[i][j], 0, 0, 0);
+          accI[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(gi[j], a[i], accI[i][j], 0, 0, 0);
+        }
+      }
+  };
   copy(0, 0);
   int buf = 0;
   // The issue order of a step is pinned with scheduling barriers: B reads,
@@ -527,31 +550,13 @@ __device__ __forceinline__ void exchange_bf16_body(const v8bf* __restrict__ rcs,
 #pragma unroll
     for (int b = 0; b < NB; ++b) nh[b] = Ah[b * 64 + (long)kn * ROWS];
     __builtin_amdgcn_sched_barrier(0);
-#pragma unroll
-    for (int i = 0; i < NB; ++i)
-#pragma unroll
-      for (int j = 0; j < NB; ++j) {
-        accR[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[i], brh[j], accR[i][j], 0, 0, 0);
-        accI[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[i], bih[j], accI[i][j], 0, 0, 0);
-      }
+    pass(al, brh, bih);
     __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
     for (int b = 0; b < NB; ++b) al[b] = Al[b * 64 + (long)kn * ROWS];
     __builtin_amdgcn_sched_barrier(0);
-#pragma unroll
-    for (int i = 0; i < NB; ++i)
-#pragma unroll
-      for (int j = 0; j < NB; ++j) {
-        accR[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], brh[j], accR[i][j], 0, 0, 0);
-        accI[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bih[j], accI[i][j], 0, 0, 0);
-      }
-#pragma unroll
-    for (int i = 0; i < NB; ++i)
-#pragma unroll
-      for (int j = 0; j < NB; ++j) {
-        accR[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], brl[j], accR[i][j], 0, 0, 0);
-        accI[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bil[j], accI[i][j], 0, 0, 0);
-      }
+    pass(ah, brh, bih);
+    pass(ah, brl, bil);
 #pragma unroll
     for (int b = 0; b < NB; ++b) ah[b] = nh[b];
     __builtin_amdgcn_sched_barrier(0);
@@ -562,32 +567,46 @@ __device__ __forceinline__ void exchange_bf16_body(const v8bf* __restrict__ rcs,
   // accumulator element v of lane l sits at (4(l/16) + v, l%16)
   const int cc = lane & 15;
   double kr = 0.0, ki = 0.0, cr = 0.0, ci = 0.0;
-  double2* S = scr[wave];
+  // off-diagonal pairs: block (bi, bj) and, in the same lane and element, its
+  // transposed partner from block (bj, bi); each pair counts twice in the sum
 #pragma unroll
   for (int bi = 0; bi < NB; ++bi)
 #pragma unroll
-    for (int bj = bi; bj < NB; ++bj) {
-#pragma unroll
-      for (int v = 0; v < 4; ++v) S[(4 * q4 + v) * 17 + cc] = double2{(double)accR[bj][bi][v], (double)accI[bj][bi][v]};
-      wave_lds_sync();
-      const double wgt = (bi == bj) ? 1.0 : 2.0;
+    for (int bj = bi + 1; bj < NB; ++bj)
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
-        const int r = 4 * q4 + v;
-        const int gi = bi * 16 + r, gj = bj * 16 + cc;
+        const int gi = bi * 16 + 4 * q4 + v, gj = bj * 16 + cc;
         if (gi < O && gj < O) {
           const double xr = accR[bi][bj][v], xi = accI[bi][bj][v];
-          const double2 y = S[cc * 17 + r];
-          kr += wgt * (xr * y.x - xi * y.y);
-          ki += wgt * (xr * y.y + xi * y.x);
-          if (bi == bj && r == cc) {
-            cr += xr;
-            ci += xi;
-          }
+          const double yr = accR[bj][bi][v], yi = accI[bj][bi][v];
+          kr += 2.0 * (xr * yr - xi * yi);
+          ki += 2.0 * (xr * yi + xi * yr);
         }
       }
-      wave_lds_sync();
+  // diagonal blocks through the wave's transpose tile
+  double2* S = scr[wave];
+#pragma unroll
+  for (int bi = 0; bi < NB; ++bi) {
+#pragma unroll
+    for (int v = 0; v < 4; ++v) S[(4 * q4 + v) * 17 + cc] = double2{(double)accR[bi][bi][v], (double)accI[bi][bi][v]};
+    wave_lds_sync();
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const int r = 4 * q4 + v;
+      const int gi = bi * 16 + r, gj = bi * 16 + cc;
+      if (gi < O && gj < O) {
+        const double xr = accR[bi][bi][v], xi = accI[bi][bi][v];
+        const double2 y = S[cc * 17 + r];
+        kr += xr * y.x - xi * y.y;
+        ki += xr * y.y + xi * y.x;
+        if (r == cc) {
+          cr += xr;
+          ci += xi;
+        }
+      }
     }
+    wave_lds_sync();
+  }
   kr = wave_sum(kr);
   ki = wave_sum(ki);
   cr = wave_sum(cr);
