@@ -30,12 +30,21 @@ __device__ inline v4f mma(v8bf a, v8bf b, v4f c) { return __builtin_amdgcn_mfma_
 // and 2r + 1 (im), so a wave's 16 x 16 blocks cover re and im rows alike and
 // the accumulator rows 4(l/16) + v pair up as (re, im) of one output element.
 // Per 32-deep k step the work-group copies its A (BMR real rows) and B (BN
-// rows) hi/lo planes to LDS once (16-byte loads, rows padded to 144 bytes so
-// that the 16 rows of a fragment read fall in distinct banks), and the waves
-// read their fragments from there; two LDS stages, one barrier per step.
+// rows) hi/lo planes to LDS once (16-byte loads), and the waves read their
+// fragments from there; two LDS stages, one barrier per step.  In LDS each
+// 16-row block and plane is one 1 KB fragment of 64 16-byte slots in MFMA
+// lane order, slot 16 q + (r ^ (2 q + 8 plane)) holding row r, k-group q: a
+// fragment read covers 16 consecutive slots per 16 lanes, and the stash
+// writes (8 lanes per row, both planes) also fall in 16 distinct bank groups
+// per 16 lanes (rows of 144 bytes read by row were 1.4x slower).
 // Per accumulator the products run in a fixed order (hi*hi, lo*hi, hi*lo per
 // step, steps ascending), so results do not depend on the tiling.
-constexpr int LROW = 72;  // bf16 per LDS row: hi (32), lo (32), pad (8)
+constexpr int LFRAG = 512;  // bf16 per LDS fragment: 16 rows x 32 k
+
+// bf16 offset of (row, plane, k-group part) in a stage of 16-row fragments
+__device__ __forceinline__ int lslot(int row, int plane, int part) {
+  return ((row >> 4) * 2 + plane) * LFRAG + (16 * part + ((row & 15) ^ (2 * part + 8 * plane))) * 8;
+}
 
 template <int BMR, int BN, int WM, int WN>
 __device__ __forceinline__ void gemm_bf3_body(const Bf3& p, int tiles_m, int tiles_n, long tile, long b, int kz,
@@ -45,8 +54,8 @@ __device__ __forceinline__ void gemm_bf3_body(const Bf3& p, int tiles_m, int til
   constexpr int CA = BMR * 8, CB = BN * 8;  // 16-byte chunks per stage
   constexpr int RA = (CA + NT - 1) / NT, RB = (CB + NT - 1) / NT;
   static_assert(TM * 16 * WM == BMR && TN * 16 * WN == BN, "tile");
-  __bf16* As = lds;                      // [2][BMR][LROW]
-  __bf16* Bs = lds + 2 * BMR * LROW;     // [2][BN][LROW]
+  __bf16* As = lds;                // [2][BMR / 16][2 planes][LFRAG]
+  __bf16* Bs = lds + 2 * BMR * 64;  // [2][BN / 16][2 planes][LFRAG]
 
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wm = wave / WN, wn = wave % WN;
@@ -69,7 +78,7 @@ __device__ __forceinline__ void gemm_bf3_body(const Bf3& p, int tiles_m, int til
     const int w = min(gr >> 1, p.a_rows - 1);
     const int q = 2 * (gr & 1) + plane;  // re_hi, re_lo, im_hi, im_lo
     ga[i] = reinterpret_cast<const v8bf*>(Ab + (long)w * p.lda + q * p.pA + part * 8);
-    la[i] = c < CA ? row * LROW + plane * 32 + part * 8 : -1;
+    la[i] = c < CA ? lslot(row, plane, part) : -1;
   }
   const v8bf* gb[RB];
   int lb[RB];
@@ -79,7 +88,7 @@ __device__ __forceinline__ void gemm_bf3_body(const Bf3& p, int tiles_m, int til
     const int row = c >> 3, plane = (c >> 2) & 1, part = c & 3;
     const int gn = min(n0 + row, p.b_rows - 1);
     gb[i] = reinterpret_cast<const v8bf*>(Bb + (long)gn * p.ldb + plane * p.pB + part * 8);
-    lb[i] = c < CB ? row * LROW + plane * 32 + part * 8 : -1;
+    lb[i] = c < CB ? lslot(row, plane, part) : -1;
   }
 
   v4f acc[TM][TN];
@@ -100,8 +109,8 @@ __device__ __forceinline__ void gemm_bf3_body(const Bf3& p, int tiles_m, int til
       if (lb[i] >= 0) rb[i] = gb[i][ks * kvb];
   };
   auto stash = [&](int st) {
-    __bf16* a = As + st * BMR * LROW;
-    __bf16* bb = Bs + st * BN * LROW;
+    __bf16* a = As + st * BMR * 64;
+    __bf16* bb = Bs + st * BN * 64;
 #pragma unroll
     for (int i = 0; i < RA; ++i)
       if (la[i] >= 0) *reinterpret_cast<v8bf*>(a + la[i]) = ra[i];
@@ -113,23 +122,25 @@ __device__ __forceinline__ void gemm_bf3_body(const Bf3& p, int tiles_m, int til
   fetch(ks0);
   stash(0);
   __syncthreads();
-  const int r16 = lane & 15, q8 = 8 * (lane >> 4);
+  const int r16 = lane & 15, q4 = lane >> 4, q8 = 8 * q4;
   for (int ks = ks0; ks < nk; ++ks) {
     const bool more = ks + 1 < nk;
     if (more) fetch(ks + 1);
     const int st = (ks - ks0) & 1;
-    const __bf16* a = As + st * BMR * LROW + (wm * TM * 16 + r16) * LROW + q8;
-    const __bf16* bb = Bs + st * BN * LROW + (wn * TN * 16 + r16) * LROW + q8;
+    // this lane's slot in a hi and a lo fragment
+    const int sh = (16 * q4 + (r16 ^ (2 * q4))) * 8, sl = LFRAG + (16 * q4 + (r16 ^ (2 * q4 + 8))) * 8;
+    const __bf16* a = As + st * BMR * 64 + wm * TM * 2 * LFRAG;
+    const __bf16* bb = Bs + st * BN * 64 + wn * TN * 2 * LFRAG;
     v8bf ah[TM], al[TM], bh[TN], bl[TN];
 #pragma unroll
     for (int i = 0; i < TM; ++i) {
-      ah[i] = *reinterpret_cast<const v8bf*>(a + i * 16 * LROW);
-      al[i] = *reinterpret_cast<const v8bf*>(a + i * 16 * LROW + 32);
+      ah[i] = *reinterpret_cast<const v8bf*>(a + i * 2 * LFRAG + sh);
+      al[i] = *reinterpret_cast<const v8bf*>(a + i * 2 * LFRAG + sl);
     }
 #pragma unroll
     for (int j = 0; j < TN; ++j) {
-      bh[j] = *reinterpret_cast<const v8bf*>(bb + j * 16 * LROW);
-      bl[j] = *reinterpret_cast<const v8bf*>(bb + j * 16 * LROW + 32);
+      bh[j] = *reinterpret_cast<const v8bf*>(bb + j * 2 * LFRAG + sh);
+      bl[j] = *reinterpret_cast<const v8bf*>(bb + j * 2 * LFRAG + sl);
     }
 #pragma unroll
     for (int i = 0; i < TM; ++i)
@@ -235,7 +246,7 @@ int launch_lds(const Bf3& p, hipStream_t s) {
     if (ksplit < 1) return ksplit;
     ws = static_cast<double*>(p.ws);
   }
-  const size_t lds = sizeof(__bf16) * 2 * (BMR + BN) * LROW;
+  const size_t lds = sizeof(__bf16) * 2 * (BMR + BN) * 64;
   static bool raised = false;
   if (!raised && lds > 65536) {
     hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(gemm_bf3_lds_kernel<BMR, BN, WM, WN>),
@@ -373,7 +384,7 @@ long bf3_group_items(const Bf3Job& j) { return ((long)j.tx * j.ksplit + 7) / 8 *
 
 int gemm_bf3_grouped(const GroupTab<Bf3Job>& t, hipStream_t s) {
   if (t.njobs < 1 || t.total < 1) return AFQ_OK;
-  const size_t lds = sizeof(__bf16) * 2 * (128 + 128) * LROW;
+  const size_t lds = sizeof(__bf16) * 2 * (128 + 128) * 64;
   static bool raised = false;
   if (!raised && lds > 65536) {
     hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(gemm_bf3_grouped_kernel),
