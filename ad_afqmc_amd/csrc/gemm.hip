@@ -389,34 +389,6 @@ __global__ void splitk_reduce_kernel(Gemm g, const double* ws, int ksplit) {
   }
 }
 
-// Grouped complex reduction (batch 1 per job).
-__global__ void splitk_reduce_grouped_kernel(const ReduceJob* __restrict__ jobs, const int* __restrict__ prefix,
-                                             int njobs) {
-  const int item = blockIdx.x;
-  const int j = group_find(prefix, njobs, item);
-  const Gemm& g = jobs[j].g;
-  const double* ws = jobs[j].ws;
-  const int ksplit = jobs[j].ksplit;
-  const long plane = (long)g.M * g.N;
-  const long idx = (long)(item - prefix[j]) * 256 + threadIdx.x;
-  if (idx >= plane) return;
-  const int m = (int)(idx / g.N), n = (int)(idx % g.N);
-  double2 sum = {0.0, 0.0};
-  for (int z = 0; z < ksplit; ++z) {
-    const double2 v = reinterpret_cast<const double2*>(ws)[(long)z * plane + idx];
-    sum.x += v.x;
-    sum.y += v.y;
-  }
-  double2 r = cmul(g.alpha, sum);
-  double2* Cc = static_cast<double2*>(g.C) + (long)m * g.ldc + n;
-  if (g.beta.x != 0.0 || g.beta.y != 0.0) {
-    const double2 o = cmul(g.beta, *Cc);
-    r.x += o.x;
-    r.y += o.y;
-  }
-  *Cc = r;
-}
-
 template <bool AC, bool BC, bool AT, bool BT, int WM, int WN, int TM, int TN>
 int launch(const Gemm& g, hipStream_t s) {
   using C = Cfg<AC, BC, AT, BT, WM, WN, TM, TN>;
@@ -657,13 +629,6 @@ int gemm_f64_grouped(const GroupTab<GemmJob>& t, bool a_cplx, bool b_cplx, bool 
     return fail(AFQ_E_UNSUPPORTED, "gemm_f64_grouped: operand form");
 #undef GG
   return launch_status("gemm_f64_grouped_kernel");
-}
-
-int splitk_reduce_grouped(const GroupTab<ReduceJob>& t, hipStream_t s) {
-  if (t.njobs < 1 || t.total < 1) return AFQ_OK;
-  hipLaunchKernelGGL(splitk_reduce_grouped_kernel, dim3((unsigned)t.total), dim3(256), 0, s, t.params, t.prefix,
-                     t.njobs);
-  return launch_status("splitk_reduce_grouped_kernel");
 }
 
 int gemm_f64(const Gemm& g, hipStream_t s) {

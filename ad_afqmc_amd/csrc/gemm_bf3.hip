@@ -14,7 +14,11 @@
 // Outputs: complex64 C[b*sC + m*ldc + n] = alpha * acc (the VHS product,
 // feeding the complex64 Taylor terms), or complex128 C = alpha * acc + beta * C
 // through K slices written to a workspace and a fixed-order reduction (the
-// force bias: few output tiles, a long K).
+// force bias: few output tiles, a long K).  A K slice's partial is its FP32
+// accumulator, stored as complex64 (exactly) and summed in FP64.  The K slices
+// of a split product are spread over the XCDs, each slice's tiles on one XCD,
+// so its A and B slabs are fetched into one L2 (tiles in the other order made
+// every XCD read all of A).
 #include "afq_internal.h"
 #include "gemm_bf3.h"
 #include "group.h"
@@ -48,7 +52,7 @@ __device__ __forceinline__ int lslot(int row, int plane, int part) {
 
 template <int BMR, int BN, int WM, int WN>
 __device__ __forceinline__ void gemm_bf3_body(const Bf3& p, int tiles_m, int tiles_n, long tile, long b, int kz,
-                                              int kchunk, double* ws, __bf16* __restrict__ lds) {
+                                              int kchunk, float* ws, __bf16* __restrict__ lds) {
   constexpr int NT = 64 * WM * WN;
   constexpr int TM = BMR / (16 * WM), TN = BN / (16 * WN);
   constexpr int CA = BMR * 8, CB = BN * 8;  // 16-byte chunks per stage
@@ -172,8 +176,8 @@ __device__ __forceinline__ void gemm_bf3_body(const Bf3& p, int tiles_m, int til
         const int n = n0 + (wn * TN + j) * 16 + r16;
         if (m >= p.M || n >= p.N) continue;
         const double xr = acc[i][j][2 * h], xi = acc[i][j][2 * h + 1];
-        if (ws) {  // raw partial of K slice kz (complex128), reduced by the split-K kernel
-          reinterpret_cast<double2*>(ws)[((long)kz * p.M + m) * p.N + n] = double2{xr, xi};
+        if (ws) {  // raw partial of K slice kz (the FP32 accumulator as complex64), reduced in FP64
+          reinterpret_cast<float2*>(ws)[((long)kz * p.M + m) * p.N + n] = float2{acc[i][j][2 * h], acc[i][j][2 * h + 1]};
           continue;
         }
         const double orr = p.alpha.x * xr - p.alpha.y * xi, oi = p.alpha.x * xi + p.alpha.y * xr;
@@ -181,18 +185,33 @@ __device__ __forceinline__ void gemm_bf3_body(const Bf3& p, int tiles_m, int til
       }
 }
 
+// Split-K order: item i runs on XCD i mod 8 and takes, of the K slices
+// kz = XCD, XCD + 8, ..., slice (i / 8) / tiles and tile (i / 8) mod tiles
+// (the M-tiles of an N-tile consecutive).  Returns false past the last slice.
+__device__ __forceinline__ bool slice_item(long i, long tiles, int ksplit, long* tile, int* kz) {
+  const long j = i >> 3;
+  *kz = (int)(8 * (j / tiles) + (i & 7));
+  *tile = j % tiles;
+  return *kz < ksplit;
+}
+
 template <int BMR, int BN, int WM, int WN>
 __global__ __launch_bounds__(64 * WM * WN) void gemm_bf3_lds_kernel(Bf3 p, int tiles_m, int tiles_n, int xcd_remap,
-                                                                     long T8, int kchunk, double* ws) {
+                                                                     long T8, int ksplit, int kchunk, float* ws) {
   extern __shared__ __align__(16) __bf16 lds_bf3[];
   long tile = blockIdx.x;
-  if (xcd_remap) tile = (tile & 7) * T8 + (tile >> 3);
-  gemm_bf3_body<BMR, BN, WM, WN>(p, tiles_m, tiles_n, tile, blockIdx.y, blockIdx.z, kchunk, ws, lds_bf3);
+  int kz = 0;
+  if (ws) {
+    if (!slice_item(blockIdx.x, (long)tiles_m * tiles_n, ksplit, &tile, &kz)) return;  // uniform
+  } else if (xcd_remap) {
+    tile = (tile & 7) * T8 + (tile >> 3);
+  }
+  gemm_bf3_body<BMR, BN, WM, WN>(p, tiles_m, tiles_n, tile, blockIdx.y, kz, kchunk, ws, lds_bf3);
 }
 
 // Grouped (batch-1 products): work-group i belongs to job j with
-// prefix[j] <= i < prefix[j + 1] (a multiple of 8); local index = t + tx * kz,
-// t remapped to the XCD-aware tile order as gemm_bf3_lds_kernel's.
+// prefix[j] <= i < prefix[j + 1] (a multiple of 8), in gemm_bf3_lds_kernel's
+// orders: split-K jobs by slice_item, the others by the XCD-aware tile remap.
 __global__ __launch_bounds__(512) void gemm_bf3_grouped_kernel(const Bf3Job* __restrict__ jobs,
                                                                const int* __restrict__ prefix, int njobs) {
   extern __shared__ __align__(16) __bf16 lds_bf3[];
@@ -200,26 +219,68 @@ __global__ __launch_bounds__(512) void gemm_bf3_grouped_kernel(const Bf3Job* __r
   const int j = group_find(prefix, njobs, item);
   const Bf3Job J = jobs[j];
   const int local = item - prefix[j];
-  const int kz = local / J.tx;
-  if (kz >= J.ksplit) return;  // padding past the last K slice (uniform)
-  long t = local - (long)kz * J.tx;
-  if (J.T8 > 0) t = (t & 7) * J.T8 + (t >> 3);
-  if (t >= (long)J.tiles_m * J.tiles_n) return;
+  const long tiles = (long)J.tiles_m * J.tiles_n;
+  long t = local;
+  int kz = 0;
+  if (J.ws) {
+    if (!slice_item(local, tiles, J.ksplit, &t, &kz)) return;  // uniform
+  } else {
+    if (J.T8 > 0) t = (t & 7) * J.T8 + (t >> 3);
+    if (t >= tiles) return;
+  }
   gemm_bf3_body<128, 128, 2, 4>(J.p, J.tiles_m, J.tiles_n, t, 0, kz, J.kchunk, J.ws, lds_bf3);
 }
 
+// C = alpha sum_kz partial_kz + beta C for a complex128 output from complex64
+// K-slice partials, summed in FP64 in slice order.
+__device__ __forceinline__ void reduce_f32_body(const Gemm& g, const float* __restrict__ ws, int ksplit, long idx) {
+  const long plane = (long)g.M * g.N;
+  if (idx >= plane) return;
+  const int m = (int)(idx / g.N), n = (int)(idx % g.N);
+  double2 sum = {0.0, 0.0};
+  for (int z = 0; z < ksplit; ++z) {
+    const float2 v = reinterpret_cast<const float2*>(ws)[(long)z * plane + idx];
+    sum.x += (double)v.x;
+    sum.y += (double)v.y;
+  }
+  const double2 a = g.alpha;
+  double2 r = {a.x * sum.x - a.y * sum.y, a.x * sum.y + a.y * sum.x};
+  double2* Cc = static_cast<double2*>(g.C) + (long)m * g.ldc + n;
+  if (g.beta.x != 0.0 || g.beta.y != 0.0) {
+    const double2 o = *Cc, bt = g.beta;
+    r.x += bt.x * o.x - bt.y * o.y;
+    r.y += bt.x * o.y + bt.y * o.x;
+  }
+  *Cc = r;
+}
+
+__global__ __launch_bounds__(256) void reduce_f32_kernel(Gemm g, const float* ws, int ksplit) {
+  reduce_f32_body(g, ws, ksplit, blockIdx.x * 256L + threadIdx.x);
+}
+
+__global__ __launch_bounds__(256) void reduce_f32_grouped_kernel(const ReduceJob* __restrict__ jobs,
+                                                                 const int* __restrict__ prefix, int njobs) {
+  const int item = blockIdx.x;
+  const int j = group_find(prefix, njobs, item);
+  reduce_f32_body(jobs[j].g, jobs[j].ws, jobs[j].ksplit, (long)(item - prefix[j]) * 256 + threadIdx.x);
+}
+
 // K slices for a complex128 output: split until about 1024 work-groups are in
-// flight; the fixed-order reduction applies alpha and beta.
+// flight; the fixed-order reduction applies alpha and beta.  A partial takes
+// one complex64 (the size of one double of the workspace) per output element.
 int bf3_ksplit(const Bf3& p, long tiles, int* kchunk) {
   if (p.batch != 1 || !p.ws) return fail(AFQ_E_INVALID, "gemm_bf3: complex128 output needs batch 1 and a workspace");
   int ksplit = (int)std::max<long>(1, std::min<long>((1024 + tiles - 1) / tiles, p.K / 256));
-  while (ksplit > 1 && (long)ksplit * p.M * p.N * 2 > p.ws_elems) --ksplit;
-  if ((long)ksplit * p.M * p.N * 2 > p.ws_elems) return fail(AFQ_E_INVALID, "gemm_bf3: workspace too small");
+  while (ksplit > 1 && (long)ksplit * p.M * p.N > p.ws_elems) --ksplit;
+  if ((long)ksplit * p.M * p.N > p.ws_elems) return fail(AFQ_E_INVALID, "gemm_bf3: workspace too small");
   *kchunk = ((p.K + ksplit - 1) / ksplit + 31) / 32 * 32;
   return (p.K + *kchunk - 1) / *kchunk;
 }
 
-int bf3_reduce(const Bf3& p, double* ws, int ksplit, hipStream_t s) {
+// work-groups of a split-K product in slice_item's order
+long slice_items(long tiles, int ksplit) { return 8L * ((ksplit + 7) / 8) * tiles; }
+
+int bf3_reduce(const Bf3& p, float* ws, int ksplit, hipStream_t s) {
   Gemm g;
   g.M = p.M;
   g.N = p.N;
@@ -228,7 +289,9 @@ int bf3_reduce(const Bf3& p, double* ws, int ksplit, hipStream_t s) {
   g.ldc = p.ldc;
   g.alpha = p.alpha;
   g.beta = p.beta;
-  return gemm_splitk_reduce(g, ws, ksplit, true, s);
+  const long n = (long)p.M * p.N;
+  hipLaunchKernelGGL(reduce_f32_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, g, ws, ksplit);
+  return launch_status("reduce_f32_kernel");
 }
 
 template <int BMR, int BN, int WM, int WN>
@@ -240,11 +303,14 @@ int launch_lds(const Bf3& p, hipStream_t s) {
   const long gx = remap ? T8 * 8 : tiles;
   if (gx > 0x7fffffffL || p.batch > 65535) return fail(AFQ_E_UNSUPPORTED, "gemm_bf3: grid too large");
   int ksplit = 1, kchunk = p.K;
-  double* ws = nullptr;
+  float* ws = nullptr;
+  long items = gx;
   if (p.c_f64) {
     ksplit = bf3_ksplit(p, tiles, &kchunk);
     if (ksplit < 1) return ksplit;
-    ws = static_cast<double*>(p.ws);
+    ws = static_cast<float*>(p.ws);
+    items = slice_items(tiles, ksplit);
+    if (items > 0x7fffffffL) return fail(AFQ_E_UNSUPPORTED, "gemm_bf3: grid too large");
   }
   const size_t lds = sizeof(__bf16) * 2 * (BMR + BN) * 64;
   static bool raised = false;
@@ -254,8 +320,8 @@ int launch_lds(const Bf3& p, hipStream_t s) {
     if (e != hipSuccess) return fail(AFQ_E_HIP, std::string("gemm_bf3 LDS attribute: ") + hipGetErrorString(e));
     raised = true;
   }
-  hipLaunchKernelGGL((gemm_bf3_lds_kernel<BMR, BN, WM, WN>), dim3((unsigned)gx, p.batch, ksplit), dim3(64 * WM * WN),
-                     lds, s, p, tm, tn, remap, T8, kchunk, ws);
+  hipLaunchKernelGGL((gemm_bf3_lds_kernel<BMR, BN, WM, WN>), dim3((unsigned)items, p.batch), dim3(64 * WM * WN), lds,
+                     s, p, tm, tn, remap, T8, ksplit, kchunk, ws);
   AFQ_TRY(launch_status("gemm_bf3_lds_kernel"));
   return ws ? bf3_reduce(p, ws, ksplit, s) : AFQ_OK;
 }
@@ -374,13 +440,23 @@ int bf3_group_job(const Bf3& p, Bf3Job* out) {
     const int ks = bf3_ksplit(p, (long)j.tiles_m * j.tiles_n, &j.kchunk);
     if (ks < 1) return ks;
     j.ksplit = ks;
-    j.ws = static_cast<double*>(p.ws);
+    j.ws = static_cast<float*>(p.ws);
   }
   *out = j;
   return AFQ_OK;
 }
 
-long bf3_group_items(const Bf3Job& j) { return ((long)j.tx * j.ksplit + 7) / 8 * 8; }
+long bf3_group_items(const Bf3Job& j) {
+  if (j.ws) return slice_items((long)j.tiles_m * j.tiles_n, j.ksplit);
+  return ((long)j.tx + 7) / 8 * 8;
+}
+
+int reduce_f32_grouped(const GroupTab<ReduceJob>& t, hipStream_t s) {
+  if (t.njobs < 1 || t.total < 1) return AFQ_OK;
+  hipLaunchKernelGGL(reduce_f32_grouped_kernel, dim3((unsigned)t.total), dim3(256), 0, s, t.params, t.prefix,
+                     t.njobs);
+  return launch_status("reduce_f32_grouped_kernel");
+}
 
 int gemm_bf3_grouped(const GroupTab<Bf3Job>& t, hipStream_t s) {
   if (t.njobs < 1 || t.total < 1) return AFQ_OK;

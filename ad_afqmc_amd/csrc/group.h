@@ -75,27 +75,28 @@ int vec_grouped(const GroupTab<VecJob>& t, hipStream_t s);
 
 // Split-BF16 x3 products of batch 1 (gemm_bf3's 128 x 128 tiles, 2 x 4
 // waves); a complex128 output takes K slices into ws, reduced afterwards by
-// splitk_reduce_grouped.  Work items: tiles_m * tiles_n * ksplit.
+// reduce_f32_grouped.
 struct Bf3Job {
   Bf3 p;
   int tiles_m = 0, tiles_n = 0, ksplit = 1, kchunk = 0;
-  int tx = 0;    // work items per K slice (tiles, rounded up to 8 when remapped)
+  int tx = 0;    // work items of an unsplit product (tiles, rounded up to 8 when remapped)
   long T8 = 0;   // > 0: gemm_bf3's XCD-aware tile order (the M-tiles of an L panel on one XCD)
-  double* ws = nullptr;
+  float* ws = nullptr;  // split-K partials (complex64); null: unsplit
 };
 int bf3_group_job(const Bf3& p, Bf3Job* out);
-// work items: tx * ksplit rounded up to 8 (every job starts on XCD 0)
+// work items: gemm_bf3's split-K slice order, or tx rounded up to 8 (every job
+// starts on XCD 0)
 long bf3_group_items(const Bf3Job& j);
 int gemm_bf3_grouped(const GroupTab<Bf3Job>& t, hipStream_t s);
 
-// C = alpha sum_kz ws[kz] + beta C for batch-1 complex outputs; work items:
-// ceil(M N / 256).
+// C = alpha sum_kz ws[kz] + beta C for batch-1 complex128 outputs from the
+// complex64 K-slice partials of gemm_bf3; work items: ceil(M N / 256).
 struct ReduceJob {
   Gemm g;
-  const double* ws;
+  const float* ws;
   int ksplit;
 };
-int splitk_reduce_grouped(const GroupTab<ReduceJob>& t, hipStream_t s);
+int reduce_f32_grouped(const GroupTab<ReduceJob>& t, hipStream_t s);
 
 // Split-BF16 x3 exchange of each job (k_exchange_bf16's operands: rot_chol
 // planes rcs, Green's-function planes gs); work items: W * nchunk, all jobs of

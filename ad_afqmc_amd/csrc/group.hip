@@ -314,7 +314,7 @@ int build(afq_group* g, afq_job* const* jobs, void* const* walkers, double* cons
     red[i].g.ldc = G;
     red[i].g.alpha = f.alpha;
     red[i].g.beta = f.beta;
-    red[i].ws = static_cast<const double*>(fbj[i].ws);
+    red[i].ws = fbj[i].ws;
     red[i].ksplit = fbj[i].ksplit;
     it_red[i] = ((long)W * G + 255) / 256;  // the complex128 output always passes through ws
 
@@ -413,7 +413,7 @@ int g_block(afq_group* g, hipStream_t st) {
     AFQ_TRY(g_green(g, true, true, i > 0, st));
     AFQ_TRY(split_bf16_grouped(g->gsplit_fb.view(), st));
     AFQ_TRY(gemm_bf3_grouped(g->fb.view(), st));
-    AFQ_TRY(splitk_reduce_grouped(g->fb_red.view(), st));
+    AFQ_TRY(reduce_f32_grouped(g->fb_red.view(), st));
     AFQ_TRY(g_field_shift(g->bj[BI_WALKER].view(), i, st));
     AFQ_TRY(split_rows_grouped(g->xsplit.view(), st));
     AFQ_TRY(gemm_bf3_grouped(g->vhs.view(), st));
