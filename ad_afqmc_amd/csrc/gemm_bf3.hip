@@ -19,6 +19,8 @@
 // of a split product are spread over the XCDs, each slice's tiles on one XCD,
 // so its A and B slabs are fetched into one L2 (tiles in the other order made
 // every XCD read all of A).
+#include <type_traits>
+
 #include "afq_internal.h"
 #include "gemm_bf3.h"
 #include "group.h"
@@ -57,6 +59,9 @@ __device__ __forceinline__ void gemm_bf3_body(const Bf3& p, int tiles_m, int til
   constexpr int TM = BMR / (16 * WM), TN = BN / (16 * WN);
   constexpr int CA = BMR * 8, CB = BN * 8;  // 16-byte chunks per stage
   constexpr int RA = (CA + NT - 1) / NT, RB = (CB + NT - 1) / NT;
+  // every thread's chunks exist when the tile's chunks divide evenly: no
+  // guard (and no branch) around those loads and stores
+  constexpr bool EVENA = CA % NT == 0, EVENB = CB % NT == 0;
   static_assert(TM * 16 * WM == BMR && TN * 16 * WN == BN, "tile");
   __bf16* As = lds;                // [2][BMR / 16][2 planes][LFRAG]
   __bf16* Bs = lds + 2 * BMR * 64;  // [2][BN / 16][2 planes][LFRAG]
@@ -103,38 +108,43 @@ __device__ __forceinline__ void gemm_bf3_body(const Bf3& p, int tiles_m, int til
 
   const int ks0 = kz * kchunk / 32, nk = min(p.K, (kz + 1) * kchunk) / 32;
   const long kva = p.kA >> 3, kvb = p.kB >> 3;  // k step strides in units of 8 bf16
-  v8bf ra[RA], rb[RB];
-  auto fetch = [&](int ks) {
+  // Two register sets: a step's slabs are fetched two steps before they are
+  // used, stashed into LDS one step before.  Past the last step the fetches
+  // repeat it (no branch around a load, so the compiler's count of the loads
+  // still in flight stays exact).
+  v8bf ra[2][RA], rb[2][RB];
+  auto fetch = [&](int ks, v8bf(&xa)[RA], v8bf(&xb)[RB]) {
+    ks = min(ks, nk - 1);
 #pragma unroll
     for (int i = 0; i < RA; ++i)
-      if (la[i] >= 0) ra[i] = ga[i][ks * kva];
+      if (EVENA || la[i] >= 0) xa[i] = ga[i][ks * kva];
 #pragma unroll
     for (int i = 0; i < RB; ++i)
-      if (lb[i] >= 0) rb[i] = gb[i][ks * kvb];
+      if (EVENB || lb[i] >= 0) xb[i] = gb[i][ks * kvb];
   };
-  auto stash = [&](int st) {
+  auto stash = [&](int st, const v8bf(&xa)[RA], const v8bf(&xb)[RB]) {
     __bf16* a = As + st * BMR * 64;
     __bf16* bb = Bs + st * BN * 64;
 #pragma unroll
     for (int i = 0; i < RA; ++i)
-      if (la[i] >= 0) *reinterpret_cast<v8bf*>(a + la[i]) = ra[i];
+      if (EVENA || la[i] >= 0) *reinterpret_cast<v8bf*>(a + la[i]) = xa[i];
 #pragma unroll
     for (int i = 0; i < RB; ++i)
-      if (lb[i] >= 0) *reinterpret_cast<v8bf*>(bb + lb[i]) = rb[i];
+      if (EVENB || lb[i] >= 0) *reinterpret_cast<v8bf*>(bb + lb[i]) = xb[i];
   };
 
-  fetch(ks0);
-  stash(0);
-  __syncthreads();
   const int r16 = lane & 15, q4 = lane >> 4, q8 = 8 * q4;
-  for (int ks = ks0; ks < nk; ++ks) {
-    const bool more = ks + 1 < nk;
-    if (more) fetch(ks + 1);
-    const int st = (ks - ks0) & 1;
-    // this lane's slot in a hi and a lo fragment
-    const int sh = (16 * q4 + (r16 ^ (2 * q4))) * 8, sl = LFRAG + (16 * q4 + (r16 ^ (2 * q4 + 8))) * 8;
-    const __bf16* a = As + st * BMR * 64 + wm * TM * 2 * LFRAG;
-    const __bf16* bb = Bs + st * BN * 64 + wn * TN * 2 * LFRAG;
+  // this lane's slot in a hi and a lo fragment
+  const int sh = (16 * q4 + (r16 ^ (2 * q4))) * 8, sl = LFRAG + (16 * q4 + (r16 ^ (2 * q4 + 8))) * 8;
+  // step ks on LDS stage P (= (ks - ks0) mod 2) and register set P, whose
+  // slabs were stashed the step before: fetch step ks + 2 into set P, multiply
+  // stage P, stash set 1 - P (step ks + 1) into stage 1 - P
+  auto step = [&](int ks, auto par) {
+    constexpr int P = decltype(par)::value;
+    fetch(ks + 2, ra[P], rb[P]);
+    __builtin_amdgcn_sched_barrier(0);  // the fetch is issued first, not sunk towards its use
+    const __bf16* a = As + P * BMR * 64 + wm * TM * 2 * LFRAG;
+    const __bf16* bb = Bs + P * BN * 64 + wn * TN * 2 * LFRAG;
     v8bf ah[TM], al[TM], bh[TN], bl[TN];
 #pragma unroll
     for (int i = 0; i < TM; ++i) {
@@ -158,9 +168,21 @@ __device__ __forceinline__ void gemm_bf3_body(const Bf3& p, int tiles_m, int til
     for (int i = 0; i < TM; ++i)
 #pragma unroll
       for (int j = 0; j < TN; ++j) acc[i][j] = mma(ah[i], bl[j], acc[i][j]);
-    if (more) stash(st ^ 1);
+    __builtin_amdgcn_sched_barrier(0);
+    stash(1 - P, ra[1 - P], rb[1 - P]);
     __syncthreads();
+  };
+
+  fetch(ks0, ra[0], rb[0]);
+  fetch(ks0 + 1, ra[1], rb[1]);
+  stash(0, ra[0], rb[0]);
+  __syncthreads();
+  int ks = ks0;
+  for (; ks + 1 < nk; ks += 2) {
+    step(ks, std::integral_constant<int, 0>{});
+    step(ks + 1, std::integral_constant<int, 1>{});
   }
+  if (ks < nk) step(ks, std::integral_constant<int, 0>{});
 
   // accumulator element v of lane l is real row 4(l/16) + v of the block:
   // v = 0, 1 are (re, im) of one output row, v = 2, 3 of the next
