@@ -47,15 +47,17 @@ __device__ inline double2 crecip(double2 a) {
 // at W = 256).
 //
 // n <= 64 (gj_reg_body): the matrix stays in registers.  Lane i of every wave
-// holds row i, wave w the columns w + 4t (t < 16).  Rows are never swapped:
-// the pivot of column k is the first (lowest) not yet used row of maximal
-// measure, found by the wave that holds column k (a DPP maximum and a ballot),
-// which publishes the pivot row index, the pivot and its copy of column k (the
-// factors) through LDS, double-buffered, so one barrier per column suffices,
-// and replaces its column k by the unit vector of the pivot row.  Each wave
-// then passes its part of the pivot row a_p through LDS and updates all its
-// columns with one branch-free formula (pivot row: a_p / d; other rows:
-// a - (f / d) a_p).  The result R relates to the
+// holds row i, wave w the columns 16 w .. 16 w + 15 (one panel).  Rows are
+// never swapped: the pivot of column k is the first (lowest) not yet used row
+// of maximal measure (a DPP maximum and a ballot).  Panel P is factored by
+// wave P alone, column by column, on its own 16 columns (the pivot row through
+// LDS within the wave; column k replaced by the unit vector of its pivot row,
+// so one branch-free update leaves a_p / d in the pivot row and a - (f / d) a_p
+// elsewhere).  The panel's steps compose to M = I + sum_t U_t e_{p_t}^T, and
+// because of the unit vectors the panel's columns end up holding M e_{p_t}:
+// after one barrier every other wave applies M to its columns as one rank-16
+// update, a += U a[p_t] (rows p_t as they were before the panel).  Four
+// barriers per matrix instead of one per column.  The result R relates to the
 // inverse through the pivot sequence piv (row piv[m] was the pivot of column
 // m): A^-1[pinv[r]][piv[c]] = R[r][c], pinv the inverse permutation; det A =
 // sign(piv) prod of the pivots (the determinant alone runs the same sweep).
@@ -110,98 +112,123 @@ __device__ inline void wave_lds_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-#define AFQ_GJ_CASES(X) X(0) X(1) X(2) X(3) X(4) X(5) X(6) X(7) X(8) X(9) X(10) X(11) X(12) X(13) X(14) X(15)
-
 __device__ __forceinline__ void gj_reg_body(const DetInvSet& set, long b) {
-  __shared__ double2 fcol[2][64];
-  __shared__ double2 pdd[2];  // pivot
-  __shared__ int pks[2];
+  __shared__ double2 ubuf[2][16][64];   // panel columns minus the unit vectors, U[t][row], by panel parity
+  __shared__ double2 prow[4][16][16];   // per wave: [column][t] of the panel's pivot rows before the panel
+  __shared__ double2 rrow[16];          // the panel wave's current pivot row
+  __shared__ double2 pdet[4];           // product of each panel's pivots
   __shared__ int piv[64], pinv[64];
-  __shared__ double2 prow[4][16];  // per wave: its part of the pivot row
   const int n = set.n;
   const int tid = threadIdx.x, lane = tid & 63;
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int c0 = 16 * wave;  // this wave's columns c0 .. c0 + 15
   const bool want_inv = set.inv != nullptr;
   const double2* src = set.S + b * set.strideS;
+  for (int e = tid; e < 4 * 16 * 16; e += 256) (&prow[0][0][0])[e] = double2{0.0, 0.0};
   double2 a[16];
 #pragma unroll
   for (int t = 0; t < 16; ++t) {
-    const int c = wave + 4 * t;
+    const int c = c0 + t;
     a[t] = (lane < n && c < n) ? src[lane * n + c] : double2{0.0, 0.0};
   }
   unsigned long long used = 0;  // rows already used as pivots (wave-uniform)
-  double2 dprod = {1.0, 0.0};
-  for (int k = 0; k < n; ++k) {
-    const int buf = k & 1;
-    if (wave == (k & 3)) {
-      // pivot search on column k (register k / 4 of this wave)
-      const int tk = k >> 2;
-      double2 x = a[0];
-      switch (tk) {
-#define AFQ_GET(t) \
-  case t:          \
-    x = a[t];      \
-    break;
-        AFQ_GJ_CASES(AFQ_GET)
-#undef AFQ_GET
+  const int npan = (n + 15) >> 4;
+  for (int P = 0; P < npan; ++P) {
+    const int np = min(16, n - 16 * P);
+    double2(&U)[16][64] = ubuf[P & 1];
+    if (wave == P) {
+      // ---- the panel, by this wave alone: Gauss-Jordan steps on its 16 columns
+      double2 dp = {1.0, 0.0};
+      int pj[16];
+#pragma unroll
+      for (int t = 0; t < 16; ++t) {
+        pj[t] = -1;
+        if (t < np) {
+          const double2 x = a[t];
+          const bool cand = lane < n && !((used >> lane) & 1ull);
+          const double v = cand ? fabs(x.x) + fabs(x.y) : -1.0;
+          const double m = wave_fmax(v);
+          // lowest row of maximal measure (a column of NaNs falls back to the
+          // lowest unused row, so p is always a valid row)
+          const unsigned long long best = __ballot(cand && v == m), open = __ballot(cand);
+          const int p = __builtin_ctzll(best ? best : open);
+          pj[t] = p;
+          const double2 d = {readlane_d(x.x, p), readlane_d(x.y, p)};
+          dp = cmul(dp, d);
+          used |= 1ull << p;
+          // column t becomes the unit vector of row p: the common update
+          // below then leaves 1 / d in row p and -f / d in the others
+          const bool isp = lane == p;
+          a[t] = isp ? double2{1.0, 0.0} : double2{0.0, 0.0};
+          if (isp) {
+#pragma unroll
+            for (int s = 0; s < 16; ++s) rrow[s] = a[s];
+          }
+          const double2 dinv = crecip_fast(d);
+          const double sc = isp ? 0.0 : 1.0;
+          const double2 fd = cmul(x, dinv);
+          const double2 g = isp ? dinv : double2{-fd.x, -fd.y};
+          wave_lds_sync();
+          double2 r[16];
+#pragma unroll
+          for (int s = 0; s < 16; ++s) r[s] = rrow[s];
+#pragma unroll
+          for (int s = 0; s < 16; ++s) {
+            const double2 y = a[s];
+            a[s].x = fma(g.x, r[s].x, fma(-g.y, r[s].y, sc * y.x));
+            a[s].y = fma(g.x, r[s].y, fma(g.y, r[s].x, sc * y.y));
+          }
+          wave_lds_sync();
+        }
       }
-      const bool cand = lane < n && !((used >> lane) & 1ull);
-      const double v = cand ? fabs(x.x) + fabs(x.y) : -1.0;
-      const double m = wave_fmax(v);
-      // lowest row of maximal measure (a column of NaNs falls back to the
-      // lowest unused row, so p is always a valid row)
-      const unsigned long long best = __ballot(cand && v == m), open = __ballot(cand);
-      const int p = __builtin_ctzll(best ? best : open);
-      const double2 d = {readlane_d(x.x, p), readlane_d(x.y, p)};
-      fcol[buf][lane] = x;
       if (lane == 0) {
-        pks[buf] = p;
-        pdd[buf] = d;
-        piv[k] = p;
+        pdet[P] = dp;
+#pragma unroll
+        for (int t = 0; t < 16; ++t)
+          if (t < np) piv[16 * P + t] = pj[t];
       }
-      // column k becomes the unit vector of row p: the common update below
-      // then leaves 1 / d in row p and -f / d in the others
-      const double2 e = (lane == p) ? double2{1.0, 0.0} : double2{0.0, 0.0};
-      switch (tk) {
-#define AFQ_SET(t) \
-  case t:          \
-    a[t] = e;      \
-    break;
-        AFQ_GJ_CASES(AFQ_SET)
-#undef AFQ_SET
-      }
+      // the panel's transform M = I + sum_t U_t e_{p_t}^T: column t of the
+      // panel now holds M e_{p_t} (zero past the panel's width)
+#pragma unroll
+      for (int t = 0; t < 16; ++t) U[t][lane] = (lane == pj[t]) ? double2{a[t].x - 1.0, a[t].y} : a[t];
     }
     __syncthreads();
-    const int p = __builtin_amdgcn_readfirstlane(pks[buf]);
-    const double2 d = pdd[buf];
-    const double2 f = fcol[buf][lane];
-    used |= 1ull << p;
-    if (tid == 0) dprod = cmul(dprod, d);
-    // row p of this wave's columns through LDS (the reciprocal is formed
-    // while the copies are in flight)
-    if (lane == p) {
+    if (wave != P) {
+      int pj[16];
 #pragma unroll
-      for (int t = 0; t < 16; ++t) prow[wave][t] = a[t];
-    }
-    const double2 dinv = crecip_fast(d);
-    // row p: a / d; other rows: a - (f / d) a_p (column k: 1 / d and -f / d)
-    const bool isp = lane == p;
-    const double sc = isp ? 0.0 : 1.0;
-    const double2 fd = cmul(f, dinv);
-    const double2 g = isp ? dinv : double2{-fd.x, -fd.y};
-    wave_lds_sync();
-    double2 r[16];
+      for (int t = 0; t < 16; ++t) pj[t] = t < np ? __builtin_amdgcn_readfirstlane(piv[16 * P + t]) : -1;
 #pragma unroll
-    for (int t = 0; t < 16; ++t) r[t] = prow[wave][t];
+      for (int t = 0; t < 16; ++t)
+        if (pj[t] >= 0) used |= 1ull << pj[t];
+      if (c0 < n) {
+        // ---- apply the panel to this wave's columns: a += U a[p_t] (rows
+        // p_t before the panel, through LDS: the lane holding row p_t writes it)
+        int jj = -1;
 #pragma unroll
-    for (int t = 0; t < 16; ++t) {
-      const double2 y = a[t];
-      a[t].x = fma(g.x, r[t].x, fma(-g.y, r[t].y, sc * y.x));
-      a[t].y = fma(g.x, r[t].y, fma(g.y, r[t].x, sc * y.y));
+        for (int t = 0; t < 16; ++t) jj = (pj[t] == lane) ? t : jj;
+        if (jj >= 0) {
+#pragma unroll
+          for (int s = 0; s < 16; ++s) prow[wave][s][jj] = a[s];
+        }
+        wave_lds_sync();
+#pragma unroll
+        for (int t = 0; t < 16; ++t) {
+          const double2 u = U[t][lane];
+#pragma unroll
+          for (int s = 0; s < 16; ++s) {
+            const double2 r = prow[wave][s][t];
+            a[s].x = fma(u.x, r.x, fma(-u.y, r.y, a[s].x));
+            a[s].y = fma(u.x, r.y, fma(u.y, r.x, a[s].y));
+          }
+        }
+        wave_lds_sync();
+      }
     }
   }
-  __syncthreads();  // piv complete
+  __syncthreads();  // piv and pdet complete
   if (tid == 0) {
+    double2 dprod = {1.0, 0.0};
+    for (int q = 0; q < npan; ++q) dprod = cmul(dprod, pdet[q]);
     // sign of the pivot permutation: n minus its number of cycles transpositions
     unsigned long long seen = 0;
     int cycles = 0;
@@ -221,12 +248,11 @@ __device__ __forceinline__ void gj_reg_body(const DetInvSet& set, long b) {
     const long row = (long)pinv[lane] * n;
 #pragma unroll
     for (int t = 0; t < 16; ++t) {
-      const int c = wave + 4 * t;
+      const int c = c0 + t;
       if (c < n) dst[row + piv[c]] = a[t];
     }
   }
 }
-#undef AFQ_GJ_CASES
 
 template <int NMAX>
 __device__ __forceinline__ void gj_det_inv_body(const DetInvSet& set, long b, double2* __restrict__ lds) {
