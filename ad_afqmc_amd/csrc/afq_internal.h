@@ -76,6 +76,7 @@ __device__ inline long sym_off(int i, int k) {
 struct Gemm {
   int M = 0, N = 0, K = 0, batch = 1;
   bool a_cplx = false, b_cplx = false, a_trans = false, b_trans = false;
+  bool a_conj = false;  // complex A taken conjugated (op(A) = conj(A) or A^H)
   const void* A = nullptr;
   long lda = 0, sA = 0;
   // a_sym > 0: A is a symmetric a_sym x a_sym matrix held as its packed lower
@@ -138,7 +139,6 @@ int batched_det_inv_sets(const DetInvSet* sets, int nsets, int count, hipStream_
 // (above n = 64 factored in place in R; R may be S).
 int batched_cholesky_upper(const double2* S, long strideS, int n, int count, double2* R, long strideR,
                            hipStream_t stream);
-int k_conj(const double2* x, double2* y, long n, hipStream_t stream);
 
 // Classical Gram-Schmidt with re-orthogonalisation (CGS2) on `count` row-major
 // n_rows x n_cols complex panels (leading dimension ld, batch stride sW), in place.

@@ -572,7 +572,8 @@ int propagate(afq_job* j, void* up, void* dn, const double* fields, double* weig
 // with a positive real diagonal, W <- W R^-1.  This is the QR factorisation
 // whose R has a positive real diagonal -- the one CGS2 computes -- and is
 // accurate for the well-conditioned panels of a block (re-orthonormalised
-// every block); the second pass restores orthogonality to rounding.
+// every block); the second pass restores orthogonality to rounding.  The
+// first pass writes the scratch panel and the second reads it and writes W.
 int cholqr2_panel(afq_job* j, double2* Wp, int c, int s, hipStream_t st) {
   const int N = j->N, W = j->W;
   const long per = (long)N * c, cc = (long)c * c;
@@ -580,7 +581,8 @@ int cholqr2_panel(afq_job* j, double2* Wp, int c, int s, hipStream_t st) {
   double2* S = j->S[s];
   double2* Rinv = j->Sinv[s];
   for (int pass = 0; pass < 2; ++pass) {
-    AFQ_TRY(k_conj(Wp, tmp, per * W, st));
+    double2* in = pass == 0 ? Wp : tmp;
+    double2* out = pass == 0 ? tmp : Wp;
     Gemm g;
     g.M = c;
     g.N = c;
@@ -588,11 +590,12 @@ int cholqr2_panel(afq_job* j, double2* Wp, int c, int s, hipStream_t st) {
     g.batch = W;
     g.a_cplx = true;
     g.a_trans = true;
-    g.A = tmp;
+    g.a_conj = true;
+    g.A = in;
     g.lda = c;
     g.sA = per;
     g.b_cplx = true;
-    g.B = Wp;
+    g.B = in;
     g.ldb = c;
     g.sB = per;
     g.C = S;
@@ -607,18 +610,17 @@ int cholqr2_panel(afq_job* j, double2* Wp, int c, int s, hipStream_t st) {
     q.K = c;
     q.batch = W;
     q.a_cplx = true;
-    q.A = Wp;
+    q.A = in;
     q.lda = c;
     q.sA = per;
     q.b_cplx = true;
     q.B = Rinv;
     q.ldb = c;
     q.sB = cc;
-    q.C = tmp;
+    q.C = out;
     q.ldc = c;
     q.sC = per;
     AFQ_TRY(gemm_f64(q, st));
-    AFQ_HIP(hipMemcpyAsync(Wp, tmp, sizeof(double2) * per * W, hipMemcpyDeviceToDevice, st));
   }
   return AFQ_OK;
 }

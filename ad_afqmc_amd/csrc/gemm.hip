@@ -193,6 +193,7 @@ __device__ __forceinline__ void gemm_f64_body(const Gemm& g, int tiles_m, int ti
       accR[i][j] = v4d{0.0, 0.0, 0.0, 0.0};
       accI[i][j] = v4d{0.0, 0.0, 0.0, 0.0};
     }
+  const double asg = g.a_conj ? -1.0 : 1.0;  // sign of A's imaginary part
 
   double ra[C::RA][4][2], rb[C::RB][4][2];
   // K is processed as [kbeg, kend); the fetch helpers clip against kend.
@@ -218,7 +219,7 @@ __device__ __forceinline__ void gemm_f64_body(const Gemm& g, int tiles_m, int ti
       for (int i = 0; i < TM; ++i) {
         const int col = (wm * TM + i) * 16 + c;
         ar[i] = as[kr * C::LDA + col];
-        ai[i] = AC ? as[BK * C::LDA + kr * C::LDA + col] : 0.0;
+        ai[i] = AC ? asg * as[BK * C::LDA + kr * C::LDA + col] : 0.0;
       }
 #pragma unroll
       for (int j = 0; j < TN; ++j) {
@@ -571,7 +572,7 @@ int dispatch_shape(const Gemm& g, hipStream_t s) {
   // N=200, 512 walkers); wider panels stay on the LDS-staged tiles, which were
   // faster at 118 columns
   if constexpr (!AT && !BT && BC)
-    if (g.batch > 1 && g.N > 32 && g.N <= 48 && g.M > 64 && !g.ws) return launch_panel<AC, 8, 3>(g, s);
+    if (g.batch > 1 && g.N > 32 && g.N <= 48 && g.M > 64 && !g.ws && !g.a_conj) return launch_panel<AC, 8, 3>(g, s);
   // panel tiles for N <= 64 (walker-column products)
   if (g.N <= 16) return launch<AC, BC, AT, BT, 4, 1, 1, 1>(g, s);
   if (g.N <= 32) return launch<AC, BC, AT, BT, 4, 1, 1, 2>(g, s);
@@ -635,6 +636,7 @@ int gemm_f64(const Gemm& g, hipStream_t s) {
   if (g.M <= 0 || g.N <= 0 || g.batch <= 0) return AFQ_OK;
   if (g.K <= 0) return fail(AFQ_E_INVALID, "gemm_f64: K must be positive");
   if (g.E) return fail(AFQ_E_UNSUPPORTED, "gemm_f64: no separate addend");
+  if (g.a_conj && !g.a_cplx) return fail(AFQ_E_INVALID, "gemm_f64: only a complex A can be conjugated");
   if (g.a_sym && (g.a_trans || g.a_sym != g.M || g.a_sym != g.K))
     return fail(AFQ_E_INVALID, "gemm_f64: a packed symmetric A is square and not transposed");
   if (g.a_cplx) return g.b_cplx ? dispatch_t<true, true>(g, s) : dispatch_t<true, false>(g, s);

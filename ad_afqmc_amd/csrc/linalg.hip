@@ -580,10 +580,6 @@ int raise_lds(const void* kernel, bool* done, const char* what) {
   return AFQ_OK;
 }
 
-__global__ void conj_kernel(const double2* __restrict__ x, double2* __restrict__ y, long n) {
-  const long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
-  if (i < n) y[i] = double2{x[i].x, -x[i].y};
-}
 
 // Grouped copies / conjugate copies: work-group i covers elements
 // [(i - prefix[j]) * 1024, +1024) of job j's vector.
@@ -708,11 +704,6 @@ int vec_grouped(const GroupTab<VecJob>& t, hipStream_t stream) {
   return launch_status("vec_grouped_kernel");
 }
 
-int k_conj(const double2* x, double2* y, long n, hipStream_t stream) {
-  if (n <= 0) return AFQ_OK;
-  hipLaunchKernelGGL(conj_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, stream, x, y, n);
-  return launch_status("conj_kernel");
-}
 
 int batched_cgs2(double2* Wp, long ld, long sW, int n_rows, int n_cols, int count, hipStream_t stream) {
   if (n_cols < 1 || n_cols > 128 || n_cols > n_rows) return fail(AFQ_E_UNSUPPORTED, "CGS2 supports 1 <= nocc <= 128 <= N");
