@@ -8,14 +8,22 @@
 // population control -- the function the reference differentiates with
 // jax.vjp in its "reverse" AD mode (sampling.py:300-341 _nosr_norot variant,
 // driver.py:865-876).  The segment is checkpointed once per block (walkers,
-// weights, population-control shift); in the reverse sweep each block's steps
-// are recomputed from its checkpoint and then pulled back in reverse order.
+// weights, population-control shift).  The steps' tapes (walkers, weights,
+// overlaps, shifted fields before and after each step) are recorded for every
+// block in the forward sweep when they fit the memory limit set in
+// afq_segment_energy_vjp; otherwise the reverse sweep recomputes each block's
+// steps from its checkpoint.  Either way the steps are then pulled back in
+// reverse order.
 //
 // Adjoint convention: a complex intermediate z carries g_z = dF/dRe z - i dF/dIm z,
 // so a holomorphic product y = A x pulls back as g_x = A^T g_y (transposes only,
 // which the GEMM supports directly); a real parameter p in z gets Re(g_z dz/dp).
-// Complex accumulators for real parameters are reduced with Re at the end.
+// Products that feed the real parameters L and rot_chol compute only the part of
+// the complex product that Re(alpha A.B) needs (the GEMM's real-part mode); the
+// rot_h1 accumulator stays complex and is reduced with Re at the end.
+#include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 
 #include "kernels.h"
@@ -107,23 +115,35 @@ __global__ void adj_gxt_init_kernel(const double2* gd, const double2* mf, double
 }
 
 // gs = alpha xt - gxt (fb_term and xt = x - s), gfb = -i sqrt(dt) gs (s = -i sqrt(dt)(fb - m)),
-// and the m adjoint from shift_term and s, summed over walkers (one thread per gamma).
-__global__ void adj_fb_kernel(const double2* xt, const double2* gxt, const double2* gd, const double* alpha, double sqdt,
-                              int G, int W, double2* gfb, double* mbar) {
-  const int g = blockIdx.x * blockDim.x + threadIdx.x;
-  if (g >= G) return;
+// and the m adjoint from shift_term and s, summed over walkers.  A work-group
+// of 1024 threads takes 64 gammas: 16 walker lanes per gamma, lane l taking
+// walkers l, l + 16, ...; the lanes' partial sums are added in a fixed order.
+__global__ __launch_bounds__(1024) void adj_fb_kernel(const double2* xt, const double2* gxt, const double2* gd,
+                                                      const double* alpha, double sqdt, int G, int W, double2* gfb,
+                                                      double* mbar) {
+  __shared__ double part[16][64];
+  const int gl = threadIdx.x & 63, wl = threadIdx.x >> 6;
+  const int g = blockIdx.x * 64 + gl;
   double macc = 0.0;
-  for (int w = 0; w < W; ++w) {
-    const long i = (long)w * G + g;
-    const double2 x = xt[i], gx = gxt[i];
-    const double a = alpha[w];
-    const double2 gs = {a * x.x - gx.x, a * x.y - gx.y};
-    gfb[i] = double2{sqdt * gs.y, -sqdt * gs.x};
-    const double2 gsh = {-sqdt * gd[w].x, -sqdt * gd[w].y};
-    const double2 p = zmul(gsh, x);
-    macc += -p.y - sqdt * gs.y;  // Re(i gsh xt) + Re(i sqrt(dt) gs)
+  if (g < G) {
+    for (int w = wl; w < W; w += 16) {
+      const long i = (long)w * G + g;
+      const double2 x = xt[i], gx = gxt[i];
+      const double a = alpha[w];
+      const double2 gs = {a * x.x - gx.x, a * x.y - gx.y};
+      gfb[i] = double2{sqdt * gs.y, -sqdt * gs.x};
+      const double2 gsh = {-sqdt * gd[w].x, -sqdt * gd[w].y};
+      const double2 p = zmul(gsh, x);
+      macc += -p.y - sqdt * gs.y;  // Re(i gsh xt) + Re(i sqrt(dt) gs)
+    }
   }
-  mbar[g] += macc;
+  part[wl][gl] = macc;
+  __syncthreads();
+  if (wl == 0 && g < G) {
+    double t = 0.0;
+    for (int l = 0; l < 16; ++l) t += part[l][gl];
+    mbar[g] += t;
+  }
 }
 
 // X_w = c gd_w S_w^-T (O x O per walker)
@@ -349,6 +369,10 @@ struct GB {
     g.ws_elems = j->gemm_ws_elems;
     return *this;
   }
+  GB& part(int p) {  // real C: Re(alpha A.B) from the real (1) or imaginary (2) part of A.B
+    g.c_part = p;
+    return *this;
+  }
   int run(hipStream_t st) { return gemm_f64(g, st); }
 };
 
@@ -365,6 +389,18 @@ struct Arena {
   }
 };
 
+// Tapes of one block: walkers, weights, overlaps and scalars before each step
+// and after the last (n_steps + 1 entries), shifted fields and weight terms of
+// each step.
+struct Tape {
+  double2* W;
+  double* w;
+  double* sc;
+  double2* ov;
+  double2* xt;
+  double2* terms;
+};
+
 struct Buffers {
   // checkpoints and block results
   double2* ckW;
@@ -372,13 +408,15 @@ struct Buffers {
   double* cksc;
   double* bo;
   double* seg;
-  // tapes of one block
-  double2* Wt;
-  double* wt;
-  double* sct;
-  double2* ovt;
-  double2* xtt;
-  double2* termst;
+  // forward state
+  double* fw;
+  double2* fov;
+  double* fsc;
+  // tapes: one block's, or every block's when the forward records them
+  Tape tp;
+  // with recorded tapes: each block's orthonormalised end walkers and energies
+  double2* qk;
+  double2* ek;
   // reverse sweep
   double2 *Q, *Qc, *R, *Rinv, *detR;
   double2* Tb;
@@ -386,7 +424,8 @@ struct Buffers {
   double2 *gvhs, *EbarW;
   double2 *gxt, *gfb, *gd, *gG, *P, *gS, *X;
   double *alpha, *gw, *kappa, *mbar, *scal;  // scal: gesh, gh0p
-  double2 *LbarC, *rcbarC, *rh1barC;
+  double* Lbar;  // L adjoint (accumulated over the steps)
+  double2* rh1barC;
   // energy chunks
   int Wc;
   double2 *F, *Fp, *Gp, *Cc;
@@ -394,20 +433,26 @@ struct Buffers {
   double *Ebar, *v0, *v1, *hmod, *Amat, *Mblk, *Mexp, *Mwork, *hmodbar, *h1sbar, *rh1bar, *rcbar, *rho, *tg;
 };
 
-void carve(Arena& ar, Buffers& b, const afq_job* j, int nb, int ns, int K) {
+void carve(Arena& ar, Buffers& b, const afq_job* j, int nb, int ns, int K, bool keep) {
   const long N = j->N, G = j->G, W = j->W, O = j->nocc[0];
   const long per = N * O, NN = N * N;
+  const long slots = keep ? nb : 1;
   b.ckW = ar.take<double2>(nb * W * per);
   b.ckw = ar.take<double>(nb * W);
   b.cksc = ar.take<double>(2 * nb);
   b.bo = ar.take<double>(2 * nb);
   b.seg = ar.take<double>(2);
-  b.Wt = ar.take<double2>((ns + 1) * W * per);
-  b.wt = ar.take<double>((ns + 1) * W);
-  b.sct = ar.take<double>(2 * (ns + 1));
-  b.ovt = ar.take<double2>((ns + 1) * W);
-  b.xtt = ar.take<double2>(ns * W * G);
-  b.termst = ar.take<double2>(ns * 2 * W);
+  b.fw = ar.take<double>(W);
+  b.fov = ar.take<double2>(W);
+  b.fsc = ar.take<double>(2);
+  b.tp.W = ar.take<double2>(slots * (ns + 1) * W * per);
+  b.tp.w = ar.take<double>(slots * (ns + 1) * W);
+  b.tp.sc = ar.take<double>(slots * 2 * (ns + 1));
+  b.tp.ov = ar.take<double2>(slots * (ns + 1) * W);
+  b.tp.xt = ar.take<double2>(slots * ns * W * G);
+  b.tp.terms = ar.take<double2>(slots * ns * 2 * W);
+  b.qk = ar.take<double2>(keep ? nb * W * per : 0);
+  b.ek = ar.take<double2>(keep ? nb * W : 0);
   b.Q = ar.take<double2>(W * per);
   b.Qc = ar.take<double2>(W * per);
   b.R = ar.take<double2>(W * O * O);
@@ -434,12 +479,11 @@ void carve(Arena& ar, Buffers& b, const afq_job* j, int nb, int ns, int K) {
   b.kappa = ar.take<double>(W);
   b.mbar = ar.take<double>(G);
   b.scal = ar.take<double>(4);
-  b.LbarC = ar.take<double2>(G * NN);
-  b.rcbarC = ar.take<double2>(G * per);
+  b.Lbar = ar.take<double>(G * NN);
   b.rh1barC = ar.take<double2>(per);
-  // exchange-adjoint chunks: F and its permuted copy at most ~512 MB each
+  // exchange-adjoint chunks: F and its permuted copy at most ~2 GB each
   const long per_w = G * O * O * (long)sizeof(double2);
-  b.Wc = (int)std::max(1L, std::min(W, (512L << 20) / std::max(1L, per_w)));
+  b.Wc = (int)std::max(1L, std::min(W, (2048L << 20) / std::max(1L, per_w)));
   b.F = ar.take<double2>(G * O * b.Wc * O);
   b.Fp = ar.take<double2>(G * O * b.Wc * O);
   b.Gp = ar.take<double2>(b.Wc * per);
@@ -475,6 +519,14 @@ struct Seg {
   long N, G, W, O, per, NN;
   double dt, sqdt, e_est;
   const double* fields;
+  bool keep = false;  // the forward records every block's tapes
+  Tape t;             // tapes of the block being pulled back
+
+  Tape tape_at(int blk) const {
+    const long s = keep ? blk : 0;
+    return Tape{b.tp.W + s * (ns + 1) * W * per, b.tp.w + s * (ns + 1) * W, b.tp.sc + s * 2 * (ns + 1),
+                b.tp.ov + s * (ns + 1) * W,       b.tp.xt + s * ns * W * G,  b.tp.terms + s * ns * 2 * W};
+  }
 
   const double* T() const { return j->mo[0]; }
   const double* E() const { return j->exp_h1[0]; }
@@ -493,12 +545,13 @@ struct Seg {
     return overlap_from_dets(j, ov, st);
   }
 
-  // Forward over all blocks with checkpoints (afq_block with the given fields).
+  // Forward over all blocks with checkpoints (afq_block with the given fields),
+  // with keep also every block's tapes, end walkers and energies.
   int forward(const void* walkers, const double* weights, double* energy_out) {
     double2* Wc = b.Q;  // current walkers
-    double* wc = b.wt;
-    double2* ov = b.ovt;
-    double* sc = b.sct;
+    double* wc = b.fw;
+    double2* ov = b.fov;
+    double* sc = b.fsc;
     AFQ_TRY(cpy(Wc, walkers, sizeof(double2) * W * per));
     AFQ_TRY(cpy(wc, weights, sizeof(double) * W));
     const double sc0[2] = {e_est, e_est};
@@ -508,34 +561,48 @@ struct Seg {
       AFQ_TRY(cpy(b.ckW + blk * W * per, Wc, sizeof(double2) * W * per));
       AFQ_TRY(cpy(b.ckw + blk * W, wc, sizeof(double) * W));
       AFQ_TRY(cpy(b.cksc + 2 * blk, sc, 2 * sizeof(double)));
-      for (int k = 0; k < ns; ++k) AFQ_TRY(propagate(j, Wc, nullptr, fields_at(blk, k), wc, ov, sc, st));
+      if (keep) {  // the same steps, on the block's tape
+        AFQ_TRY(tape(blk));
+        const Tape tb = tape_at(blk);
+        AFQ_TRY(cpy(Wc, tb.W + ns * W * per, sizeof(double2) * W * per));
+        AFQ_TRY(cpy(wc, tb.w + ns * W, sizeof(double) * W));
+        AFQ_TRY(cpy(ov, tb.ov + ns * W, sizeof(double2) * W));
+        AFQ_TRY(cpy(sc, tb.sc + 2 * ns, 2 * sizeof(double)));
+      } else {
+        for (int k = 0; k < ns; ++k) AFQ_TRY(propagate(j, Wc, nullptr, fields_at(blk, k), wc, ov, sc, st));
+      }
       AFQ_TRY(orthonormalize(j, Wc, nullptr, st));
       AFQ_TRY(overlap(Wc, ov));
       AFQ_TRY(energy_pipeline(j, Wc, nullptr, j->energies, st));
+      if (keep) {
+        AFQ_TRY(cpy(b.qk + blk * W * per, Wc, sizeof(double2) * W * per));
+        AFQ_TRY(cpy(b.ek + blk * W, j->energies, sizeof(double2) * W));
+      }
       AFQ_TRY(k_block_finalize(j->energies, wc, sc, std::sqrt(2.0 / dt), (int)W, b.bo + 2 * blk, st));
     }
     hipLaunchKernelGGL(adj_seed_kernel, dim3(1), dim3(64), 0, st, b.bo, nb, b.seg, energy_out);
     return launch_status("adj_seed_kernel");
   }
 
-  // Recomputes the tapes of block blk from its checkpoint.
+  // Computes the tapes of block blk from its checkpoint.
   int tape(int blk) {
-    AFQ_TRY(cpy(b.Wt, b.ckW + blk * W * per, sizeof(double2) * W * per));
-    AFQ_TRY(cpy(b.wt, b.ckw + blk * W, sizeof(double) * W));
-    AFQ_TRY(cpy(b.sct, b.cksc + 2 * blk, 2 * sizeof(double)));
-    AFQ_TRY(overlap(b.Wt, b.ovt));
+    const Tape tb = tape_at(blk);
+    AFQ_TRY(cpy(tb.W, b.ckW + blk * W * per, sizeof(double2) * W * per));
+    AFQ_TRY(cpy(tb.w, b.ckw + blk * W, sizeof(double) * W));
+    AFQ_TRY(cpy(tb.sc, b.cksc + 2 * blk, 2 * sizeof(double)));
+    AFQ_TRY(overlap(tb.W, tb.ov));
     for (int k = 0; k < ns; ++k) {
-      double2* Wn = b.Wt + (k + 1) * W * per;
-      double* wn = b.wt + (k + 1) * W;
-      double2* ovn = b.ovt + (k + 1) * W;
-      double* scn = b.sct + 2 * (k + 1);
-      AFQ_TRY(cpy(Wn, b.Wt + k * W * per, sizeof(double2) * W * per));
-      AFQ_TRY(cpy(wn, b.wt + k * W, sizeof(double) * W));
-      AFQ_TRY(cpy(ovn, b.ovt + k * W, sizeof(double2) * W));
-      AFQ_TRY(cpy(scn, b.sct + 2 * k, 2 * sizeof(double)));
+      double2* Wn = tb.W + (k + 1) * W * per;
+      double* wn = tb.w + (k + 1) * W;
+      double2* ovn = tb.ov + (k + 1) * W;
+      double* scn = tb.sc + 2 * (k + 1);
+      AFQ_TRY(cpy(Wn, tb.W + k * W * per, sizeof(double2) * W * per));
+      AFQ_TRY(cpy(wn, tb.w + k * W, sizeof(double) * W));
+      AFQ_TRY(cpy(ovn, tb.ov + k * W, sizeof(double2) * W));
+      AFQ_TRY(cpy(scn, tb.sc + 2 * k, 2 * sizeof(double)));
       AFQ_TRY(propagate(j, Wn, nullptr, fields_at(blk, k), wn, ovn, scn, st));
-      AFQ_TRY(cpy(b.xtt + k * W * G, j->xs, sizeof(double2) * W * G));
-      AFQ_TRY(cpy(b.termst + k * 2 * W, j->terms, sizeof(double2) * 2 * W));
+      AFQ_TRY(cpy(tb.xt + k * W * G, j->xs, sizeof(double2) * W * G));
+      AFQ_TRY(cpy(tb.terms + k * 2 * W, j->terms, sizeof(double2) * 2 * W));
     }
     return AFQ_OK;
   }
@@ -574,21 +641,24 @@ struct Seg {
       LAUNCH(adj_cc_kernel, (long)wc * G, b.F, (int)G, (int)O, wc, b.Cc);
       // gG_w = sum_g 4 kappa c_g rc_g - 2 kappa sum_g f_g rc_g + 2 kappa rot_h1
       AFQ_TRY(GB(wc, per, G).a(b.Cc, G, 0, true).b(j->rot_chol[0], per, 0, false).c(b.gG + w0 * per, per, 0).run(st));
-      AFQ_TRY(GB(O, N, G * O, wc)
-                  .a(b.Fp, G * O, O * G * O, true)
+      // all wc walkers in one product (row (w, i) of Fp starts at (w O + i) G O), split over K
+      AFQ_TRY(GB((int)(wc * O), N, G * O)
+                  .a(b.Fp, G * O, 0, true)
                   .b(j->rot_chol[0], N, 0, false)
-                  .c(b.gG + w0 * per, N, per)
+                  .c(b.gG + w0 * per, N, 0)
                   .beta(1.0)
+                  .ws(j)
                   .run(st));
       LAUNCH(adj_outer_kernel, wc * per, b.gG + w0 * per, b.kappa + w0, j->rot_h1[0], 2.0, per, wc * per);
       // rc_g adjoint: sum_w 4 kappa c_wg G_w - 2 kappa f_wg^T G_w ; rot_h1 adjoint: 2 sum_w kappa G_w
-      AFQ_TRY(GB(G, per, wc).a(b.Cc, G, 0, true, true).b(Gc, per, 0, true).c(b.rcbarC, per, 0).beta(1.0).run(st));
+      AFQ_TRY(GB(G, per, wc).a(b.Cc, G, 0, true, true).b(Gc, per, 0, true).c(b.rcbar, per, 0).beta(1.0).part(1).run(st));
       LAUNCH(adj_gperm_kernel, wc * per, Gc, (int)O, (int)N, wc, b.Gp);
       AFQ_TRY(GB(O, N, O * wc, G)
                   .a(b.F, O, O * wc * O, true, true)
                   .b(b.Gp, N, 0, true)
-                  .c(b.rcbarC, N, per)
+                  .c(b.rcbar, N, per)
                   .beta(1.0)
+                  .part(1)
                   .run(st));
       AFQ_TRY(GB(1, per, wc).a(b.kappa + w0, wc, 0, false).b(Gc, per, 0, true).c(b.rh1barC, per, 0).alpha(2.0).beta(1.0).run(
           st));
@@ -601,11 +671,11 @@ struct Seg {
   // on exit those of W_k, w_k, E_k.
   int step_backward(int k) {
     const long OO = O * O;
-    double2* Wk = b.Wt + k * W * per;
-    double2* Wn = b.Wt + (k + 1) * W * per;
-    const double2* xt = b.xtt + k * W * G;
-    hipLaunchKernelGGL(adj_weight_kernel, dim3(1), dim3(1024), 0, st, b.ovt + k * W, b.ovt + (k + 1) * W,
-                       b.termst + k * 2 * W, b.wt + k * W, b.wt + (k + 1) * W, b.sct + 2 * k, j->h0_prop, dt, (int)W,
+    double2* Wk = t.W + k * W * per;
+    double2* Wn = t.W + (k + 1) * W * per;
+    const double2* xt = t.xt + k * W * G;
+    hipLaunchKernelGGL(adj_weight_kernel, dim3(1), dim3(1024), 0, st, t.ov + k * W, t.ov + (k + 1) * W,
+                       t.terms + k * 2 * W, t.w + k * W, t.w + (k + 1) * W, t.sc + 2 * k, j->h0_prop, dt, (int)W,
                        b.gw, b.scal, b.gd, b.alpha, b.scal + 1);
     AFQ_TRY(launch_status("adj_weight_kernel"));
     // log ov' = 2 log det(T^T W'): gW' += T (2 gd S'^-T)
@@ -654,18 +724,30 @@ struct Seg {
     if (K > 0) {
       AFQ_TRY(GB(W, G, NN).a(b.gvhs, NN, 0, true).b(j->chol, NN, 0, false, true).c(b.gxt, G, 0).alpha(0.0, sqdt).beta(1.0).ws(
           j).run(st));
-      AFQ_TRY(GB(G, NN, W).a(xt, G, 0, true, true).b(b.gvhs, NN, 0, true).c(b.LbarC, NN, 0).alpha(0.0, sqdt).beta(1.0).run(
-          st));
+      AFQ_TRY(GB(G, NN, W)
+                  .a(xt, G, 0, true, true)
+                  .b(b.gvhs, NN, 0, true)
+                  .c(b.Lbar, NN, 0)
+                  .alpha(0.0, sqdt)
+                  .beta(1.0)
+                  .part(2)
+                  .run(st));
     }
     // shifts, force bias and the m adjoint
-    hipLaunchKernelGGL(adj_fb_kernel, dim3(nblk(G)), dim3(256), 0, st, xt, b.gxt, b.gd, b.alpha, sqdt, (int)G, (int)W,
-                       b.gfb, b.mbar);
+    hipLaunchKernelGGL(adj_fb_kernel, dim3((unsigned)((G + 63) / 64)), dim3(1024), 0, st, xt, b.gxt, b.gd, b.alpha, sqdt,
+                       (int)G, (int)W, b.gfb, b.mbar);
     AFQ_TRY(launch_status("adj_fb_kernel"));
     // fb = 2 sum rc . G at W_k
     AFQ_TRY(green_pipeline(j, Wk, nullptr, true, true, st));
     AFQ_TRY(GB(W, per, G).a(b.gfb, G, 0, true).b(j->rot_chol[0], per, 0, false).c(b.gG, per, 0).alpha(2.0).run(st));
-    AFQ_TRY(GB(G, per, W).a(b.gfb, G, 0, true, true).b(j->green[0], per, 0, true).c(b.rcbarC, per, 0).alpha(2.0).beta(
-        1.0).run(st));
+    AFQ_TRY(GB(G, per, W)
+                .a(b.gfb, G, 0, true, true)
+                .b(j->green[0], per, 0, true)
+                .c(b.rcbar, per, 0)
+                .alpha(2.0)
+                .beta(1.0)
+                .part(1)
+                .run(st));
     // Green's function and -log ov at W_k
     AFQ_TRY(green_backward(b.gG, b.gd, b.gWk));
     std::swap(b.gW, b.gWk);
@@ -674,8 +756,8 @@ struct Seg {
 
   int reverse() {
     const size_t wbytes = sizeof(double2) * W * per;
-    AFQ_TRY(zero(b.LbarC, sizeof(double2) * G * NN));
-    AFQ_TRY(zero(b.rcbarC, sizeof(double2) * G * per));
+    AFQ_TRY(zero(b.Lbar, sizeof(double) * G * NN));
+    AFQ_TRY(zero(b.rcbar, sizeof(double) * G * per));
     AFQ_TRY(zero(b.rh1barC, sizeof(double2) * per));
     AFQ_TRY(zero(b.EbarW, sizeof(double2) * W * NN));
     AFQ_TRY(zero(b.mbar, sizeof(double) * G));
@@ -683,13 +765,20 @@ struct Seg {
     AFQ_TRY(zero(b.gW, wbytes));
     AFQ_TRY(zero(b.gw, sizeof(double) * W));
     for (int blk = nb - 1; blk >= 0; --blk) {
-      AFQ_TRY(tape(blk));
-      // block end: Q = qr(W_end), energies at Q
-      double2* Wend = b.Wt + ns * W * per;
-      AFQ_TRY(cpy(b.Q, Wend, wbytes));
-      AFQ_TRY(orthonormalize(j, b.Q, nullptr, st));
-      AFQ_TRY(energy_pipeline(j, b.Q, nullptr, j->energies, st));
-      hipLaunchKernelGGL(adj_block_end_kernel, dim3(1), dim3(1024), 0, st, j->energies, b.wt + ns * W, b.sct,
+      if (!keep) AFQ_TRY(tape(blk));
+      t = tape_at(blk);
+      // block end: Q = qr(W_end), energies at Q (recorded, or recomputed)
+      double2* Wend = t.W + ns * W * per;
+      if (keep) {
+        AFQ_TRY(cpy(b.Q, b.qk + blk * W * per, wbytes));
+        AFQ_TRY(cpy(j->energies, b.ek + blk * W, sizeof(double2) * W));
+        AFQ_TRY(green_pipeline(j, b.Q, nullptr, true, true, st));
+      } else {
+        AFQ_TRY(cpy(b.Q, Wend, wbytes));
+        AFQ_TRY(orthonormalize(j, b.Q, nullptr, st));
+        AFQ_TRY(energy_pipeline(j, b.Q, nullptr, j->energies, st));
+      }
+      hipLaunchKernelGGL(adj_block_end_kernel, dim3(1), dim3(1024), 0, st, j->energies, t.w + ns * W, t.sc,
                          b.bo + 2 * blk, b.seg, std::sqrt(2.0 / dt), (int)W, b.scal, b.kappa, b.gw);
       AFQ_TRY(launch_status("adj_block_end_kernel"));
       AFQ_TRY(energy_backward());
@@ -728,12 +817,11 @@ struct Seg {
     AFQ_TRY(GB(G, 1, NN).a(j->chol, NN, 0, false).b(b.hmodbar, 1, 0, false).c(b.tg, 1, 0).run(st));
     LAUNCH(adj_mbar_final_kernel, G, b.mbar, j->mreal, b.scal + 1, b.tg, (int)G);
     // L adjoint
-    LAUNCH(adj_re_kernel, G * NN, b.LbarC, G * NN, chol_bar);
+    AFQ_TRY(cpy(chol_bar, b.Lbar, sizeof(double) * G * NN));
     AFQ_TRY(GB(N, N, N, G).a(b.hmodbar, N, 0, false).b(j->chol, N, NN, false, true).c(chol_bar, N, NN).alpha(-0.5).beta(
         1.0).run(st));
     AFQ_TRY(GB(N, N, N, G).a(j->chol, N, NN, false, true).b(b.hmodbar, N, 0, false).c(chol_bar, N, NN).alpha(-0.5).beta(
         1.0).run(st));
-    LAUNCH(adj_re_kernel, G * per_, b.rcbarC, G * per_, b.rcbar);
     AFQ_TRY(GB(N, N, O, G).a(T(), O, 0, false).b(b.rcbar, N, per_, false).c(chol_bar, N, NN).beta(1.0).run(st));
     AFQ_TRY(k_axpby4(b.rho, j->rdm1, j->rdm1 + NN, 1.0, nullptr, nullptr, 0.0, NN, st));
     LAUNCH(adj_lfinal_kernel, G * NN, chol_bar, j->mreal, b.mbar, b.hmodbar, b.rho, (int)G, NN);
@@ -776,13 +864,33 @@ extern "C" int afq_segment_energy_vjp(afq_job_t j, const void* walkers, const do
   s.sqdt = std::sqrt(j->d.dt);
   s.e_est = e_estimate;
   s.fields = fields;
+  // Record every block's tapes in the forward (instead of recomputing them
+  // block by block in the reverse sweep) when that arena takes at most half of
+  // the free device memory, or AFQ_SEGMENT_TAPE_LIMIT bytes when that is set.
   Arena sizer;
-  carve(sizer, s.b, j, n_blocks, n_steps, s.K);
+  bool keep = false;
+  if (h1_bar) {
+    carve(sizer, s.b, j, n_blocks, n_steps, s.K, true);
+    if (n_blocks == 1) {
+      keep = true;  // a single block's tape takes the same memory either way
+    } else {
+      size_t free_b = 0, total_b = 0;
+      AFQ_HIP(hipMemGetInfo(&free_b, &total_b));
+      double limit = 0.5 * (double)free_b;
+      if (const char* e = std::getenv("AFQ_SEGMENT_TAPE_LIMIT")) limit = std::min(limit, std::atof(e));
+      keep = (double)sizer.off <= limit;
+    }
+  }
+  if (!keep) {
+    sizer = Arena{};
+    carve(sizer, s.b, j, n_blocks, n_steps, s.K, false);
+  }
+  s.keep = keep;
   void* base = nullptr;
   AFQ_TRY(dalloc(&base, sizer.off));
   Arena ar;
   ar.base = static_cast<char*>(base);
-  carve(ar, s.b, j, n_blocks, n_steps, s.K);
+  carve(ar, s.b, j, n_blocks, n_steps, s.K, s.keep);
   int rc = s.forward(walkers, weights, energy_out);
   if (rc == AFQ_OK && h1_bar) rc = s.reverse();
   if (rc == AFQ_OK && h1_bar) rc = s.setup_backward(h1_bar, chol_bar);

@@ -98,6 +98,10 @@ struct Gemm {
   // with too few output tiles to fill the GPU
   void* ws = nullptr;
   long ws_elems = 0;
+  // complex A and B with real C = Re(alpha A.B) + beta C, computing only the
+  // part of A.B that alpha selects: 1 = real part (alpha real), 2 = imaginary
+  // part (alpha imaginary); ldc and sC then count doubles
+  int c_part = 0;
 };
 int gemm_f64(const Gemm& g, hipStream_t stream);
 // C = alpha * sum of `ksplit` split-K partial planes in ws + beta * C (+ D), fixed order.

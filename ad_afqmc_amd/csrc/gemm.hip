@@ -166,9 +166,11 @@ __device__ inline void store_b(double* Bs, const double (&r)[C::RB][4][2], int t
 
 // One work-group's tile `tile` of batch entry b, K slice kz (the whole
 // product when kchunk >= K and ws is null).
-template <bool AC, bool BC, bool AT, bool BT, int WM, int WN, int TM, int TN>
+// PART (complex A and B only): 0 complex C; 1 real C from Re(A.B); 2 real C from Im(A.B)
+template <bool AC, bool BC, bool AT, bool BT, int WM, int WN, int TM, int TN, int PART = 0>
 __device__ __forceinline__ void gemm_f64_body(const Gemm& g, int tiles_m, int tiles_n, long tile, long b, int kz,
                                               int kchunk, double* ws) {
+  static_assert(PART == 0 || (AC && BC), "a real part of a complex product needs complex A and B");
   using C = Cfg<AC, BC, AT, BT, WM, WN, TM, TN>;
   constexpr bool CC = AC || BC;
   __shared__ double As[2][C::PA * BK * C::LDA];
@@ -231,11 +233,13 @@ __device__ __forceinline__ void gemm_f64_body(const Gemm& g, int tiles_m, int ti
       for (int i = 0; i < TM; ++i)
 #pragma unroll
         for (int j = 0; j < TN; ++j) {
-          accR[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar[i], br[j], accR[i][j], 0, 0, 0);
+          if (PART != 2) accR[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar[i], br[j], accR[i][j], 0, 0, 0);
           if (AC && BC) {
-            accR[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(-ai[i], bi[j], accR[i][j], 0, 0, 0);
-            accI[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar[i], bi[j], accI[i][j], 0, 0, 0);
-            accI[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(ai[i], br[j], accI[i][j], 0, 0, 0);
+            if (PART != 2) accR[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(-ai[i], bi[j], accR[i][j], 0, 0, 0);
+            if (PART != 1) {
+              accI[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar[i], bi[j], accI[i][j], 0, 0, 0);
+              accI[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(ai[i], br[j], accI[i][j], 0, 0, 0);
+            }
           } else if (AC) {
             accI[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(ai[i], br[j], accI[i][j], 0, 0, 0);
           } else if (BC) {
@@ -273,6 +277,24 @@ __device__ __forceinline__ void gemm_f64_body(const Gemm& g, int tiles_m, int ti
     return;
   }
   const bool has_beta = (g.beta.x != 0.0 || g.beta.y != 0.0);
+  if constexpr (PART != 0) {  // real C (no D, no split K)
+    double* Cr = static_cast<double*>(g.C) + b * g.sC;
+#pragma unroll
+    for (int i = 0; i < TM; ++i)
+#pragma unroll
+      for (int j = 0; j < TN; ++j)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          const int m = m0 + (wm * TM + i) * 16 + (lane >> 4) + 4 * v;
+          const int n = n0 + (wn * TN + j) * 16 + (lane & 15);
+          if (m >= g.M || n >= g.N) continue;
+          const long off = (long)m * g.ldc + n;
+          double r = PART == 1 ? g.alpha.x * accR[i][j][v] : -g.alpha.y * accI[i][j][v];
+          if (has_beta) r += g.beta.x * Cr[off];
+          Cr[off] = r;
+        }
+    return;
+  }
   double* Cp = static_cast<double*>(g.C) + b * g.sC * (CC ? 2 : 1);
   double* Dp = g.D ? static_cast<double*>(g.D) + b * g.sD * (CC ? 2 : 1) : nullptr;
 #pragma unroll
@@ -312,7 +334,7 @@ __device__ __forceinline__ void gemm_f64_body(const Gemm& g, int tiles_m, int ti
 
 // grid: x = tiles (XCD-remapped when xcd_remap == 1), y = batch, z = K split;
 // xcd_remap == 2: x = tiles x batch rounded up to 8 entries, y = z = 1.
-template <bool AC, bool BC, bool AT, bool BT, int WM, int WN, int TM, int TN>
+template <bool AC, bool BC, bool AT, bool BT, int WM, int WN, int TM, int TN, int PART = 0>
 __global__ __launch_bounds__(64 * WM * WN) void gemm_f64_kernel(Gemm g, int tiles_m, int tiles_n, int xcd_remap,
                                                                  long T8, int kchunk, double* ws) {
   long tile = blockIdx.x;
@@ -328,7 +350,7 @@ __global__ __launch_bounds__(64 * WM * WN) void gemm_f64_kernel(Gemm g, int tile
     tile = p % T8;
     if (b >= g.batch) return;  // uniform per work-group
   }
-  gemm_f64_body<AC, BC, AT, BT, WM, WN, TM, TN>(g, tiles_m, tiles_n, tile, b, (int)blockIdx.z, kchunk, ws);
+  gemm_f64_body<AC, BC, AT, BT, WM, WN, TM, TN, PART>(g, tiles_m, tiles_n, tile, b, (int)blockIdx.z, kchunk, ws);
 }
 
 // Grouped batched products: work-group i belongs to the job j with
@@ -390,14 +412,14 @@ __global__ void splitk_reduce_kernel(Gemm g, const double* ws, int ksplit) {
   }
 }
 
-template <bool AC, bool BC, bool AT, bool BT, int WM, int WN, int TM, int TN>
+template <bool AC, bool BC, bool AT, bool BT, int WM, int WN, int TM, int TN, int PART = 0>
 int launch(const Gemm& g, hipStream_t s) {
   using C = Cfg<AC, BC, AT, BT, WM, WN, TM, TN>;
   const int tm = (g.M + C::BM - 1) / C::BM, tn = (g.N + C::BN - 1) / C::BN;
   const long tiles = (long)tm * tn;
   int ksplit = 1;
   const long wgs = tiles * g.batch;
-  if (g.ws && g.batch == 1 && wgs < 1024 && g.K >= 8 * BK) {
+  if (PART == 0 && g.ws && g.batch == 1 && wgs < 1024 && g.K >= 8 * BK) {
     ksplit = (int)std::min<long>((2048 + wgs - 1) / wgs, (long)g.K / (4 * BK));
     const long need = (long)ksplit * g.M * g.N * ((AC || BC) ? 2 : 1);
     while (ksplit > 1 && need > g.ws_elems) {
@@ -417,8 +439,8 @@ int launch(const Gemm& g, hipStream_t s) {
   if (gx > 0x7fffffffL || gy > 65535u || ksplit > 65535) return fail(AFQ_E_UNSUPPORTED, "gemm_f64: grid too large");
   dim3 grid((unsigned)gx, gy, ksplit);
   double* ws = ksplit > 1 ? static_cast<double*>(g.ws) : nullptr;
-  hipLaunchKernelGGL((gemm_f64_kernel<AC, BC, AT, BT, WM, WN, TM, TN>), grid, dim3(C::NT), 0, s, g, tm, tn, remap, T8,
-                     kchunk, ws);
+  hipLaunchKernelGGL((gemm_f64_kernel<AC, BC, AT, BT, WM, WN, TM, TN, PART>), grid, dim3(C::NT), 0, s, g, tm, tn, remap,
+                     T8, kchunk, ws);
   AFQ_TRY(launch_status("gemm_f64_kernel"));
   if (ksplit > 1) return gemm_splitk_reduce(g, ws, ksplit, AC || BC, s);
   return AFQ_OK;
@@ -632,10 +654,27 @@ int gemm_f64_grouped(const GroupTab<GemmJob>& t, bool a_cplx, bool b_cplx, bool 
   return launch_status("gemm_f64_grouped_kernel");
 }
 
+// Real C from one part of a complex product (64 x 64 tiles, no K split).
+template <int PART>
+int dispatch_part(const Gemm& g, hipStream_t s) {
+  if (g.a_trans)
+    return g.b_trans ? launch<true, true, true, true, 2, 2, 2, 2, PART>(g, s)
+                     : launch<true, true, true, false, 2, 2, 2, 2, PART>(g, s);
+  return g.b_trans ? launch<true, true, false, true, 2, 2, 2, 2, PART>(g, s)
+                   : launch<true, true, false, false, 2, 2, 2, 2, PART>(g, s);
+}
+
 int gemm_f64(const Gemm& g, hipStream_t s) {
   if (g.M <= 0 || g.N <= 0 || g.batch <= 0) return AFQ_OK;
   if (g.K <= 0) return fail(AFQ_E_INVALID, "gemm_f64: K must be positive");
   if (g.E) return fail(AFQ_E_UNSUPPORTED, "gemm_f64: no separate addend");
+  if (g.c_part) {
+    if (!g.a_cplx || !g.b_cplx || g.D || g.a_sym || g.a_conj || g.c_part < 1 || g.c_part > 2)
+      return fail(AFQ_E_INVALID, "gemm_f64: a real-part product takes complex A and B, no D, no packed or conjugated A");
+    if (g.beta.y != 0.0 || (g.c_part == 1 ? g.alpha.y : g.alpha.x) != 0.0)
+      return fail(AFQ_E_INVALID, "gemm_f64: real-part product: alpha must be real (part 1) or imaginary (part 2), beta real");
+    return g.c_part == 1 ? dispatch_part<1>(g, s) : dispatch_part<2>(g, s);
+  }
   if (g.a_conj && !g.a_cplx) return fail(AFQ_E_INVALID, "gemm_f64: only a complex A can be conjugated");
   if (g.a_sym && (g.a_trans || g.a_sym != g.M || g.a_sym != g.K))
     return fail(AFQ_E_INVALID, "gemm_f64: a packed symmetric A is square and not transposed");

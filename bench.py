@@ -588,6 +588,45 @@ def h10_timing(steps=20, warmup=3):
             "workload": "H10 chain, STO-6G, 1.6 bohr, RHF trial, 50 walkers, dt 0.005, 10 steps per block, FP64"}
 
 
+def adjoint_timing(name="rhf_n200_o40_g1200_w512", n_blocks=2, n_steps=10, reps=2, l_coef=0.3):
+    """Reverse pass of a phaseless segment (afq_segment_energy_vjp) on
+    workload `name` (RHF trial, restricted walkers, FP64): n_blocks blocks of
+    n_steps steps with given fields, the value alone and the value with dF/dh1
+    and dF/dchol."""
+    import numpy as np
+    import torch
+
+    from ad_afqmc_amd.adjoint import _job
+
+    cfg = syn_config(name)
+    N, O, G, W = cfg["norb"], cfg["nocc_up"], cfg["nchol"], cfg["n_walkers"]
+    s = make_system(name, l_coef)
+    from ad_afqmc_amd import synthetic as syn
+
+    walkers = torch.as_tensor(syn.perturbed_walkers(s.mo_up, W, 0.3 / math.sqrt(N), seed=1)).cuda()
+    weights = torch.ones(W, dtype=torch.float64, device="cuda")
+    gen = torch.Generator(device="cuda").manual_seed(5)
+    fields = torch.randn((n_blocks, n_steps, W, G), dtype=torch.float64, device="cuda", generator=gen)
+    job = _job(s.h0, np.array(s.h1), np.array(s.chol).reshape(G, N * N), s.mo_up, W, cfg["dt"], 6)
+    try:
+        e_est = float(torch.mean(job.energy(walkers).real))
+        out = {}
+        for want in (False, True, False, True):  # the first pair warms up
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            for _ in range(reps):
+                F, _, _ = job.segment_energy_vjp(walkers, weights, e_est, fields, want_grad=want)
+            torch.cuda.synchronize()
+            out["with_gradient" if want else "value_only"] = (time.perf_counter() - t0) / reps * 1e3
+    finally:
+        job.close()
+    ws = W * n_blocks * n_steps
+    return {"workload": f"{name}, {n_blocks} blocks x {n_steps} steps, FP64",
+            "ms_value_only": out["value_only"], "ms_with_gradient": out["with_gradient"],
+            "gradient_over_value": out["with_gradient"] / out["value_only"],
+            "walker_steps_per_s_with_gradient": ws / out["with_gradient"] * 1e3, "segment_energy": float(F)}
+
+
 def h10_cpu_baseline(n_steps=20):
     """The FP64 oracle on the H10 chain, 50 walkers, one core: n_steps steps
     and one measurement, scaled to blocks of 10 steps."""
@@ -802,6 +841,9 @@ def main():
         print("[bench] H10 chain", file=sys.stderr, flush=True)
         line["h10_chain"] = h10_timing()
         line["h10_chain"]["cpu_baseline"] = h10_cpu
+        print("[bench] segment reverse pass", file=sys.stderr, flush=True)
+        line["segment_adjoint"] = adjoint_timing(l_coef=l_coef or 0.3)
+        torch.cuda.empty_cache()
     if args.serving_jobs > 0 and not args.only:
         from ad_afqmc_amd import serving
 
