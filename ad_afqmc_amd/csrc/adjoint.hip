@@ -233,16 +233,6 @@ __global__ void adj_cc_kernel(const double2* F, int G, int O, int Wc, double2* C
   Cc[idx] = double2{-2.0 * s.x, -2.0 * s.y};
 }
 
-// Gp[i][w][n] = G_w[i][n]
-__global__ void adj_gperm_kernel(const double2* Gm, int O, int N, int Wc, double2* Gp) {
-  const long idx = blockIdx.x * (long)blockDim.x + threadIdx.x;
-  if (idx >= (long)Wc * O * N) return;
-  const int n = (int)(idx % N);
-  const long t = idx / N;
-  const int i = (int)(t % O), wl = (int)(t / O);
-  Gp[((long)i * Wc + wl) * N + n] = Gm[idx];
-}
-
 // gG_w += c kappa_w rot_h1
 __global__ void adj_outer_kernel(double2* gG, const double* kappa, const double* rh1, double c, long per, long n) {
   const long idx = blockIdx.x * (long)blockDim.x + threadIdx.x;
@@ -428,7 +418,7 @@ struct Buffers {
   double2* rh1barC;
   // energy chunks
   int Wc;
-  double2 *F, *Fp, *Gp, *Cc;
+  double2 *F, *Fp, *Cc;
   // setup
   double *Ebar, *v0, *v1, *hmod, *Amat, *Mblk, *Mexp, *Mwork, *hmodbar, *h1sbar, *rh1bar, *rcbar, *rho, *tg;
 };
@@ -486,7 +476,6 @@ void carve(Arena& ar, Buffers& b, const afq_job* j, int nb, int ns, int K, bool 
   b.Wc = (int)std::max(1L, std::min(W, (2048L << 20) / std::max(1L, per_w)));
   b.F = ar.take<double2>(G * O * b.Wc * O);
   b.Fp = ar.take<double2>(G * O * b.Wc * O);
-  b.Gp = ar.take<double2>(b.Wc * per);
   b.Cc = ar.take<double2>(b.Wc * G);
   b.Ebar = ar.take<double>(NN);
   b.v0 = ar.take<double>(NN);
@@ -631,11 +620,11 @@ struct Seg {
       const int wc = (int)std::min<long>(Wc, W - w0);
       const double2* Gc = j->green[0] + w0 * per;
       const long fsz = G * O * wc * O;
-      // f[g][i][(w,j)] = rc_g G_w^T
-      AFQ_TRY(GB(O, wc * O, N, G)
-                  .a(j->rot_chol[0], N, per, false)
+      // f[g][i][(w,j)] = rc_g G_w^T, all gammas in one product: rows (g, i) of rot_chol
+      AFQ_TRY(GB((int)(G * O), (int)(wc * O), N)
+                  .a(j->rot_chol[0], N, 0, false)
                   .b(Gc, N, 0, true, true)
-                  .c(b.F, wc * O, O * wc * O)
+                  .c(b.F, wc * O, 0)
                   .run(st));
       LAUNCH(adj_fscale_kernel, fsz, b.F, b.kappa + w0, (int)G, (int)O, wc, b.Fp);
       LAUNCH(adj_cc_kernel, (long)wc * G, b.F, (int)G, (int)O, wc, b.Cc);
@@ -652,11 +641,11 @@ struct Seg {
       LAUNCH(adj_outer_kernel, wc * per, b.gG + w0 * per, b.kappa + w0, j->rot_h1[0], 2.0, per, wc * per);
       // rc_g adjoint: sum_w 4 kappa c_wg G_w - 2 kappa f_wg^T G_w ; rot_h1 adjoint: 2 sum_w kappa G_w
       AFQ_TRY(GB(G, per, wc).a(b.Cc, G, 0, true, true).b(Gc, per, 0, true).c(b.rcbar, per, 0).beta(1.0).part(1).run(st));
-      LAUNCH(adj_gperm_kernel, wc * per, Gc, (int)O, (int)N, wc, b.Gp);
-      AFQ_TRY(GB(O, N, O * wc, G)
-                  .a(b.F, O, O * wc * O, true, true)
-                  .b(b.Gp, N, 0, true)
-                  .c(b.rcbar, N, per)
+      // rows (g, j) of rot_chol: sum over (w, i) of Fp[w][i][g][j] G_w[i][n], one product
+      AFQ_TRY(GB((int)(G * O), N, (int)(wc * O))
+                  .a(b.Fp, G * O, 0, true, true)
+                  .b(Gc, N, 0, true)
+                  .c(b.rcbar, N, 0)
                   .beta(1.0)
                   .part(1)
                   .run(st));
