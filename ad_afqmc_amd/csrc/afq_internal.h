@@ -219,6 +219,10 @@ struct afq_job {
   uint16_t* rcs[2] = {nullptr, nullptr};
   uint16_t* rcs_fb[2] = {nullptr, nullptr};
   uint16_t* gsplit = nullptr;
+  // FP32 exchange operands (nocc <= 64): rot_chol (G, Opad, Npad) per spin and
+  // one walker batch of G (W, 2, Opad, Npad), in the FP32 tiled layout
+  float* rct32[2] = {nullptr, nullptr};
+  float* gt32 = nullptr;
   // split-BF16 x3 VHS operands: L^T (NNr, 2, Gpad) and the shifted fields
   // (Wr, 4, Gpad); rows padded to 16, k to 32
   uint16_t* lt_split = nullptr;

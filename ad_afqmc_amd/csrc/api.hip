@@ -92,6 +92,9 @@ void release(afq_job* j) {
   fr(j->rcs_fb[0]);
   if (!j->rhf) fr(j->rcs_fb[1]);
   fr(j->gsplit);
+  fr(j->rct32[0]);
+  if (!j->rhf) fr(j->rct32[1]);
+  fr(j->gt32);
   fr(j->lt_split);
   fr(j->xs_split);
   fr(j->dscratch);
@@ -277,12 +280,19 @@ int energy_pipeline(afq_job* j, const void* up, const void* dn, double2* out, hi
       AFQ_TRY(k_exchange_bf16(j->rcs[s], j->gsplit, j->nocc[s], j->N, j->G, j->W, coul, kpart, &nch[s], st));
       continue;
     }
+    if (j->lp && j->nocc[s] <= kExchangeRegMaxO) {  // FP32
+      {
+        ProfScope pm(j, AFQ_K_ENERGY_MISC, st);
+        AFQ_TRY(k_split_f32(reinterpret_cast<const double*>(j->green[s]), true, j->nocc[s], j->N, j->W, j->gt32, st));
+      }
+      ProfScope ps(j, AFQ_K_EXCHANGE, st);
+      AFQ_TRY(k_exchange_f32(j->rct32[s], j->gt32, j->nocc[s], j->N, j->G, j->W, coul, kpart, &nch[s], st));
+      continue;
+    }
     ProfScope ps(j, AFQ_K_EXCHANGE, st);
     if (j->nocc[s] > kExchangeRegMaxO)
       AFQ_TRY(k_exchange_wide(j->rot_chol[s], j->green[s], j->nocc[s], j->N, j->G, j->W, j->xwide, j->xwide_elems,
                               coul, kpart, &nch[s], st));
-    else if (j->lp)
-      AFQ_TRY(k_exchange_lp(j->rot_chol32[s], j->green[s], j->nocc[s], j->N, j->G, j->W, coul, kpart, &nch[s], st));
     else
       AFQ_TRY(k_exchange(j->rot_chol[s], j->green[s], j->nocc[s], j->N, j->G, j->W, coul, kpart, &nch[s], st));
   }
@@ -851,6 +861,14 @@ int alloc_job(afq_job* j, bool dry, int64_t* total) {
       }
     } else {
       A_(j->chol32, G * N * N);
+      const long np = bf16_npad(j->N);
+      int omax = 0;
+      for (int s = 0; s < (j->rhf ? 1 : 2); ++s)
+        if (j->nocc[s] <= kExchangeRegMaxO) {
+          A_(j->rct32[s], G * bf16_opad(j->nocc[s]) * np);
+          omax = std::max(omax, j->nocc[s]);
+        }
+      if (omax > 0) A_(j->gt32, W * 2 * bf16_opad(omax) * np);
     }
     A_(j->rot_chol32[0], G * j->nocc[0] * N);
     if (!j->rhf) A_(j->rot_chol32[1], G * j->nocc[1] * N);
@@ -924,6 +942,7 @@ int afq_job_create(afq_job_t* out, const afq_job_desc* d) {
     j->rot_chol32[1] = j->rot_chol32[0];
     j->rcs[1] = j->rcs[0];
     j->rcs_fb[1] = j->rcs_fb[0];
+    j->rct32[1] = j->rct32[0];
   }
   *out = j;
   return AFQ_OK;
@@ -1046,6 +1065,7 @@ int afq_build_measurement_intermediates(afq_job_t j, void* stream) {
   if (j->lp)
     for (int s = 0; s < nsp; ++s) {
       AFQ_TRY(k_to_f32(j->rot_chol[s], j->rot_chol32[s], (long)j->G * j->nocc[s] * N, st));
+      if (j->rct32[s]) AFQ_TRY(k_split_f32(j->rot_chol[s], false, j->nocc[s], N, j->G, j->rct32[s], st));
       if (j->bf16x3) {
         AFQ_TRY(k_split_bf16(j->rot_chol[s], false, j->nocc[s], N, j->G, false, j->rcs[s], st));
         AFQ_TRY(k_split_bf16(j->rot_chol[s], false, j->nocc[s], N, j->G, true, j->rcs_fb[s], st));
@@ -1168,6 +1188,8 @@ int afq_set_intermediate(afq_job_t j, int which, int spin, const void* src, void
     case AFQ_INT_ROT_CHOL:
       j->have_rot_chol[spin] = true;
       if (j->lp) AFQ_TRY(k_to_f32(j->rot_chol[spin], j->rot_chol32[spin], G * O * N, S(stream)));
+      if (j->rct32[spin])
+        AFQ_TRY(k_split_f32(j->rot_chol[spin], false, (int)O, (int)N, G, j->rct32[spin], S(stream)));
       if (j->bf16x3) {
         AFQ_TRY(k_split_bf16(j->rot_chol[spin], false, (int)O, (int)N, G, false, j->rcs[spin], S(stream)));
         AFQ_TRY(k_split_bf16(j->rot_chol[spin], false, (int)O, (int)N, G, true, j->rcs_fb[spin], S(stream)));

@@ -59,6 +59,15 @@ __host__ __device__ inline int bf16_npad_d(int N) { return 32 * ((N + 31) / 32);
 __host__ __device__ inline long bf16_tile(int o, int k, int Opad) {
   return ((long)(k >> 5) * Opad + o) * 32 + (k & 31);
 }
+// Position of (row o, column k) in an Opad x Npad FP32 plane for the 16x16x4
+// f32 MFMA: 32-deep k steps of Opad x 32 elements; within a step, with
+// k % 32 = 8q + 4h + e, [h][o][q][e].  In a step lane (r, q) of a 16-row
+// fragment feeds k = 8q + t to MFMA t (t = 0..7), so its 8 k are the two
+// 16-byte vectors h = 0, 1, and the 64 lanes of one vector load read 1 KB of
+// consecutive bytes.
+__host__ __device__ inline long f32_tile(int o, int k, int Opad) {
+  return (long)(k >> 5) * Opad * 32 + (long)((k >> 2) & 1) * Opad * 16 + o * 16 + ((k >> 3) & 3) * 4 + (k & 3);
+}
 
 // Permutation of the four 16-byte k-groups of LDS row r: rows r and r' < 16
 // of a fragment then occupy distinct 16-byte bank groups both within every 8
@@ -419,6 +428,78 @@ __global__ __launch_bounds__(512, 1) void exchange_split_kernel(const double* __
   }
 }
 
+// Exchange epilogue of one wave: c_g = tr f_g and K_g = sum_ab f_ab f_ba from
+// the NB x NB accumulator blocks (block (i, j), i <= j, holds f[16i + r][16j + c]
+// at accumulator position (r, c); block (j, i) its transposed partner), then
+// coul[w, g] and the work-group's sum of K into kpart[w, chunk].  S is the
+// wave's 16 x 17 transpose tile, kred 4 shared partial sums.
+template <int NB>
+__device__ __forceinline__ void exchange_finish(const v4f (&accR)[NB][NB], const v4f (&accI)[NB][NB], int O, int Gn,
+                                                int nchunk, double2* __restrict__ coul, double2* __restrict__ kpart,
+                                                int w, int chunk, int g, bool valid, double2* S, double2* kred) {
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int q4 = lane >> 4;
+  // accumulator element v of lane l sits at (4(l/16) + v, l%16)
+  const int cc = lane & 15;
+  double kr = 0.0, ki = 0.0, cr = 0.0, ci = 0.0;
+  // off-diagonal pairs: block (bi, bj) and, in the same lane and element, its
+  // transposed partner from block (bj, bi); each pair counts twice in the sum
+#pragma unroll
+  for (int bi = 0; bi < NB; ++bi)
+#pragma unroll
+    for (int bj = bi + 1; bj < NB; ++bj)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int gi = bi * 16 + 4 * q4 + v, gj = bj * 16 + cc;
+        if (gi < O && gj < O) {
+          const double xr = accR[bi][bj][v], xi = accI[bi][bj][v];
+          const double yr = accR[bj][bi][v], yi = accI[bj][bi][v];
+          kr += 2.0 * (xr * yr - xi * yi);
+          ki += 2.0 * (xr * yi + xi * yr);
+        }
+      }
+  // diagonal blocks through the wave's transpose tile
+#pragma unroll
+  for (int bi = 0; bi < NB; ++bi) {
+#pragma unroll
+    for (int v = 0; v < 4; ++v) S[(4 * q4 + v) * 17 + cc] = double2{(double)accR[bi][bi][v], (double)accI[bi][bi][v]};
+    wave_lds_sync();
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const int r = 4 * q4 + v;
+      const int gi = bi * 16 + r, gj = bi * 16 + cc;
+      if (gi < O && gj < O) {
+        const double xr = accR[bi][bi][v], xi = accI[bi][bi][v];
+        const double2 y = S[cc * 17 + r];
+        kr += xr * y.x - xi * y.y;
+        ki += xr * y.y + xi * y.x;
+        if (r == cc) {
+          cr += xr;
+          ci += xi;
+        }
+      }
+    }
+    wave_lds_sync();
+  }
+  kr = wave_sum(kr);
+  ki = wave_sum(ki);
+  cr = wave_sum(cr);
+  ci = wave_sum(ci);
+  if (lane == 0) {
+    if (valid) coul[(long)w * Gn + g] = double2{cr, ci};
+    kred[wave] = valid ? double2{kr, ki} : double2{0.0, 0.0};
+  }
+  __syncthreads();
+  if (tid == 0) {
+    double2 s = {0.0, 0.0};
+    for (int i = 0; i < 4; ++i) {
+      s.x += kred[i].x;
+      s.y += kred[i].y;
+    }
+    kpart[(long)w * nchunk + chunk] = s;
+  }
+}
+
 // Split-BF16 x3 exchange on pre-split operands.  rot_chol is held as bf16
 // (hi, lo) planes and each walker's Green's function as (re_hi, re_lo, im_hi,
 // im_lo) planes, rows padded to 16*NB and columns to a multiple of 32 with
@@ -564,66 +645,7 @@ __device__ __forceinline__ void exchange_bf16_body(const v8bf* __restrict__ rcs,
   }
   // no copy into LDS may be in flight when the work-group ends
   copies_done();
-  // accumulator element v of lane l sits at (4(l/16) + v, l%16)
-  const int cc = lane & 15;
-  double kr = 0.0, ki = 0.0, cr = 0.0, ci = 0.0;
-  // off-diagonal pairs: block (bi, bj) and, in the same lane and element, its
-  // transposed partner from block (bj, bi); each pair counts twice in the sum
-#pragma unroll
-  for (int bi = 0; bi < NB; ++bi)
-#pragma unroll
-    for (int bj = bi + 1; bj < NB; ++bj)
-#pragma unroll
-      for (int v = 0; v < 4; ++v) {
-        const int gi = bi * 16 + 4 * q4 + v, gj = bj * 16 + cc;
-        if (gi < O && gj < O) {
-          const double xr = accR[bi][bj][v], xi = accI[bi][bj][v];
-          const double yr = accR[bj][bi][v], yi = accI[bj][bi][v];
-          kr += 2.0 * (xr * yr - xi * yi);
-          ki += 2.0 * (xr * yi + xi * yr);
-        }
-      }
-  // diagonal blocks through the wave's transpose tile
-  double2* S = scr[wave];
-#pragma unroll
-  for (int bi = 0; bi < NB; ++bi) {
-#pragma unroll
-    for (int v = 0; v < 4; ++v) S[(4 * q4 + v) * 17 + cc] = double2{(double)accR[bi][bi][v], (double)accI[bi][bi][v]};
-    wave_lds_sync();
-#pragma unroll
-    for (int v = 0; v < 4; ++v) {
-      const int r = 4 * q4 + v;
-      const int gi = bi * 16 + r, gj = bi * 16 + cc;
-      if (gi < O && gj < O) {
-        const double xr = accR[bi][bi][v], xi = accI[bi][bi][v];
-        const double2 y = S[cc * 17 + r];
-        kr += xr * y.x - xi * y.y;
-        ki += xr * y.y + xi * y.x;
-        if (r == cc) {
-          cr += xr;
-          ci += xi;
-        }
-      }
-    }
-    wave_lds_sync();
-  }
-  kr = wave_sum(kr);
-  ki = wave_sum(ki);
-  cr = wave_sum(cr);
-  ci = wave_sum(ci);
-  if (lane == 0) {
-    if (valid) coul[(long)w * Gn + g] = double2{cr, ci};
-    kred[wave] = valid ? double2{kr, ki} : double2{0.0, 0.0};
-  }
-  __syncthreads();
-  if (tid == 0) {
-    double2 s = {0.0, 0.0};
-    for (int i = 0; i < 4; ++i) {
-      s.x += kred[i].x;
-      s.y += kred[i].y;
-    }
-    kpart[(long)w * nchunk + chunk] = s;
-  }
+  exchange_finish<NB>(accR, accI, O, Gn, nchunk, coul, kpart, w, chunk, g, valid, scr[wave], kred);
 }
 
 template <int NB>
@@ -661,6 +683,162 @@ __global__ __launch_bounds__(256, 2) void exchange_bf16_grouped_kernel(const Exc
   if (chunk >= e.nchunk) return;  // uniform for the whole work-group
   exchange_bf16_body<NB>(reinterpret_cast<const v8bf*>(e.rcs), reinterpret_cast<const v8bf*>(e.gs), e.O, e.Opad,
                          e.Npad, e.Gn, e.nchunk, e.coul, e.kpart, w, chunk);
+}
+
+// FP32 exchange on pre-converted operands: rot_chol as one FP32 plane per
+// gamma and each walker's Green's function as (re, im) FP32 planes, both in
+// the f32_tile layout, rows padded to 16 NB and columns to a multiple of 32
+// with zeros.  The work-group layout is the split-BF16 kernel's: four waves,
+// one gamma each, share one walker, whose Green's-function planes for the
+// next 32-deep step are copied into LDS by global_load_lds (double-buffered,
+// one barrier per step), in MFMA lane order, so each fragment read is 1 KB of
+// consecutive LDS bytes; each wave reads its rot_chol fragments into registers
+// one step ahead.  A step is 8 MFMAs (16x16x4 f32) per block pair and part;
+// for each column block j the loop runs t outermost, so the 2 NB products
+// between two updates of one accumulator never wait on each other.  Blocks
+// below the diagonal take the operands swapped, as in the split-BF16 kernel.
+__device__ __forceinline__ float f4c(const float4& v, int e) {
+  return e == 0 ? v.x : e == 1 ? v.y : e == 2 ? v.z : v.w;
+}
+
+template <int NB>
+__device__ __forceinline__ void exchange_f32_body(const float4* __restrict__ rct, const float4* __restrict__ gt,
+                                                  int O, int Npad, int Gn, int nchunk, double2* __restrict__ coul,
+                                                  double2* __restrict__ kpart, int w, int chunk) {
+  constexpr int ROWS = NB * 16;
+  constexpr int STEP = ROWS * 8;           // float4 of one 32-deep step of a plane
+  constexpr int PER = 2 * STEP;            // float4 of one step's stage: re and im planes
+  static_assert(PER % 256 == 0, "stage must split into whole 64-lane copies per wave");
+  constexpr int CPW = PER / 256;           // 64-lane copies per wave per step
+  __shared__ float4 bsh[2][PER];
+  __shared__ double2 scr[4][16 * 17];
+  __shared__ double2 kred[4];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int g = chunk * 4 + wave;
+  const bool valid = g < Gn;
+  const int gg = valid ? g : Gn - 1;
+  const long plane = (long)ROWS * Npad / 4;  // float4 per plane
+  const int nsteps = Npad / 32;
+  const int r16 = lane & 15, q4 = lane >> 4;
+  // fragment (block b, half h) of step kv: kv STEP + h 4 ROWS + 64 b + 4 r + q
+  const float4* A = rct + (long)gg * plane + r16 * 4 + q4;
+  const float4* Bw = gt + (long)w * 2 * plane;
+  // LDS slot 64 u + l, u = (plane p, half h, block j) = (2 p + h) NB + j, holds
+  // the float4 that lane l = 16 q + c of that fragment reads: row 16 j + c,
+  // vector q.  Copy t of this wave fills slots [64 (4 t + wave), +64).
+  long src[CPW];
+#pragma unroll
+  for (int t = 0; t < CPW; ++t) {
+    const int u = 4 * t + wave;
+    const int j = u % NB, h = (u / NB) & 1, p = u / (2 * NB);
+    src[t] = p * plane + h * 4 * ROWS + (16 * j + r16) * 4 + q4;
+  }
+  auto copy = [&](int kv, int buf) {
+#pragma unroll
+    for (int t = 0; t < CPW; ++t)
+      __builtin_amdgcn_global_load_lds(static_cast<const void*>(Bw + src[t] + (long)kv * STEP),
+                                       (__attribute__((address_space(3))) void*)&bsh[buf][64 * (4 * t + wave)], 16,
+                                       0, 0);
+  };
+  // s_waitcnt vmcnt(0): this wave's copies and rot_chol fragments have landed
+  auto copies_done = [] { __builtin_amdgcn_s_waitcnt(0x0F70); };
+
+  v4f accR[NB][NB], accI[NB][NB];
+#pragma unroll
+  for (int i = 0; i < NB; ++i)
+#pragma unroll
+    for (int j = 0; j < NB; ++j) {
+      accR[i][j] = v4f{0.f, 0.f, 0.f, 0.f};
+      accI[i][j] = v4f{0.f, 0.f, 0.f, 0.f};
+    }
+  float4 a[NB][2], an[NB][2];
+#pragma unroll
+  for (int b = 0; b < NB; ++b) {
+    a[b][0] = A[b * 64];
+    a[b][1] = A[4 * ROWS + b * 64];
+  }
+  copy(0, 0);
+  int buf = 0;
+  for (int kv = 0; kv < nsteps; ++kv) {
+    copies_done();
+    __syncthreads();
+    // the last step fetches its own again (into the idle buffer and
+    // registers), which keeps the loop free of branches
+    const int kn = kv + 1 < nsteps ? kv + 1 : kv;
+    copy(kn, buf ^ 1);
+#pragma unroll
+    for (int b = 0; b < NB; ++b) {
+      an[b][0] = A[(long)kn * STEP + b * 64];
+      an[b][1] = A[(long)kn * STEP + 4 * ROWS + b * 64];
+    }
+    const float4* B = bsh[buf] + lane;
+#pragma unroll
+    for (int j = 0; j < NB; ++j) {
+      const float4 br0 = B[(0 * NB + j) * 64], br1 = B[(1 * NB + j) * 64];
+      const float4 bi0 = B[(2 * NB + j) * 64], bi1 = B[(3 * NB + j) * 64];
+#pragma unroll
+      for (int t = 0; t < 8; ++t) {
+        const float br = f4c(t < 4 ? br0 : br1, t & 3);
+        const float bi = f4c(t < 4 ? bi0 : bi1, t & 3);
+#pragma unroll
+        for (int i = 0; i < NB; ++i) {
+          const float av = f4c(a[i][t >> 2], t & 3);
+          if (i <= j) {
+            accR[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(av, br, accR[i][j], 0, 0, 0);
+            accI[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(av, bi, accI[i][j], 0, 0, 0);
+          } else {
+            accR[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(br, av, accR[i][j], 0, 0, 0);
+            accI[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(bi, av, accI[i][j], 0, 0, 0);
+          }
+        }
+      }
+    }
+#pragma unroll
+    for (int b = 0; b < NB; ++b) {
+      a[b][0] = an[b][0];
+      a[b][1] = an[b][1];
+    }
+    buf ^= 1;
+  }
+  // no copy into LDS may be in flight when the work-group ends
+  copies_done();
+  exchange_finish<NB>(accR, accI, O, Gn, nchunk, coul, kpart, w, chunk, g, valid, scr[wave], kred);
+}
+
+template <int NB>
+__global__ __launch_bounds__(256, 2) void exchange_f32_kernel(const float4* __restrict__ rct,
+                                                              const float4* __restrict__ gt, int O, int Npad, int Gn,
+                                                              int W, int nchunk, int cpg, long T8,
+                                                              double2* __restrict__ coul,
+                                                              double2* __restrict__ kpart) {
+  const long bid = blockIdx.x;
+  const long L = (bid & 7) * T8 + (bid >> 3);
+  const long per_group = (long)W * cpg;
+  const long grp = L / per_group;
+  const long rem = L - grp * per_group;
+  const int w = (int)(rem / cpg);
+  const int chunk = (int)(grp * cpg + rem % cpg);
+  if (chunk >= nchunk) return;  // uniform for the whole work-group
+  exchange_f32_body<NB>(rct, gt, O, Npad, Gn, nchunk, coul, kpart, w, chunk);
+}
+
+// out[r][c][f32_tile(o, k)] (o < Opad, k < Npad, zero padded) = float of
+// src[r][o][k] (o < O, k < N; real, or interleaved complex with c = re, im).
+__global__ void split_f32_kernel(const double* __restrict__ src, int cplx, int O, int N, int Opad, int Npad,
+                                 long rows, float* __restrict__ out) {
+  const long per = (long)Opad * Npad;
+  const long idx = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (idx >= rows * per) return;
+  const long r = idx / per;
+  const int o = (int)((idx % per) / Npad), k = (int)(idx % Npad);
+  const bool in = o < O && k < N;
+  const int np = cplx ? 2 : 1;
+  float* base = out + r * np * per + f32_tile(o, k, Opad);
+  for (int c = 0; c < np; ++c) {
+    double x = 0.0;
+    if (in) x = cplx ? src[2 * ((r * O + o) * (long)N + k) + c] : src[(r * O + o) * (long)N + k];
+    base[c * per] = (float)x;
+  }
 }
 
 // out[r][plane][tile(o, k)] (o < Opad, k < Npad, zero padded) from
@@ -948,6 +1126,43 @@ int k_exchange_bf16(const uint16_t* rcs, const uint16_t* gs, int O, int N, int G
   }
 #undef EXB
   return launch_status("exchange_bf16_kernel");
+}
+
+int k_split_f32(const double* src, bool cplx, int O, int N, long rows, float* out, hipStream_t s) {
+  const int Opad = bf16_opad(O), Npad = bf16_npad(N);
+  const long n = rows * Opad * (long)Npad;
+  if (n <= 0) return AFQ_OK;
+  hipLaunchKernelGGL(split_f32_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, src, cplx ? 1 : 0, O, N,
+                     Opad, Npad, rows, out);
+  return launch_status("split_f32_kernel");
+}
+
+int k_exchange_f32(const float* rct, const float* gt, int O, int N, int Gn, int W, double2* coul, double2* kpart,
+                   int* nchunk_out, hipStream_t s) {
+  if (O < 1 || O > 64) return fail(AFQ_E_UNSUPPORTED, "exchange kernel supports 1 <= nocc <= 64");
+  const int nchunk = exchange_nchunk(Gn);
+  *nchunk_out = nchunk;
+  const int Opad = bf16_opad(O), Npad = bf16_npad(N);
+  const long chunk_bytes = 4L * Opad * Npad * 4;
+  const int cpg = (int)std::max(1L, std::min((long)nchunk, (kGroupSliceBytes) / std::max(1L, chunk_bytes)));
+  const long ngroups = (nchunk + cpg - 1) / cpg;
+  const long T = ngroups * cpg * (long)W;
+  const long T8 = (T + 7) / 8;
+  const long grid = T8 * 8;
+  if (grid > 0x7fffffffL) return fail(AFQ_E_UNSUPPORTED, "exchange grid too large");
+  const float4* a = reinterpret_cast<const float4*>(rct);
+  const float4* b = reinterpret_cast<const float4*>(gt);
+#define EXF(NB)                                                                                                   \
+  hipLaunchKernelGGL(exchange_f32_kernel<NB>, dim3((unsigned)grid), dim3(256), 0, s, a, b, O, Npad, Gn, W, nchunk, \
+                     cpg, T8, coul, kpart)
+  switch (Opad / 16) {
+    case 1: EXF(1); break;
+    case 2: EXF(2); break;
+    case 3: EXF(3); break;
+    default: EXF(4); break;
+  }
+#undef EXF
+  return launch_status("exchange_f32_kernel");
 }
 
 }  // namespace afq

@@ -38,6 +38,12 @@ int bf16_npad(int N);
 int k_split_bf16(const double* src, bool cplx, int O, int N, long rows, bool ilv, uint16_t* out, hipStream_t s);
 int k_exchange_bf16(const uint16_t* rcs, const uint16_t* gs, int O, int N, int Gn, int W, double2* coul,
                     double2* kpart, int* nchunk_out, hipStream_t s);
+// FP32 operands of the exchange: rows x (O, N) real (1 plane) or complex (re,
+// im planes) doubles -> FP32 planes of bf16_opad(O) x bf16_npad(N), zero
+// padded, in the FP32 exchange's tiled layout.
+int k_split_f32(const double* src, bool cplx, int O, int N, long rows, float* out, hipStream_t s);
+int k_exchange_f32(const float* rct, const float* gt, int O, int N, int Gn, int W, double2* coul, double2* kpart,
+                   int* nchunk_out, hipStream_t s);
 int k_to_f32(const double* src, float* dst, long n, hipStream_t s);
 // Symmetric Cholesky vectors L_g = L_g^T: the VHS product runs over the 16 x 16
 // orbital blocks (ib, jb) with ib >= jb only, and the Taylor products read that
