@@ -9,23 +9,23 @@
 // are reduced in FP64 in every mode.
 //
 // Arithmetic of f:
-//   F64    : v_mfma_f64_16x16x4f64
-//   F32    : v_mfma_f32_16x16x4f32 (rot_chol read as FP32, G rounded to FP32)
+//   FP64   : v_mfma_f64_16x16x4f64 (exchange_kernel, exchange_split_kernel)
+//   FP32   : v_mfma_f32_16x16x4f32 on pre-converted FP32 planes
+//            (exchange_f32_kernel)
 //   BF16X3 : v_mfma_f32_16x16x32_bf16 on pre-split hi/lo bf16 planes,
 //            hi*hi + hi*lo + lo*hi (exchange_bf16_kernel)
 //
-// Fragments go straight from global memory to registers in MFMA layout.  The
-// K dimension (orbital index p) is permuted within each chunk so that every
-// lane reads consecutive elements of one row: for the 4-deep MFMAs lane l of
-// slab s reads p = c + KS*(l/16) + s (KS slabs per chunk); the 32-deep bf16
-// MFMA takes 8 consecutive k per lane.  In the F64/F32 kernel the next chunk
-// is loaded while the current one is multiplied.
+// In the FP64 kernels fragments go straight from global memory to registers
+// in MFMA layout.  The K dimension (orbital index p) is permuted within each
+// chunk so that every lane reads consecutive elements of one row: lane l of
+// slab s reads p = c + KS*(l/16) + s (KS slabs per chunk), and the next chunk
+// is loaded while the current one is multiplied.  The FP32 and bf16 kernels
+// take 8 consecutive k per lane from planes laid out for it.
 //
 // Work-group order is XCD-aware: work-groups are handed to the eight XCDs
 // round-robin, so the logical index L = xcd * (T/8) + (bid/8) gives each XCD a
 // contiguous range of the (gamma-group, walker, chunk) order, keeping one
 // rot_chol group and the active walkers' G in that XCD's L2.
-#include <type_traits>
 
 #include "kernels.h"
 #include "group.h"
@@ -35,7 +35,6 @@ typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
 namespace afq {
 namespace {
 
-enum Mode { F64 = 0, F32 = 1 };
 
 // Bytes of rot_chol a gamma group spans (its chunks run back to back on one
 // XCD, so the slice stays in that XCD's L2 while all walkers pass over it;
@@ -80,15 +79,13 @@ __device__ inline double wave_sum(double v) {
   return v;
 }
 
-// Per-mode fragment of one chunk: NB row blocks of A (rot_chol rows) and of
-// B (Green's function rows), KE elements per lane.
-template <int MODE, int NB, int KS>
+// FP64 fragment of one chunk: NB row blocks of A (rot_chol rows) and of B
+// (Green's function rows), KS elements per lane.
+template <int NB, int KS>
 struct Frag {
-  static constexpr int KE = KS;
-  using R = std::conditional_t<MODE == F64, double, float>;
-  R a[NB][KE];
-  R br[NB][KE];
-  R bi[NB][KE];
+  double a[NB][KS];
+  double br[NB][KS];
+  double bi[NB][KS];
 };
 
 // One chunk as loaded: raw values at clamped (always valid) addresses, rows
@@ -99,14 +96,14 @@ struct Frag {
 // MFMAs of the current chunk are issued, so the next chunk's loads overlap them
 // (masking or converting at load time made the compiler wait for every load
 // first).
-template <int NB, int KS, typename TR>
+template <int NB, int KS>
 struct Raw {
-  TR a[NB][KS];
+  double a[NB][KS];
   double2 b[NB][KS];
 };
 
-template <int NB, int KS, typename TR>
-__device__ inline void load_raw(Raw<NB, KS, TR>& X, const TR* __restrict__ A, const double2* __restrict__ B, int O,
+template <int NB, int KS>
+__device__ inline void load_raw(Raw<NB, KS>& X, const double* __restrict__ A, const double2* __restrict__ B, int O,
                                 int N, int c, int lane) {
   const int kb = c + KS * (lane >> 4);
   const int r16 = lane & 15;
@@ -122,29 +119,29 @@ __device__ inline void load_raw(Raw<NB, KS, TR>& X, const TR* __restrict__ A, co
   }
 }
 
-template <int MODE, int NB, int KS, typename TR>
-__device__ inline void take(Frag<MODE, NB, KS>& F, const Raw<NB, KS, TR>& X, int O, int N, int c, int lane) {
-  using R = typename Frag<MODE, NB, KS>::R;
+template <int NB, int KS>
+__device__ inline void take(Frag<NB, KS>& F, const Raw<NB, KS>& X, int O, int N, int c, int lane) {
   const int kb = c + KS * (lane >> 4);
 #pragma unroll
   for (int bi = 0; bi < NB; ++bi) {
     const bool rok = bi * 16 + (lane & 15) < O;
 #pragma unroll
     for (int s = 0; s < KS; ++s) {
-      F.a[bi][s] = (rok && kb + s < N) ? (R)X.a[bi][s] : (R)0;
-      F.br[bi][s] = (R)X.b[bi][s].x;
-      F.bi[bi][s] = (R)X.b[bi][s].y;
+      F.a[bi][s] = (rok && kb + s < N) ? X.a[bi][s] : 0.0;
+      F.br[bi][s] = X.b[bi][s].x;
+      F.bi[bi][s] = X.b[bi][s].y;
     }
   }
 }
 
-// NB <= 3, and F32 at NB = 4, fit two work-groups per CU (<= 256 VGPRs per
-// lane); F64 at NB = 4 needs the whole file (exchange_split_kernel serves it).
-template <int MODE, int NB, int KS, typename TR>
-__global__ __launch_bounds__(256, (NB <= 3 || MODE == F32 ? 2 : 1)) void exchange_kernel(const TR* __restrict__ rc, const double2* __restrict__ Gm, int O,
+// NB <= 3 fits two work-groups per CU (<= 256 VGPRs per lane); NB = 4 would
+// need the whole register file (exchange_split_kernel serves 48 < O <= 64).
+template <int NB, int KS>
+__global__ __launch_bounds__(256, 2) void exchange_kernel(const double* __restrict__ rc, const double2* __restrict__ Gm, int O,
                                                        int N, int Gn, int W, int nchunk, int cpg, long T8,
                                                        double2* __restrict__ coul, double2* __restrict__ kpart) {
-  using Acc = std::conditional_t<MODE == F64, v4d, v4f>;
+  static_assert(NB <= 3, "48 < O <= 64 runs in exchange_split_kernel");
+  using Acc = v4d;
   __shared__ double2 scr[4][16 * 17];
   __shared__ double2 kred[4];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -162,7 +159,7 @@ __global__ __launch_bounds__(256, (NB <= 3 || MODE == F32 ? 2 : 1)) void exchang
   const bool valid = g < Gn;
   const int gg = valid ? g : Gn - 1;
 
-  const TR* A = rc + (long)gg * O * N;
+  const double* A = rc + (long)gg * O * N;
   const double2* B = Gm + (long)w * O * N;
 
   Acc accR[NB][NB], accI[NB][NB];
@@ -174,33 +171,26 @@ __global__ __launch_bounds__(256, (NB <= 3 || MODE == F32 ? 2 : 1)) void exchang
       accI[i][j] = Acc{0, 0, 0, 0};
     }
 
-  using Fr = Frag<MODE, NB, KS>;
-  constexpr int CW = 4 * Fr::KE;
-  Fr cur;
-  Raw<NB, KS, TR> nxt;
-  load_raw<NB, KS, TR>(nxt, A, B, O, N, 0, lane);
-  take<MODE, NB, KS, TR>(cur, nxt, O, N, 0, lane);
+  constexpr int CW = 4 * KS;
+  Frag<NB, KS> cur;
+  Raw<NB, KS> nxt;
+  load_raw<NB, KS>(nxt, A, B, O, N, 0, lane);
+  take<NB, KS>(cur, nxt, O, N, 0, lane);
   for (int c = 0; c < N; c += CW) {
-    if (c + CW < N) load_raw<NB, KS, TR>(nxt, A, B, O, N, c + CW, lane);
+    if (c + CW < N) load_raw<NB, KS>(nxt, A, B, O, N, c + CW, lane);
 #pragma unroll
     for (int s = 0; s < KS; ++s)
 #pragma unroll
       for (int bi = 0; bi < NB; ++bi)
 #pragma unroll
         for (int bj = 0; bj < NB; ++bj) {
-          if constexpr (MODE == F64) {
-            accR[bi][bj] = __builtin_amdgcn_mfma_f64_16x16x4f64(cur.a[bi][s], cur.br[bj][s], accR[bi][bj], 0, 0, 0);
-            accI[bi][bj] = __builtin_amdgcn_mfma_f64_16x16x4f64(cur.a[bi][s], cur.bi[bj][s], accI[bi][bj], 0, 0, 0);
-          } else {
-            accR[bi][bj] = __builtin_amdgcn_mfma_f32_16x16x4f32(cur.a[bi][s], cur.br[bj][s], accR[bi][bj], 0, 0, 0);
-            accI[bi][bj] = __builtin_amdgcn_mfma_f32_16x16x4f32(cur.a[bi][s], cur.bi[bj][s], accI[bi][bj], 0, 0, 0);
-          }
+          accR[bi][bj] = __builtin_amdgcn_mfma_f64_16x16x4f64(cur.a[bi][s], cur.br[bj][s], accR[bi][bj], 0, 0, 0);
+          accI[bi][bj] = __builtin_amdgcn_mfma_f64_16x16x4f64(cur.a[bi][s], cur.bi[bj][s], accI[bi][bj], 0, 0, 0);
         }
-    if (c + CW < N) take<MODE, NB, KS, TR>(cur, nxt, O, N, c + CW, lane);
+    if (c + CW < N) take<NB, KS>(cur, nxt, O, N, c + CW, lane);
   }
 
-  // element v of lane l sits at (row, col) = (l/16 + 4v, l%16) for f64 and
-  // (4(l/16) + v, l%16) for f32 accumulators
+  // element v of lane l sits at (row, col) = (l/16 + 4v, l%16)
   const int cc = lane & 15;
   double kr = 0.0, ki = 0.0, cr = 0.0, ci = 0.0;
   double2* S = scr[wave];
@@ -210,14 +200,14 @@ __global__ __launch_bounds__(256, (NB <= 3 || MODE == F32 ? 2 : 1)) void exchang
     for (int bj = bi; bj < NB; ++bj) {
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
-        const int r = MODE == F64 ? (lane >> 4) + 4 * v : 4 * (lane >> 4) + v;
+        const int r = (lane >> 4) + 4 * v;
         S[r * 17 + cc] = double2{(double)accR[bj][bi][v], (double)accI[bj][bi][v]};
       }
       wave_lds_sync();
       const double wgt = (bi == bj) ? 1.0 : 2.0;
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
-        const int r = MODE == F64 ? (lane >> 4) + 4 * v : 4 * (lane >> 4) + v;
+        const int r = (lane >> 4) + 4 * v;
         const int gi = bi * 16 + r, gj = bj * 16 + cc;
         if (gi < O && gj < O) {
           const double xr = accR[bi][bj][v], xi = accI[bi][bj][v];
@@ -937,19 +927,19 @@ __global__ __launch_bounds__(256) void exchange_wide_reduce_kernel(const double2
   if (tid == 0) kpart[(long)w * nchunk + g0 / 4 + c] = double2{ksr, ksi};
 }
 
-template <int MODE, int NB, int KS, typename TR>
-int launch_nb(const TR* rc, const double2* Gm, int O, int N, int Gn, int W, int nchunk, double2* coul, double2* kpart,
-              hipStream_t s) {
+template <int NB, int KS>
+int launch_nb(const double* rc, const double2* Gm, int O, int N, int Gn, int W, int nchunk, double2* coul,
+              double2* kpart, hipStream_t s) {
   // gamma chunks per group: a group's rot_chol slice ~2.5 MB of L2
-  const long chunk_bytes = 4L * O * N * (long)sizeof(TR);
+  const long chunk_bytes = 4L * O * N * (long)sizeof(double);
   const int cpg = (int)std::max(1L, std::min((long)nchunk, (kGroupSliceBytes) / std::max(1L, chunk_bytes)));
   const long ngroups = (nchunk + cpg - 1) / cpg;
   const long T = ngroups * cpg * (long)W;
   const long T8 = (T + 7) / 8;
   const long grid = T8 * 8;
   if (grid > 0x7fffffffL) return fail(AFQ_E_UNSUPPORTED, "exchange grid too large");
-  hipLaunchKernelGGL((exchange_kernel<MODE, NB, KS, TR>), dim3((unsigned)grid), dim3(256), 0, s, rc, Gm, O, N, Gn, W,
-                     nchunk, cpg, T8, coul, kpart);
+  hipLaunchKernelGGL((exchange_kernel<NB, KS>), dim3((unsigned)grid), dim3(256), 0, s, rc, Gm, O, N, Gn, W, nchunk,
+                     cpg, T8, coul, kpart);
   return launch_status("exchange_kernel");
 }
 
@@ -976,17 +966,14 @@ int launch_split(const double* rc, const double2* Gm, int O, int N, int Gn, int 
   return launch_status("exchange_split_kernel");
 }
 
-template <int MODE, typename TR>
-int launch_mode(const TR* rc, const double2* Gm, int O, int N, int Gn, int W, int nchunk, double2* coul,
-                double2* kpart, hipStream_t s) {
-  // chunk widths chosen so the fragments fit next to the accumulators: NB = 3
-  // uses 8-wide chunks (two work-groups per CU), NB = 4 (49 <= O <= 64) 4-wide
-  // ones in FP64 and 8-wide ones in F32
+// O <= 48; chunk widths chosen so the fragments fit next to the accumulators
+// (NB = 3: 8-wide chunks, two work-groups per CU)
+int launch_f64(const double* rc, const double2* Gm, int O, int N, int Gn, int W, int nchunk, double2* coul,
+               double2* kpart, hipStream_t s) {
   switch ((O + 15) / 16) {
-    case 1: return launch_nb<MODE, 1, 4, TR>(rc, Gm, O, N, Gn, W, nchunk, coul, kpart, s);
-    case 2: return launch_nb<MODE, 2, 4, TR>(rc, Gm, O, N, Gn, W, nchunk, coul, kpart, s);
-    case 3: return launch_nb<MODE, 3, 2, TR>(rc, Gm, O, N, Gn, W, nchunk, coul, kpart, s);
-    default: return launch_nb<MODE, 4, (MODE == F64 ? 1 : 2), TR>(rc, Gm, O, N, Gn, W, nchunk, coul, kpart, s);
+    case 1: return launch_nb<1, 4>(rc, Gm, O, N, Gn, W, nchunk, coul, kpart, s);
+    case 2: return launch_nb<2, 4>(rc, Gm, O, N, Gn, W, nchunk, coul, kpart, s);
+    default: return launch_nb<3, 2>(rc, Gm, O, N, Gn, W, nchunk, coul, kpart, s);
   }
 }
 
@@ -1001,7 +988,7 @@ int k_exchange(const double* rc, const double2* Gm, int O, int N, int Gn, int W,
   *nchunk_out = nchunk;
   // 48 < O <= 64: two waves per f-block (the 64 x 64 block alone would take 256 registers)
   if (O > 48) return launch_split<2>(rc, Gm, O, N, Gn, W, nchunk, coul, kpart, s);
-  return launch_mode<F64, double>(rc, Gm, O, N, Gn, W, nchunk, coul, kpart, s);
+  return launch_f64(rc, Gm, O, N, Gn, W, nchunk, coul, kpart, s);
 }
 
 long exchange_wide_ws_elems(int O, int Gn, int W) {
@@ -1046,14 +1033,6 @@ int k_exchange_wide(const double* rc, const double2* Gm, int O, int N, int Gn, i
     AFQ_TRY(launch_status("exchange_wide_reduce_kernel"));
   }
   return AFQ_OK;
-}
-
-int k_exchange_lp(const float* rc, const double2* Gm, int O, int N, int Gn, int W, double2* coul, double2* kpart,
-                  int* nchunk_out, hipStream_t s) {
-  if (O < 1 || O > 64) return fail(AFQ_E_UNSUPPORTED, "exchange kernel supports 1 <= nocc <= 64");
-  const int nchunk = exchange_nchunk(Gn);
-  *nchunk_out = nchunk;
-  return launch_mode<F32, float>(rc, Gm, O, N, Gn, W, nchunk, coul, kpart, s);
 }
 
 int bf16_opad(int O) { return bf16_opad_d(O); }

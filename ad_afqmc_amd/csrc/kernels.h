@@ -19,8 +19,6 @@ int k_ene1(const double2* G0, const double* rh0, int n0, const double2* G1, cons
 int exchange_nchunk(int Gn);
 int k_exchange(const double* rc, const double2* Gm, int O, int N, int Gn, int W, double2* coul, double2* kpart,
                int* nchunk_out, hipStream_t s);
-int k_exchange_lp(const float* rc, const double2* Gm, int O, int N, int Gn, int W, double2* coul, double2* kpart,
-                  int* nchunk_out, hipStream_t s);
 // nocc > 64 (any precision, FP64 arithmetic): f_g of a pass of g's for all
 // walkers by the batched FP64 product into ws, then c_g and the K_g of each
 // 4-g chunk; ws holds exchange_wide_ws_elems(O, Gn, W) complex elements.
