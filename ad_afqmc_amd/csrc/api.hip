@@ -134,12 +134,13 @@ int green_pipeline(afq_job* j, const void* up, const void* dn, bool want_inv, bo
   const int N = j->N, W = j->W;
   if (!have_inv) {
     DetInvSet sets[2];
+    Gemm gs[2];
     for (int s = 0; s < j->nspin_meas; ++s) {
       const int O = j->nocc[s];
       const double2* P;
       long ld;
       panel(j, s, up, dn, &P, &ld);
-      Gemm g;
+      Gemm& g = gs[s];
       g.M = O;
       g.N = O;
       g.K = N;
@@ -155,24 +156,24 @@ int green_pipeline(afq_job* j, const void* up, const void* dn, bool want_inv, bo
       g.C = j->S[s];
       g.ldc = O;
       g.sC = (long)O * O;
-      {
-        ProfScope ps(j, AFQ_K_OVLP_GEMM, st);
-        AFQ_TRY(gemm_f64(g, st));
-      }
       // above n = 64 a determinant alone is factored in place in S
       sets[s] = DetInvSet{j->S[s], (long)O * O, O, j->det[s], want_inv ? j->Sinv[s] : nullptr, (long)O * O, j->S[s]};
+    }
+    {
+      ProfScope ps(j, AFQ_K_OVLP_GEMM, st);
+      AFQ_TRY(j->nspin_meas == 2 ? gemm_f64_pair(gs[0], gs[1], st) : gemm_f64(gs[0], st));
     }
     ProfScope ps(j, AFQ_K_DETINV, st);
     AFQ_TRY(batched_det_inv_sets(sets, j->nspin_meas, W, st));
   }
   if (!want_green) return AFQ_OK;
+  Gemm hs[2];
   for (int s = 0; s < j->nspin_meas; ++s) {
     const int O = j->nocc[s];
     const double2* P;
     long ld;
     panel(j, s, up, dn, &P, &ld);
-    ProfScope ps(j, AFQ_K_GREEN, st);
-    Gemm h;
+    Gemm& h = hs[s];
     h.M = O;
     h.N = N;
     h.K = O;
@@ -190,9 +191,9 @@ int green_pipeline(afq_job* j, const void* up, const void* dn, bool want_inv, bo
     h.C = j->green[s];
     h.ldc = N;
     h.sC = (long)O * N;
-    AFQ_TRY(gemm_f64(h, st));
   }
-  return AFQ_OK;
+  ProfScope ps(j, AFQ_K_GREEN, st);
+  return j->nspin_meas == 2 ? gemm_f64_pair(hs[0], hs[1], st) : gemm_f64(hs[0], st);
 }
 
 int overlap_from_dets(afq_job* j, double2* out, hipStream_t st) {

@@ -353,6 +353,25 @@ __global__ __launch_bounds__(64 * WM * WN) void gemm_f64_kernel(Gemm g, int tile
   gemm_f64_body<AC, BC, AT, BT, WM, WN, TM, TN, PART>(g, tiles_m, tiles_n, tile, b, (int)blockIdx.z, kchunk, ws);
 }
 
+// Two batched products with the same operand form and tile configuration in
+// one launch (both spins of a step): work-groups [0, items0) run g0 and the
+// rest g1, each in gemm_f64_kernel's batched XCD-aware order (items0 is a
+// multiple of 8, so the order holds for g1 too).  No K split.
+template <bool AC, bool BC, bool AT, bool BT, int WM, int WN, int TM, int TN>
+__global__ __launch_bounds__(64 * WM * WN) void gemm_f64_pair_kernel(Gemm g0, Gemm g1, int tm0, int tn0, int tm1,
+                                                                      int tn1, long items0) {
+  const long item = blockIdx.x;
+  const bool second = item >= items0;  // uniform per work-group
+  const Gemm g = second ? g1 : g0;
+  const int tm = second ? tm1 : tm0, tn = second ? tn1 : tn0;
+  const long local = second ? item - items0 : item;
+  const long tiles = (long)tm * tn;
+  const long p = local >> 3;
+  const long b = (local & 7) + 8 * (p / tiles);
+  if (b >= g.batch) return;  // uniform per work-group
+  gemm_f64_body<AC, BC, AT, BT, WM, WN, TM, TN>(g, tm, tn, p % tiles, b, 0, g.K, nullptr);
+}
+
 // Grouped batched products: work-group i belongs to the job j with
 // prefix[j] <= i < prefix[j + 1] (a multiple of 8); within it the local index
 // is mapped as gemm_f64_kernel's xcd_remap == 2: XCD local % 8 takes the batch
@@ -612,6 +631,45 @@ int dispatch_t(const Gemm& g, hipStream_t s) {
   return g.b_trans ? dispatch_shape<AC, BC, false, true>(g, s) : dispatch_shape<AC, BC, false, false>(g, s);
 }
 
+// The tile configuration dispatch_shape picks for a batched product without
+// K split, packed or conjugated A: 0 = 2x2x2x2, 1..4 = 4x1x1xTN, 5 = 4x4x1x2,
+// -1 = a shape served otherwise (the wide VHS tiles, the walker-panel kernel).
+int batched_config(const Gemm& g) {
+  if (!g.b_cplx && !g.b_trans && g.batch == 1 && g.N >= 4096 && g.M >= 64 && g.K >= 64) return -1;
+  if (!g.a_trans && !g.b_trans && g.b_cplx && g.batch > 1 && g.N > 32 && g.N <= 48 && g.M > 64) return -1;
+  if (g.N <= 16) return 1;
+  if (g.N <= 32) return 2;
+  if (g.N <= 48) return 3;
+  if (g.N <= 64 && g.M > 64) return 4;
+  if (g.N <= 128 && g.M > 64 && g.batch > 1) return 5;
+  return 0;
+}
+
+template <bool AC, bool BC, bool AT, bool BT, int WM, int WN, int TM, int TN>
+int launch_pair(const Gemm& g0, const Gemm& g1, hipStream_t s) {
+  using C = Cfg<AC, BC, AT, BT, WM, WN, TM, TN>;
+  const int tm0 = (g0.M + C::BM - 1) / C::BM, tn0 = (g0.N + C::BN - 1) / C::BN;
+  const int tm1 = (g1.M + C::BM - 1) / C::BM, tn1 = (g1.N + C::BN - 1) / C::BN;
+  const long items0 = (long)tm0 * tn0 * ((g0.batch + 7) / 8 * 8);
+  const long items1 = (long)tm1 * tn1 * ((g1.batch + 7) / 8 * 8);
+  if (items0 + items1 > 0x7fffffffL) return fail(AFQ_E_UNSUPPORTED, "gemm_f64_pair: grid too large");
+  hipLaunchKernelGGL((gemm_f64_pair_kernel<AC, BC, AT, BT, WM, WN, TM, TN>), dim3((unsigned)(items0 + items1)),
+                     dim3(C::NT), 0, s, g0, g1, tm0, tn0, tm1, tn1, items0);
+  return launch_status("gemm_f64_pair_kernel");
+}
+
+template <bool AC, bool BC, bool AT, bool BT>
+int dispatch_pair(const Gemm& g0, const Gemm& g1, int cfg, hipStream_t s) {
+  switch (cfg) {
+    case 1: return launch_pair<AC, BC, AT, BT, 4, 1, 1, 1>(g0, g1, s);
+    case 2: return launch_pair<AC, BC, AT, BT, 4, 1, 1, 2>(g0, g1, s);
+    case 3: return launch_pair<AC, BC, AT, BT, 4, 1, 1, 3>(g0, g1, s);
+    case 4: return launch_pair<AC, BC, AT, BT, 4, 1, 1, 4>(g0, g1, s);
+    case 5: return launch_pair<AC, BC, AT, BT, 4, 4, 1, 2>(g0, g1, s);
+    default: return launch_pair<AC, BC, AT, BT, 2, 2, 2, 2>(g0, g1, s);
+  }
+}
+
 }  // namespace
 
 int gemm_splitk_reduce(const Gemm& g, const double* ws, int ksplit, bool cplx, hipStream_t s) {
@@ -680,6 +738,23 @@ int gemm_f64(const Gemm& g, hipStream_t s) {
     return fail(AFQ_E_INVALID, "gemm_f64: a packed symmetric A is square and not transposed");
   if (g.a_cplx) return g.b_cplx ? dispatch_t<true, true>(g, s) : dispatch_t<true, false>(g, s);
   return g.b_cplx ? dispatch_t<false, true>(g, s) : dispatch_t<false, false>(g, s);
+}
+
+int gemm_f64_pair(const Gemm& g0, const Gemm& g1, hipStream_t s) {
+  auto plain = [](const Gemm& g) {
+    return g.batch > 1 && !g.ws && !g.a_sym && !g.a_conj && !g.D && !g.E && !g.c_part && g.K > 0 && g.M > 0 &&
+           g.N > 0;
+  };
+  const bool same = g0.a_cplx == g1.a_cplx && g0.b_cplx == g1.b_cplx && g0.a_trans == g1.a_trans &&
+                    g0.b_trans == g1.b_trans;
+  const int cfg = batched_config(g0);
+  if (plain(g0) && plain(g1) && same && cfg >= 0 && cfg == batched_config(g1)) {
+    // the operand forms of the overlap and Green's-function products
+    if (!g0.a_cplx && g0.b_cplx && g0.a_trans && !g0.b_trans) return dispatch_pair<false, true, true, false>(g0, g1, cfg, s);
+    if (g0.a_cplx && g0.b_cplx && g0.a_trans && g0.b_trans) return dispatch_pair<true, true, true, true>(g0, g1, cfg, s);
+  }
+  AFQ_TRY(gemm_f64(g0, s));
+  return gemm_f64(g1, s);
 }
 
 }  // namespace afq
