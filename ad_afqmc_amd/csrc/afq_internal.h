@@ -105,7 +105,7 @@ struct Gemm {
 };
 int gemm_f64(const Gemm& g, hipStream_t stream);
 // Two batched products in one launch when their operand form and tile
-// configuration match (the overlap and Green's-function products of both
+// configuration match (the overlap, Green's-function and QR products of both
 // spins); otherwise two gemm_f64 calls.  Same results as the separate calls.
 int gemm_f64_pair(const Gemm& g0, const Gemm& g1, hipStream_t stream);
 // C = alpha * sum of `ksplit` split-K partial planes in ws + beta * C (+ D), fixed order.

@@ -6,6 +6,7 @@
 #include <string>
 
 #include "gemm_bf3.h"
+#include "group.h"
 #include "kernels.h"
 #include "trot_fused.h"
 #include "pipeline.h"
@@ -636,11 +637,71 @@ int cholqr2_panel(afq_job* j, double2* Wp, int c, int s, hipStream_t st) {
   return AFQ_OK;
 }
 
+// cholqr2_panel for both spins of unrestricted walkers at once: every product
+// and factorisation of a pass covers the two spins in one launch (spin 1's
+// scratch panel follows spin 0's in wgather).  Same arithmetic per matrix.
+int cholqr2_pair(afq_job* j, double2* W0, double2* W1, hipStream_t st) {
+  const int N = j->N, W = j->W;
+  double2* Wp[2] = {W0, W1};
+  double2* tmp[2] = {j->wgather, j->wgather + (long)W * N * j->nocc[0]};
+  for (int pass = 0; pass < 2; ++pass) {
+    Gemm g[2], q[2];
+    CholSet cs[2];
+    DetInvSet ds[2];
+    for (int s = 0; s < 2; ++s) {
+      const int c = j->nocc[s];
+      const long per = (long)N * c, cc = (long)c * c;
+      double2* in = pass == 0 ? Wp[s] : tmp[s];
+      double2* out = pass == 0 ? tmp[s] : Wp[s];
+      Gemm& a = g[s];
+      a.M = c;
+      a.N = c;
+      a.K = N;
+      a.batch = W;
+      a.a_cplx = true;
+      a.a_trans = true;
+      a.a_conj = true;
+      a.A = in;
+      a.lda = c;
+      a.sA = per;
+      a.b_cplx = true;
+      a.B = in;
+      a.ldb = c;
+      a.sB = per;
+      a.C = j->S[s];
+      a.ldc = c;
+      a.sC = cc;
+      cs[s] = CholSet{j->S[s], cc, c, j->S[s], cc};
+      ds[s] = DetInvSet{j->S[s], cc, c, j->det[s], j->Sinv[s], cc};
+      Gemm& b = q[s];
+      b.M = N;
+      b.N = c;
+      b.K = c;
+      b.batch = W;
+      b.a_cplx = true;
+      b.A = in;
+      b.lda = c;
+      b.sA = per;
+      b.b_cplx = true;
+      b.B = j->Sinv[s];
+      b.ldb = c;
+      b.sB = cc;
+      b.C = out;
+      b.ldc = c;
+      b.sC = per;
+    }
+    AFQ_TRY(gemm_f64_pair(g[0], g[1], st));
+    AFQ_TRY(batched_cholesky_upper_sets(cs, 2, W, st));
+    AFQ_TRY(batched_det_inv_sets(ds, 2, W, st));
+    AFQ_TRY(gemm_f64_pair(q[0], q[1], st));
+  }
+  return AFQ_OK;
+}
+
 int orthonormalize(afq_job* j, void* up, void* dn, hipStream_t st) {
   ProfScope ps(j, AFQ_K_QR, st);
   if (j->restricted) return cholqr2_panel(j, static_cast<double2*>(up), j->wcols, 0, st);
-  AFQ_TRY(cholqr2_panel(j, static_cast<double2*>(up), j->nocc[0], 0, st));
-  return cholqr2_panel(j, static_cast<double2*>(dn), j->nocc[1], 1, st);
+  return cholqr2_pair(j, static_cast<double2*>(up), static_cast<double2*>(dn), st);
 }
 
 int require(const afq_job* j, bool meas, bool prop) {
@@ -888,7 +949,9 @@ int alloc_job(afq_job* j, bool dry, int64_t* total) {
   A_(j->kpart, 2 * W * kch);
   A_(j->ene1, W);
   A_(j->energies, W);
-  A_(j->wgather, W * N * wmax);
+  // gather buffer of the SR copy; for unrestricted walkers also both spins'
+  // scratch panels of the paired QR
+  A_(j->wgather, W * N * (j->restricted ? wmax : (long)j->nocc[0] + j->nocc[1]));
   A_(j->sr_idx, W);
   A_(j->cum, W);
   A_(j->tmp_nn, 8 * N * N);

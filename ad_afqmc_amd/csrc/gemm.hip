@@ -632,11 +632,12 @@ int dispatch_t(const Gemm& g, hipStream_t s) {
 }
 
 // The tile configuration dispatch_shape picks for a batched product without
-// K split, packed or conjugated A: 0 = 2x2x2x2, 1..4 = 4x1x1xTN, 5 = 4x4x1x2,
+// K split or packed A: 0 = 2x2x2x2, 1..4 = 4x1x1xTN, 5 = 4x4x1x2,
 // -1 = a shape served otherwise (the wide VHS tiles, the walker-panel kernel).
 int batched_config(const Gemm& g) {
   if (!g.b_cplx && !g.b_trans && g.batch == 1 && g.N >= 4096 && g.M >= 64 && g.K >= 64) return -1;
-  if (!g.a_trans && !g.b_trans && g.b_cplx && g.batch > 1 && g.N > 32 && g.N <= 48 && g.M > 64) return -1;
+  if (!g.a_trans && !g.b_trans && g.b_cplx && g.batch > 1 && g.N > 32 && g.N <= 48 && g.M > 64 && !g.a_conj)
+    return -1;
   if (g.N <= 16) return 1;
   if (g.N <= 32) return 2;
   if (g.N <= 48) return 3;
@@ -742,16 +743,19 @@ int gemm_f64(const Gemm& g, hipStream_t s) {
 
 int gemm_f64_pair(const Gemm& g0, const Gemm& g1, hipStream_t s) {
   auto plain = [](const Gemm& g) {
-    return g.batch > 1 && !g.ws && !g.a_sym && !g.a_conj && !g.D && !g.E && !g.c_part && g.K > 0 && g.M > 0 &&
-           g.N > 0;
+    return g.batch > 1 && !g.ws && !g.a_sym && !g.D && !g.E && !g.c_part && g.K > 0 && g.M > 0 && g.N > 0 &&
+           (!g.a_conj || g.a_cplx);
   };
   const bool same = g0.a_cplx == g1.a_cplx && g0.b_cplx == g1.b_cplx && g0.a_trans == g1.a_trans &&
-                    g0.b_trans == g1.b_trans;
+                    g0.b_trans == g1.b_trans && g0.a_conj == g1.a_conj;
   const int cfg = batched_config(g0);
   if (plain(g0) && plain(g1) && same && cfg >= 0 && cfg == batched_config(g1)) {
-    // the operand forms of the overlap and Green's-function products
-    if (!g0.a_cplx && g0.b_cplx && g0.a_trans && !g0.b_trans) return dispatch_pair<false, true, true, false>(g0, g1, cfg, s);
-    if (g0.a_cplx && g0.b_cplx && g0.a_trans && g0.b_trans) return dispatch_pair<true, true, true, true>(g0, g1, cfg, s);
+    // the operand forms of the overlap, Green's-function and QR products
+    const bool ac = g0.a_cplx, bc = g0.b_cplx, at = g0.a_trans, bt = g0.b_trans;
+    if (!ac && bc && at && !bt) return dispatch_pair<false, true, true, false>(g0, g1, cfg, s);
+    if (ac && bc && at && bt) return dispatch_pair<true, true, true, true>(g0, g1, cfg, s);
+    if (ac && bc && at && !bt) return dispatch_pair<true, true, true, false>(g0, g1, cfg, s);
+    if (ac && bc && !at && !bt) return dispatch_pair<true, true, false, false>(g0, g1, cfg, s);
   }
   AFQ_TRY(gemm_f64(g0, s));
   return gemm_f64(g1, s);

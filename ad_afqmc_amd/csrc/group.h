@@ -62,6 +62,8 @@ struct CholSet {
   long strideR;
 };
 int cholesky_grouped(const GroupTab<CholSet>& t, int nmax, hipStream_t s);
+// One or two sets of `count` matrices in one launch (both spins of a QR pass).
+int batched_cholesky_upper_sets(const CholSet* sets, int nsets, int count, hipStream_t s);
 
 // dst = src or conj(src) over n complex elements per job; work items:
 // ceil(n / 1024) per job.

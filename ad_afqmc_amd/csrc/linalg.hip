@@ -554,11 +554,12 @@ __device__ __forceinline__ void cholesky_upper_body(const double2* __restrict__ 
   }
 }
 
+// grid (count, sets): matrix blockIdx.x of set blockIdx.y
 template <int NMAX>
-__global__ __launch_bounds__(256) void cholesky_upper_kernel(const double2* __restrict__ S, long strideS, int n,
-                                                             double2* __restrict__ R, long strideR) {
+__global__ __launch_bounds__(256) void cholesky_upper_kernel(CholSet c0, CholSet c1) {
   extern __shared__ double2 lds_ch[];
-  cholesky_upper_body<NMAX>(S, strideS, n, R, strideR, blockIdx.x, lds_ch);
+  const CholSet c = blockIdx.y == 0 ? c0 : c1;
+  cholesky_upper_body<NMAX>(c.S, c.strideS, c.n, c.R, c.strideR, blockIdx.x, lds_ch);
 }
 
 template <int NMAX>
@@ -662,22 +663,32 @@ int batched_det_inv(const double2* S, long strideS, int n, int count, double2* d
 }
 
 template <int NMAX>
-static int launch_cholesky(const double2* S, long strideS, int n, int count, double2* R, long strideR, size_t lds,
-                           hipStream_t stream) {
+static int launch_cholesky(const CholSet* sets, int nsets, int count, size_t lds, hipStream_t stream) {
   static bool raised = false;
   AFQ_TRY(raise_lds(reinterpret_cast<const void*>(cholesky_upper_kernel<NMAX>), &raised, "Cholesky"));
-  hipLaunchKernelGGL(cholesky_upper_kernel<NMAX>, dim3(count), dim3(256), lds, stream, S, strideS, n, R, strideR);
+  hipLaunchKernelGGL(cholesky_upper_kernel<NMAX>, dim3(count, nsets), dim3(256), lds, stream, sets[0],
+                     sets[nsets > 1 ? 1 : 0]);
   return launch_status("cholesky_upper_kernel");
+}
+
+int batched_cholesky_upper_sets(const CholSet* sets, int nsets, int count, hipStream_t stream) {
+  if (nsets < 1 || nsets > 2) return fail(AFQ_E_INVALID, "Cholesky: one or two matrix sets");
+  int nmax = 0;
+  for (int i = 0; i < nsets; ++i) {
+    if (sets[i].n < 1 || sets[i].n > kDetInvMaxN)
+      return fail(AFQ_E_UNSUPPORTED, "Cholesky kernel supports 1 <= n <= " + std::to_string(kDetInvMaxN));
+    nmax = std::max(nmax, sets[i].n);
+  }
+  if (count <= 0) return AFQ_OK;
+  const size_t lds = matrix_lds_bytes(nmax);
+  return nmax <= kLdsN ? launch_cholesky<kLdsN>(sets, nsets, count, lds, stream)
+                       : launch_cholesky<kDetInvMaxN>(sets, nsets, count, lds, stream);
 }
 
 int batched_cholesky_upper(const double2* S, long strideS, int n, int count, double2* R, long strideR,
                            hipStream_t stream) {
-  if (n < 1 || n > kDetInvMaxN)
-    return fail(AFQ_E_UNSUPPORTED, "Cholesky kernel supports 1 <= n <= " + std::to_string(kDetInvMaxN));
-  if (count <= 0) return AFQ_OK;
-  const size_t lds = matrix_lds_bytes(n);
-  return n <= kLdsN ? launch_cholesky<kLdsN>(S, strideS, n, count, R, strideR, lds, stream)
-                    : launch_cholesky<kDetInvMaxN>(S, strideS, n, count, R, strideR, lds, stream);
+  const CholSet set{S, strideS, n, R, strideR};
+  return batched_cholesky_upper_sets(&set, 1, count, stream);
 }
 
 template <int NMAX>
