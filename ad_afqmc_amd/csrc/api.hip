@@ -90,8 +90,7 @@ void release(afq_job* j) {
   fr(j->t32[1]);
   fr(j->rcs[0]);
   if (!j->rhf) fr(j->rcs[1]);
-  fr(j->rcs_fb[0]);
-  if (!j->rhf) fr(j->rcs_fb[1]);
+  fr(j->rcs_fb[0]);  // rcs_fb[1] points into it
   fr(j->gsplit);
   fr(j->rct32[0]);
   if (!j->rhf) fr(j->rct32[1]);
@@ -202,8 +201,44 @@ int overlap_from_dets(afq_job* j, double2* out, hipStream_t st) {
   return k_overlap_combine(j->det[0], j->one_spin ? nullptr : j->det[1], out, j->W, st);
 }
 
+// Split-BF16 force bias of a UHF trial: both spins as one product over the
+// concatenated zero-padded (Opad, Npad) orbital pairs, spin 1's k steps after
+// spin 0's in every row of the split Green's functions and of rcs_fb (one
+// allocation; rcs_fb[1] points at spin 1's part).  fb_kcat is that depth.
+bool fb_cat(const afq_job* j) { return j->bf16x3 && !j->rhf; }
+long fb_kspin(const afq_job* j, int s) { return (long)bf16_opad(j->nocc[s]) * bf16_npad(j->N); }
+long fb_kcat(const afq_job* j) { return fb_kspin(j, 0) + (j->rhf ? 0 : fb_kspin(j, 1)); }
+
 // fb = sum_s f_s * Gflat_s (W x O_s N) . rot_chol_s^T (O_s N x G), f = 2 for one spin channel.
 int force_bias_gemm(afq_job* j, double2* out, hipStream_t st) {
+  if (fb_cat(j)) {
+    ProfScope ps(j, AFQ_K_FORCE_BIAS, st);
+    const long K = fb_kcat(j);
+    for (int s = 0; s < 2; ++s)
+      AFQ_TRY(k_split_bf16(reinterpret_cast<const double*>(j->green[s]), true, j->nocc[s], j->N, j->W, true,
+                           j->gsplit + (s ? 4 * fb_kspin(j, 0) : 0), st, 4 * K));
+    Bf3 b;
+    b.M = j->W;
+    b.N = j->G;
+    b.K = (int)K;
+    b.a_cplx = true;
+    b.A = j->gsplit;
+    b.lda = 4 * K;
+    b.pA = 32;
+    b.kA = 4 * 32;
+    b.a_rows = j->Wcap;
+    b.B = j->rcs_fb[0];
+    b.ldb = 2 * K;
+    b.pB = 32;
+    b.kB = 2 * 32;
+    b.b_rows = j->G;
+    b.C = out;
+    b.ldc = j->G;
+    b.c_f64 = true;
+    b.ws = j->gemm_ws;
+    b.ws_elems = j->gemm_ws_elems;
+    return gemm_bf3(b, st);
+  }
   for (int s = 0; s < j->nspin_meas; ++s) {
     const int O = j->nocc[s];
     Gemm g;
@@ -911,9 +946,8 @@ int alloc_job(afq_job* j, bool dry, int64_t* total) {
       const long np = bf16_npad(j->N), gp = bf16_npad(j->G);
       A_(j->rcs[0], G * 2 * bf16_opad(j->nocc[0]) * np);
       if (!j->rhf) A_(j->rcs[1], G * 2 * bf16_opad(j->nocc[1]) * np);
-      A_(j->rcs_fb[0], G * 2 * bf16_opad(j->nocc[0]) * np);
-      if (!j->rhf) A_(j->rcs_fb[1], G * 2 * bf16_opad(j->nocc[1]) * np);
-      A_(j->gsplit, W * 4 * bf16_opad((int)std::max(j->nocc[0], j->nocc[1])) * np);
+      A_(j->rcs_fb[0], G * 2 * fb_kcat(j));  // both spins' rows side by side for a UHF trial
+      A_(j->gsplit, W * 4 * std::max(fb_kcat(j), (long)bf16_opad((int)std::max(j->nocc[0], j->nocc[1])) * np));
       A_(j->lt_split, 16 * ((N * N + 15) / 16) * 2 * gp);
       A_(j->xs_split, bf16_opad((int)W) * 4 * gp);
       if (trot_fused_supported((int)N)) {
@@ -1007,6 +1041,8 @@ int afq_job_create(afq_job_t* out, const afq_job_desc* d) {
     j->rcs[1] = j->rcs[0];
     j->rcs_fb[1] = j->rcs_fb[0];
     j->rct32[1] = j->rct32[0];
+  } else if (j->rcs_fb[0]) {
+    j->rcs_fb[1] = j->rcs_fb[0] + 2 * fb_kspin(j, 0);
   }
   *out = j;
   return AFQ_OK;
@@ -1132,7 +1168,8 @@ int afq_build_measurement_intermediates(afq_job_t j, void* stream) {
       if (j->rct32[s]) AFQ_TRY(k_split_f32(j->rot_chol[s], false, j->nocc[s], N, j->G, j->rct32[s], st));
       if (j->bf16x3) {
         AFQ_TRY(k_split_bf16(j->rot_chol[s], false, j->nocc[s], N, j->G, false, j->rcs[s], st));
-        AFQ_TRY(k_split_bf16(j->rot_chol[s], false, j->nocc[s], N, j->G, true, j->rcs_fb[s], st));
+        AFQ_TRY(k_split_bf16(j->rot_chol[s], false, j->nocc[s], N, j->G, true, j->rcs_fb[s], st,
+                             fb_cat(j) ? 2 * fb_kcat(j) : 0));
       }
     }
   j->have_meas = true;
@@ -1256,7 +1293,8 @@ int afq_set_intermediate(afq_job_t j, int which, int spin, const void* src, void
         AFQ_TRY(k_split_f32(j->rot_chol[spin], false, (int)O, (int)N, G, j->rct32[spin], S(stream)));
       if (j->bf16x3) {
         AFQ_TRY(k_split_bf16(j->rot_chol[spin], false, (int)O, (int)N, G, false, j->rcs[spin], S(stream)));
-        AFQ_TRY(k_split_bf16(j->rot_chol[spin], false, (int)O, (int)N, G, true, j->rcs_fb[spin], S(stream)));
+        AFQ_TRY(k_split_bf16(j->rot_chol[spin], false, (int)O, (int)N, G, true, j->rcs_fb[spin], S(stream),
+                             fb_cat(j) ? 2 * fb_kcat(j) : 0));
       }
       break;
     case AFQ_INT_MF_SHIFTS: j->have_mf = true; break;

@@ -840,8 +840,12 @@ __global__ void split_f32_kernel(const double* __restrict__ src, int cplx, int O
 // planes of a row are instead interleaved every 32 elements of that order,
 // out[r][t / 32][plane][t % 32] with t = bf16_tile(o, k): the force-bias
 // product reads a row's 32-deep step of all planes as whole 128-byte lines.
+// Rows of out are rst elements apart (rst = 0: the row's own size), so that
+// the planes of two spins can share a row (both spins' force bias as one
+// product over the concatenated k steps).
 __device__ __forceinline__ void split_bf16_body(const double* __restrict__ src, int cplx, int O, int N, int Opad,
-                                                int Npad, long rows, int ilv, __bf16* __restrict__ out, long idx) {
+                                                int Npad, long rows, int ilv, long rst, __bf16* __restrict__ out,
+                                                long idx) {
   const long per = (long)Opad * Npad;
   if (idx >= rows * per) return;
   const long r = idx / per;
@@ -850,7 +854,7 @@ __device__ __forceinline__ void split_bf16_body(const double* __restrict__ src, 
   const int np = cplx ? 4 : 2;
   const long t = bf16_tile(o, k, Opad);
   const long pst = ilv ? 32 : per;  // plane stride
-  __bf16* base = out + r * np * per + (ilv ? (t >> 5) * (32 * np) + (t & 31) : t);
+  __bf16* base = out + r * (rst > 0 ? rst : np * per) + (ilv ? (t >> 5) * (32 * np) + (t & 31) : t);
   for (int c = 0; c < (cplx ? 2 : 1); ++c) {
     double x = 0.0;
     if (in) x = cplx ? src[2 * ((r * O + o) * (long)N + k) + c] : src[(r * O + o) * (long)N + k];
@@ -862,8 +866,8 @@ __device__ __forceinline__ void split_bf16_body(const double* __restrict__ src, 
 }
 
 __global__ void split_bf16_kernel(const double* __restrict__ src, int cplx, int O, int N, int Opad, int Npad,
-                                  long rows, int ilv, __bf16* __restrict__ out) {
-  split_bf16_body(src, cplx, O, N, Opad, Npad, rows, ilv, out, blockIdx.x * (long)blockDim.x + threadIdx.x);
+                                  long rows, int ilv, long rst, __bf16* __restrict__ out) {
+  split_bf16_body(src, cplx, O, N, Opad, Npad, rows, ilv, rst, out, blockIdx.x * (long)blockDim.x + threadIdx.x);
 }
 
 __global__ void split_bf16_grouped_kernel(const SplitJob* __restrict__ jobs, const int* __restrict__ prefix,
@@ -871,7 +875,7 @@ __global__ void split_bf16_grouped_kernel(const SplitJob* __restrict__ jobs, con
   const int item = blockIdx.x;
   const int j = group_find(prefix, njobs, item);
   const SplitJob J = jobs[j];
-  split_bf16_body(J.src, J.cplx, J.O, J.N, bf16_opad_d(J.O), bf16_npad_d(J.N), J.rows, J.ilv,
+  split_bf16_body(J.src, J.cplx, J.O, J.N, bf16_opad_d(J.O), bf16_npad_d(J.N), J.rows, J.ilv, J.rst,
                   reinterpret_cast<__bf16*>(J.out), (long)(item - prefix[j]) * 256 + threadIdx.x);
 }
 
@@ -1049,12 +1053,15 @@ int split_bf16_grouped(const GroupTab<SplitJob>& t, hipStream_t s) {
   return launch_status("split_bf16_grouped_kernel");
 }
 
-int k_split_bf16(const double* src, bool cplx, int O, int N, long rows, bool ilv, uint16_t* out, hipStream_t s) {
+int k_split_bf16(const double* src, bool cplx, int O, int N, long rows, bool ilv, uint16_t* out, hipStream_t s,
+                 long row_stride) {
   const int Opad = bf16_opad(O), Npad = bf16_npad(N);
   const long n = rows * Opad * (long)Npad;
   if (n <= 0) return AFQ_OK;
+  if (row_stride != 0 && row_stride < (cplx ? 4L : 2L) * Opad * Npad)
+    return fail(AFQ_E_INVALID, "k_split_bf16: row stride below the row size");
   hipLaunchKernelGGL(split_bf16_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, src, cplx ? 1 : 0, O, N,
-                     Opad, Npad, rows, ilv ? 1 : 0, reinterpret_cast<__bf16*>(out));
+                     Opad, Npad, rows, ilv ? 1 : 0, row_stride, reinterpret_cast<__bf16*>(out));
   return launch_status("split_bf16_kernel");
 }
 

@@ -123,6 +123,7 @@ struct SplitJob {
   long rows;
   uint16_t* out;
   int ilv;  // planes interleaved by 32-deep k step (the force-bias product's layout)
+  long rst = 0;  // elements between rows of out (0: the row's own size)
 };
 long split_items(const SplitJob& j);
 int split_bf16_grouped(const GroupTab<SplitJob>& t, hipStream_t s);

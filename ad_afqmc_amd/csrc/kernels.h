@@ -33,7 +33,9 @@ constexpr int kExchangeRegMaxO = 64;  // largest nocc of the register-blocked ke
 // 32 elements (the force-bias product's layout).
 int bf16_opad(int O);
 int bf16_npad(int N);
-int k_split_bf16(const double* src, bool cplx, int O, int N, long rows, bool ilv, uint16_t* out, hipStream_t s);
+// row_stride: elements from one row of out to the next (0: the row's own size).
+int k_split_bf16(const double* src, bool cplx, int O, int N, long rows, bool ilv, uint16_t* out, hipStream_t s,
+                 long row_stride = 0);
 int k_exchange_bf16(const uint16_t* rcs, const uint16_t* gs, int O, int N, int Gn, int W, double2* coul,
                     double2* kpart, int* nchunk_out, hipStream_t s);
 // FP32 operands of the exchange: rows x (O, N) real (1 plane) or complex (re,
