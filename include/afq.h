@@ -284,7 +284,12 @@ int afq_group_destroy(afq_group_t group);
  * h1_bar = chol_bar = NULL evaluates F only.  The fields are constants (the
  * reference draws them from a key that carries no tangent); clipping masks
  * are held fixed.  RHF trial, restricted walkers, FP64 jobs.  Allocates a
- * device workspace for the call and synchronises the stream. */
+ * device workspace for the call and synchronises the stream.  With a gradient
+ * requested, the steps' tapes of every block are kept from the forward sweep
+ * when that workspace takes at most half of the free device memory (or at
+ * most AFQ_SEGMENT_TAPE_LIMIT bytes, from the environment, when set);
+ * otherwise each block is recomputed from its checkpoint in the reverse sweep
+ * (O(n_blocks + n_steps) walker arrays).  Both give the same result. */
 int afq_segment_energy_vjp(afq_job_t job, const void* walkers, const double* weights, double e_estimate,
                            const double* fields, int32_t n_blocks, int32_t n_steps, double* energy_out,
                            double* h1_bar, double* chol_bar, void* stream);
