@@ -599,6 +599,8 @@ int launch_panel(const Gemm& g, hipStream_t s) {
   return launch_status("panel_f64_kernel");
 }
 
+// batched_config (below) restates these choices for gemm_f64_pair: a change
+// here needs the same change there.
 template <bool AC, bool BC, bool AT, bool BT>
 int dispatch_shape(const Gemm& g, hipStream_t s) {
   // Large unbatched products with a wide N (the VHS product, N = norb^2): 16 waves
