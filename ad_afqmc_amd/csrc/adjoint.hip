@@ -202,8 +202,8 @@ __global__ void adj_seed_kernel(const double* bo, int nb, double* seg, double* e
   *energy_out = num / den;
 }
 
-// F~ = -2 kappa_w f (in place, layout [g][i][w][j]) and its copy in layout [w][i][g][j].
-__global__ void adj_fscale_kernel(double2* F, const double* kappa, int G, int O, int Wc, double2* Fp) {
+// F~ = c kappa_w f (in place, layout [g][i][w][j]) and its copy in layout [w][i][g][j].
+__global__ void adj_fscale_kernel(double2* F, const double* kappa, double c, int G, int O, int Wc, double2* Fp) {
   const long n = (long)G * O * Wc * O;
   const long idx = blockIdx.x * (long)blockDim.x + threadIdx.x;
   if (idx >= n) return;
@@ -213,24 +213,18 @@ __global__ void adj_fscale_kernel(double2* F, const double* kappa, int G, int O,
   t /= Wc;
   const int i = (int)(t % O);
   const int g = (int)(t / O);
-  const double s = -2.0 * kappa[wl];
+  const double s = c * kappa[wl];
   const double2 v = {s * F[idx].x, s * F[idx].y};
   F[idx] = v;
   Fp[(((long)wl * O + i) * G + g) * O + j] = v;
 }
 
-// Cc[w][g] = -2 tr F~_wg = 4 kappa_w c_wg
-__global__ void adj_cc_kernel(const double2* F, int G, int O, int Wc, double2* Cc) {
-  const long idx = blockIdx.x * (long)blockDim.x + threadIdx.x;
-  if (idx >= (long)Wc * G) return;
-  const int wl = (int)(idx / G), g = (int)(idx % G);
-  double2 s = {0.0, 0.0};
-  for (int i = 0; i < O; ++i) {
-    const double2 v = F[(((long)g * O + i) * Wc + wl) * O + i];
-    s.x += v.x;
-    s.y += v.y;
-  }
-  Cc[idx] = double2{-2.0 * s.x, -2.0 * s.y};
+// Cc[w][g] = c kappa_w Ct[w][g] (the Coulomb adjoint from the traces Ct)
+__global__ void adj_cscale_kernel(const double2* Ct, const double* kappa, double c, int G, long n, double2* Cc) {
+  const long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double s = c * kappa[i / G];
+  Cc[i] = double2{s * Ct[i].x, s * Ct[i].y};
 }
 
 // gG_w += c kappa_w rot_h1
@@ -298,7 +292,7 @@ __global__ void adj_lfinal_kernel(double* Lbar, const double* m, const double* m
   Lbar[idx] += m[g] * hmodbar[e] + mbar[g] * rho[e];
 }
 
-// h1bar[s] = (X + X^T)/4 for s = 0, 1 (h1s = ((h1_0 + h1_0^T)/2 + (h1_1 + h1_1^T)/2)/2)
+// h1bar[s] = (X + X^T)/4 for s = 0, 1 (restricted walkers: h1s = ((h1_0 + h1_0^T)/2 + (h1_1 + h1_1^T)/2)/2)
 __global__ void adj_h1_out_kernel(const double* X, int n, double* out) {
   const long idx = blockIdx.x * (long)blockDim.x + threadIdx.x;
   const long nn = (long)n * n;
@@ -307,6 +301,14 @@ __global__ void adj_h1_out_kernel(const double* X, int n, double* out) {
   const double v = 0.25 * (X[idx] + X[(long)c * n + r]);
   out[idx] = v;
   out[nn + idx] = v;
+}
+
+// h1bar_s = (X + X^T)/2 (unrestricted walkers: h1s_s = (h1_s + h1_s^T)/2)
+__global__ void adj_h1_out_spin_kernel(const double* X, int n, double* out) {
+  const long idx = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (idx >= (long)n * n) return;
+  const int r = (int)(idx / n), c = (int)(idx % n);
+  out[idx] = 0.5 * (X[idx] + X[(long)c * n + r]);
 }
 
 // ---------------------------------------------------------------- GEMM builder
@@ -391,6 +393,29 @@ struct Tape {
   double2* terms;
 };
 
+// Spin layout.  A walker set holds spin 0's (W, N, O_0) panel and, for
+// unrestricted walkers of a UHF trial, spin 1's (W, N, O_1) panel after it
+// (ws elements in all); O x O per-walker matrices likewise (wq elements).
+struct Spins {
+  int nsp = 1;
+  long O[2] = {0, 0}, pw[2] = {0, 0};  // occupied orbitals, N O_s per walker
+  long wo[2] = {0, 0}, qo[2] = {0, 0};  // spin offsets in a walker set / O x O set
+  long ws = 0, wq = 0, pt = 0;          // walker set, O x O set, sum_s N O_s
+  void init(const afq_job* j) {
+    nsp = j->restricted ? 1 : 2;
+    const long N = j->N, W = j->W;
+    for (int s = 0; s < nsp; ++s) {
+      O[s] = j->nocc[s];
+      pw[s] = N * O[s];
+    }
+    wo[1] = W * pw[0];
+    qo[1] = W * O[0] * O[0];
+    ws = W * (pw[0] + pw[1]);
+    wq = W * (O[0] * O[0] + O[1] * O[1]);
+    pt = pw[0] + pw[1];
+  }
+};
+
 struct Buffers {
   // checkpoints and block results
   double2* ckW;
@@ -418,16 +443,16 @@ struct Buffers {
   double2* rh1barC;
   // energy chunks
   int Wc;
-  double2 *F, *Fp, *Cc;
+  double2 *F, *Fp, *Cc, *Ct;
   // setup
   double *Ebar, *v0, *v1, *hmod, *Amat, *Mblk, *Mexp, *Mwork, *hmodbar, *h1sbar, *rh1bar, *rcbar, *rho, *tg;
 };
 
-void carve(Arena& ar, Buffers& b, const afq_job* j, int nb, int ns, int K, bool keep) {
-  const long N = j->N, G = j->G, W = j->W, O = j->nocc[0];
-  const long per = N * O, NN = N * N;
+void carve(Arena& ar, Buffers& b, const afq_job* j, const Spins& sp, int nb, int ns, int K, bool keep) {
+  const long N = j->N, G = j->G, W = j->W;
+  const long NN = N * N, ws = sp.ws, wq = sp.wq, ns_ = sp.nsp;
   const long slots = keep ? nb : 1;
-  b.ckW = ar.take<double2>(nb * W * per);
+  b.ckW = ar.take<double2>(nb * ws);
   b.ckw = ar.take<double>(nb * W);
   b.cksc = ar.take<double>(2 * nb);
   b.bo = ar.take<double>(2 * nb);
@@ -435,48 +460,50 @@ void carve(Arena& ar, Buffers& b, const afq_job* j, int nb, int ns, int K, bool 
   b.fw = ar.take<double>(W);
   b.fov = ar.take<double2>(W);
   b.fsc = ar.take<double>(2);
-  b.tp.W = ar.take<double2>(slots * (ns + 1) * W * per);
+  b.tp.W = ar.take<double2>(slots * (ns + 1) * ws);
   b.tp.w = ar.take<double>(slots * (ns + 1) * W);
   b.tp.sc = ar.take<double>(slots * 2 * (ns + 1));
   b.tp.ov = ar.take<double2>(slots * (ns + 1) * W);
   b.tp.xt = ar.take<double2>(slots * ns * W * G);
   b.tp.terms = ar.take<double2>(slots * ns * 2 * W);
-  b.qk = ar.take<double2>(keep ? nb * W * per : 0);
+  b.qk = ar.take<double2>(keep ? nb * ws : 0);
   b.ek = ar.take<double2>(keep ? nb * W : 0);
-  b.Q = ar.take<double2>(W * per);
-  b.Qc = ar.take<double2>(W * per);
-  b.R = ar.take<double2>(W * O * O);
-  b.Rinv = ar.take<double2>(W * O * O);
-  b.detR = ar.take<double2>(W);
-  b.Tb = ar.take<double2>((K + 1) * W * per);
-  b.W2 = ar.take<double2>(W * per);
-  b.gW = ar.take<double2>(W * per);
-  b.gWk = ar.take<double2>(W * per);
-  b.gW2 = ar.take<double2>(W * per);
-  b.gTa = ar.take<double2>(W * per);
-  b.gTb = ar.take<double2>(W * per);
+  b.Q = ar.take<double2>(ws);
+  b.Qc = ar.take<double2>(ws);
+  b.R = ar.take<double2>(wq);
+  b.Rinv = ar.take<double2>(wq);
+  b.detR = ar.take<double2>(2 * W);
+  b.Tb = ar.take<double2>((K + 1) * ws);
+  b.W2 = ar.take<double2>(ws);
+  b.gW = ar.take<double2>(ws);
+  b.gWk = ar.take<double2>(ws);
+  b.gW2 = ar.take<double2>(ws);
+  b.gTa = ar.take<double2>(ws);
+  b.gTb = ar.take<double2>(ws);
   b.gvhs = ar.take<double2>(W * NN);
-  b.EbarW = ar.take<double2>(W * NN);
+  b.EbarW = ar.take<double2>(ns_ * W * NN);
   b.gxt = ar.take<double2>(W * G);
   b.gfb = ar.take<double2>(W * G);
   b.gd = ar.take<double2>(W);
-  b.gG = ar.take<double2>(W * per);
-  b.P = ar.take<double2>(W * O * O);
-  b.gS = ar.take<double2>(W * O * O);
-  b.X = ar.take<double2>(W * O * O);
+  b.gG = ar.take<double2>(ws);
+  b.P = ar.take<double2>(wq);
+  b.gS = ar.take<double2>(wq);
+  b.X = ar.take<double2>(wq);
   b.alpha = ar.take<double>(W);
   b.gw = ar.take<double>(W);
   b.kappa = ar.take<double>(W);
   b.mbar = ar.take<double>(G);
   b.scal = ar.take<double>(4);
   b.Lbar = ar.take<double>(G * NN);
-  b.rh1barC = ar.take<double2>(per);
+  b.rh1barC = ar.take<double2>(sp.pt);
   // exchange-adjoint chunks: F and its permuted copy at most ~2 GB each
-  const long per_w = G * O * O * (long)sizeof(double2);
+  const long omax = std::max(sp.O[0], sp.O[1]);
+  const long per_w = G * omax * omax * (long)sizeof(double2);
   b.Wc = (int)std::max(1L, std::min(W, (2048L << 20) / std::max(1L, per_w)));
-  b.F = ar.take<double2>(G * O * b.Wc * O);
-  b.Fp = ar.take<double2>(G * O * b.Wc * O);
-  b.Cc = ar.take<double2>(b.Wc * G);
+  b.F = ar.take<double2>(G * omax * b.Wc * omax);
+  b.Fp = ar.take<double2>(G * omax * b.Wc * omax);
+  b.Cc = ar.take<double2>(W * G);
+  b.Ct = ar.take<double2>(W * G);
   b.Ebar = ar.take<double>(NN);
   b.v0 = ar.take<double>(NN);
   b.v1 = ar.take<double>(NN);
@@ -485,10 +512,10 @@ void carve(Arena& ar, Buffers& b, const afq_job* j, int nb, int ns, int K, bool 
   b.Mblk = ar.take<double>(4 * NN);
   b.Mexp = ar.take<double>(4 * NN);
   b.Mwork = ar.take<double>(12 * NN);
-  b.hmodbar = ar.take<double>(NN);
+  b.hmodbar = ar.take<double>(ns_ * NN);
   b.h1sbar = ar.take<double>(NN);
-  b.rh1bar = ar.take<double>(per);
-  b.rcbar = ar.take<double>(G * per);
+  b.rh1bar = ar.take<double>(sp.pt);
+  b.rcbar = ar.take<double>(G * sp.pt);
   b.rho = ar.take<double>(NN);
   b.tg = ar.take<double>(G);
 }
@@ -503,23 +530,33 @@ void carve(Arena& ar, Buffers& b, const afq_job* j, int nb, int ns, int K, bool 
 struct Seg {
   afq_job* j;
   Buffers b;
+  Spins sp;
   hipStream_t st;
   int nb, ns, K;
-  long N, G, W, O, per, NN;
+  long N, G, W, NN;
   double dt, sqdt, e_est;
+  // spin weights: restricted walkers stand for both spins (force bias, log
+  // overlap, one-body and exchange energy counted twice, Coulomb 2c^2)
+  double cs = 1.0, ccoul = 1.0;
   const double* fields;
   bool keep = false;  // the forward records every block's tapes
   Tape t;             // tapes of the block being pulled back
 
   Tape tape_at(int blk) const {
     const long s = keep ? blk : 0;
-    return Tape{b.tp.W + s * (ns + 1) * W * per, b.tp.w + s * (ns + 1) * W, b.tp.sc + s * 2 * (ns + 1),
+    return Tape{b.tp.W + s * (ns + 1) * sp.ws, b.tp.w + s * (ns + 1) * W, b.tp.sc + s * 2 * (ns + 1),
                 b.tp.ov + s * (ns + 1) * W,       b.tp.xt + s * ns * W * G,  b.tp.terms + s * ns * 2 * W};
   }
 
-  const double* T() const { return j->mo[0]; }
-  const double* E() const { return j->exp_h1[0]; }
+  const double* T(int s) const { return j->mo[s]; }
+  const double* E(int s) const { return j->exp_h1[s]; }
   const double* fields_at(int blk, int k) const { return fields + ((long)blk * ns + k) * W * G; }
+  // spin s of a walker set / of an O x O set
+  template <typename P>
+  P* wsp(P* set, int s) const { return set + sp.wo[s]; }
+  template <typename P>
+  P* qsp(P* set, int s) const { return set + sp.qo[s]; }
+  void* dn(double2* set) const { return sp.nsp == 2 ? static_cast<void*>(set + sp.wo[1]) : nullptr; }
 
   int cpy(void* dst, const void* src, size_t bytes) {
     AFQ_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, st));
@@ -529,42 +566,44 @@ struct Seg {
     AFQ_HIP(hipMemsetAsync(p, 0, bytes, st));
     return AFQ_OK;
   }
+  int green(double2* Wp, bool want_green) { return green_pipeline(j, Wp, dn(Wp), true, want_green, st); }
   int overlap(double2* Wp, double2* ov) {
-    AFQ_TRY(green_pipeline(j, Wp, nullptr, false, false, st));
+    AFQ_TRY(green_pipeline(j, Wp, dn(Wp), false, false, st));
     return overlap_from_dets(j, ov, st);
   }
 
   // Forward over all blocks with checkpoints (afq_block with the given fields),
   // with keep also every block's tapes, end walkers and energies.
   int forward(const void* walkers, const double* weights, double* energy_out) {
+    const size_t wbytes = sizeof(double2) * sp.ws;
     double2* Wc = b.Q;  // current walkers
     double* wc = b.fw;
     double2* ov = b.fov;
     double* sc = b.fsc;
-    AFQ_TRY(cpy(Wc, walkers, sizeof(double2) * W * per));
+    AFQ_TRY(cpy(Wc, walkers, wbytes));
     AFQ_TRY(cpy(wc, weights, sizeof(double) * W));
     const double sc0[2] = {e_est, e_est};
     AFQ_HIP(hipMemcpyAsync(sc, sc0, sizeof(sc0), hipMemcpyHostToDevice, st));
     AFQ_TRY(overlap(Wc, ov));
     for (int blk = 0; blk < nb; ++blk) {
-      AFQ_TRY(cpy(b.ckW + blk * W * per, Wc, sizeof(double2) * W * per));
+      AFQ_TRY(cpy(b.ckW + blk * sp.ws, Wc, wbytes));
       AFQ_TRY(cpy(b.ckw + blk * W, wc, sizeof(double) * W));
       AFQ_TRY(cpy(b.cksc + 2 * blk, sc, 2 * sizeof(double)));
       if (keep) {  // the same steps, on the block's tape
         AFQ_TRY(tape(blk));
         const Tape tb = tape_at(blk);
-        AFQ_TRY(cpy(Wc, tb.W + ns * W * per, sizeof(double2) * W * per));
+        AFQ_TRY(cpy(Wc, tb.W + ns * sp.ws, wbytes));
         AFQ_TRY(cpy(wc, tb.w + ns * W, sizeof(double) * W));
         AFQ_TRY(cpy(ov, tb.ov + ns * W, sizeof(double2) * W));
         AFQ_TRY(cpy(sc, tb.sc + 2 * ns, 2 * sizeof(double)));
       } else {
-        for (int k = 0; k < ns; ++k) AFQ_TRY(propagate(j, Wc, nullptr, fields_at(blk, k), wc, ov, sc, st));
+        for (int k = 0; k < ns; ++k) AFQ_TRY(propagate(j, Wc, dn(Wc), fields_at(blk, k), wc, ov, sc, st));
       }
-      AFQ_TRY(orthonormalize(j, Wc, nullptr, st));
+      AFQ_TRY(orthonormalize(j, Wc, dn(Wc), st));
       AFQ_TRY(overlap(Wc, ov));
-      AFQ_TRY(energy_pipeline(j, Wc, nullptr, j->energies, st));
+      AFQ_TRY(energy_pipeline(j, Wc, dn(Wc), j->energies, st));
       if (keep) {
-        AFQ_TRY(cpy(b.qk + blk * W * per, Wc, sizeof(double2) * W * per));
+        AFQ_TRY(cpy(b.qk + blk * sp.ws, Wc, wbytes));
         AFQ_TRY(cpy(b.ek + blk * W, j->energies, sizeof(double2) * W));
       }
       AFQ_TRY(k_block_finalize(j->energies, wc, sc, std::sqrt(2.0 / dt), (int)W, b.bo + 2 * blk, st));
@@ -575,82 +614,109 @@ struct Seg {
 
   // Computes the tapes of block blk from its checkpoint.
   int tape(int blk) {
+    const size_t wbytes = sizeof(double2) * sp.ws;
     const Tape tb = tape_at(blk);
-    AFQ_TRY(cpy(tb.W, b.ckW + blk * W * per, sizeof(double2) * W * per));
+    AFQ_TRY(cpy(tb.W, b.ckW + blk * sp.ws, wbytes));
     AFQ_TRY(cpy(tb.w, b.ckw + blk * W, sizeof(double) * W));
     AFQ_TRY(cpy(tb.sc, b.cksc + 2 * blk, 2 * sizeof(double)));
     AFQ_TRY(overlap(tb.W, tb.ov));
     for (int k = 0; k < ns; ++k) {
-      double2* Wn = tb.W + (k + 1) * W * per;
+      double2* Wn = tb.W + (k + 1) * sp.ws;
       double* wn = tb.w + (k + 1) * W;
       double2* ovn = tb.ov + (k + 1) * W;
       double* scn = tb.sc + 2 * (k + 1);
-      AFQ_TRY(cpy(Wn, tb.W + k * W * per, sizeof(double2) * W * per));
+      AFQ_TRY(cpy(Wn, tb.W + k * sp.ws, wbytes));
       AFQ_TRY(cpy(wn, tb.w + k * W, sizeof(double) * W));
       AFQ_TRY(cpy(ovn, tb.ov + k * W, sizeof(double2) * W));
       AFQ_TRY(cpy(scn, tb.sc + 2 * k, 2 * sizeof(double)));
-      AFQ_TRY(propagate(j, Wn, nullptr, fields_at(blk, k), wn, ovn, scn, st));
+      AFQ_TRY(propagate(j, Wn, dn(Wn), fields_at(blk, k), wn, ovn, scn, st));
       AFQ_TRY(cpy(tb.xt + k * W * G, j->xs, sizeof(double2) * W * G));
       AFQ_TRY(cpy(tb.terms + k * 2 * W, j->terms, sizeof(double2) * 2 * W));
     }
     return AFQ_OK;
   }
 
-  // gWout += adjoint through G^T = W S^-1 and log det S (coefficient gd, or none),
-  // with j->green / j->Sinv holding G and S^-1 of the walkers concerned.
+  // gWout += adjoint through G_s^T = W_s S_s^-1 and cs log det S_s (coefficient
+  // gd, or none) for every spin, with j->green / j->Sinv holding G and S^-1 of
+  // the walkers concerned.
   int green_backward(const double2* gG, const double2* gd, double2* gWout) {
-    const long OO = O * O;
-    AFQ_TRY(GB(O, O, N, W).a(j->green[0], N, per, true).b(gG, N, per, true, true).c(b.P, O, OO).run(st));
-    if (gd) LAUNCH(adj_diag_add_kernel, W * O, b.P, gd, 2.0, (int)O, W * O);
-    AFQ_TRY(GB(O, O, O, W).a(b.P, O, OO, true).b(j->Sinv[0], O, OO, true, true).c(b.gS, O, OO).alpha(-1.0).run(st));
-    AFQ_TRY(GB(N, O, O, W)
-                .a(gG, N, per, true, true)
-                .b(j->Sinv[0], O, OO, true, true)
-                .c(gWout, O, per)
-                .beta(1.0)
-                .run(st));
-    return GB(N, O, O, W).a(T(), O, 0, false).b(b.gS, O, OO, true).c(gWout, O, per).beta(1.0).run(st);
+    for (int s = 0; s < sp.nsp; ++s) {
+      const long O = sp.O[s], per = sp.pw[s], OO = O * O;
+      double2* P = qsp(b.P, s);
+      double2* gS = qsp(b.gS, s);
+      const double2* gGs = wsp(gG, s);
+      double2* gWs = wsp(gWout, s);
+      AFQ_TRY(GB(O, O, N, W).a(j->green[s], N, per, true).b(gGs, N, per, true, true).c(P, O, OO).run(st));
+      if (gd) LAUNCH(adj_diag_add_kernel, W * O, P, gd, cs, (int)O, W * O);
+      AFQ_TRY(GB(O, O, O, W).a(P, O, OO, true).b(j->Sinv[s], O, OO, true, true).c(gS, O, OO).alpha(-1.0).run(st));
+      AFQ_TRY(GB(N, O, O, W)
+                  .a(gGs, N, per, true, true)
+                  .b(j->Sinv[s], O, OO, true, true)
+                  .c(gWs, O, per)
+                  .beta(1.0)
+                  .run(st));
+      AFQ_TRY(GB(N, O, O, W).a(T(s), O, 0, false).b(gS, O, OO, true).c(gWs, O, per).beta(1.0).run(st));
+    }
+    return AFQ_OK;
   }
 
   // Pulls kappa_w Re E_L(Q_w) back to gG (into b.gG) and the parameter accumulators;
-  // j->green / j->Sinv hold G and S^-1 at Q.  wavefunctions.py:1128-1150.
+  // j->green / j->Sinv hold G and S^-1 at Q.  wavefunctions.py:1128-1150, 1331-1363.
+  // E_L = cs sum_s tr(rot_h1_s G_s^T) + (ccoul / 2) sum_g c_g^2 - (cs / 2) sum_sg K_sg,
+  // c_g = sum_s tr f_sg, K_sg = sum_ij f_sg,ij f_sg,ji, f_sg = rot_chol_sg G_s^T.
   int energy_backward() {
     const int Wc = b.Wc;
-    for (long w0 = 0; w0 < W; w0 += Wc) {
-      const int wc = (int)std::min<long>(Wc, W - w0);
-      const double2* Gc = j->green[0] + w0 * per;
-      const long fsz = G * O * wc * O;
-      // f[g][i][(w,j)] = rc_g G_w^T, all gammas in one product: rows (g, i) of rot_chol
-      AFQ_TRY(GB((int)(G * O), (int)(wc * O), N)
-                  .a(j->rot_chol[0], N, 0, false)
-                  .b(Gc, N, 0, true, true)
-                  .c(b.F, wc * O, 0)
-                  .run(st));
-      LAUNCH(adj_fscale_kernel, fsz, b.F, b.kappa + w0, (int)G, (int)O, wc, b.Fp);
-      LAUNCH(adj_cc_kernel, (long)wc * G, b.F, (int)G, (int)O, wc, b.Cc);
-      // gG_w = sum_g 4 kappa c_g rc_g - 2 kappa sum_g f_g rc_g + 2 kappa rot_h1
-      AFQ_TRY(GB(wc, per, G).a(b.Cc, G, 0, true).b(j->rot_chol[0], per, 0, false).c(b.gG + w0 * per, per, 0).run(st));
-      // all wc walkers in one product (row (w, i) of Fp starts at (w O + i) G O), split over K
-      AFQ_TRY(GB((int)(wc * O), N, G * O)
-                  .a(b.Fp, G * O, 0, true)
-                  .b(j->rot_chol[0], N, 0, false)
-                  .c(b.gG + w0 * per, N, 0)
-                  .beta(1.0)
+    // Coulomb adjoint Cc[w][g] = ccoul kappa_w c_wg, c from one product per spin over all walkers
+    for (int s = 0; s < sp.nsp; ++s)
+      AFQ_TRY(GB(W, G, sp.pw[s])
+                  .a(j->green[s], sp.pw[s], 0, true)
+                  .b(j->rot_chol[s], sp.pw[s], 0, false, true)
+                  .c(b.Ct, G, 0)
+                  .beta(s > 0 ? 1.0 : 0.0)
                   .ws(j)
                   .run(st));
-      LAUNCH(adj_outer_kernel, wc * per, b.gG + w0 * per, b.kappa + w0, j->rot_h1[0], 2.0, per, wc * per);
-      // rc_g adjoint: sum_w 4 kappa c_wg G_w - 2 kappa f_wg^T G_w ; rot_h1 adjoint: 2 sum_w kappa G_w
-      AFQ_TRY(GB(G, per, wc).a(b.Cc, G, 0, true, true).b(Gc, per, 0, true).c(b.rcbar, per, 0).beta(1.0).part(1).run(st));
-      // rows (g, j) of rot_chol: sum over (w, i) of Fp[w][i][g][j] G_w[i][n], one product
-      AFQ_TRY(GB((int)(G * O), N, (int)(wc * O))
-                  .a(b.Fp, G * O, 0, true, true)
-                  .b(Gc, N, 0, true)
-                  .c(b.rcbar, N, 0)
-                  .beta(1.0)
-                  .part(1)
-                  .run(st));
-      AFQ_TRY(GB(1, per, wc).a(b.kappa + w0, wc, 0, false).b(Gc, per, 0, true).c(b.rh1barC, per, 0).alpha(2.0).beta(1.0).run(
-          st));
+    LAUNCH(adj_cscale_kernel, W * G, b.Ct, b.kappa, ccoul, (int)G, W * G, b.Cc);
+    for (int s = 0; s < sp.nsp; ++s) {
+      const long O = sp.O[s], per = sp.pw[s];
+      double* rcbar = b.rcbar + (s ? G * sp.pw[0] : 0);
+      double2* rh1bar = b.rh1barC + (s ? sp.pw[0] : 0);
+      for (long w0 = 0; w0 < W; w0 += Wc) {
+        const int wc = (int)std::min<long>(Wc, W - w0);
+        const double2* Gc = j->green[s] + w0 * per;
+        double2* gG = wsp(b.gG, s) + w0 * per;
+        const double2* Cc = b.Cc + w0 * G;
+        const long fsz = G * O * wc * O;
+        // f[g][i][(w,j)] = rc_g G_w^T, all gammas in one product: rows (g, i) of rot_chol
+        AFQ_TRY(GB((int)(G * O), (int)(wc * O), N)
+                    .a(j->rot_chol[s], N, 0, false)
+                    .b(Gc, N, 0, true, true)
+                    .c(b.F, wc * O, 0)
+                    .run(st));
+        LAUNCH(adj_fscale_kernel, fsz, b.F, b.kappa + w0, -cs, (int)G, (int)O, wc, b.Fp);
+        // gG_w = sum_g Cc_wg rc_g - cs kappa sum_g f_g rc_g + cs kappa rot_h1
+        AFQ_TRY(GB(wc, per, G).a(Cc, G, 0, true).b(j->rot_chol[s], per, 0, false).c(gG, per, 0).run(st));
+        // all wc walkers in one product (row (w, i) of Fp starts at (w O + i) G O), split over K
+        AFQ_TRY(GB((int)(wc * O), N, G * O)
+                    .a(b.Fp, G * O, 0, true)
+                    .b(j->rot_chol[s], N, 0, false)
+                    .c(gG, N, 0)
+                    .beta(1.0)
+                    .ws(j)
+                    .run(st));
+        LAUNCH(adj_outer_kernel, wc * per, gG, b.kappa + w0, j->rot_h1[s], cs, per, wc * per);
+        // rc_g adjoint: sum_w Cc_wg G_w - cs kappa f_wg^T G_w ; rot_h1 adjoint: cs sum_w kappa G_w
+        AFQ_TRY(GB(G, per, wc).a(Cc, G, 0, true, true).b(Gc, per, 0, true).c(rcbar, per, 0).beta(1.0).part(1).run(st));
+        // rows (g, j) of rot_chol: sum over (w, i) of Fp[w][i][g][j] G_w[i][n], one product
+        AFQ_TRY(GB((int)(G * O), N, (int)(wc * O))
+                    .a(b.Fp, G * O, 0, true, true)
+                    .b(Gc, N, 0, true)
+                    .c(rcbar, N, 0)
+                    .beta(1.0)
+                    .part(1)
+                    .run(st));
+        AFQ_TRY(GB(1, per, wc).a(b.kappa + w0, wc, 0, false).b(Gc, per, 0, true).c(rh1bar, per, 0).alpha(cs).beta(1.0).run(
+            st));
+      }
     }
     return AFQ_OK;
   }
@@ -659,56 +725,69 @@ struct Seg {
   // On entry b.gW / b.gw / scal[0] hold the adjoints of W_{k+1}, w_{k+1}, E_{k+1};
   // on exit those of W_k, w_k, E_k.
   int step_backward(int k) {
-    const long OO = O * O;
-    double2* Wk = t.W + k * W * per;
-    double2* Wn = t.W + (k + 1) * W * per;
+    double2* Wk = t.W + k * sp.ws;
+    double2* Wn = t.W + (k + 1) * sp.ws;
     const double2* xt = t.xt + k * W * G;
     hipLaunchKernelGGL(adj_weight_kernel, dim3(1), dim3(1024), 0, st, t.ov + k * W, t.ov + (k + 1) * W,
                        t.terms + k * 2 * W, t.w + k * W, t.w + (k + 1) * W, t.sc + 2 * k, j->h0_prop, dt, (int)W,
                        b.gw, b.scal, b.gd, b.alpha, b.scal + 1);
     AFQ_TRY(launch_status("adj_weight_kernel"));
-    // log ov' = 2 log det(T^T W'): gW' += T (2 gd S'^-T)
-    AFQ_TRY(green_pipeline(j, Wn, nullptr, true, false, st));
-    LAUNCH(adj_scaled_transpose_kernel, W * OO, j->Sinv[0], b.gd, 2.0, (int)O, W * OO, b.X);
-    AFQ_TRY(GB(N, O, O, W).a(T(), O, 0, false).b(b.X, O, OO, true).c(b.gW, O, per).beta(1.0).run(st));
+    // log ov' = cs sum_s log det(T_s^T W'_s): gW'_s += T_s (cs gd S'_s^-T)
+    AFQ_TRY(green(Wn, false));
+    for (int s = 0; s < sp.nsp; ++s) {
+      const long O = sp.O[s], OO = O * O;
+      LAUNCH(adj_scaled_transpose_kernel, W * OO, j->Sinv[s], b.gd, cs, (int)O, W * OO, qsp(b.X, s));
+      AFQ_TRY(GB(N, O, O, W).a(T(s), O, 0, false).b(qsp(b.X, s), O, OO, true).c(wsp(b.gW, s), O, sp.pw[s]).beta(1.0).run(
+          st));
+    }
     LAUNCH(adj_gxt_init_kernel, W * G, b.gd, j->mf_shifts, sqdt, (int)G, W * G, b.gxt);
-    // recompute VHS, the Taylor terms and their sum
-    const size_t wbytes = sizeof(double2) * W * per;
-    AFQ_TRY(GB(N, O, N, W).a(E(), N, 0, false).b(Wk, O, per, true).c(b.Tb, O, per).run(st));
-    AFQ_TRY(cpy(b.W2, b.Tb, wbytes));
-    if (K > 0) {
+    // recompute VHS, the Taylor terms and their sum, per spin
+    if (K > 0)
       AFQ_TRY(GB(W, NN, G).a(xt, G, 0, true).b(j->chol, NN, 0, false).c(j->vhs, NN, 0).alpha(0.0, sqdt).run(st));
+    for (int s = 0; s < sp.nsp; ++s) {
+      const long O = sp.O[s], per = sp.pw[s];
+      double2* Tb0 = wsp(b.Tb, s);
+      AFQ_TRY(GB(N, O, N, W).a(E(s), N, 0, false).b(wsp(Wk, s), O, per, true).c(Tb0, O, per).run(st));
+      AFQ_TRY(cpy(wsp(b.W2, s), Tb0, sizeof(double2) * W * per));
       for (int n = 1; n <= K; ++n)
         AFQ_TRY(GB(N, O, N, W)
                     .a(j->vhs, N, NN, true)
-                    .b(b.Tb + (n - 1) * W * per, O, per, true)
-                    .c(b.Tb + n * W * per, O, per)
-                    .d(b.W2, O, per)
+                    .b(Tb0 + (n - 1) * sp.ws, O, per, true)
+                    .c(Tb0 + n * sp.ws, O, per)
+                    .d(wsp(b.W2, s), O, per)
                     .alpha(1.0 / n)
                     .run(st));
     }
-    // W' = E W2
-    AFQ_TRY(GB(N, O, N, W).a(E(), N, 0, false, true).b(b.gW, O, per, true).c(b.gW2, O, per).run(st));
-    AFQ_TRY(GB(N, N, O, W).a(b.gW, O, per, true).b(b.W2, O, per, true, true).c(b.EbarW, N, NN).beta(1.0).run(st));
-    // Taylor: gT_K = gW2 ; gT_{n-1} = gW2 + vhs^T gT_n / n ; gvhs = sum_n gT_n T_{n-1}^T / n
-    const double2* cur = b.gW2;
-    for (int n = K; n >= 1; --n) {
-      AFQ_TRY(GB(N, N, O, W)
-                  .a(cur, O, per, true)
-                  .b(b.Tb + (n - 1) * W * per, O, per, true, true)
-                  .c(b.gvhs, N, NN)
-                  .alpha(1.0 / n)
-                  .beta(n == K ? 0.0 : 1.0)
-                  .run(st));
-      double2* nxt = (cur == b.gTa) ? b.gTb : b.gTa;
-      AFQ_TRY(cpy(nxt, b.gW2, wbytes));
-      AFQ_TRY(GB(N, O, N, W).a(j->vhs, N, NN, true, true).b(cur, O, per, true).c(nxt, O, per).alpha(1.0 / n).beta(1.0).run(
+    for (int s = 0; s < sp.nsp; ++s) {
+      const long O = sp.O[s], per = sp.pw[s];
+      double2* Tb0 = wsp(b.Tb, s);
+      double2* EbarW = b.EbarW + s * W * NN;
+      // W' = E W2
+      AFQ_TRY(GB(N, O, N, W).a(E(s), N, 0, false, true).b(wsp(b.gW, s), O, per, true).c(wsp(b.gW2, s), O, per).run(st));
+      AFQ_TRY(GB(N, N, O, W).a(wsp(b.gW, s), O, per, true).b(wsp(b.W2, s), O, per, true, true).c(EbarW, N, NN).beta(1.0).run(
           st));
-      cur = nxt;
+      // Taylor: gT_K = gW2 ; gT_{n-1} = gW2 + vhs^T gT_n / n ; gvhs = sum_n gT_n T_{n-1}^T / n
+      const double2* cur = wsp(b.gW2, s);
+      double2* ga = wsp(b.gTa, s);
+      double2* gb = wsp(b.gTb, s);
+      for (int n = K; n >= 1; --n) {
+        AFQ_TRY(GB(N, N, O, W)
+                    .a(cur, O, per, true)
+                    .b(Tb0 + (n - 1) * sp.ws, O, per, true, true)
+                    .c(b.gvhs, N, NN)
+                    .alpha(1.0 / n)
+                    .beta(n == K && s == 0 ? 0.0 : 1.0)
+                    .run(st));
+        double2* nxt = (cur == ga) ? gb : ga;
+        AFQ_TRY(cpy(nxt, wsp(b.gW2, s), sizeof(double2) * W * per));
+        AFQ_TRY(GB(N, O, N, W).a(j->vhs, N, NN, true, true).b(cur, O, per, true).c(nxt, O, per).alpha(1.0 / n).beta(1.0).run(
+            st));
+        cur = nxt;
+      }
+      // W1 = E W_k
+      AFQ_TRY(GB(N, N, O, W).a(cur, O, per, true).b(wsp(Wk, s), O, per, true, true).c(EbarW, N, NN).beta(1.0).run(st));
+      AFQ_TRY(GB(N, O, N, W).a(E(s), N, 0, false, true).b(cur, O, per, true).c(wsp(b.gWk, s), O, per).run(st));
     }
-    // W1 = E W_k
-    AFQ_TRY(GB(N, N, O, W).a(cur, O, per, true).b(Wk, O, per, true, true).c(b.EbarW, N, NN).beta(1.0).run(st));
-    AFQ_TRY(GB(N, O, N, W).a(E(), N, 0, false, true).b(cur, O, per, true).c(b.gWk, O, per).run(st));
     // vhs = i sqrt(dt) sum_g xt_g L_g
     if (K > 0) {
       AFQ_TRY(GB(W, G, NN).a(b.gvhs, NN, 0, true).b(j->chol, NN, 0, false, true).c(b.gxt, G, 0).alpha(0.0, sqdt).beta(1.0).ws(
@@ -726,17 +805,20 @@ struct Seg {
     hipLaunchKernelGGL(adj_fb_kernel, dim3((unsigned)((G + 63) / 64)), dim3(1024), 0, st, xt, b.gxt, b.gd, b.alpha, sqdt,
                        (int)G, (int)W, b.gfb, b.mbar);
     AFQ_TRY(launch_status("adj_fb_kernel"));
-    // fb = 2 sum rc . G at W_k
-    AFQ_TRY(green_pipeline(j, Wk, nullptr, true, true, st));
-    AFQ_TRY(GB(W, per, G).a(b.gfb, G, 0, true).b(j->rot_chol[0], per, 0, false).c(b.gG, per, 0).alpha(2.0).run(st));
-    AFQ_TRY(GB(G, per, W)
-                .a(b.gfb, G, 0, true, true)
-                .b(j->green[0], per, 0, true)
-                .c(b.rcbar, per, 0)
-                .alpha(2.0)
-                .beta(1.0)
-                .part(1)
-                .run(st));
+    // fb = cs sum_s rc_s . G_s at W_k
+    AFQ_TRY(green(Wk, true));
+    for (int s = 0; s < sp.nsp; ++s) {
+      const long per = sp.pw[s];
+      AFQ_TRY(GB(W, per, G).a(b.gfb, G, 0, true).b(j->rot_chol[s], per, 0, false).c(wsp(b.gG, s), per, 0).alpha(cs).run(st));
+      AFQ_TRY(GB(G, per, W)
+                  .a(b.gfb, G, 0, true, true)
+                  .b(j->green[s], per, 0, true)
+                  .c(b.rcbar + (s ? G * sp.pw[0] : 0), per, 0)
+                  .alpha(cs)
+                  .beta(1.0)
+                  .part(1)
+                  .run(st));
+    }
     // Green's function and -log ov at W_k
     AFQ_TRY(green_backward(b.gG, b.gd, b.gWk));
     std::swap(b.gW, b.gWk);
@@ -744,11 +826,11 @@ struct Seg {
   }
 
   int reverse() {
-    const size_t wbytes = sizeof(double2) * W * per;
+    const size_t wbytes = sizeof(double2) * sp.ws;
     AFQ_TRY(zero(b.Lbar, sizeof(double) * G * NN));
-    AFQ_TRY(zero(b.rcbar, sizeof(double) * G * per));
-    AFQ_TRY(zero(b.rh1barC, sizeof(double2) * per));
-    AFQ_TRY(zero(b.EbarW, sizeof(double2) * W * NN));
+    AFQ_TRY(zero(b.rcbar, sizeof(double) * G * sp.pt));
+    AFQ_TRY(zero(b.rh1barC, sizeof(double2) * sp.pt));
+    AFQ_TRY(zero(b.EbarW, sizeof(double2) * sp.nsp * W * NN));
     AFQ_TRY(zero(b.mbar, sizeof(double) * G));
     AFQ_TRY(zero(b.scal, sizeof(double) * 4));
     AFQ_TRY(zero(b.gW, wbytes));
@@ -757,27 +839,34 @@ struct Seg {
       if (!keep) AFQ_TRY(tape(blk));
       t = tape_at(blk);
       // block end: Q = qr(W_end), energies at Q (recorded, or recomputed)
-      double2* Wend = t.W + ns * W * per;
+      double2* Wend = t.W + ns * sp.ws;
       if (keep) {
-        AFQ_TRY(cpy(b.Q, b.qk + blk * W * per, wbytes));
+        AFQ_TRY(cpy(b.Q, b.qk + blk * sp.ws, wbytes));
         AFQ_TRY(cpy(j->energies, b.ek + blk * W, sizeof(double2) * W));
-        AFQ_TRY(green_pipeline(j, b.Q, nullptr, true, true, st));
+        AFQ_TRY(green(b.Q, true));
       } else {
         AFQ_TRY(cpy(b.Q, Wend, wbytes));
-        AFQ_TRY(orthonormalize(j, b.Q, nullptr, st));
-        AFQ_TRY(energy_pipeline(j, b.Q, nullptr, j->energies, st));
+        AFQ_TRY(orthonormalize(j, b.Q, dn(b.Q), st));
+        AFQ_TRY(energy_pipeline(j, b.Q, dn(b.Q), j->energies, st));
       }
       hipLaunchKernelGGL(adj_block_end_kernel, dim3(1), dim3(1024), 0, st, j->energies, t.w + ns * W, t.sc,
                          b.bo + 2 * blk, b.seg, std::sqrt(2.0 / dt), (int)W, b.scal, b.kappa, b.gw);
       AFQ_TRY(launch_status("adj_block_end_kernel"));
       AFQ_TRY(energy_backward());
       AFQ_TRY(green_backward(b.gG, nullptr, b.gW));  // gQ = incoming gW + energy part
-      // gauge: Q = W_end R^-1 with R = Q^H W_end ; g_Wend = g_Q R^-T
-      const long OO = O * O;
-      LAUNCH(adj_conj_kernel, W * per, b.Q, b.Qc, W * per);
-      AFQ_TRY(GB(O, O, N, W).a(b.Qc, O, per, true, true).b(Wend, O, per, true).c(b.R, O, OO).run(st));
-      AFQ_TRY(batched_det_inv(b.R, OO, (int)O, (int)W, b.detR, b.Rinv, OO, st));
-      AFQ_TRY(GB(N, O, O, W).a(b.gW, O, per, true).b(b.Rinv, O, OO, true, true).c(b.gWk, O, per).run(st));
+      // gauge per spin: Q = W_end R^-1 with R = Q^H W_end ; g_Wend = g_Q R^-T
+      LAUNCH(adj_conj_kernel, sp.ws, b.Q, b.Qc, sp.ws);
+      for (int s = 0; s < sp.nsp; ++s) {
+        const long O = sp.O[s], per = sp.pw[s], OO = O * O;
+        AFQ_TRY(GB(O, O, N, W).a(wsp(b.Qc, s), O, per, true, true).b(wsp(Wend, s), O, per, true).c(qsp(b.R, s), O, OO).run(
+            st));
+        AFQ_TRY(batched_det_inv(qsp(b.R, s), OO, (int)O, (int)W, b.detR + s * W, qsp(b.Rinv, s), OO, st));
+        AFQ_TRY(GB(N, O, O, W)
+                    .a(wsp(b.gW, s), O, per, true)
+                    .b(qsp(b.Rinv, s), O, OO, true, true)
+                    .c(wsp(b.gWk, s), O, per)
+                    .run(st));
+      }
       std::swap(b.gW, b.gWk);
       for (int k = ns - 1; k >= 0; --k) AFQ_TRY(step_backward(k));
     }
@@ -785,35 +874,53 @@ struct Seg {
   }
 
   // Setup intermediates pulled back to h1 and L (wavefunctions.py:1215-1232,
-  // propagation.py:378-438).
+  // 1472-1496, propagation.py:378-438).
   int setup_backward(double* h1_bar, double* chol_bar) {
-    const long per_ = per;
-    LAUNCH(adj_reduce_re_kernel, NN, b.EbarW, (int)W, NN, b.Ebar);
-    // h1_mod = (h1_0 + h1_1)/2 - v0 - v1, A = -dt/2 h1_mod
+    double* hbar_tot = b.hmodbar + (sp.nsp - 1) * NN;  // sum over the spins' h1_mod adjoints
     AFQ_TRY(compute_v0(j, b.v0, st));
     AFQ_TRY(GB(1, NN, G).a(j->mreal + G, G, 0, false).b(j->chol, NN, 0, false).c(b.v1, NN, 0).run(st));
-    AFQ_TRY(k_axpby4(b.hmod, j->h1, j->h1 + NN, 0.5, b.v0, b.v1, -1.0, NN, st));
-    AFQ_TRY(k_scale(b.Amat, b.hmod, -0.5 * dt, (int)N, st));
-    LAUNCH(adj_block_matrix_kernel, 4 * NN, b.Amat, b.Ebar, (int)N, b.Mblk);
-    AFQ_TRY(expm_device(b.Mblk, 2 * (int)N, b.Mexp, b.Mwork, j->dscratch, st));
-    LAUNCH(adj_extract_ur_kernel, NN, b.Mexp, (int)N, -0.5 * dt, b.hmodbar);
-    // h1s adjoint = hmodbar + T rot_h1bar
-    LAUNCH(adj_re_kernel, per_, b.rh1barC, per_, b.rh1bar);
-    AFQ_TRY(cpy(b.h1sbar, b.hmodbar, sizeof(double) * NN));
-    AFQ_TRY(GB(N, N, O).a(T(), O, 0, false).b(b.rh1bar, N, 0, false).c(b.h1sbar, N, 0).beta(1.0).run(st));
-    LAUNCH(adj_h1_out_kernel, NN, b.h1sbar, (int)N, h1_bar);
-    // m adjoint: + h0_prop' m (h0_prop = -h0 + sum m^2/2) + sum L_g . hmodbar (v1 = -sum m_g L_g)
-    AFQ_TRY(GB(G, 1, NN).a(j->chol, NN, 0, false).b(b.hmodbar, 1, 0, false).c(b.tg, 1, 0).run(st));
+    for (int s = 0; s < sp.nsp; ++s) {
+      double* hmodbar = b.hmodbar + s * NN;
+      LAUNCH(adj_reduce_re_kernel, NN, b.EbarW + s * W * NN, (int)W, NN, b.Ebar);
+      // h1_mod = (h1_0 + h1_1)/2 - v0 - v1 (restricted walkers) or h1_s - v0 - v1, A = -dt/2 h1_mod
+      if (sp.nsp == 1)
+        AFQ_TRY(k_axpby4(b.hmod, j->h1, j->h1 + NN, 0.5, b.v0, b.v1, -1.0, NN, st));
+      else
+        AFQ_TRY(k_axpby4(b.hmod, j->h1 + s * NN, nullptr, 1.0, b.v0, b.v1, -1.0, NN, st));
+      AFQ_TRY(k_scale(b.Amat, b.hmod, -0.5 * dt, (int)N, st));
+      LAUNCH(adj_block_matrix_kernel, 4 * NN, b.Amat, b.Ebar, (int)N, b.Mblk);
+      AFQ_TRY(expm_device(b.Mblk, 2 * (int)N, b.Mexp, b.Mwork, j->dscratch, st));
+      LAUNCH(adj_extract_ur_kernel, NN, b.Mexp, (int)N, -0.5 * dt, hmodbar);
+      // h1s adjoint = hmodbar + T rot_h1bar
+      const long per = sp.pw[s];
+      double* rh1bar = b.rh1bar + (s ? sp.pw[0] : 0);
+      LAUNCH(adj_re_kernel, per, b.rh1barC + (s ? sp.pw[0] : 0), per, rh1bar);
+      AFQ_TRY(cpy(b.h1sbar, hmodbar, sizeof(double) * NN));
+      AFQ_TRY(GB(N, N, sp.O[s]).a(T(s), sp.O[s], 0, false).b(rh1bar, N, 0, false).c(b.h1sbar, N, 0).beta(1.0).run(st));
+      if (sp.nsp == 1)
+        LAUNCH(adj_h1_out_kernel, NN, b.h1sbar, (int)N, h1_bar);
+      else
+        LAUNCH(adj_h1_out_spin_kernel, NN, b.h1sbar, (int)N, h1_bar + s * NN);
+    }
+    if (sp.nsp == 2) AFQ_TRY(k_axpby4(hbar_tot, b.hmodbar, b.hmodbar + NN, 1.0, nullptr, nullptr, 0.0, NN, st));
+    // m adjoint: + h0_prop' m (h0_prop = -h0 + sum m^2/2) + sum L_g . hbar (v1 = -sum m_g L_g)
+    AFQ_TRY(GB(G, 1, NN).a(j->chol, NN, 0, false).b(hbar_tot, 1, 0, false).c(b.tg, 1, 0).run(st));
     LAUNCH(adj_mbar_final_kernel, G, b.mbar, j->mreal, b.scal + 1, b.tg, (int)G);
     // L adjoint
     AFQ_TRY(cpy(chol_bar, b.Lbar, sizeof(double) * G * NN));
-    AFQ_TRY(GB(N, N, N, G).a(b.hmodbar, N, 0, false).b(j->chol, N, NN, false, true).c(chol_bar, N, NN).alpha(-0.5).beta(
+    AFQ_TRY(GB(N, N, N, G).a(hbar_tot, N, 0, false).b(j->chol, N, NN, false, true).c(chol_bar, N, NN).alpha(-0.5).beta(
         1.0).run(st));
-    AFQ_TRY(GB(N, N, N, G).a(j->chol, N, NN, false, true).b(b.hmodbar, N, 0, false).c(chol_bar, N, NN).alpha(-0.5).beta(
+    AFQ_TRY(GB(N, N, N, G).a(j->chol, N, NN, false, true).b(hbar_tot, N, 0, false).c(chol_bar, N, NN).alpha(-0.5).beta(
         1.0).run(st));
-    AFQ_TRY(GB(N, N, O, G).a(T(), O, 0, false).b(b.rcbar, N, per_, false).c(chol_bar, N, NN).beta(1.0).run(st));
+    for (int s = 0; s < sp.nsp; ++s)
+      AFQ_TRY(GB(N, N, sp.O[s], G)
+                  .a(T(s), sp.O[s], 0, false)
+                  .b(b.rcbar + (s ? G * sp.pw[0] : 0), N, sp.pw[s], false)
+                  .c(chol_bar, N, NN)
+                  .beta(1.0)
+                  .run(st));
     AFQ_TRY(k_axpby4(b.rho, j->rdm1, j->rdm1 + NN, 1.0, nullptr, nullptr, 0.0, NN, st));
-    LAUNCH(adj_lfinal_kernel, G * NN, chol_bar, j->mreal, b.mbar, b.hmodbar, b.rho, (int)G, NN);
+    LAUNCH(adj_lfinal_kernel, G * NN, chol_bar, j->mreal, b.mbar, hbar_tot, b.rho, (int)G, NN);
     return AFQ_OK;
   }
 };
@@ -833,33 +940,39 @@ extern "C" int afq_segment_energy_vjp(afq_job_t j, const void* walkers, const do
   if ((h1_bar == nullptr) != (chol_bar == nullptr))
     return fail(AFQ_E_INVALID, "h1_bar and chol_bar must both be given or both be NULL");
   if (n_blocks < 1 || n_steps < 1) return fail(AFQ_E_INVALID, "n_blocks and n_steps must be >= 1");
-  if (!j->rhf || !j->restricted) return fail(AFQ_E_UNSUPPORTED, "segment derivative: RHF trial and restricted walkers only");
+  if (j->rhf != j->restricted)
+    return fail(AFQ_E_UNSUPPORTED,
+                "segment derivative: RHF trial with restricted walkers, or UHF trial with unrestricted walkers");
   if (j->lp) return fail(AFQ_E_UNSUPPORTED, "segment derivative: FP64 jobs only");
-  if (j->have_rot_h1[0] || j->have_rot_chol[0] || j->have_mf || j->have_h0p || j->have_exp[0])
+  for (int s = 0; s < 2; ++s)
+    if (j->have_rot_h1[s] || j->have_rot_chol[s] || j->have_exp[s])
+      return fail(AFQ_E_STATE, "segment derivative needs intermediates built from the job's h1 and chol");
+  if (j->have_mf || j->have_h0p)
     return fail(AFQ_E_STATE, "segment derivative needs intermediates built from the job's h1 and chol");
   Seg s;
   s.j = j;
   s.st = reinterpret_cast<hipStream_t>(stream);
+  s.sp.init(j);
   s.nb = n_blocks;
   s.ns = n_steps;
   s.K = j->d.n_exp_terms - 1;
   s.N = j->N;
   s.G = j->G;
   s.W = j->W;
-  s.O = j->nocc[0];
-  s.per = s.N * s.O;
   s.NN = s.N * s.N;
   s.dt = j->d.dt;
   s.sqdt = std::sqrt(j->d.dt);
   s.e_est = e_estimate;
   s.fields = fields;
+  s.cs = s.sp.nsp == 1 ? 2.0 : 1.0;
+  s.ccoul = s.sp.nsp == 1 ? 4.0 : 1.0;
   // Record every block's tapes in the forward (instead of recomputing them
   // block by block in the reverse sweep) when that arena takes at most half of
   // the free device memory, or AFQ_SEGMENT_TAPE_LIMIT bytes when that is set.
   Arena sizer;
   bool keep = false;
   if (h1_bar) {
-    carve(sizer, s.b, j, n_blocks, n_steps, s.K, true);
+    carve(sizer, s.b, j, s.sp, n_blocks, n_steps, s.K, true);
     if (n_blocks == 1) {
       keep = true;  // a single block's tape takes the same memory either way
     } else {
@@ -872,14 +985,14 @@ extern "C" int afq_segment_energy_vjp(afq_job_t j, const void* walkers, const do
   }
   if (!keep) {
     sizer = Arena{};
-    carve(sizer, s.b, j, n_blocks, n_steps, s.K, false);
+    carve(sizer, s.b, j, s.sp, n_blocks, n_steps, s.K, false);
   }
   s.keep = keep;
   void* base = nullptr;
   AFQ_TRY(dalloc(&base, sizer.off));
   Arena ar;
   ar.base = static_cast<char*>(base);
-  carve(ar, s.b, j, n_blocks, n_steps, s.K, s.keep);
+  carve(ar, s.b, j, s.sp, n_blocks, n_steps, s.K, s.keep);
   int rc = s.forward(walkers, weights, energy_out);
   if (rc == AFQ_OK && h1_bar) rc = s.reverse();
   if (rc == AFQ_OK && h1_bar) rc = s.setup_backward(h1_bar, chol_bar);

@@ -335,17 +335,33 @@ class EngineJob:
             stream_ptr(),
         )
 
-    def segment_energy_vjp(self, walkers, weights, e_estimate: float, fields, want_grad: bool = True):
+    def segment_energy_vjp(self, walkers, weights, e_estimate: float, fields, want_grad: bool = True,
+                           walkers_dn=None):
         """Segment energy F over fields (n_blocks, n_steps, W, G) and, with
         want_grad, dF/dh1 (2,N,N) and dF/dchol (G,N*N) (afq_segment_energy_vjp).
+        RHF trial with restricted walkers (W,N,nocc), or UHF trial with
+        unrestricted walkers (walkers (W,N,nocc_up) and walkers_dn (W,N,nocc_dn)).
         Synchronises the stream."""
-        if not self.restricted:
-            raise NotImplementedError("the segment derivative supports restricted walkers only")
+        if self.restricted != (self.trial_type == "rhf"):
+            raise NotImplementedError("the segment derivative takes an RHF trial with restricted walkers "
+                                      "or a UHF trial with unrestricted walkers")
         f = as_f64(fields)
         if f.dim() != 4 or f.shape[2] != self.batch or f.shape[3] != self.nchol:
             raise ValueError(f"fields must be (n_blocks, n_steps, {self.batch}, {self.nchol}), got {tuple(f.shape)}")
+        su, sd = walker_shapes(self.norb, self.nelec, self.walker_type, self.batch)
         w = as_c128(walkers)
+        check_tensor("walkers" if sd is None else "walkers_up", w, su, torch.complex128)
+        if sd is None:
+            if walkers_dn is not None:
+                raise ValueError("restricted walkers take one buffer (walkers_dn must be None)")
+        else:
+            if walkers_dn is None:
+                raise ValueError("unrestricted walkers need walkers_dn")
+            wd = as_c128(walkers_dn)
+            check_tensor("walkers_dn", wd, sd, torch.complex128)
+            w = torch.cat([w.reshape(-1), wd.reshape(-1)])  # spin 1's panel after spin 0's
         wt = as_f64(weights)
+        check_tensor("weights", wt, (self.batch,), torch.float64)
         out = torch.empty(1, dtype=torch.float64, device=w.device)
         gh1 = gL = None
         if want_grad:

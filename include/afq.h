@@ -274,7 +274,9 @@ int afq_group_destroy(afq_group_t group);
  * over n_blocks blocks of n_steps phaseless steps (afq_block's step, QR,
  * overlap, energy and block bookkeeping) with the given fields
  * (n_blocks, n_steps, W, G) and no population control, starting from
- * `walkers` (restricted, not modified) and `weights` (W,), with the
+ * `walkers` (not modified; restricted walkers (W, N, nocc), or for
+ * unrestricted walkers spin up's (W, N, nocc_up) panel followed by spin
+ * down's (W, N, nocc_dn) in the same buffer) and `weights` (W,), with the
  * population-control shift starting at, and the energy clipping centred on,
  * e_estimate.  The intermediates are
  * functions of the job's h1 and Cholesky vectors (they must have been built by
@@ -283,7 +285,8 @@ int afq_group_destroy(afq_group_t group);
  *   energy_out (1,) device, h1_bar (2, N, N), chol_bar (G, N*N).
  * h1_bar = chol_bar = NULL evaluates F only.  The fields are constants (the
  * reference draws them from a key that carries no tangent); clipping masks
- * are held fixed.  RHF trial, restricted walkers, FP64 jobs.  Allocates a
+ * are held fixed.  RHF trial with restricted walkers or UHF trial with
+ * unrestricted walkers, FP64 jobs.  Allocates a
  * device workspace for the call and synchronises the stream.  With a gradient
  * requested, the steps' tapes of every block are kept from the forward sweep
  * when that workspace takes at most half of the free device memory (or at
