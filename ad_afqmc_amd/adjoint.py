@@ -63,7 +63,9 @@ class SegmentEnergy(torch.autograd.Function):
 
 def segment_energy(h1: torch.Tensor, chol: torch.Tensor, mo, walkers, weights, e_estimate: float, fields, dt: float,
                    h0: float = 0.0, n_exp_terms: int = 6) -> torch.Tensor:
-    """Differentiable segment energy (RHF trial, restricted walkers, FP64).
+    """Differentiable segment energy (RHF trial, restricted walkers, FP64; a
+    UHF trial with unrestricted walkers goes through
+    EngineJob.segment_energy_vjp with walkers_dn).
 
     h1 (2,N,N), chol (G,N*N), mo (N,O), walkers (W,N,O) c128, weights (W,),
     fields (n_blocks, n_steps, W, G)."""

@@ -129,6 +129,7 @@ def test_uhf_vjp_matches_central_differences():
         fd = (_gpu_vjp_uhf(cp, False)[0] - _gpu_vjp_uhf(cm, False)[0]) / (2 * h)
         an = float(np.sum(gh1 * D))
         print(f"[adjoint] UHF h1 direction {trial}: fd={fd:.12e} an={an:.12e}", flush=True)
+        assert abs(an) > 1e-3  # a derivative the check can resolve
         assert abs(fd - an) <= 1e-6 * max(1.0, abs(an))
         E = rng.standard_normal((G, N, N))
         E = ((E + E.transpose(0, 2, 1)) / 2).reshape(G, N * N) * 0.01
@@ -137,6 +138,7 @@ def test_uhf_vjp_matches_central_differences():
         fd = (_gpu_vjp_uhf(cp, False)[0] - _gpu_vjp_uhf(cm, False)[0]) / (2 * h)
         an = float(np.sum(gL * E))
         print(f"[adjoint] UHF chol direction {trial}: fd={fd:.12e} an={an:.12e}", flush=True)
+        assert abs(an) > 1e-5
         assert abs(fd - an) <= 1e-6 * max(1.0, abs(an))
 
 
