@@ -590,32 +590,39 @@ def h10_timing(steps=20, warmup=3):
 
 def adjoint_timing(name="rhf_n200_o40_g1200_w512", n_blocks=2, n_steps=10, reps=2, l_coef=0.3):
     """Reverse pass of a phaseless segment (afq_segment_energy_vjp) on
-    workload `name` (RHF trial, restricted walkers, FP64): n_blocks blocks of
-    n_steps steps with given fields, the value alone and the value with dF/dh1
-    and dF/dchol."""
+    workload `name` (RHF trial with restricted walkers, or UHF trial with
+    unrestricted walkers; FP64): n_blocks blocks of n_steps steps with given
+    fields, the value alone and the value with dF/dh1 and dF/dchol."""
     import numpy as np
     import torch
 
-    from ad_afqmc_amd.adjoint import _job
+    from ad_afqmc_amd import synthetic as syn
+    from ad_afqmc_amd.engine import EngineJob
 
     cfg = syn_config(name)
-    N, O, G, W = cfg["norb"], cfg["nocc_up"], cfg["nchol"], cfg["n_walkers"]
+    N, G, W = cfg["norb"], cfg["nchol"], cfg["n_walkers"]
+    uhf = cfg["uhf"]
     s = make_system(name, l_coef)
-    from ad_afqmc_amd import synthetic as syn
-
-    walkers = torch.as_tensor(syn.perturbed_walkers(s.mo_up, W, 0.3 / math.sqrt(N), seed=1)).cuda()
+    mo_dn = s.mo_dn if uhf else s.mo_up
+    up = torch.as_tensor(syn.perturbed_walkers(s.mo_up, W, 0.3 / math.sqrt(N), seed=1)).cuda()
+    dn = torch.as_tensor(syn.perturbed_walkers(mo_dn, W, 0.3 / math.sqrt(N), seed=2)).cuda() if uhf else None
     weights = torch.ones(W, dtype=torch.float64, device="cuda")
     gen = torch.Generator(device="cuda").manual_seed(5)
     fields = torch.randn((n_blocks, n_steps, W, G), dtype=torch.float64, device="cuda", generator=gen)
-    job = _job(s.h0, np.array(s.h1), np.array(s.chol).reshape(G, N * N), s.mo_up, W, cfg["dt"], 6)
+    wtype, ttype = ("unrestricted", "uhf") if uhf else ("restricted", "rhf")
+    job = EngineJob(N, (s.mo_up.shape[1], mo_dn.shape[1]), G, W, wtype, ttype, 6, cfg["dt"], "fp64")
     try:
-        e_est = float(torch.mean(job.energy(walkers).real))
+        job.set_hamiltonian(float(s.h0), np.array(s.h1), np.array(s.chol).reshape(G, N * N))
+        job.set_trial(s.mo_up, mo_dn if uhf else None, s.rdm1())
+        job.build_measurement()
+        job.build_propagation()
+        e_est = float(torch.mean(job.energy(up, dn).real))
         out = {}
         for want in (False, True, False, True):  # the first pair warms up
             torch.cuda.synchronize()
             t0 = time.perf_counter()
             for _ in range(reps):
-                F, _, _ = job.segment_energy_vjp(walkers, weights, e_est, fields, want_grad=want)
+                F, _, _ = job.segment_energy_vjp(up, weights, e_est, fields, want_grad=want, walkers_dn=dn)
             torch.cuda.synchronize()
             out["with_gradient" if want else "value_only"] = (time.perf_counter() - t0) / reps * 1e3
     finally:
@@ -843,6 +850,8 @@ def main():
         line["h10_chain"]["cpu_baseline"] = h10_cpu
         print("[bench] segment reverse pass", file=sys.stderr, flush=True)
         line["segment_adjoint"] = adjoint_timing(l_coef=l_coef or 0.3)
+        torch.cuda.empty_cache()
+        line["segment_adjoint_uhf"] = adjoint_timing("uhf_n300_o60_58_g2000_w256", l_coef=l_coef or 0.3)
         torch.cuda.empty_cache()
     if args.serving_jobs > 0 and not args.only:
         from ad_afqmc_amd import serving
