@@ -405,3 +405,28 @@ def test_reduced_precision_against_oracle(case, precision):
     assert rel(npy(pd["weights"]), st["weights"]) < 10 * tol
     # the FP64-kept parts are untouched: overlap of the same walkers is exact
     assert rel(npy(trial.calc_overlap(walkers, wave_lp)), otrial.calc_overlap(owalk)) < TOL
+
+
+@pytest.mark.parametrize("kind", ["rhf", "uhf"])
+def test_init_walkers_span_the_trial_space(kind):
+    """The façade's get_init_walkers (wavefunctions.py:427-466) on the device:
+    the walkers' projector per spin is the trial's T T^T, as for the oracle's."""
+    print(f"[init walkers] {kind}", flush=True)
+    rng = np.random.default_rng(22)
+    N, nup, ndn, Wn = 10, 4, 3, 5
+    Tu = np.linalg.qr(rng.standard_normal((N, nup)))[0]
+    if kind == "rhf":
+        trial, wtype, nelec, Ts = wf.rhf(N, (nup, nup)), "restricted", (nup, nup), [Tu]
+        wave = {"mo_coeff": Tu, "rdm1": np.array([Tu @ Tu.T, Tu @ Tu.T])}
+    else:
+        Td = np.linalg.qr(rng.standard_normal((N, ndn)))[0]
+        trial, wtype, nelec, Ts = wf.uhf(N, (nup, ndn)), "unrestricted", (nup, ndn), [Tu, Td]
+        wave = {"mo_coeff": [Tu, Td], "rdm1": np.array([Tu @ Tu.T, Td @ Td.T])}
+    tensors = [t for t in trial.get_init_walkers(wave, Wn, wtype).tensors() if t is not None]
+    assert len(tensors) == len(Ts)
+    for t, T in zip(tensors, Ts):
+        W = t.cpu().numpy()
+        assert W.shape == (Wn, N, T.shape[1])
+        for w in W:
+            assert np.allclose(np.real(w) @ np.real(w).T, T @ T.T, atol=1e-12)
+            assert np.allclose(np.imag(w), 0.0)

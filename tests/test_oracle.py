@@ -257,3 +257,28 @@ def test_blocking_and_outliers():
     data[17, 1] = 1e3
     clean, mask = O.reject_outliers(data, 1)
     assert clean.shape[0] == 99 and not mask[17]
+
+
+@pytest.mark.parametrize("walker_type", ["restricted", "unrestricted"])
+def test_init_walkers_span_the_trial_space(walker_type):
+    """get_init_walkers (wavefunctions.py:427-466): the natural orbitals of the
+    trial 1-RDM span the trial's occupied space, so the walker projector equals
+    the trial projector T T^T per spin and every walker overlaps the trial."""
+    rng = np.random.default_rng(21)
+    N, nup, ndn, Wn = 9, 4, 3, 3
+    Tu = np.linalg.qr(rng.standard_normal((N, nup)))[0]
+    Td = Tu[:, :ndn] if walker_type == "restricted" else np.linalg.qr(rng.standard_normal((N, ndn)))[0]
+    rdm1 = np.array([Tu @ Tu.T, Td @ Td.T])
+    nelec = (nup, nup) if walker_type == "restricted" else (nup, ndn)
+    if walker_type == "restricted":
+        rdm1 = np.array([Tu @ Tu.T, Tu @ Tu.T])
+        spins = [(O.init_walkers(rdm1, nelec, Wn, walker_type), Tu)]
+    else:
+        up, dn = O.init_walkers(rdm1, nelec, Wn, walker_type)
+        spins = [(up, Tu), (dn, Td)]
+    for W, T in spins:
+        assert W.shape == (Wn, N, T.shape[1]) and np.iscomplexobj(W)
+        for w in W:
+            assert np.allclose(w, W[0])
+            assert np.allclose(np.real(w) @ np.real(w).T, T @ T.T, atol=1e-12)
+            assert abs(abs(np.linalg.det(T.T @ w)) - 1.0) < 1e-12
